@@ -1,0 +1,715 @@
+// api.hip -- C ABI of libgca.so: contexts, clouds, argument validation and the small utility
+// kernels (bounds, transform, upload/copy, scans).  The heavy kernels live in frontend.hip,
+// voxel.hip, grid.hip, normals.hip and icp.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <new>
+
+#include "gca_device.hpp"
+#include "gca_internal.hpp"
+
+namespace gca {
+
+int32_t hip_status(hipError_t e)
+{
+    if (e == hipErrorOutOfMemory) return GCA_ERR_OUT_OF_MEMORY;
+    return GCA_ERR_DEVICE;
+}
+
+void *ctx_scratch(gca_ctx ctx, int slot, size_t bytes, bool zeroed)
+{
+    if (bytes == 0) bytes = 16;
+    if (ctx->scratch_bytes[slot] < bytes) {
+        if (ctx->scratch[slot]) {
+            hipStreamSynchronize(ctx->stream);
+            hipFree(ctx->scratch[slot]);
+            ctx->scratch[slot] = nullptr;
+        }
+        size_t want = bytes + bytes / 2;
+        want = (want + 255) & ~size_t(255);
+        if (hipMalloc(&ctx->scratch[slot], want) != hipSuccess) {
+            ctx->scratch[slot] = nullptr;
+            ctx->scratch_bytes[slot] = 0;
+            return nullptr;
+        }
+        ctx->scratch_bytes[slot] = want;
+        ctx->scratch_zeroed[slot] = false;
+    }
+    if (zeroed && !ctx->scratch_zeroed[slot]) {
+        if (hipMemsetAsync(ctx->scratch[slot], 0, ctx->scratch_bytes[slot], ctx->stream) != hipSuccess)
+            return nullptr;
+        ctx->scratch_zeroed[slot] = true;
+    }
+    return ctx->scratch[slot];
+}
+
+// ---------------------------------------------------------------------------------------
+// utility kernels
+
+__global__ void k_bounds_init(CloudHdr *h)
+{
+    for (int a = 0; a < 3; ++a) {
+        h->bmin_key[a] = float_to_key(FLT_MAX);
+        h->bmax_key[a] = float_to_key(FLT_MIN);  // geometry_util.cuh:92 initialises max with FLT_MIN
+    }
+}
+
+// Block-reduced AABB, one atomic per block and component.
+__global__ __launch_bounds__(256) void k_bounds_reduce(const float4 *__restrict__ xyzi,
+                                                       CloudHdr *__restrict__ h)
+{
+    __shared__ float smin[3][4], smax[3][4];
+    const int n = h->n;
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 p = xyzi[i];
+        mn[0] = fminf(mn[0], p.x), mn[1] = fminf(mn[1], p.y), mn[2] = fminf(mn[2], p.z);
+        mx[0] = fmaxf(mx[0], p.x), mx[1] = fmaxf(mx[1], p.y), mx[2] = fmaxf(mx[2], p.z);
+    }
+    const int w = threadIdx.x / kWave;
+    for (int a = 0; a < 3; ++a) {
+        float a0 = wave_min(mn[a]), a1 = wave_max(mx[a]);
+        if (lane_id() == 0) smin[a][w] = a0, smax[a][w] = a1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int a = threadIdx.x;
+        float m0 = smin[a][0], m1 = smax[a][0];
+        for (int k = 1; k < 4; ++k) m0 = fminf(m0, smin[a][k]), m1 = fmaxf(m1, smax[a][k]);
+        if (m0 <= m1 || n > 0) {
+            atomicMin(&h->bmin_key[a], float_to_key(m0));
+            atomicMax(&h->bmax_key[a], float_to_key(m1));
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_transform(float4 *__restrict__ xyzi, float4 *__restrict__ nrm,
+                                                   float4 *__restrict__ grad, const CloudHdr *__restrict__ h,
+                                                   double t00, double t01, double t02, double t03,
+                                                   double t10, double t11, double t12, double t13,
+                                                   double t20, double t21, double t22, double t23,
+                                                   int do_n, int do_g)
+{
+    const int n = h->n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 p = xyzi[i];
+        double x = p.x, y = p.y, z = p.z;
+        p.x = (float)(((t00 * x + t01 * y) + t02 * z) + t03);
+        p.y = (float)(((t10 * x + t11 * y) + t12 * z) + t13);
+        p.z = (float)(((t20 * x + t21 * y) + t22 * z) + t23);
+        xyzi[i] = p;
+        if (do_n) {
+            float4 v = nrm[i];
+            double a = v.x, b = v.y, c = v.z;
+            v.x = (float)((t00 * a + t01 * b) + t02 * c);
+            v.y = (float)((t10 * a + t11 * b) + t12 * c);
+            v.z = (float)((t20 * a + t21 * b) + t22 * c);
+            nrm[i] = v;
+        }
+        if (do_g) {
+            float4 v = grad[i];
+            double a = v.x, b = v.y, c = v.z;
+            v.x = (float)((t00 * a + t01 * b) + t02 * c);
+            v.y = (float)((t10 * a + t11 * b) + t12 * c);
+            v.z = (float)((t20 * a + t21 * b) + t22 * c);
+            grad[i] = v;
+        }
+    }
+}
+
+// ---- exclusive scan of uint32 (reduce-then-scan, 4096 elements per chunk) -----------------
+constexpr int kScanBlock = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanChunk = kScanBlock * kScanPer;
+
+template <bool POPC>
+__device__ __forceinline__ uint32_t scan_val(uint32_t v) { return POPC ? (uint32_t)__popc(v) : v; }
+
+template <bool POPC>
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n,
+                                                            uint32_t *__restrict__ chunk_tot)
+{
+    __shared__ uint32_t lds[kScanBlock / kWave + 1];
+    int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k)
+        if (base + k < n) s += scan_val<POPC>(in[base + k]);
+    uint32_t tot;
+    block_excl_scan<kScanBlock>(s, lds, &tot);
+    if (threadIdx.x == 0) chunk_tot[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_chunks(uint32_t *__restrict__ chunk, int nchunks,
+                                                      uint32_t *__restrict__ total_dev,
+                                                      uint32_t *__restrict__ out_end)
+{
+    __shared__ uint32_t lds[1024 / kWave + 1];
+    const int per = (nchunks + 1023) / 1024;
+    const int b = threadIdx.x * per;
+    uint32_t s = 0;
+    for (int k = 0; k < per; ++k)
+        if (b + k < nchunks) s += chunk[b + k];
+    uint32_t tot;
+    uint32_t run = block_excl_scan<1024>(s, lds, &tot);
+    for (int k = 0; k < per; ++k)
+        if (b + k < nchunks) {
+            uint32_t t = chunk[b + k];
+            chunk[b + k] = run;
+            run += t;
+        }
+    if (threadIdx.x == 0) {
+        if (total_dev) *total_dev = tot;
+        if (out_end) *out_end = tot;
+    }
+}
+
+template <bool POPC>
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t *__restrict__ in, int64_t n,
+                                                           const uint32_t *__restrict__ chunk_pre,
+                                                           uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t lds[kScanBlock / kWave + 1];
+    int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+    uint32_t v[kScanPer];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        v[k] = (base + k < n) ? scan_val<POPC>(in[base + k]) : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan<kScanBlock>(s, lds, &tot) + chunk_pre[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k)
+        if (base + k < n) {
+            out[base + k] = run;
+            run += v[k];
+        }
+}
+
+int32_t exclusive_scan_u32(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n,
+                           uint32_t *total_dev, bool popcount)
+{
+    int64_t nchunks = (n + kScanChunk - 1) / kScanChunk;
+    if (nchunks < 1) nchunks = 1;
+    if (nchunks > 1024 * 64) return GCA_ERR_INVALID_VALUE;
+    uint32_t *chunk = (uint32_t *)ctx_scratch(ctx, 15, sizeof(uint32_t) * nchunks, false);
+    if (!chunk) return GCA_ERR_OUT_OF_MEMORY;
+    if (popcount)
+        hipLaunchKernelGGL(k_scan_reduce<true>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream,
+                           in, n, chunk);
+    else
+        hipLaunchKernelGGL(k_scan_reduce<false>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream,
+                           in, n, chunk);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, ctx->stream, chunk, (int)nchunks,
+                       total_dev, out + n);
+    if (popcount)
+        hipLaunchKernelGGL(k_scan_apply<true>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream,
+                           in, n, chunk, out);
+    else
+        hipLaunchKernelGGL(k_scan_apply<false>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream,
+                           in, n, chunk, out);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+
+int32_t cloud_reserve(gca_cloud c, int64_t cap)
+{
+    if (cap <= c->cap) return GCA_OK;
+    if (cap > (int64_t)INT_MAX / 2) return GCA_ERR_INVALID_VALUE;
+    GCA_HIP(hipStreamSynchronize(c->ctx->stream));
+    hipFree(c->xyzi);
+    hipFree(c->rgb);
+    hipFree(c->nrm);
+    hipFree(c->grad);
+    c->xyzi = c->rgb = c->nrm = c->grad = nullptr;
+    c->cap = 0;
+    size_t b = sizeof(float4) * (size_t)cap;
+    GCA_HIP(hipMalloc(&c->xyzi, b));
+    GCA_HIP(hipMalloc(&c->rgb, b));
+    GCA_HIP(hipMalloc(&c->nrm, b));
+    GCA_HIP(hipMalloc(&c->grad, b));
+    c->cap = cap;
+    return GCA_OK;
+}
+
+int32_t cloud_sync_header(gca_cloud c)
+{
+    if (c->host_valid) return GCA_OK;
+    GCA_HIP(hipMemcpyAsync(&c->host_hdr, c->hdr, sizeof(CloudHdr), hipMemcpyDeviceToHost,
+                           c->ctx->stream));
+    GCA_HIP(hipStreamSynchronize(c->ctx->stream));
+    c->host_valid = true;
+    return GCA_OK;
+}
+
+void cloud_host_bounds(gca_cloud c, float mn[3], float mx[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = key_to_float(c->host_hdr.bmin_key[a]);
+        mx[a] = key_to_float(c->host_hdr.bmax_key[a]);
+    }
+}
+
+void cloud_points_changed(gca_cloud c)
+{
+    c->host_valid = false;
+    c->grid.valid = false;
+}
+
+int32_t launch_bounds(gca_ctx ctx, gca_cloud c)
+{
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(1), 0, ctx->stream, c->hdr);
+    int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (c->cap + 255) / 256));
+    hipLaunchKernelGGL(k_bounds_reduce, dim3((unsigned)nb), dim3(256), 0, ctx->stream, c->xyzi, c->hdr);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
+}
+
+int32_t launch_transform(gca_ctx ctx, gca_cloud c, const double *T)
+{
+    int64_t nb = std::min<int64_t>(2048, std::max<int64_t>(1, (c->cap + 255) / 256));
+    hipLaunchKernelGGL(k_transform, dim3((unsigned)nb), dim3(256), 0, ctx->stream, c->xyzi, c->nrm,
+                       c->grad, c->hdr, T[0], T[1], T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9],
+                       T[10], T[11], c->has_normals ? 1 : 0, c->has_gradient ? 1 : 0);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
+}
+
+// Host-side canonical grid geometry; identical FP32 formulas to the oracle.
+void grid_params(const float bmin[3], const float bmax[3], float r, GridDesc *d)
+{
+    float pad = 1.5f * r;
+    float top[3];
+    for (int a = 0; a < 3; ++a) {
+        d->origin[a] = bmin[a] - pad;
+        top[a] = bmax[a] + pad;
+    }
+    float s = r;
+    for (;;) {
+        double total = 1.0;
+        for (int a = 0; a < 3; ++a) {
+            float q = (top[a] - d->origin[a]) / s;
+            d->dims[a] = (q < 2.0e9f) ? (int32_t)q + 1 : 2000000000;
+            total *= (double)d->dims[a];
+        }
+        if (total <= (double)kGridCellCap) break;
+        s = s * 2.0f;
+    }
+    d->cell = s;
+    d->radius = r;
+    d->ncells = (int64_t)d->dims[0] * d->dims[1] * d->dims[2];
+}
+
+int32_t grid_build(gca_ctx ctx, gca_cloud c, float radius, bool exact)
+{
+    if (c->grid.valid) {
+        if (c->grid.desc.radius == radius) return GCA_OK;
+        if (!exact && c->grid.desc.cell >= radius && c->grid.desc.cell <= 1.5f * radius) return GCA_OK;
+    }
+    GCA_TRY(cloud_sync_header(c));
+    float mn[3], mx[3];
+    cloud_host_bounds(c, mn, mx);
+    GridDesc d;
+    grid_params(mn, mx, radius, &d);
+    GCA_TRY(launch_grid_build(ctx, c, d));
+    return GCA_OK;
+}
+
+}  // namespace gca
+
+using namespace gca;
+
+// =========================================================================================
+// C ABI
+
+extern "C" {
+
+const char *gca_status_string(int32_t s)
+{
+    switch (s) {
+    case GCA_OK: return "ok";
+    case GCA_ERR_INVALID_VALUE: return "invalid value";
+    case GCA_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case GCA_ERR_DEVICE: return "device (HIP) error";
+    case GCA_ERR_NOT_SPD: return "normal equations not positive definite (identity step)";
+    case GCA_ERR_NOT_READY: return "missing prerequisite (normals / gradient / correspondences)";
+    default: return "unknown status";
+    }
+}
+
+int32_t gca_abi_version(void) { return GCA_ABI_VERSION; }
+
+int32_t gca_ctx_create(int32_t device, gca_ctx *out)
+{
+    if (!out) return GCA_ERR_INVALID_VALUE;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return GCA_ERR_DEVICE;
+    if (device < 0 || device >= ndev) return GCA_ERR_INVALID_VALUE;
+    GCA_HIP(hipSetDevice(device));
+    gca_ctx c = new (std::nothrow) gca_ctx_s();
+    if (!c) return GCA_ERR_OUT_OF_MEMORY;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess) {
+        delete c;
+        return GCA_ERR_DEVICE;
+    }
+    *out = c;
+    return GCA_OK;
+}
+
+int32_t gca_ctx_destroy(gca_ctx c)
+{
+    if (!c) return GCA_ERR_INVALID_VALUE;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 16; ++i)
+        if (c->level_clouds[i]) gca_cloud_destroy(c->level_clouds[i]);
+    for (int i = 0; i < 16; ++i) hipFree(c->scratch[i]);
+    hipFree(c->bitmap);
+    hipFree(c->lut);
+    hipFree(c->icp_state);
+    hipFree(c->partials);
+    hipEventDestroy(c->ev_a);
+    hipEventDestroy(c->ev_b);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return GCA_OK;
+}
+
+int32_t gca_ctx_synchronize(gca_ctx c)
+{
+    if (!c) return GCA_ERR_INVALID_VALUE;
+    GCA_HIP(hipStreamSynchronize(c->stream));
+    return GCA_OK;
+}
+
+int32_t gca_ctx_get_stream(gca_ctx c, void **stream)
+{
+    if (!c || !stream) return GCA_ERR_INVALID_VALUE;
+    *stream = (void *)c->stream;
+    return GCA_OK;
+}
+
+int32_t gca_cloud_create(gca_ctx ctx, int64_t capacity, gca_cloud *out)
+{
+    if (!ctx || !out || capacity < 0) return GCA_ERR_INVALID_VALUE;
+    *out = nullptr;
+    gca_cloud c = new (std::nothrow) gca_cloud_s();
+    if (!c) return GCA_ERR_OUT_OF_MEMORY;
+    c->ctx = ctx;
+    if (hipMalloc(&c->hdr, sizeof(CloudHdr)) != hipSuccess) {
+        delete c;
+        return GCA_ERR_OUT_OF_MEMORY;
+    }
+    CloudHdr h{};
+    h.n = 0;
+    for (int a = 0; a < 3; ++a) {
+        h.bmin_key[a] = float_to_key(FLT_MAX);
+        h.bmax_key[a] = float_to_key(FLT_MIN);
+    }
+    if (hipMemcpy(c->hdr, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(c->hdr);
+        delete c;
+        return GCA_ERR_DEVICE;
+    }
+    c->host_hdr = h;
+    c->host_valid = true;
+    int32_t s = cloud_reserve(c, std::max<int64_t>(capacity, 1));
+    if (s != GCA_OK) {
+        gca_cloud_destroy(c);
+        return s;
+    }
+    *out = c;
+    return GCA_OK;
+}
+
+static void grid_free(Grid &g)
+{
+    hipFree(g.count);
+    hipFree(g.start);
+    hipFree(g.order);
+    hipFree(g.sxyzi);
+    hipFree(g.cell_of);
+    g = Grid();
+}
+
+int32_t gca_cloud_destroy(gca_cloud c)
+{
+    if (!c) return GCA_ERR_INVALID_VALUE;
+    hipStreamSynchronize(c->ctx->stream);
+    hipFree(c->xyzi);
+    hipFree(c->rgb);
+    hipFree(c->nrm);
+    hipFree(c->grad);
+    hipFree(c->hdr);
+    grid_free(c->grid);
+    delete c;
+    return GCA_OK;
+}
+
+int32_t gca_cloud_size(gca_cloud c, int64_t *n)
+{
+    if (!c || !n) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(cloud_sync_header(c));
+    *n = c->host_hdr.n;
+    return GCA_OK;
+}
+
+int32_t gca_cloud_capacity(gca_cloud c, int64_t *cap)
+{
+    if (!c || !cap) return GCA_ERR_INVALID_VALUE;
+    *cap = c->cap;
+    return GCA_OK;
+}
+
+int32_t gca_cloud_has_normals(gca_cloud c, int32_t *flag)
+{
+    if (!c || !flag) return GCA_ERR_INVALID_VALUE;
+    *flag = c->has_normals ? 1 : 0;
+    return GCA_OK;
+}
+
+int32_t gca_cloud_upload(gca_cloud c, const float *xyz, const float *rgb, int64_t n)
+{
+    if (!c || n < 0 || (n > 0 && (!xyz || !rgb))) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(cloud_reserve(c, std::max<int64_t>(n, 1)));
+    std::vector<float4> a((size_t)n), b((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        float r = rgb[3 * i], g = rgb[3 * i + 1], bb = rgb[3 * i + 2];
+        // color3::to_intensity (type.hpp:45-51), evaluated exactly as on the device
+        float I = (float)(0.2126 * (double)r + 0.7152 * (double)g + 0.0722 * (double)bb);
+        a[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], I);
+        b[i] = make_float4(r, g, bb, 0.0f);
+    }
+    GCA_HIP(hipStreamSynchronize(c->ctx->stream));
+    if (n > 0) {
+        GCA_HIP(hipMemcpy(c->xyzi, a.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        GCA_HIP(hipMemcpy(c->rgb, b.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    }
+    int32_t n32 = (int32_t)n;
+    GCA_HIP(hipMemcpy(&c->hdr->n, &n32, sizeof(int32_t), hipMemcpyHostToDevice));
+    c->has_normals = c->has_gradient = false;
+    cloud_points_changed(c);
+    GCA_TRY(launch_bounds(c->ctx, c));
+    return GCA_OK;
+}
+
+static int32_t upload_vec3(gca_cloud c, float4 *dst, const float *v, int64_t n)
+{
+    int64_t have;
+    GCA_TRY(gca_cloud_size(c, &have));
+    if (n != have || (n > 0 && !v)) return GCA_ERR_INVALID_VALUE;
+    GCA_HIP(hipStreamSynchronize(c->ctx->stream));
+    std::vector<float4> a((size_t)n);
+    for (int64_t i = 0; i < n; ++i) a[i] = make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0f);
+    if (n > 0) GCA_HIP(hipMemcpy(dst, a.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    return GCA_OK;
+}
+
+int32_t gca_cloud_upload_normals(gca_cloud c, const float *normals, int64_t n)
+{
+    if (!c) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(upload_vec3(c, c->nrm, normals, n));
+    c->has_normals = true;
+    c->has_gradient = false;
+    return GCA_OK;
+}
+
+int32_t gca_cloud_upload_gradient(gca_cloud c, const float *gradient, int64_t n)
+{
+    if (!c) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(upload_vec3(c, c->grad, gradient, n));
+    c->has_gradient = true;
+    return GCA_OK;
+}
+
+int32_t gca_cloud_download(gca_cloud c, float *xyz, float *rgb, float *intensity, float *normals,
+                           float *gradient)
+{
+    if (!c) return GCA_ERR_INVALID_VALUE;
+    int64_t n;
+    GCA_TRY(gca_cloud_size(c, &n));
+    // the ctx stream is non-blocking: wait for queued producers before reading on the host
+    GCA_HIP(hipStreamSynchronize(c->ctx->stream));
+    std::vector<float4> a((size_t)n);
+    auto get = [&](const float4 *src, float *dst, bool with_w) -> int32_t {
+        if (!dst || n == 0) return GCA_OK;
+        GCA_HIP(hipMemcpy(a.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) {
+            dst[3 * i] = a[i].x;
+            dst[3 * i + 1] = a[i].y;
+            dst[3 * i + 2] = a[i].z;
+        }
+        (void)with_w;
+        return GCA_OK;
+    };
+    GCA_TRY(get(c->xyzi, xyz, false));
+    if (intensity && n > 0) {
+        GCA_HIP(hipMemcpy(a.data(), c->xyzi, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) intensity[i] = a[i].w;
+    }
+    GCA_TRY(get(c->rgb, rgb, false));
+    if (normals && !c->has_normals) return GCA_ERR_NOT_READY;
+    GCA_TRY(get(c->nrm, normals, false));
+    if (gradient && !c->has_gradient) return GCA_ERR_NOT_READY;
+    GCA_TRY(get(c->grad, gradient, false));
+    return GCA_OK;
+}
+
+int32_t gca_cloud_device_arrays(gca_cloud c, void **xyzi, void **rgb, void **normals,
+                                void **gradient, void **count)
+{
+    if (!c) return GCA_ERR_INVALID_VALUE;
+    if (xyzi) *xyzi = c->xyzi;
+    if (rgb) *rgb = c->rgb;
+    if (normals) *normals = c->nrm;
+    if (gradient) *gradient = c->grad;
+    if (count) *count = &c->hdr->n;
+    return GCA_OK;
+}
+
+int32_t gca_cloud_bounds(gca_cloud c, float mn[3], float mx[3])
+{
+    if (!c || !mn || !mx) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(cloud_sync_header(c));
+    cloud_host_bounds(c, mn, mx);
+    return GCA_OK;
+}
+
+int32_t gca_cloud_copy(gca_ctx ctx, gca_cloud src, gca_cloud dst)
+{
+    if (!ctx || !src || !dst || src == dst) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(cloud_reserve(dst, src->cap));
+    size_t b = sizeof(float4) * (size_t)src->cap;
+    GCA_HIP(hipMemcpyAsync(dst->xyzi, src->xyzi, b, hipMemcpyDeviceToDevice, ctx->stream));
+    GCA_HIP(hipMemcpyAsync(dst->rgb, src->rgb, b, hipMemcpyDeviceToDevice, ctx->stream));
+    if (src->has_normals)
+        GCA_HIP(hipMemcpyAsync(dst->nrm, src->nrm, b, hipMemcpyDeviceToDevice, ctx->stream));
+    if (src->has_gradient)
+        GCA_HIP(hipMemcpyAsync(dst->grad, src->grad, b, hipMemcpyDeviceToDevice, ctx->stream));
+    GCA_HIP(hipMemcpyAsync(dst->hdr, src->hdr, sizeof(CloudHdr), hipMemcpyDeviceToDevice, ctx->stream));
+    dst->has_normals = src->has_normals;
+    dst->has_gradient = src->has_gradient;
+    cloud_points_changed(dst);
+    return GCA_OK;
+}
+
+int32_t gca_cloud_from_rgbd(gca_ctx ctx, const uint16_t *depth_dev, const uint8_t *bgr_dev,
+                            const gca_camera *cam, const gca_bilateral *filter, float tmin,
+                            float tmax, gca_cloud out)
+{
+    // cuda_point_cloud_factory.cu:286-290
+    if (!ctx || !out || !cam || !depth_dev || !bgr_dev) return GCA_ERR_INVALID_VALUE;
+    if (cam->width <= 0 || cam->height <= 0 || !(cam->depth_scale > 0.0f)) return GCA_ERR_INVALID_VALUE;
+    if ((int64_t)cam->width * cam->height > (int64_t)INT_MAX / 4) return GCA_ERR_INVALID_VALUE;
+    gca_bilateral none{GCA_BILATERAL_NONE, 0, 1.0f, 1.0f};
+    const gca_bilateral *f = filter ? filter : &none;
+    if (f->mode < 0 || f->mode > 2) return GCA_ERR_INVALID_VALUE;
+    if (f->mode == GCA_BILATERAL_FIXED &&
+        (f->radius < 0 || f->radius > 15 || !(f->sigma_space > 0.0f) || !(f->sigma_depth > 0.0f)))
+        return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(launch_from_rgbd(ctx, depth_dev, bgr_dev, cam, f, tmin, tmax, out));
+    out->has_normals = out->has_gradient = false;
+    cloud_points_changed(out);
+    return GCA_OK;
+}
+
+int32_t gca_voxel_downsample(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out)
+{
+    if (!ctx || !in || !out || in == out) return GCA_ERR_INVALID_VALUE;
+    if (!(voxel > 0.0f)) return GCA_ERR_INVALID_VALUE;  // point_cloud.cu:200-204
+    GCA_TRY(launch_voxel(ctx, in, voxel, out));
+    out->has_normals = out->has_gradient = false;
+    return GCA_OK;
+}
+
+int32_t gca_radius_outlier_removal(gca_ctx ctx, gca_cloud in, float radius, int32_t min_nn,
+                                   gca_cloud out)
+{
+    if (!ctx || !in || !out || in == out) return GCA_ERR_INVALID_VALUE;
+    if (!(radius > 0.0f)) return GCA_ERR_INVALID_VALUE;  // point_cloud.cu:243-247
+    GCA_TRY(grid_build(ctx, in, radius));
+    GCA_TRY(launch_outlier(ctx, in, radius, min_nn, out));
+    out->has_normals = out->has_gradient = false;
+    cloud_points_changed(out);
+    return GCA_OK;
+}
+
+int32_t gca_nn_search(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t *out_dev)
+{
+    if (!ctx || !q || !ref || !out_dev) return GCA_ERR_INVALID_VALUE;
+    if (!(radius > 0.0f)) return GCA_ERR_INVALID_VALUE;  // point_cloud.cu:512-516
+    GCA_TRY(grid_build(ctx, ref, radius));
+    return launch_nn(ctx, q, ref, radius, out_dev);
+}
+
+int32_t gca_radius_search(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t max_nn,
+                          int32_t *offsets_dev, int32_t *idx_dev, int64_t idx_capacity,
+                          int64_t *total)
+{
+    if (!ctx || !q || !ref || !offsets_dev || !total || max_nn < 0 || idx_capacity < 0)
+        return GCA_ERR_INVALID_VALUE;
+    if (!(radius > 0.0f)) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(grid_build(ctx, ref, radius));
+    return launch_radius(ctx, q, ref, radius, max_nn, offsets_dev, idx_dev, idx_capacity, total);
+}
+
+int32_t gca_grid_cells(gca_ctx ctx, gca_cloud c, float radius, int32_t *cells_dev, float origin[3],
+                       float *cell_size, int32_t dims[3])
+{
+    if (!ctx || !c || !cells_dev) return GCA_ERR_INVALID_VALUE;
+    if (!(radius > 0.0f)) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(grid_build(ctx, c, radius, true));
+    GCA_TRY(launch_grid_cells(ctx, c, cells_dev));
+    for (int a = 0; a < 3; ++a) {
+        if (origin) origin[a] = c->grid.desc.origin[a];
+        if (dims) dims[a] = c->grid.desc.dims[a];
+    }
+    if (cell_size) *cell_size = c->grid.desc.cell;
+    return GCA_OK;
+}
+
+int32_t gca_estimate_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn)
+{
+    if (!ctx || !c || max_nn < 0) return GCA_ERR_INVALID_VALUE;
+    if (!(radius > 0.0f)) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(grid_build(ctx, c, radius));
+    GCA_TRY(launch_normals(ctx, c, radius, max_nn));
+    c->has_normals = true;
+    c->has_gradient = false;
+    return GCA_OK;
+}
+
+int32_t gca_compute_color_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn)
+{
+    if (!ctx || !c || max_nn < 0) return GCA_ERR_INVALID_VALUE;
+    if (!(radius > 0.0f)) return GCA_ERR_INVALID_VALUE;
+    if (!c->has_normals) return GCA_ERR_NOT_READY;  // color_icp.cu:168-174
+    GCA_TRY(grid_build(ctx, c, radius));
+    GCA_TRY(launch_gradient(ctx, c, radius, max_nn));
+    c->has_gradient = true;
+    return GCA_OK;
+}
+
+int32_t gca_transform(gca_ctx ctx, gca_cloud c, const double T[16])
+{
+    if (!ctx || !c || !T) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(launch_transform(ctx, c, T));
+    cloud_points_changed(c);
+    GCA_TRY(launch_bounds(ctx, c));  // point_cloud.cu:96-100 recomputes the bounds
+    return GCA_OK;
+}
+
+}  // extern "C"

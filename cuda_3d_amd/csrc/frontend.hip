@@ -1,0 +1,312 @@
+// frontend.hip -- depth + colour frame -> point cloud (K1).
+//
+// Replaces __kernel_make_pointcloud_Z16_BGR8 + remove_invalid_points
+// (src/geometry/cuda_point_cloud_factory.cu:189-313, src/geometry/geometry_util.cuh:124-130).
+//
+// Kernel A (one block = 256 consecutive pixels of one row): the depth rows y-R..y+R of the
+// segment are staged in LDS with coalesced 16-bit loads, the bilateral filter runs from LDS
+// with FP64 weights taken from host-built tables (bit-identical to exp() of the exact tap
+// arguments), then back-projection, depth->colour re-projection and BGR sampling in the
+// reference's FP32 operation order.  Results go to a per-pixel staging buffer and each block
+// records its count of valid pixels.
+// Kernel B: exclusive scan of the block counts (raster order == block order).
+// Kernel C: stable compaction (wave ballot + in-block prefix) into the SoA cloud, plus the
+// bounding box (block min/max, one atomic per block and axis).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+#include "gca_device.hpp"
+#include "gca_internal.hpp"
+
+namespace gca {
+
+constexpr int kRowSeg = 256;
+constexpr int kMaxR = 15;
+
+struct FrontParams {
+    int32_t W, H, nbx;
+    float scale, tmin, tmax;
+    gca_intrinsics di, ci;
+    float R[9];
+    float t[3];
+    int32_t mode;
+    int32_t radius;           // fixed mode
+    int32_t ws_off[6];        // per adaptive step (fixed mode uses [0])
+    int32_t wr_off[6];
+    int32_t wr_len[6];
+};
+
+__global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restrict__ depth,
+                                                         const uint8_t *__restrict__ bgr,
+                                                         const double *__restrict__ lut, FrontParams fp,
+                                                         float4 *__restrict__ st_xyzi,
+                                                         float4 *__restrict__ st_rgb,
+                                                         uint8_t *__restrict__ st_valid,
+                                                         uint32_t *__restrict__ block_cnt,
+                                                         CloudHdr *__restrict__ hdr)
+{
+    __shared__ uint16_t tile[2 * kMaxR + 1][kRowSeg + 2 * kMaxR];
+    __shared__ uint32_t wcount[kRowSeg / kWave];
+
+    const int W = fp.W, H = fp.H;
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * kRowSeg;
+    const int x = x0 + threadIdx.x;
+    int R = 0;
+    if (fp.mode == GCA_BILATERAL_FIXED) R = fp.radius;
+    else if (fp.mode == GCA_BILATERAL_REFERENCE_ADAPTIVE) R = 5;
+
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        for (int a = 0; a < 3; ++a) {
+            hdr->bmin_key[a] = float_to_key(FLT_MAX);
+            hdr->bmax_key[a] = float_to_key(FLT_MIN);  // geometry_util.cuh:92
+        }
+    }
+
+    // stage rows y-R .. y+R, columns x0-R .. x0+255+R (out of frame -> 0, which the filter
+    // skips exactly like the reference skips out-of-frame and zero taps)
+    const int rows = 2 * R + 1, cols = kRowSeg + 2 * R;
+    for (int k = threadIdx.x; k < rows * cols; k += kRowSeg) {
+        int ry = k / cols, rx = k - ry * cols;
+        int gy = y - R + ry, gx = x0 - R + rx;
+        uint16_t v = 0;
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W) v = depth[(int64_t)gy * W + gx];
+        tile[ry][rx] = v;
+    }
+    __syncthreads();
+
+    bool valid = false;
+    float4 oxyz = make_float4(0.f, 0.f, 0.f, 0.f), orgb = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (x < W) {
+        const uint16_t d0 = tile[R][threadIdx.x + R];
+        const float z0 = (float)d0 * fp.scale;
+        if (d0 != 0 && !(z0 < fp.tmin || z0 > fp.tmax)) {
+            double f = (double)d0;
+            int r = -1, s = 0;
+            if (fp.mode == GCA_BILATERAL_FIXED) {
+                r = fp.radius;
+            } else if (fp.mode == GCA_BILATERAL_REFERENCE_ADAPTIVE) {
+                // cuda_point_cloud_factory.cu:119-125
+                unsigned step = (unsigned)((z0 - fp.tmin) / 0.5f);
+                if (step > 5u) step = 5u;
+                s = (int)step;
+                r = (int)(1.0f * (float)step);
+            }
+            if (r >= 0) {
+                const double *ws = lut + fp.ws_off[s];
+                const double *wr = lut + fp.wr_off[s];
+                const int wl = fp.wr_len[s];
+                const int side = 2 * r + 1;
+                double sw = 0.0, swd = 0.0;
+                for (int dy = -r; dy <= r; ++dy) {
+                    const uint16_t *row = &tile[R + dy][threadIdx.x + R];
+                    for (int dx = -r; dx <= r; ++dx) {
+                        uint16_t dn = row[dx];
+                        if (!dn) continue;
+                        int diff = abs((int)d0 - (int)dn);
+                        double wrv = diff < wl ? wr[diff] : 0.0;
+                        double w = __dmul_rn(ws[(dy + r) * side + (dx + r)], wrv);
+                        sw = __dadd_rn(sw, w);
+                        swd = __dadd_rn(swd, __dmul_rn(w, (double)dn));
+                    }
+                }
+                f = swd / sw;
+            }
+            const float z = (float)f * fp.scale;
+            if (!(z < fp.tmin || z > fp.tmax)) {
+                const float u = (float)x - 0.5f, v = (float)y - 0.5f;
+                const float X = __fdiv_rn(__fmul_rn(__fsub_rn(u, fp.di.cx), z), fp.di.fx);
+                const float Y = __fdiv_rn(__fmul_rn(__fsub_rn(v, fp.di.cy), z), fp.di.fy);
+                const float Z = z;
+                const float c0 = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(fp.R[0], X), __fmul_rn(fp.R[3], Y)),
+                                                     __fmul_rn(fp.R[6], Z)), fp.t[0]);
+                const float c1 = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(fp.R[1], X), __fmul_rn(fp.R[4], Y)),
+                                                     __fmul_rn(fp.R[7], Z)), fp.t[1]);
+                const float c2 = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(fp.R[2], X), __fmul_rn(fp.R[5], Y)),
+                                                     __fmul_rn(fp.R[8], Z)), fp.t[2]);
+                if (c2 > 0.0f) {
+                    float fu = __fadd_rn(__fdiv_rn(__fmul_rn(c0, fp.ci.fx), c2), fp.ci.cx);
+                    float fv = __fadd_rn(__fdiv_rn(__fmul_rn(c1, fp.ci.fy), c2), fp.ci.cy);
+                    fu = __fadd_rn(fu, 0.5f);
+                    fv = __fadd_rn(fv, 0.5f);
+                    if (fu > -1.0f && fu < (float)W && fv > -1.0f && fv < (float)H) {
+                        const int tx = (int)fu, ty = (int)fv;
+                        const uint8_t *c = bgr + 3 * ((int64_t)ty * W + tx);
+                        const float inv255 = 1.0f / 255.0f;
+                        const float b = __fmul_rn((float)c[0], inv255);
+                        const float g = __fmul_rn((float)c[1], inv255);
+                        const float rr = __fmul_rn((float)c[2], inv255);
+                        oxyz = make_float4(X, Y, Z, intensity_of(rr, g, b));
+                        orgb = make_float4(rr, g, b, 0.0f);
+                        valid = true;
+                    }
+                }
+            }
+        }
+        const int64_t p = (int64_t)y * W + x;
+        st_xyzi[p] = oxyz;
+        st_rgb[p] = orgb;
+        st_valid[p] = valid ? 1 : 0;
+    }
+    uint64_t m = __ballot(valid);
+    if (lane_id() == 0) wcount[threadIdx.x / kWave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int i = 0; i < kRowSeg / kWave; ++i) s += wcount[i];
+        block_cnt[(int64_t)y * fp.nbx + blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kRowSeg) void k_rgbd_compact(const float4 *__restrict__ st_xyzi,
+                                                          const float4 *__restrict__ st_rgb,
+                                                          const uint8_t *__restrict__ st_valid,
+                                                          const uint32_t *__restrict__ block_off,
+                                                          int32_t W, int32_t nbx,
+                                                          float4 *__restrict__ out_xyzi,
+                                                          float4 *__restrict__ out_rgb,
+                                                          CloudHdr *__restrict__ hdr)
+{
+    __shared__ uint32_t woff[kRowSeg / kWave];
+    __shared__ float smin[3][kRowSeg / kWave], smax[3][kRowSeg / kWave];
+    const int y = blockIdx.y;
+    const int x = blockIdx.x * kRowSeg + threadIdx.x;
+    const int64_t p = (int64_t)y * W + x;
+    bool valid = (x < W) && st_valid[p];
+    uint64_t m = __ballot(valid);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) woff[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int i = 0; i < kRowSeg / kWave; ++i) {
+            uint32_t t = woff[i];
+            woff[i] = s;
+            s += t;
+        }
+    }
+    __syncthreads();
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    if (valid) {
+        uint64_t below = m & ((1ull << lane_id()) - 1ull);
+        uint32_t dst = block_off[(int64_t)y * nbx + blockIdx.x] + woff[w] + (uint32_t)__popcll(below);
+        float4 a = st_xyzi[p];
+        out_xyzi[dst] = a;
+        out_rgb[dst] = st_rgb[p];
+        mn[0] = mx[0] = a.x;
+        mn[1] = mx[1] = a.y;
+        mn[2] = mx[2] = a.z;
+    }
+    for (int a = 0; a < 3; ++a) {
+        float v0 = wave_min(mn[a]), v1 = wave_max(mx[a]);
+        if (lane_id() == 0) smin[a][w] = v0, smax[a][w] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int a = threadIdx.x;
+        float v0 = smin[a][0], v1 = smax[a][0];
+        for (int k = 1; k < kRowSeg / kWave; ++k) v0 = fminf(v0, smin[a][k]), v1 = fmaxf(v1, smax[a][k]);
+        if (v0 <= v1) {
+            atomicMin(&hdr->bmin_key[a], float_to_key(v0));
+            atomicMax(&hdr->bmax_key[a], float_to_key(v1));
+        }
+    }
+}
+
+// Host: FP64 weight tables, exactly exp() of the tap arguments the oracle evaluates.
+static int32_t build_lut(gca_ctx ctx, const gca_bilateral *f, FrontParams *fp)
+{
+    bool same = ctx->lut && ctx->lut_key_mode == f->mode;
+    if (same && f->mode == GCA_BILATERAL_FIXED)
+        same = ctx->lut_key_radius == f->radius && ctx->lut_key_ss == f->sigma_space &&
+               ctx->lut_key_sd == f->sigma_depth;
+    std::vector<double> tab;
+    int nsteps = (f->mode == GCA_BILATERAL_REFERENCE_ADAPTIVE) ? 6 : 1;
+    for (int s = 0; s < nsteps; ++s) {
+        int R;
+        double ss, sd;
+        if (f->mode == GCA_BILATERAL_REFERENCE_ADAPTIVE) {
+            R = (int)(1.0f * (float)s);
+            ss = (double)(8.0f + (float)s * 1.0f);
+            sd = (double)(100.0f + (float)s * 10.0f);
+        } else {
+            R = f->radius;
+            ss = (double)f->sigma_space;
+            sd = (double)f->sigma_depth;
+        }
+        fp->ws_off[s] = (int32_t)tab.size();
+        for (int dy = -R; dy <= R; ++dy)
+            for (int dx = -R; dx <= R; ++dx) tab.push_back(exp(-((double)(dx * dx + dy * dy)) / (ss * ss)));
+        fp->wr_off[s] = (int32_t)tab.size();
+        int len = 0;
+        for (int k = 0; k < 65536; ++k) {
+            double w = exp(-((double)k * (double)k) / (sd * sd));
+            tab.push_back(w);
+            ++len;
+            if (w == 0.0) break;
+        }
+        fp->wr_len[s] = len;
+    }
+    if (!same) {
+        size_t bytes = tab.size() * sizeof(double);
+        if (ctx->lut_bytes < bytes) {
+            GCA_HIP(hipStreamSynchronize(ctx->stream));
+            hipFree(ctx->lut);
+            ctx->lut = nullptr;
+            ctx->lut_bytes = 0;
+            GCA_HIP(hipMalloc(&ctx->lut, bytes));
+            ctx->lut_bytes = bytes;
+        }
+        GCA_HIP(hipStreamSynchronize(ctx->stream));
+        GCA_HIP(hipMemcpy(ctx->lut, tab.data(), bytes, hipMemcpyHostToDevice));
+        ctx->lut_key_mode = f->mode;
+        ctx->lut_key_radius = f->radius;
+        ctx->lut_key_ss = f->sigma_space;
+        ctx->lut_key_sd = f->sigma_depth;
+    }
+    return GCA_OK;
+}
+
+int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr, const gca_camera *cam,
+                         const gca_bilateral *bil, float tmin, float tmax, gca_cloud out)
+{
+    const int W = cam->width, H = cam->height;
+    const int64_t P = (int64_t)W * H;
+    FrontParams fp{};
+    fp.W = W;
+    fp.H = H;
+    fp.nbx = (W + kRowSeg - 1) / kRowSeg;
+    fp.scale = cam->depth_scale;
+    fp.tmin = tmin;
+    fp.tmax = tmax;
+    fp.di = cam->depth_intrin;
+    fp.ci = cam->color_intrin;
+    for (int k = 0; k < 9; ++k) fp.R[k] = cam->depth_to_color.rotation[k];
+    for (int k = 0; k < 3; ++k) fp.t[k] = cam->depth_to_color.translation[k];
+    fp.mode = bil->mode;
+    fp.radius = bil->radius;
+    if (bil->mode != GCA_BILATERAL_NONE) GCA_TRY(build_lut(ctx, bil, &fp));
+    GCA_TRY(cloud_reserve(out, P));
+    const int64_t nblocks = (int64_t)fp.nbx * H;
+    float4 *st_xyzi = (float4 *)ctx_scratch(ctx, 0, sizeof(float4) * P, false);
+    float4 *st_rgb = (float4 *)ctx_scratch(ctx, 1, sizeof(float4) * P, false);
+    uint8_t *st_valid = (uint8_t *)ctx_scratch(ctx, 2, (size_t)P, false);
+    uint32_t *bcnt = (uint32_t *)ctx_scratch(ctx, 3, sizeof(uint32_t) * (nblocks + 1), false);
+    uint32_t *boff = (uint32_t *)ctx_scratch(ctx, 4, sizeof(uint32_t) * (nblocks + 1), false);
+    if (!st_xyzi || !st_rgb || !st_valid || !bcnt || !boff) return GCA_ERR_OUT_OF_MEMORY;
+    dim3 grid(fp.nbx, H);
+    hipLaunchKernelGGL(k_rgbd_pixels, grid, dim3(kRowSeg), 0, ctx->stream, depth, bgr,
+                       (const double *)ctx->lut, fp, st_xyzi, st_rgb, st_valid, bcnt, out->hdr);
+    GCA_HIP(hipGetLastError());
+    GCA_TRY(exclusive_scan_u32(ctx, bcnt, boff, nblocks, (uint32_t *)&out->hdr->n));
+    hipLaunchKernelGGL(k_rgbd_compact, grid, dim3(kRowSeg), 0, ctx->stream, st_xyzi, st_rgb, st_valid,
+                       boff, W, fp.nbx, out->xyzi, out->rgb, out->hdr);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
+}
+
+}  // namespace gca

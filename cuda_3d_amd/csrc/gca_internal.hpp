@@ -1,0 +1,177 @@
+// gca_internal.hpp -- shared internals of libgca.so (HIP, gfx950 only).
+//
+// Data layout in HBM (structure of arrays, 16-byte aligned float4 records so that every hot
+// read is one dwordx4 load):
+//   xyzi[n]  float4 (x, y, z, intensity)      rgb[n]  float4 (r, g, b, 0)
+//   nrm[n]   float4 (nx, ny, nz, 0)           grad[n] float4 (dx, dy, dz, 0)
+// The point count and the bounding box live in a small device header (CloudHdr) so kernels
+// read sizes from device memory and launches are sized by capacity: no host round trip is
+// needed between pipeline stages.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/gca/gca.h"
+
+namespace gca {
+
+constexpr int kWave = 64;
+constexpr double kFx30 = 1073741824.0;       // 2^30: voxel sums fixed point
+constexpr double kFx40 = 1099511627776.0;    // 2^40: neighbourhood sums fixed point
+constexpr int64_t kGridCellCap = 1LL << 25;  // max cells of a neighbour grid
+
+// ---------------------------------------------------------------------------------------
+// Device header of a cloud.  Bounds are stored as order-preserving uint32 keys so that
+// atomicMin/atomicMax on unsigned ints give float min/max.
+struct CloudHdr {
+    int32_t n;
+    uint32_t bmin_key[3];
+    uint32_t bmax_key[3];
+    int32_t pad;
+};
+
+__host__ __device__ inline uint32_t float_to_key(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__host__ __device__ inline float key_to_float(uint32_t k)
+{
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// ---------------------------------------------------------------------------------------
+// Neighbour grid: dense cells of edge `cell` >= radius over the padded bounds, points
+// sorted by cell (counting sort).  Cell of a point = floor(((double)p - origin) / cell) in
+// FP64; a query at q scans the cells overlapped by [q - r, q + r], which makes the
+// neighbour set exactly { j : d2_fp64 < r^2 } whatever the cell edge.
+struct GridDesc {
+    float origin[3];
+    float cell;
+    int32_t dims[3];
+    float radius;
+    int64_t ncells;
+};
+
+struct Grid {
+    GridDesc desc{};
+    bool valid = false;
+    int64_t cap_cells = 0, cap_pts = 0;
+    uint32_t *count = nullptr;   // ncells, kept at zero between builds
+    uint32_t *start = nullptr;   // ncells + 1 (exclusive prefix)
+    int32_t *order = nullptr;    // original index of each sorted slot
+    float4 *sxyzi = nullptr;     // points in cell order
+    uint32_t *cell_of = nullptr; // linear cell of each original point (cap_pts), 32-bit
+};
+
+}  // namespace gca
+
+struct gca_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // scratch buffers owned by the ctx, grown on demand
+    void *scratch[16] = {nullptr};
+    size_t scratch_bytes[16] = {0};
+    bool scratch_zeroed[16] = {false};
+    // persistent bitmap of the voxel downsampler (kept all-zero between calls)
+    uint32_t *bitmap = nullptr;
+    int64_t bitmap_words = 0;
+    bool bitmap_dirty = false;
+    // bilateral weight tables (FP64, host-computed), keyed by their parameters
+    double *lut = nullptr;
+    size_t lut_bytes = 0;
+    int32_t lut_key_mode = -1, lut_key_radius = -1;
+    float lut_key_ss = -1.f, lut_key_sd = -1.f;
+    int32_t lut_len = 0;
+    // ICP device state
+    void *icp_state = nullptr;
+    size_t icp_state_bytes = 0;
+    double *partials = nullptr;
+    size_t partials_bytes = 0;
+    // per-level working clouds of the multi-scale driver (created on first use)
+    gca_cloud level_clouds[16] = {nullptr};
+};
+
+struct gca_cloud_s {
+    gca_ctx ctx = nullptr;
+    int64_t cap = 0;
+    float4 *xyzi = nullptr, *rgb = nullptr, *nrm = nullptr, *grad = nullptr;
+    gca::CloudHdr *hdr = nullptr;
+    bool has_normals = false, has_gradient = false;
+    // host copy of the header (valid when host_valid)
+    bool host_valid = false;
+    gca::CloudHdr host_hdr{};
+    // cached neighbour grid (built for grid.desc.radius); invalidated when points change
+    gca::Grid grid;
+};
+
+namespace gca {
+
+// Internal helpers implemented in api.hip -------------------------------------------------
+int32_t hip_status(hipError_t e);
+void *ctx_scratch(gca_ctx ctx, int slot, size_t bytes, bool zeroed);
+int32_t cloud_reserve(gca_cloud c, int64_t cap);
+int32_t cloud_sync_header(gca_cloud c);  // copies hdr to host (synchronises the stream)
+void cloud_host_bounds(gca_cloud c, float mn[3], float mx[3]);
+void cloud_points_changed(gca_cloud c);
+// (Re)builds c->grid so that it serves queries of `radius`.  A cached grid whose cell edge lies
+// in [radius, 1.5 radius] is reused (results do not depend on the cell edge); exact = build
+// for exactly this radius (canonical cell binning).
+int32_t grid_build(gca_ctx ctx, gca_cloud c, float radius, bool exact = false);
+// Grid descriptor of c->grid for queries of `radius`.
+inline GridDesc query_desc(const Grid &G, float radius)
+{
+    GridDesc d = G.desc;
+    d.radius = radius;
+    return d;
+}
+
+// Kernel launchers (one translation unit each) -------------------------------------------
+int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
+                         const gca_camera *cam, const gca_bilateral *bil, float tmin, float tmax,
+                         gca_cloud out);
+int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out);
+int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d);
+int32_t launch_nn(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t *out);
+int32_t launch_radius(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t max_nn,
+                      int32_t *offsets, int32_t *idx, int64_t idx_cap, int64_t *total);
+int32_t launch_grid_cells(gca_ctx ctx, gca_cloud c, int32_t *cells);
+int32_t launch_outlier(gca_ctx ctx, gca_cloud in, float radius, int32_t min_nn, gca_cloud out);
+int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn);
+int32_t launch_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn);
+int32_t launch_transform(gca_ctx ctx, gca_cloud c, const double T[16]);
+int32_t launch_bounds(gca_ctx ctx, gca_cloud c);
+int32_t launch_copy_cloud(gca_ctx ctx, gca_cloud src, gca_cloud dst);
+int32_t launch_upload_finish(gca_ctx ctx, gca_cloud c, int64_t n);
+
+// Host-side canonical grid geometry (same FP32 formulas as the oracle)
+void grid_params(const float bmin[3], const float bmax[3], float r, GridDesc *d);
+
+// Block-wide exclusive scan of uint32 arrays (3 kernels, reduce-then-scan)
+// out has n + 1 entries (out[n] = total); popcount = scan the popcounts of the words.
+int32_t exclusive_scan_u32(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n,
+                           uint32_t *total_dev, bool popcount = false);
+
+}  // namespace gca
+
+#define GCA_HIP(expr)                                   \
+    do {                                                \
+        hipError_t _e = (expr);                         \
+        if (_e != hipSuccess) return gca::hip_status(_e); \
+    } while (0)
+
+#define GCA_TRY(expr)                  \
+    do {                               \
+        int32_t _s = (expr);           \
+        if (_s != GCA_OK) return _s;   \
+    } while (0)

@@ -1,0 +1,341 @@
+// normals.hip -- PCA normals (K7) and colour gradient (K8), one wavefront per query point.
+//
+// Replaces cuda_estimate_normals (src/geometry/cuda_estimate_normals.cu:22-199) and
+// cuda_compute_color_gradient (src/registration/cuda_compute_color_gradient.cu:13-127).
+//
+// The reference materialises every (query, neighbour) pair, reduces 36-byte raw-moment
+// records by key in FP32 and solves the 3x3 eigenproblem with an FP32 trigonometric closed
+// form.  Here a wavefront walks the query's grid runs (64 candidates per step, coalesced
+// float4 reads), and the neighbourhood sums are accumulated in 2^-40 fixed point scaled by the
+// radius, centred on the query point: integer adds are exact, so the wave reduction order does
+// not matter and the result equals the FP64 oracle's.  The covariance is diagonalised with a
+// cyclic Jacobi iteration in FP64 (only +, -, *, /, sqrt: identical on host and device).
+// With a K-nearest cap, in-radius candidates are buffered in LDS and a candidate is kept iff
+// fewer than K others precede it in (d2, index) order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+
+#include "gca_device.hpp"
+#include "gca_internal.hpp"
+
+namespace gca {
+
+constexpr int kNbCap = 256;          // LDS candidates per wave
+constexpr int kWavesPerBlock = 4;
+
+struct NbBuf {
+    double d2[kNbCap];
+    int32_t idx[kNbCap];
+    int32_t slot[kNbCap];
+};
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct GridRefs {
+    GridDesc g;
+    const uint32_t *start;
+    const int32_t *order;
+    const float4 *sxyzi;
+};
+
+// Calls f(slot, idx, point) on the lanes for every selected neighbour of q; returns the
+// number of selected neighbours (wave-uniform).  K == 0: all within radius.
+template <class F>
+__device__ __forceinline__ int32_t for_each_neighbour(const GridRefs &G, float4 q, int32_t K, NbBuf &buf, F &&f)
+{
+    int32_t lo[3], hi[3];
+    if (!grid_range(G.g, q.x, q.y, q.z, lo, hi)) return 0;
+    const double r2 = (double)G.g.radius * (double)G.g.radius;
+    const int lane = lane_id();
+    int32_t count = 0;
+    wave_lds_sync();
+    for (int32_t ix = lo[0]; ix <= hi[0]; ++ix)
+        for (int32_t iy = lo[1]; iy <= hi[1]; ++iy) {
+            const int64_t row = ((int64_t)ix * G.g.dims[1] + iy) * G.g.dims[2];
+            const uint32_t s0 = G.start[row + lo[2]], s1 = G.start[row + hi[2] + 1];
+            for (uint32_t b = s0; b < s1; b += kWave) {
+                const uint32_t s = b + lane;
+                bool in = false;
+                double d2 = 0.0;
+                float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (s < s1) {
+                    p = G.sxyzi[s];
+                    d2 = dist2(p.x, p.y, p.z, q.x, q.y, q.z);
+                    in = d2 < r2;
+                }
+                if (K == 0) {
+                    if (in) f(s, G.order[s], p);
+                    count += __popcll(__ballot(in));
+                } else {
+                    const uint64_t m = __ballot(in);
+                    if (in) {
+                        const int32_t pos = count + __popcll(m & ((1ull << lane) - 1ull));
+                        if (pos < kNbCap) {
+                            buf.d2[pos] = d2;
+                            buf.idx[pos] = G.order[s];
+                            buf.slot[pos] = (int32_t)s;
+                        }
+                    }
+                    count += __popcll(m);
+                }
+            }
+        }
+    if (K == 0) return count;
+    wave_lds_sync();
+    if (count <= K) {
+        for (int32_t e = lane; e < count; e += kWave) f(buf.slot[e], buf.idx[e], G.sxyzi[buf.slot[e]]);
+        return count;
+    }
+    if (count <= kNbCap) {
+        for (int32_t e = lane; e < count; e += kWave) {
+            const double de = buf.d2[e];
+            const int32_t ie = buf.idx[e];
+            int32_t rank = 0;
+            for (int32_t j = 0; j < count; ++j) rank += key_less(buf.d2[j], buf.idx[j], de, ie) ? 1 : 0;
+            if (rank < K) f(buf.slot[e], ie, G.sxyzi[buf.slot[e]]);
+        }
+        return K;
+    }
+    // Overflow of the LDS buffer (very dense neighbourhood): rank each candidate against a
+    // rescan of the runs.  Slow but exact; only reached with > kNbCap points within radius.
+    for (int32_t ix = lo[0]; ix <= hi[0]; ++ix)
+        for (int32_t iy = lo[1]; iy <= hi[1]; ++iy) {
+            const int64_t row = ((int64_t)ix * G.g.dims[1] + iy) * G.g.dims[2];
+            const uint32_t s0 = G.start[row + lo[2]], s1 = G.start[row + hi[2] + 1];
+            for (uint32_t s = s0 + lane; s < s1; s += kWave) {
+                const float4 p = G.sxyzi[s];
+                const double de = dist2(p.x, p.y, p.z, q.x, q.y, q.z);
+                if (!(de < r2)) continue;
+                const int32_t ie = G.order[s];
+                int32_t rank = 0;
+                for (int32_t jx = lo[0]; jx <= hi[0] && rank < K; ++jx)
+                    for (int32_t jy = lo[1]; jy <= hi[1] && rank < K; ++jy) {
+                        const int64_t rw = ((int64_t)jx * G.g.dims[1] + jy) * G.g.dims[2];
+                        for (uint32_t t = G.start[rw + lo[2]]; t < G.start[rw + hi[2] + 1]; ++t) {
+                            const float4 c = G.sxyzi[t];
+                            const double dc = dist2(c.x, c.y, c.z, q.x, q.y, q.z);
+                            if (dc < r2 && key_less(dc, G.order[t], de, ie)) ++rank;
+                        }
+                    }
+                if (rank < K) f(s, ie, p);
+            }
+        }
+    return K;
+}
+
+// Cyclic Jacobi for a symmetric 3x3 (same operation sequence as the oracle's orc_eigen3).
+__host__ __device__ inline void jacobi3(double A[3][3], double v[3][3])
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) v[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 32; ++sweep) {
+        double off = (A[0][1] * A[0][1] + A[0][2] * A[0][2]) + A[1][2] * A[1][2];
+        double dg = (A[0][0] * A[0][0] + A[1][1] * A[1][1]) + A[2][2] * A[2][2];
+        if (!(off > 1e-36 * dg)) break;
+        for (int r = 0; r < 3; ++r) {
+            const int p = r == 2 ? 1 : 0;
+            const int q = r == 0 ? 1 : 2;
+            double apq = A[p][q];
+            if (apq == 0.0) continue;
+            double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+            double t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
+            if (theta < 0.0) t = -t;
+            double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+            for (int k = 0; k < 3; ++k) {
+                double akp = A[k][p], akq = A[k][q];
+                A[k][p] = c * akp - s * akq;
+                A[k][q] = s * akp + c * akq;
+            }
+            for (int k = 0; k < 3; ++k) {
+                double apk = A[p][k], aqk = A[q][k];
+                A[p][k] = c * apk - s * aqk;
+                A[q][k] = s * apk + c * aqk;
+            }
+            A[p][q] = 0.0;
+            A[q][p] = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                double vkp = v[k][p], vkq = v[k][q];
+                v[k][p] = c * vkp - s * vkq;
+                v[k][q] = s * vkp + c * vkq;
+            }
+        }
+    }
+}
+
+__host__ __device__ inline float4 normal_from_moments(int32_t cnt, const long long S[9])
+{
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (cnt < 3) return out;  // cuda_estimate_normals.cu:89-93
+    const double n = (double)cnt;
+    double m[3], A[3][3], V[3][3];
+    for (int a = 0; a < 3; ++a) m[a] = ((double)S[a] / kFx40) / n;
+    const int IA[6] = {0, 0, 0, 1, 1, 2}, IB[6] = {0, 1, 2, 1, 2, 2};
+    for (int k = 0; k < 6; ++k) {
+        double c = ((double)S[3 + k] / kFx40) / n - m[IA[k]] * m[IB[k]];
+        A[IA[k]][IB[k]] = c;
+        A[IB[k]][IA[k]] = c;
+    }
+    jacobi3(A, V);
+    const double e0 = A[0][0], e1 = A[1][1], e2 = A[2][2];
+    int sel;  // strict-< selection (cuda_estimate_normals.cu:116-125)
+    if (e0 < e1 && e0 < e2) sel = 0;
+    else if (e1 < e0 && e1 < e2) sel = 1;
+    else sel = 2;
+    double v[3] = {V[0][sel], V[1][sel], V[2][sel]};
+    double len = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    if (!(len > 0.0)) return out;
+    for (int a = 0; a < 3; ++a) v[a] /= len;
+    if (-v[2] < 0.0)  // orient toward (0,0,-1) (cuda_estimate_normals.cu:131-151)
+        for (int a = 0; a < 3; ++a) v[a] = -v[a];
+    out.x = (float)v[0];
+    out.y = (float)v[1];
+    out.z = (float)v[2];
+    return out;
+}
+
+__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_normals(const float4 *__restrict__ xyzi,
+                                                                  const CloudHdr *__restrict__ hdr,
+                                                                  GridDesc g, const uint32_t *__restrict__ start,
+                                                                  const int32_t *__restrict__ order,
+                                                                  const float4 *__restrict__ sxyzi,
+                                                                  int32_t K, float4 *__restrict__ nrm)
+{
+    __shared__ NbBuf bufs[kWavesPerBlock];
+    NbBuf &buf = bufs[threadIdx.x / kWave];
+    const GridRefs G{g, start, order, sxyzi};
+    const int n = hdr->n;
+    const double inv_r = 1.0 / (double)g.radius;
+    const int nw = gridDim.x * kWavesPerBlock;
+    for (int qi = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; qi < n; qi += nw) {
+        const float4 q = xyzi[qi];
+        long long S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int32_t cnt = for_each_neighbour(G, q, K, buf, [&](uint32_t, int32_t, float4 p) {
+            const double u0 = ((double)p.x - (double)q.x) * inv_r;
+            const double u1 = ((double)p.y - (double)q.y) * inv_r;
+            const double u2 = ((double)p.z - (double)q.z) * inv_r;
+            S[0] += fx(u0 * kFx40);
+            S[1] += fx(u1 * kFx40);
+            S[2] += fx(u2 * kFx40);
+            S[3] += fx((u0 * u0) * kFx40);
+            S[4] += fx((u0 * u1) * kFx40);
+            S[5] += fx((u0 * u2) * kFx40);
+            S[6] += fx((u1 * u1) * kFx40);
+            S[7] += fx((u1 * u2) * kFx40);
+            S[8] += fx((u2 * u2) * kFx40);
+        });
+#pragma unroll
+        for (int k = 0; k < 9; ++k) S[k] = wave_sum(S[k]);
+        if (lane_id() == 0) nrm[qi] = normal_from_moments(cnt, S);
+    }
+}
+
+__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_gradient(const float4 *__restrict__ xyzi,
+                                                                   const float4 *__restrict__ nrm,
+                                                                   const CloudHdr *__restrict__ hdr,
+                                                                   GridDesc g, const uint32_t *__restrict__ start,
+                                                                   const int32_t *__restrict__ order,
+                                                                   const float4 *__restrict__ sxyzi,
+                                                                   int32_t K, float4 *__restrict__ grad)
+{
+    __shared__ NbBuf bufs[kWavesPerBlock];
+    NbBuf &buf = bufs[threadIdx.x / kWave];
+    const GridRefs G{g, start, order, sxyzi};
+    const int n = hdr->n;
+    const double inv_r = 1.0 / (double)g.radius;
+    const double rr = (double)g.radius * (double)g.radius;
+    const int nw = gridDim.x * kWavesPerBlock;
+    for (int qi = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; qi < n; qi += nw) {
+        const float4 q = xyzi[qi];
+        const float4 nf = nrm[qi];
+        const double nv[3] = {(double)nf.x, (double)nf.y, (double)nf.z};
+        const double Ip = (double)q.w;
+        long long S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // A: xx xy xz yy yz zz, b: x y z
+        const int32_t k = for_each_neighbour(G, q, K, buf, [&](uint32_t, int32_t idx, float4 p) {
+            if (idx == qi) return;  // cuda_compute_color_gradient.cu:61-62
+            const double e0 = (double)p.x - (double)q.x;
+            const double e1 = (double)p.y - (double)q.y;
+            const double e2 = (double)p.z - (double)q.z;
+            const double tn = (e0 * nv[0] + e1 * nv[1]) + e2 * nv[2];
+            const double w0 = (e0 - tn * nv[0]) * inv_r;
+            const double w1 = (e1 - tn * nv[1]) * inv_r;
+            const double w2 = (e2 - tn * nv[2]) * inv_r;
+            const double dI = (double)p.w - Ip;
+            S[0] += fx((w0 * w0) * kFx40);
+            S[1] += fx((w0 * w1) * kFx40);
+            S[2] += fx((w0 * w2) * kFx40);
+            S[3] += fx((w1 * w1) * kFx40);
+            S[4] += fx((w1 * w2) * kFx40);
+            S[5] += fx((w2 * w2) * kFx40);
+            S[6] += fx((w0 * dI) * kFx40);
+            S[7] += fx((w1 * dI) * kFx40);
+            S[8] += fx((w2 * dI) * kFx40);
+        });
+#pragma unroll
+        for (int c = 0; c < 9; ++c) S[c] = wave_sum(S[c]);
+        if (lane_id() == 0) {
+            float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (k >= 5) {  // cuda_compute_color_gradient.cu:41-44
+                const double kk = (double)(k - 1) * (double)(k - 1);
+                const int IA[6] = {0, 0, 0, 1, 1, 2}, IB[6] = {0, 1, 2, 1, 2, 2};
+                double A[3][3];
+                for (int c = 0; c < 6; ++c) {
+                    double v = ((double)S[c] / kFx40) * rr + kk * (nv[IA[c]] * nv[IB[c]]);
+                    A[IA[c]][IB[c]] = v;
+                    A[IB[c]][IA[c]] = v;
+                }
+                for (int a = 0; a < 3; ++a) A[a][a] += 1.0e-6;
+                double b[3];
+                for (int a = 0; a < 3; ++a) b[a] = ((double)S[6 + a] / kFx40) * (double)g.radius;
+                // unpivoted LDL^T (little_3x3_ldlt_solve.cuh:11-60)
+                double D0 = A[0][0];
+                double L10 = A[1][0] / D0, L20 = A[2][0] / D0;
+                double D1 = A[1][1] - L10 * L10 * D0;
+                double L21 = (A[2][1] - L10 * L20 * D0) / D1;
+                double D2 = A[2][2] - L20 * L20 * D0 - L21 * L21 * D1;
+                double x0 = b[0];
+                double x1 = b[1] - L10 * x0;
+                double x2 = b[2] - L20 * x0 - L21 * x1;
+                x0 /= D0;
+                x1 /= D1;
+                x2 /= D2;
+                x1 -= L21 * x2;
+                x0 -= L10 * x1 + L20 * x2;
+                out = make_float4((float)x0, (float)x1, (float)x2, 0.0f);
+            }
+            grad[qi] = out;
+        }
+    }
+}
+
+static unsigned wave_blocks(int64_t n)
+{
+    return (unsigned)std::min<int64_t>(8192, std::max<int64_t>(1, (n + kWavesPerBlock - 1) / kWavesPerBlock));
+}
+
+int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
+{
+    const Grid &G = c->grid;
+    hipLaunchKernelGGL(k_normals, dim3(wave_blocks(c->cap)), dim3(kWave * kWavesPerBlock), 0, ctx->stream,
+                       c->xyzi, c->hdr, query_desc(G, radius), G.start, G.order, G.sxyzi, K, c->nrm);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
+}
+
+int32_t launch_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
+{
+    const Grid &G = c->grid;
+    hipLaunchKernelGGL(k_gradient, dim3(wave_blocks(c->cap)), dim3(kWave * kWavesPerBlock), 0, ctx->stream,
+                       c->xyzi, c->nrm, c->hdr, query_desc(G, radius), G.start, G.order, G.sxyzi, K, c->grad);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
+}
+
+}  // namespace gca

@@ -1,0 +1,170 @@
+// voxel.hip -- voxel-grid downsample (K3).
+//
+// Replaces cuda_voxel_grid_downsample (src/geometry/cuda_voxel_grid_down_sample.cu:123-161)
+// and point_cloud::voxel_grid_down_sample (src/geometry/point_cloud.cu:198-229).
+//
+// The reference sorts int3 keys with a comparison sort and reduces by key.  Here the output
+// order (lexicographic (kx, ky, kz) == ascending linear key) comes from an occupancy bitmap
+// over the voxel lattice instead of a sort:
+//   1. mark: key in the reference's FP32 formula, set bit L = (kx*ny + ky)*nz + kz;
+//   2. popcount prefix over the bitmap words (reduce-then-scan) -> rank of every occupied
+//      voxel in lexicographic order, and the voxel count M;
+//   3. accumulate: each point adds its coordinates and colour in 2^-30 fixed point (int64
+//      atomics: exact, so the sums do not depend on the order in which points arrive);
+//   4. finalize: mean = sum / count, intensity from the mean colour; the bitmap words and
+//      accumulators are cleared on the way out, so no memset is needed before the next call.
+// The output keeps the input's bounding box, as the reference does (point_cloud.cu:224-226).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+#include "gca_device.hpp"
+#include "gca_internal.hpp"
+
+namespace gca {
+
+struct VoxelParams {
+    float bmin[3];
+    float voxel;
+    int64_t dims[3];
+};
+
+__device__ __forceinline__ int64_t voxel_key(const VoxelParams &vp, float4 p)
+{
+    const float c[3] = {p.x, p.y, p.z};
+    int64_t k[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        // __float2int_rd((p - min) / v) in FP32 (cuda_voxel_grid_down_sample.cu:37-47)
+        float q = floorf(__fdiv_rn(__fsub_rn(c[a], vp.bmin[a]), vp.voxel));
+        int64_t v = (int64_t)q;
+        v = v < 0 ? 0 : (v >= vp.dims[a] ? vp.dims[a] - 1 : v);
+        k[a] = v;
+    }
+    return (k[0] * vp.dims[1] + k[1]) * vp.dims[2] + k[2];
+}
+
+__global__ __launch_bounds__(256) void k_voxel_mark(const float4 *__restrict__ xyzi,
+                                                    const CloudHdr *__restrict__ hdr, VoxelParams vp,
+                                                    uint32_t *__restrict__ bitmap,
+                                                    int64_t *__restrict__ lin)
+{
+    const int n = hdr->n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int64_t L = voxel_key(vp, xyzi[i]);
+        lin[i] = L;
+        atomicOr(&bitmap[L >> 5], 1u << (uint32_t)(L & 31));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_voxel_accum(const float4 *__restrict__ xyzi,
+                                                     const float4 *__restrict__ rgb,
+                                                     const CloudHdr *__restrict__ hdr,
+                                                     const uint32_t *__restrict__ bitmap,
+                                                     const uint32_t *__restrict__ word_pre,
+                                                     const int64_t *__restrict__ lin,
+                                                     unsigned long long *__restrict__ acc)
+{
+    const int n = hdr->n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int64_t L = lin[i];
+        const uint32_t w = bitmap[L >> 5];
+        const uint32_t below = w & ((1u << (uint32_t)(L & 31)) - 1u);
+        const uint32_t r = word_pre[L >> 5] + (uint32_t)__popc(below);
+        unsigned long long *a = acc + 8 * (int64_t)r;
+        const float4 p = xyzi[i], c = rgb[i];
+        atomicAdd(&a[0], (unsigned long long)fx((double)p.x * kFx30));
+        atomicAdd(&a[1], (unsigned long long)fx((double)p.y * kFx30));
+        atomicAdd(&a[2], (unsigned long long)fx((double)p.z * kFx30));
+        atomicAdd(&a[3], (unsigned long long)fx((double)c.x * kFx30));
+        atomicAdd(&a[4], (unsigned long long)fx((double)c.y * kFx30));
+        atomicAdd(&a[5], (unsigned long long)fx((double)c.z * kFx30));
+        atomicAdd(&a[6], 1ull);
+        a[7] = (unsigned long long)L;  // same value from every point of the voxel
+    }
+}
+
+__global__ __launch_bounds__(256) void k_voxel_finalize(unsigned long long *__restrict__ acc,
+                                                        const CloudHdr *__restrict__ in_hdr,
+                                                        CloudHdr *__restrict__ out_hdr,
+                                                        uint32_t *__restrict__ bitmap,
+                                                        float4 *__restrict__ out_xyzi,
+                                                        float4 *__restrict__ out_rgb)
+{
+    const int m = out_hdr->n;
+    if (blockIdx.x == 0 && threadIdx.x < 3) {
+        out_hdr->bmin_key[threadIdx.x] = in_hdr->bmin_key[threadIdx.x];
+        out_hdr->bmax_key[threadIdx.x] = in_hdr->bmax_key[threadIdx.x];
+    }
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x) {
+        unsigned long long *a = acc + 8 * (int64_t)r;
+        const double cnt = (double)(long long)a[6];
+        float o[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) o[c] = (float)(((double)(long long)a[c] / cnt) * (1.0 / kFx30));
+        out_xyzi[r] = make_float4(o[0], o[1], o[2], intensity_of(o[3], o[4], o[5]));
+        out_rgb[r] = make_float4(o[3], o[4], o[5], 0.0f);
+        const int64_t L = (int64_t)a[7];
+        bitmap[L >> 5] = 0u;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) a[c] = 0ull;
+    }
+}
+
+int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out)
+{
+    GCA_TRY(cloud_sync_header(in));
+    const int64_t n = in->host_hdr.n;
+    float mn[3], mx[3];
+    cloud_host_bounds(in, mn, mx);
+    VoxelParams vp{};
+    vp.voxel = voxel;
+    double nbits = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        vp.bmin[a] = mn[a];
+        float ext = mx[a] - mn[a];
+        if (n > 0 && (double)voxel * (double)INT_MAX < (double)ext)  // point_cloud.cu:213-218
+            return GCA_ERR_INVALID_VALUE;
+        vp.dims[a] = n > 0 ? (int64_t)floorf(ext / voxel) + 1 : 1;
+        nbits *= (double)vp.dims[a];
+    }
+    if (nbits > 17179869184.0) return GCA_ERR_INVALID_VALUE;  // > 2^34 voxels: voxel too small
+    GCA_TRY(cloud_reserve(out, std::max<int64_t>(n, 1)));
+    const int64_t words = ((int64_t)nbits + 31) / 32;
+    if (ctx->bitmap_words < words || ctx->bitmap_dirty) {
+        GCA_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->bitmap_words < words) {
+            hipFree(ctx->bitmap);
+            ctx->bitmap = nullptr;
+            ctx->bitmap_words = 0;
+            int64_t want = std::max<int64_t>(words + words / 4, 1 << 16);
+            GCA_HIP(hipMalloc(&ctx->bitmap, sizeof(uint32_t) * want));
+            ctx->bitmap_words = want;
+        }
+        GCA_HIP(hipMemsetAsync(ctx->bitmap, 0, sizeof(uint32_t) * ctx->bitmap_words, ctx->stream));
+        ctx->bitmap_dirty = false;
+    }
+    int64_t *lin = (int64_t *)ctx_scratch(ctx, 5, sizeof(int64_t) * std::max<int64_t>(n, 1), false);
+    uint32_t *word_pre = (uint32_t *)ctx_scratch(ctx, 6, sizeof(uint32_t) * (words + 1), false);
+    unsigned long long *acc =
+        (unsigned long long *)ctx_scratch(ctx, 7, sizeof(unsigned long long) * 8 * std::max<int64_t>(n, 1), true);
+    if (!lin || !word_pre || !acc) return GCA_ERR_OUT_OF_MEMORY;
+    const unsigned nb = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (n + 255) / 256));
+    ctx->bitmap_dirty = true;  // until finalize has run to completion
+    hipLaunchKernelGGL(k_voxel_mark, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->hdr, vp, ctx->bitmap, lin);
+    GCA_HIP(hipGetLastError());
+    GCA_TRY(exclusive_scan_u32(ctx, ctx->bitmap, word_pre, words, (uint32_t *)&out->hdr->n, true));
+    hipLaunchKernelGGL(k_voxel_accum, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->rgb, in->hdr,
+                       ctx->bitmap, word_pre, lin, acc);
+    hipLaunchKernelGGL(k_voxel_finalize, dim3(nb), dim3(256), 0, ctx->stream, acc, in->hdr, out->hdr,
+                       ctx->bitmap, out->xyzi, out->rgb);
+    GCA_HIP(hipGetLastError());
+    ctx->bitmap_dirty = false;
+    cloud_points_changed(out);
+    return GCA_OK;
+}
+
+}  // namespace gca
