@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "libgca.so")
 HEADER = os.path.join(HERE, "..", "include", "gca", "gca.h")
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError(f"libgca.so not built ({LIB_PATH}); run python -m cuda_3d_amd.build")
+    raise ImportError(f"libgca.so not built ({LIB_PATH}); run python cuda_3d_amd/build.py")
 lib = C.CDLL(LIB_PATH)
 
 GCA_OK, GCA_ERR_INVALID_VALUE, GCA_ERR_OUT_OF_MEMORY, GCA_ERR_DEVICE, GCA_ERR_NOT_SPD, \
@@ -93,6 +93,7 @@ _SIGS = {
     "gca_cloud_capacity": (i32, [vp, P(i64)]),
     "gca_cloud_has_normals": (i32, [vp, P(i32)]),
     "gca_cloud_upload": (i32, [vp, vp, vp, i64]),
+    "gca_cloud_upload_device": (i32, [vp, vp, vp, vp, i64]),
     "gca_cloud_upload_normals": (i32, [vp, vp, i64]),
     "gca_cloud_upload_gradient": (i32, [vp, vp, i64]),
     "gca_cloud_download": (i32, [vp, vp, vp, vp, vp, vp]),

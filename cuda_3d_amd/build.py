@@ -1,7 +1,8 @@
 """Build libgca.so (HIP, gfx950) in-tree: cuda_3d_amd/libgca.so.
 
-    python -m cuda_3d_amd.build          # compile every source, link
-    python -m cuda_3d_amd.build --check  # report whether the library is present
+    python cuda_3d_amd/build.py          # compile every source, link
+    python cuda_3d_amd/build.py --check  # report whether the library is present
+(run it as a script: importing the package loads the library it is about to replace)
 
 Flags: -ffp-contract=off keeps every floating-point operation a single IEEE rounding (the
 canonical discrete rules and the bit-exact FP64 paths depend on it); packed FP32 ops are

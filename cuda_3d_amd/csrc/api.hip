@@ -506,6 +506,27 @@ int32_t gca_cloud_upload(gca_cloud c, const float *xyz, const float *rgb, int64_
     return GCA_OK;
 }
 
+__global__ void k_set_count(CloudHdr *h, int32_t n) { h->n = n; }
+
+int32_t gca_cloud_upload_device(gca_ctx ctx, gca_cloud c, const void *xyzi, const void *rgb, int64_t n)
+{
+    if (!ctx || !c || n < 0 || (n > 0 && !xyzi) || n > (int64_t)INT_MAX / 4) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(cloud_reserve(c, std::max<int64_t>(n, 1)));
+    if (n > 0) {
+        GCA_HIP(hipMemcpyAsync(c->xyzi, xyzi, sizeof(float4) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        if (rgb)
+            GCA_HIP(hipMemcpyAsync(c->rgb, rgb, sizeof(float4) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        else
+            GCA_HIP(hipMemsetAsync(c->rgb, 0, sizeof(float4) * n, ctx->stream));
+    }
+    hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, ctx->stream, c->hdr, (int32_t)n);
+    GCA_HIP(hipGetLastError());
+    c->has_normals = c->has_gradient = false;
+    cloud_points_changed(c);
+    GCA_TRY(launch_bounds(ctx, c));
+    return GCA_OK;
+}
+
 static int32_t upload_vec3(gca_cloud c, float4 *dst, const float *v, int64_t n)
 {
     int64_t have;

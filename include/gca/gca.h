@@ -153,6 +153,12 @@ int32_t gca_cloud_has_normals(gca_cloud cloud, int32_t *flag);
 /* Host xyz (n x 3) and rgb (n x 3, [0,1]) -> device cloud; intensity is derived as in
  * color3::to_intensity.  Reference: point_cloud::create_from_pcl (point_cloud.cu:481-504). */
 int32_t gca_cloud_upload(gca_cloud cloud, const float *xyz, const float *rgb, int64_t n);
+/* Device-resident points: xyzi_dev = n float4 (x, y, z, intensity), rgb_dev = n float4 or NULL
+ * (colour zeroed), copied on the ctx stream.  The caller orders its own writes to the sources
+ * before the call.  Same semantics as gca_cloud_upload (create_from_pcl, point_cloud.cu:481-504)
+ * for points that already live in device memory, e.g. torch tensors. */
+int32_t gca_cloud_upload_device(gca_ctx ctx, gca_cloud cloud, const void *xyzi_dev,
+                                const void *rgb_dev, int64_t n);
 /* Optional host normals (n x 3) for a cloud whose normals come from elsewhere. */
 int32_t gca_cloud_upload_normals(gca_cloud cloud, const float *normals, int64_t n);
 int32_t gca_cloud_upload_gradient(gca_cloud cloud, const float *gradient, int64_t n);

@@ -10,6 +10,7 @@ file:line of every step).  Arrays are numpy, float4 SoA as in the product
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -70,12 +71,35 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def build_variant(plant: int, out_dir: str) -> str:
+    """Compile the oracle with a planted mistake (-DORC_PLANT=plant) into out_dir."""
+    out = os.path.join(out_dir, f"libgca_oracle_plant{plant}.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-std=c11", "-ffp-contract=off", f"-DORC_PLANT={plant}",
+                    "-shared", "-o", out, os.path.join(_HERE, "gca_oracle.c"), "-lm"], check=True)
+    return out
+
+
+_path = _LIB_PATH
+
+
+@contextlib.contextmanager
+def using(path: str):
+    """Temporarily route every wrapper through another build of the oracle."""
+    global _lib, _path
+    old = (_lib, _path)
+    _lib, _path = None, path
+    try:
+        yield
+    finally:
+        _lib, _path = old
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+        if _path == _LIB_PATH and not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(_path)
         vp = C.c_void_p
         L.orc_cloud_from_rgbd.restype = C.c_int64
         L.orc_cloud_from_rgbd.argtypes = [u16p, u8p, C.POINTER(Camera), C.POINTER(Bilateral),

@@ -26,6 +26,12 @@
 #define FX30 1073741824.0          /* 2^30 */
 #define FX40 1099511627776.0       /* 2^40 */
 
+/* ORC_PLANT selects a deliberately wrong variant, compiled only by the test that checks the
+ * pins reject it (tests/test_oracle.py::test_planted_mistakes_are_caught); 0 = correct. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 /* ------------------------------------------------------------------------------------------ */
 /* Front end                                                                                 */
 
@@ -48,7 +54,11 @@ static double bilateral_px(const uint16_t *depth, int W, int H, int x, int y, in
             uint16_t dn = depth[(int64_t)ny * W + nx];
             if (!dn) continue;
             int diff = abs((int)d0 - (int)dn);
+#if ORC_PLANT == 6 /* textbook Gaussian with the 1/2 factor (reference has none) */
+            double ws = exp(-((double)(dx * dx + dy * dy)) / (2.0 * ss * ss));
+#else
             double ws = exp(-((double)(dx * dx + dy * dy)) / (ss * ss));
+#endif
             double wr = exp(-((double)diff * (double)diff) / (sd * sd));
             double w = ws * wr;
             sw += w;
@@ -309,7 +319,11 @@ static int cmp_cand(const void *a, const void *b)
 {
     const cand *x = (const cand *)a, *y = (const cand *)b;
     if (x->d2 != y->d2) return x->d2 < y->d2 ? -1 : 1;
+#if ORC_PLANT == 5 /* ties resolved toward the higher index */
+    return x->idx > y->idx ? -1 : (x->idx < y->idx);
+#else
     return x->idx < y->idx ? -1 : (x->idx > y->idx);
+#endif
 }
 
 /* All candidates with d2 < r^2 around q, sorted by (d2, idx) and capped. */
@@ -449,9 +463,15 @@ static void normal_from_moments(int64_t cnt, const int64_t S1[3], const int64_t 
     orc_eigen3(C, ev, V);
     /* strict-< selection of the smallest eigenvalue (cuda_estimate_normals.cu:116-125) */
     int sel;
+#if ORC_PLANT == 4 /* largest eigenvalue instead of the smallest */
+    if (ev[0] > ev[1] && ev[0] > ev[2]) sel = 0;
+    else if (ev[1] > ev[0] && ev[1] > ev[2]) sel = 1;
+    else sel = 2;
+#else
     if (ev[0] < ev[1] && ev[0] < ev[2]) sel = 0;
     else if (ev[1] < ev[0] && ev[1] < ev[2]) sel = 1;
     else sel = 2;
+#endif
     double v[3] = {V[0 + sel], V[3 + sel], V[6 + sel]};
     double len = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
     if (!(len > 0.0)) return;
@@ -533,7 +553,11 @@ void orc_color_gradient(const float *xyzi, const float *normals4, int64_t n, con
             for (int a = 0; a < 3; ++a) Sb[a] += llrint((w[a] * dI) * FX40);
         }
         /* :75-84: + (k-1)^2 n n^T + 1e-6 I */
+#if ORC_PLANT == 2 /* orthogonality constraint term dropped */
+        double kk = 0.0;
+#else
         double kk = (double)(k - 1) * (double)(k - 1);
+#endif
         double A[3][3];
         static const int IA[6] = {0, 0, 0, 1, 1, 2}, IB[6] = {0, 1, 2, 1, 2, 2};
         for (int c = 0; c < 6; ++c) {
@@ -598,7 +622,11 @@ static void residual_jac(const double p[3], const float *q4, const float *n4, co
     double dq[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
     double r_g = dot3(dq, n);
     double c[3];
+#if ORC_PLANT == 1 /* transposed cross product: n x p */
+    cross3(n, p, c);
+#else
     cross3(p, n, c);
+#endif
     for (int a = 0; a < 3; ++a) {
         Jg[a] = sl * c[a];
         Jg[3 + a] = sl * n[a];
@@ -690,9 +718,15 @@ void orc_matmul4(const double A[16], const double B[16], double C[16])
 void orc_euler_to_T(const double xi[6], double T[16])
 {
     /* transform_vec6d_to_mat4f (eigen_solver.hpp:30-40): R = Rz(xi2) Ry(xi1) Rx(xi0) */
+#if ORC_PLANT == 3 /* angles applied in the wrong order: Rx(xi2) Ry(xi1) Rz(xi0) */
+    double ca = cos(xi[2]), sa = sin(xi[2]);
+    double cb = cos(xi[1]), sb = sin(xi[1]);
+    double cc = cos(xi[0]), sc = sin(xi[0]);
+#else
     double ca = cos(xi[0]), sa = sin(xi[0]);
     double cb = cos(xi[1]), sb = sin(xi[1]);
     double cc = cos(xi[2]), sc = sin(xi[2]);
+#endif
     T[0] = cc * cb;
     T[1] = cc * sb * sa - sc * ca;
     T[2] = cc * sb * ca + sc * sa;

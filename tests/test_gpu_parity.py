@@ -1,0 +1,443 @@
+"""GPU parity: libgca.so (through the C ABI) against the FP64 oracle on the same seeded inputs.
+
+Bars: bit-exact for pixel validity, point order and coordinates, voxel assignment and means,
+grid cells, neighbour index sets and nearest neighbours; normals within 1e-5 rad (they come
+out bit-exact); poses within 1e-5 rad / 1e-5 m of the oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import oracle as O  # noqa: E402
+
+LAM = 0.968
+CONFIGS = {"cfg0_320x240": (320, 240, 11), "cfg1_640x480": (640, 480, 12)}
+
+
+def angle(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.arctan2(np.linalg.norm(np.cross(a, b), axis=-1), np.abs((a * b).sum(-1)))
+
+
+def pose_err(T, U):
+    dR = T[:3, :3] @ U[:3, :3].T
+    return (float(np.arctan2(np.linalg.norm([dR[2, 1] - dR[1, 2], dR[0, 2] - dR[2, 0], dR[1, 0] - dR[0, 1]]),
+                             np.trace(dR) - 1)), float(np.linalg.norm(T[:3, 3] - U[:3, 3])))
+
+
+class Frame:
+    def __init__(self, g, ctx, W, H, seed):
+        from cuda_3d_amd import scene
+
+        (self.td, self.tc), (self.sd, self.sc), self.G, cam = scene.frame_pair(seed, W, H)
+        self.camg = g.make_camera(W, H, cam.fx, cam.fy, cam.cx, cam.cy)
+        self.camo = O.make_camera(W, H, cam.fx, cam.fy, cam.cx, cam.cy)
+        self.bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
+        self.bilo = O.Bilateral(1, 2, 2.0, 30.0)
+        self.tgt = g.PointCloud.create_from_rgbd(self.td, self.tc, self.camg, self.bil, 0.2, 3.5, ctx=ctx)
+        self.src = g.PointCloud.create_from_rgbd(self.sd, self.sc, self.camg, self.bil, 0.2, 3.5, ctx=ctx)
+        self.otx, self.otr, _, _ = O.cloud_from_rgbd(self.td, self.tc, self.camo, self.bilo, 0.2, 3.5)
+        self.osx, self.osr, _, _ = O.cloud_from_rgbd(self.sd, self.sc, self.camo, self.bilo, 0.2, 3.5)
+        self.bmin, self.bmax = O.bounds(self.otx)
+
+
+@pytest.fixture(scope="module", params=sorted(CONFIGS))
+def frame(request, ctx):
+    import cuda_3d_amd as g
+
+    W, H, seed = CONFIGS[request.param]
+    print(f"[gpu-parity] frame {request.param}", flush=True)
+    return Frame(g, ctx, W, H, seed)
+
+
+@pytest.fixture(scope="module")
+def level(frame):
+    """Target at voxel 0.02 with normals (r 0.05, K 30) and gradient (r 0.04), GPU and oracle."""
+    v = 0.02
+    vc = frame.tgt.voxel_grid_down_sample(v)
+    ox, orgb, _ = O.voxel_downsample(frame.otx, frame.otr, v)
+    vc.estimate_normals(2.5 * v, 30)
+    vc.compute_color_gradient(2 * v, 0)
+    on = O.estimate_normals(ox, 2.5 * v, 30, frame.bmin, frame.bmax)
+    og = O.color_gradient(ox, on, 2 * v, 0, frame.bmin, frame.bmax)
+    return dict(v=v, gpu=vc, ox=ox, orgb=orgb, on=on, og=og)
+
+
+# ---- front end ---------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("mode", ["fixed", "adaptive", "none"])
+def test_frontend_bitexact(frame, ctx, mode):
+    import cuda_3d_amd as g
+
+    if mode == "fixed":
+        bg, bo = frame.bil, frame.bilo
+    elif mode == "adaptive":
+        bg, bo = g.bilateral(g.BILATERAL_REFERENCE_ADAPTIVE, 0, 0, 0), O.Bilateral(2, 0, 0, 0)
+    else:
+        bg, bo = g.bilateral(g.BILATERAL_NONE, 0, 1, 1), O.Bilateral(0, 0, 1, 1)
+    pc = g.PointCloud.create_from_rgbd(frame.td, frame.tc, frame.camg, bg, 0.2, 3.5, ctx=ctx)
+    xyz, rgb, inten = pc.download()
+    ox, orgb, _, _ = O.cloud_from_rgbd(frame.td, frame.tc, frame.camo, bo, 0.2, 3.5)
+    assert len(xyz) == len(ox) > 1000
+    np.testing.assert_array_equal(xyz.view(np.uint32), np.ascontiguousarray(ox[:, :3]).view(np.uint32))
+    np.testing.assert_array_equal(inten.view(np.uint32), np.ascontiguousarray(ox[:, 3]).view(np.uint32))
+    np.testing.assert_array_equal(rgb, orgb[:, :3])
+    mn, mx = O.bounds(ox)
+    np.testing.assert_array_equal(pc.get_min_bound(), mn)
+    np.testing.assert_array_equal(pc.get_max_bound(), mx)
+
+
+def test_frontend_edge_frames(ctx):
+    import cuda_3d_amd as g
+
+    cam = g.make_camera(64, 48, 60.0, 60.0, 32.0, 24.0)
+    camo = O.make_camera(64, 48, 60.0, 60.0, 32.0, 24.0)
+    bgr = np.full((48, 64, 3), 77, np.uint8)
+    empty = np.zeros((48, 64), np.uint16)
+    pc = g.PointCloud.create_from_rgbd(empty, bgr, cam, g.bilateral(), 0.2, 3.5, ctx=ctx)
+    assert pc.points_number() == 0
+    far = np.full((48, 64), 9000, np.uint16)  # beyond tmax
+    assert g.PointCloud.create_from_rgbd(far, bgr, cam, g.bilateral(), 0.2, 3.5, ctx=ctx).points_number() == 0
+    rng = np.random.default_rng(1)
+    sparse = np.where(rng.random((48, 64)) < 0.3, rng.integers(300, 3400, (48, 64)), 0).astype(np.uint16)
+    pc = g.PointCloud.create_from_rgbd(sparse, bgr, cam, g.bilateral(), 0.2, 3.5, ctx=ctx)
+    ox, _, _, _ = O.cloud_from_rgbd(sparse, bgr, camo, O.Bilateral(1, 2, 2.0, 30.0), 0.2, 3.5)
+    np.testing.assert_array_equal(pc.download()[0], ox[:, :3])
+    bad = g.make_camera(64, 48, 60.0, 60.0, 32.0, 24.0, depth_scale=0.0)
+    with pytest.raises(g.GcaError):
+        g.PointCloud.create_from_rgbd(sparse, bgr, bad, g.bilateral(), 0.2, 3.5, ctx=ctx)
+
+
+# ---- voxel ---------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("v", [0.04, 0.02, 0.01])
+def test_voxel_bitexact(frame, v):
+    vc = frame.tgt.voxel_grid_down_sample(v)
+    xyz, rgb, inten = vc.download()
+    ox, orgb, _ = O.voxel_downsample(frame.otx, frame.otr, v)
+    assert len(xyz) == len(ox)
+    np.testing.assert_array_equal(xyz, ox[:, :3])
+    np.testing.assert_array_equal(rgb, orgb[:, :3])
+    np.testing.assert_array_equal(inten, ox[:, 3])
+    np.testing.assert_array_equal(vc.get_min_bound(), frame.bmin)  # output keeps input bounds
+
+
+def test_voxel_edge_cases(ctx):
+    import cuda_3d_amd as g
+
+    one = g.PointCloud.from_numpy([[0.1, 0.2, 0.3]], [[0.5, 0.25, 0.125]], ctx=ctx)
+    out = one.voxel_grid_down_sample(0.01)
+    np.testing.assert_array_equal(out.download()[0], np.float32([[0.1, 0.2, 0.3]]))
+    assert one.voxel_grid_down_sample(0.0) is None
+    assert one.voxel_grid_down_sample(-1.0) is None
+    empty = g.PointCloud.from_numpy(np.zeros((0, 3)), np.zeros((0, 3)), ctx=ctx)
+    assert empty.voxel_grid_down_sample(0.02).points_number() == 0
+    same = g.PointCloud.from_numpy(np.tile([[0.5, 0.5, 1.0]], (100, 1)), np.full((100, 3), 0.5), ctx=ctx)
+    assert same.voxel_grid_down_sample(0.02).points_number() == 1
+    huge = g.PointCloud.from_numpy([[0, 0, 0], [1e4, 1e4, 1e4]], np.zeros((2, 3)), ctx=ctx)
+    assert huge.voxel_grid_down_sample(1e-6) is None  # voxel too small for the lattice
+    # repeated calls leave no state behind (bitmap and accumulators self-clear)
+    a = same.voxel_grid_down_sample(0.02).download()[0]
+    b = same.voxel_grid_down_sample(0.02).download()[0]
+    np.testing.assert_array_equal(a, b)
+
+
+# ---- neighbour search -------------------------------------------------------------------------------
+
+def test_grid_cells_bitexact(level, frame):
+    c, o, s, d = level["gpu"].grid_cells(0.05)
+    oc, oo, os_, od = O.grid_cells(level["ox"], 0.05, frame.bmin, frame.bmax)
+    np.testing.assert_array_equal(c, oc)
+    np.testing.assert_array_equal(o, oo)
+    assert s == os_
+    np.testing.assert_array_equal(d, od)
+
+
+def test_nn_bitexact(level, frame):
+    import cuda_3d_amd as g
+
+    q = frame.src.voxel_grid_down_sample(0.04)
+    oq, _, _ = O.voxel_downsample(frame.osx, frame.osr, 0.04)
+    for r in (0.05, 0.02):
+        idx = g.PointCloud.nn_search(q, level["gpu"], r)
+        np.testing.assert_array_equal(idx, O.nn_search(oq, level["ox"], r, frame.bmin, frame.bmax))
+    assert g.PointCloud.nn_search(q, level["gpu"], 0.0).size == 0  # point_cloud.cu:512-516
+
+
+@pytest.mark.parametrize("K", [0, 30, 5])
+def test_radius_search_bitexact(level, frame, K):
+    import cuda_3d_amd as g
+
+    offs, idx = g.PointCloud.radius_search(level["gpu"], level["gpu"], 0.05, K)
+    ooffs, oidx = O.radius_search(level["ox"], level["ox"], 0.05, K, frame.bmin, frame.bmax)
+    np.testing.assert_array_equal(offs, ooffs)
+    np.testing.assert_array_equal(idx, oidx)
+
+
+def test_outlier_removal_bitexact(frame):
+    vc = frame.tgt.voxel_grid_down_sample(0.01)
+    ox, orgb, _ = O.voxel_downsample(frame.otx, frame.otr, 0.01)
+    out = vc.radius_outlier_removal(0.02, 4)
+    oo, _ = O.radius_outlier_removal(ox, orgb, 0.02, 4, frame.bmin, frame.bmax)
+    np.testing.assert_array_equal(out.download()[0], oo[:, :3])
+    assert vc.radius_outlier_removal(0.0, 4).points_number() == 0
+
+
+# ---- normals and colour gradient -----------------------------------------------------------------------
+
+@pytest.mark.parametrize("K", [30, 0, 3])
+def test_normals_parity(level, frame, K):
+    c = level["gpu"].copy()
+    assert c.estimate_normals(2.5 * level["v"], K)
+    n = c.download_normals()
+    on = O.estimate_normals(level["ox"], 2.5 * level["v"], K, frame.bmin, frame.bmax)[:, :3]
+    nz = np.abs(on).sum(1) > 0
+    assert np.array_equal(nz, np.abs(n).sum(1) > 0)
+    assert angle(n[nz], on[nz]).max() <= 1e-5
+    assert np.all((n[nz] * on[nz]).sum(1) > 0)  # same orientation
+    assert np.array_equal(n, on), "normals are expected to be bit-exact"
+
+
+def test_gradient_parity(level):
+    gr = level["gpu"].download_gradient()
+    og = level["og"][:, :3]
+    scale = np.abs(og).max()
+    np.testing.assert_allclose(gr, og, rtol=0, atol=1e-6 * scale)
+    assert np.array_equal(gr, og), "colour gradients are expected to be bit-exact"
+
+
+def test_gradient_requires_normals(ctx):
+    import cuda_3d_amd as g
+
+    pc = g.PointCloud.from_numpy(np.random.default_rng(0).uniform(0, 1, (50, 3)), np.full((50, 3), 0.5), ctx=ctx)
+    assert not pc.compute_color_gradient(0.1)
+
+
+# ---- Gauss-Newton system, residuals, solve -------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def gn_case(level, frame):
+    import cuda_3d_amd as g
+
+    src = frame.src.voxel_grid_down_sample(0.02)
+    osx, _, _ = O.voxel_downsample(frame.osx, frame.osr, 0.02)
+    T = frame.G.copy()
+    T[:3, 3] += 0.003
+    moved = src.copy()
+    moved.transform(T)
+    nn = g.PointCloud.nn_search(moved, level["gpu"], 0.05)
+    return dict(src=src, osx=osx, T=T, nn=nn)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_build_gauss_newton(level, gn_case, mode):
+    import cuda_3d_amd as g
+
+    JTJ, JTr, r2, nc = g.build_gauss_newton(gn_case["src"], level["gpu"], gn_case["nn"], LAM, gn_case["T"], mode)
+    o = O.build_gn(gn_case["osx"], level["ox"], level["on"], level["og"], gn_case["nn"], LAM, gn_case["T"])
+    k = 0
+    oJ = np.zeros((6, 6))
+    for a in range(6):
+        for b in range(a, 6):
+            oJ[a, b] = oJ[b, a] = o[k]
+            k += 1
+    assert nc == int(o[28])
+    tol = {0: 2e-6, 1: 1e-3, 2: 2e-6}[mode]  # naive FP32 accumulation is the ablation
+    assert np.abs(JTJ - oJ).max() <= tol * np.abs(oJ).max()
+    assert np.abs(JTr - o[21:27]).max() <= tol * np.abs(o[21:27]).max() + 1e-9
+    assert abs(r2 - o[27]) <= tol * o[27]
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_residuals(level, gn_case, kind):
+    import cuda_3d_amd as g
+
+    vals, s = g.compute_residuals(gn_case["src"], level["gpu"], gn_case["nn"], kind, LAM, gn_case["T"])
+    ov, os_ = O.residuals(gn_case["osx"], level["ox"], level["on"], level["og"], gn_case["nn"], LAM, kind, gn_case["T"])
+    np.testing.assert_allclose(vals, ov, rtol=1e-4, atol=1e-6)
+    assert abs(s - os_) <= 1e-5 * max(1.0, abs(os_)) + 1e-6 * len(gn_case["nn"])
+
+
+def test_solver_matches_oracle():
+    import cuda_3d_amd as g
+
+    rng = np.random.default_rng(2)
+    A = rng.normal(size=(6, 6))
+    A = A @ A.T + np.eye(6)
+    b = rng.normal(size=6) * 1e-2
+    st, xi, T = g.solve_gauss_newton(A, b)
+    ost, oxi, oT = O.solve6([A[a, c] for a in range(6) for c in range(a, 6)], b)
+    assert st == ost == 0
+    np.testing.assert_allclose(xi, oxi, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(T, oT, atol=1e-15)
+
+
+# ---- registration -------------------------------------------------------------------------------------
+
+def icp_p(g, iters, corr, mode=0, conv=(0.0, 0.0)):
+    return g.icp_params(iters, corr, LAM, mode, conv[0], conv[1])
+
+
+def test_align_single_level(level, gn_case, frame):
+    import cuda_3d_amd as g
+
+    icp = g.ColorICP(30, 0.05, 0.04, conv_rotation=0.0, conv_translation_sq=0.0)
+    icp.set_source_point_cloud(gn_case["src"])
+    icp.set_target_point_cloud(level["gpu"])
+    assert icp.align()
+    T = icp.get_final_transformation_matrix()
+    To, rep = O.icp_align(gn_case["osx"], level["ox"], level["on"], level["og"],
+                          O.icp_params(30, 0.05, LAM, 0.0, 0.0), None, frame.bmin, frame.bmax)
+    er, et = pose_err(T, To)
+    print(f"[gpu-parity] align pose err vs oracle: {er:.3e} rad {et:.3e} m", flush=True)
+    assert er <= 1e-5 and et <= 1e-5
+    assert icp.report.iterations == rep.iterations == 30
+    assert abs(icp.get_RSME() - rep.rmse) <= 1e-4 * rep.rmse
+    res, mean = icp.get_abs_residual()
+    assert len(res) == gn_case["src"].points_number() and mean > 0
+
+
+def test_align_reference_stopping_rule(level, gn_case, frame):
+    import cuda_3d_amd as g
+
+    icp = g.ColorICP(100, 0.05, 0.04)  # reference thresholds (color_icp.hpp:47-48)
+    icp.set_source_point_cloud(gn_case["src"])
+    icp.set_target_point_cloud(level["gpu"])
+    assert icp.align()
+    To, rep = O.icp_align(gn_case["osx"], level["ox"], level["on"], level["og"],
+                          O.icp_params(100, 0.05, LAM), None, frame.bmin, frame.bmax)
+    assert icp.report.converged == rep.converged == 1
+    assert icp.report.iterations == rep.iterations
+    er, et = pose_err(icp.get_final_transformation_matrix(), To)
+    assert er <= 1e-5 and et <= 1e-5
+
+
+def test_align_preconditions(level, ctx):
+    import cuda_3d_amd as g
+
+    icp = g.ColorICP(10, 0.05, 0.04)
+    assert not icp.align()  # no clouds
+    icp.set_source_point_cloud(level["gpu"])
+    bare = g.PointCloud.from_numpy(np.random.default_rng(0).uniform(0, 1, (50, 3)), np.full((50, 3), 0.5), ctx=ctx)
+    icp.set_target_point_cloud(bare)
+    assert not icp.align()  # target without normals (color_icp.cu:168-174)
+    bad = g.ColorICP(10, 0.0, 0.04)
+    bad.set_source_point_cloud(level["gpu"])
+    bad.set_target_point_cloud(level["gpu"])
+    assert not bad.align()
+
+
+def test_multiscale_pose_parity(frame):
+    import cuda_3d_amd as g
+
+    lv = [g.level_params(v, icp=icp_p(g, 30, 2.5 * v)) for v in (0.04, 0.02, 0.01)]
+    lo = [O.LevelParams(v, 2.5 * v, 30, 2 * v, 0, O.icp_params(30, 2.5 * v, LAM, 0.0, 0.0)) for v in (0.04, 0.02, 0.01)]
+    T, reps = g.multiscale_icp(frame.src, frame.tgt, lv)
+    To, oreps = O.multiscale_icp(frame.osx, frame.osr, frame.otx, frame.otr, lo)
+    er, et = pose_err(T, To)
+    gr, gt = pose_err(T, frame.G)
+    print(f"[gpu-parity] multiscale vs oracle {er:.3e} rad {et:.3e} m; vs truth {gr:.3e} rad {gt:.3e} m", flush=True)
+    assert er <= 1e-5 and et <= 1e-5
+    assert [r.iterations for r in reps] == [30, 30, 30]
+    assert gr < 5e-3 and gt < 5e-3  # registration itself succeeded
+
+
+def test_accumulation_modes_pose_error(frame):
+    """Precision variant: compensated FP32 partials vs naive FP32 ablation vs FP64."""
+    import cuda_3d_amd as g
+
+    lo = [O.LevelParams(v, 2.5 * v, 30, 2 * v, 0, O.icp_params(30, 2.5 * v, LAM, 0.0, 0.0)) for v in (0.04, 0.02, 0.01)]
+    To, _ = O.multiscale_icp(frame.osx, frame.osr, frame.otx, frame.otr, lo)
+    errs = {}
+    for mode in (0, 1, 2):
+        lv = [g.level_params(v, icp=icp_p(g, 30, 2.5 * v, mode)) for v in (0.04, 0.02, 0.01)]
+        T, _ = g.multiscale_icp(frame.src, frame.tgt, lv)
+        errs[mode] = pose_err(T, To)
+    print(f"[gpu-parity] pose err by accumulation mode {errs}", flush=True)
+    assert max(errs[0]) <= 1e-5 and max(errs[2]) <= 1e-5
+
+
+def test_batch_matches_single_pairs(ctx):
+    import cuda_3d_amd as g
+    from cuda_3d_amd import scene
+
+    srcs, tgts, T0s, singles = [], [], [], []
+    for k, n in enumerate((3000, 20000, 7000, 500)):
+        sx, sr, tx, tr, G = scene.sample_surface_pair(100 + k, n)
+        s = g.PointCloud.from_numpy(sx, sr, ctx=ctx)
+        t = g.PointCloud.from_numpy(tx, tr, ctx=ctx)
+        t.estimate_normals(0.05, 30)
+        t.compute_color_gradient(0.04, 0)
+        srcs.append(s)
+        tgts.append(t)
+        T0s.append(np.eye(4))
+    p = icp_p(g, 30, 0.05)
+    Tb, reps = g.batch_icp(srcs, tgts, p, T0s)
+    for k in range(len(srcs)):
+        icp = g.ColorICP(30, 0.05, 0.04, conv_rotation=0.0, conv_translation_sq=0.0)
+        icp.set_source_point_cloud(srcs[k])
+        icp.set_target_point_cloud(tgts[k])
+        icp.align()
+        np.testing.assert_array_equal(Tb[k], icp.get_final_transformation_matrix())  # same kernel, same bits
+        assert reps[k].iterations == 30
+
+
+def test_determinism(frame):
+    import cuda_3d_amd as g
+
+    outs = []
+    for _ in range(2):
+        vc = frame.tgt.voxel_grid_down_sample(0.01)
+        vc.estimate_normals(0.025, 30)
+        vc.compute_color_gradient(0.02, 0)
+        lv = [g.level_params(v, icp=icp_p(g, 10, 2.5 * v)) for v in (0.04, 0.02)]
+        T, _ = g.multiscale_icp(frame.src, frame.tgt, lv)
+        outs.append((vc.download()[0], vc.download_normals(), vc.download_gradient(), T))
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+
+
+# ---- differentiation ---------------------------------------------------------------------------------
+
+def test_objective_adjoint(level, gn_case):
+    import cuda_3d_amd as g
+
+    xi = np.array([0.002, -0.001, 0.0015, 0.001, 0.0005, -0.001])
+    E, dxi, dp = g.objective_grad(gn_case["src"], level["gpu"], gn_case["nn"], gn_case["T"], xi, LAM)
+    oE, odxi, odp = O.objective_grad(gn_case["osx"], level["ox"], level["on"], level["og"], gn_case["nn"], LAM,
+                                     gn_case["T"], xi)
+    assert abs(E - oE) <= 1e-9 * abs(oE)
+    np.testing.assert_allclose(dxi, odxi, rtol=1e-9, atol=1e-12 * np.abs(odxi).max())
+    np.testing.assert_allclose(dp.cpu().numpy(), odp, rtol=1e-5, atol=1e-6 * np.abs(odp).max())
+
+
+def test_autograd_function_fd(level, gn_case):
+    import torch
+
+    import cuda_3d_amd as g
+    from cuda_3d_amd.autograd import colored_icp_objective
+
+    src_xyzi = torch.as_tensor(gn_case["src"].download_xyzi(), device="cuda").requires_grad_(True)
+    nn = g.device_tensor(gn_case["nn"])
+    rng = np.random.default_rng(3)
+    worst = 0.0
+    for k in range(8):
+        xi0 = np.r_[rng.uniform(-0.01, 0.01, 3), rng.uniform(-0.01, 0.01, 3)]
+        xi = torch.tensor(xi0, dtype=torch.float64, device="cuda", requires_grad=True)
+        E = colored_icp_objective(xi, src_xyzi, level["gpu"], nn, gn_case["T"], LAM)
+        E.backward()
+        h = 1e-4
+        fd = np.zeros(6)
+        for a in range(6):
+            e = np.zeros(6)
+            e[a] = h
+            Ep = colored_icp_objective(torch.tensor(xi0 + e, device="cuda"), src_xyzi.detach(), level["gpu"], nn, gn_case["T"], LAM)
+            Em = colored_icp_objective(torch.tensor(xi0 - e, device="cuda"), src_xyzi.detach(), level["gpu"], nn, gn_case["T"], LAM)
+            fd[a] = (Ep.item() - Em.item()) / (2 * h)
+        rel = np.abs(xi.grad.cpu().numpy() - fd).max() / np.abs(fd).max()
+        worst = max(worst, rel)
+        src_xyzi.grad = None
+    print(f"[gpu-parity] autograd vs central FD (h=1e-4) worst relative error {worst:.2e}", flush=True)
+    assert worst < 1e-5

@@ -1,0 +1,387 @@
+"""Pins of the FP64 oracle (CPU only).
+
+The reference ships no tests or golden vectors for this path, so the oracle is pinned by
+fixtures computed independently with numpy/scipy brute force (tests/golden/make_golden.py),
+by closed forms and invariants, and by planted-mistake checks that a plausible error (sign,
+index, dropped term) would break.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+from scipy.spatial.transform import Rotation
+
+import oracle as O
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_small.npz"))
+
+
+def f4(xyz, w=None):
+    w = np.zeros(len(xyz), np.float32) if w is None else np.asarray(w, np.float32)
+    return np.concatenate([np.asarray(xyz, np.float32), w[:, None]], 1)
+
+
+def angle(a, b, signed=False):
+    """Angle between row vectors, atan2(|a x b|, a . b) in FP64 (robust near 0)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    c = np.linalg.norm(np.cross(a, b), axis=-1)
+    d = (a * b).sum(-1)
+    return np.arctan2(c, d if signed else np.abs(d))
+
+
+# ---- front end -----------------------------------------------------------------------------
+
+def small_cam():
+    return O.make_camera(16, 12, 20.0, 20.0, 8.0, 6.0)
+
+
+def test_bilateral_matches_direct_fp64_loop():
+    d, bgr = GOLD["bil_depth"], GOLD["bil_bgr"]
+    xyzi, rgb, pix, filt = O.cloud_from_rgbd(d, bgr, small_cam(), O.Bilateral(1, 2, 2.0, 30.0), 0.2, 3.5)
+    ref = GOLD["bil_filtered"].reshape(-1)
+    m = d.reshape(-1) > 0
+    np.testing.assert_allclose(filt[m], ref[m], rtol=1e-14, atol=0)
+    # validity mask, raster order and float32 back-projection (closed form)
+    assert np.array_equal(np.sort(pix), np.nonzero(GOLD["bil_valid"].reshape(-1))[0])
+    assert np.array_equal(pix, np.sort(pix))
+    np.testing.assert_array_equal(xyzi[:, :3], GOLD["bil_xyz"])
+    np.testing.assert_array_equal(rgb[:, :3], GOLD["bil_rgb"])
+
+
+def test_bilateral_invariants():
+    cam = O.make_camera(20, 15, 30.0, 30.0, 10.0, 7.5)
+    bgr = np.zeros((15, 20, 3), np.uint8)
+    # constant depth -> unchanged
+    d = np.full((15, 20), 1234, np.uint16)
+    _, _, _, f = O.cloud_from_rgbd(d, bgr, cam, O.Bilateral(1, 3, 2.0, 30.0), 0.2, 3.5)
+    np.testing.assert_allclose(f, 1234.0, rtol=1e-14)
+    # linear ramp + huge range sigma -> symmetric normalised kernel reproduces the ramp inside
+    yy, xx = np.mgrid[0:15, 0:20]
+    d = (1000 + 3 * xx + 2 * yy).astype(np.uint16)
+    _, _, _, f = O.cloud_from_rgbd(d, bgr, cam, O.Bilateral(1, 2, 2.0, 1e6), 0.2, 3.5)
+    f = f.reshape(15, 20)
+    np.testing.assert_allclose(f[2:-2, 2:-2], d[2:-2, 2:-2], rtol=0, atol=1e-9)
+    # step edge + small range sigma -> no mixing across the edge
+    d = np.where(xx < 10, 1000, 2000).astype(np.uint16)
+    _, _, _, f = O.cloud_from_rgbd(d, bgr, cam, O.Bilateral(1, 2, 2.0, 30.0), 0.2, 3.5)
+    f = f.reshape(15, 20)
+    assert np.abs(f[:, :10] - 1000).max() < 1e-9 and np.abs(f[:, 10:] - 2000).max() < 1e-9
+    # holes are skipped, a centre hole is invalid
+    d = np.full((15, 20), 1500, np.uint16)
+    d[7, 10] = 0
+    d[3, 3] = 0
+    x, _, pix, f = O.cloud_from_rgbd(d, bgr, cam, O.Bilateral(1, 2, 2.0, 30.0), 0.2, 3.5)
+    assert 7 * 20 + 10 not in set(pix.tolist()) and len(x) == 15 * 20 - 2
+    np.testing.assert_allclose(f.reshape(15, 20)[d > 0], 1500.0, rtol=1e-14)
+
+
+def test_backprojection_identity_extrinsics_samples_own_pixel():
+    from cuda_3d_amd import scene
+
+    (td, tc), _, _, cam = scene.frame_pair(5, 64, 48)
+    c = O.make_camera(64, 48, cam.fx, cam.fy, cam.cx, cam.cy)
+    # unique colour per pixel to read back which pixel was sampled
+    ids = np.arange(64 * 48)
+    bgr = np.stack([ids % 256, (ids // 256) % 256, np.zeros_like(ids)], -1).astype(np.uint8).reshape(48, 64, 3)
+    x, rgb, pix, _ = O.cloud_from_rgbd(td, bgr, c, O.Bilateral(0, 0, 1, 1), 0.2, 3.5)
+    b = np.rint(rgb[:, 2] * 255).astype(int)
+    g = np.rint(rgb[:, 1] * 255).astype(int)
+    sampled = b + 256 * g
+    # int(u + 0.5) of an FP32 re-projection: own pixel, or one to the left / above when the
+    # rounding lands just below the integer (reference quirk, cuda_point_cloud_factory.cu:242)
+    dx, dy = (pix % 64) - (sampled % 64), (pix // 64) - (sampled // 64)
+    assert set(np.unique(dx)) <= {0, 1} and set(np.unique(dy)) <= {0, 1}
+    assert np.mean(sampled == pix) > 0.5
+    z = x[:, 2].astype(np.float64)
+    u = (pix % 64) - 0.5
+    np.testing.assert_allclose(x[:, 0], (u - cam.cx) * z / cam.fx, rtol=2e-6, atol=1e-7)
+    c = rgb.astype(np.float64)
+    inten = 0.2126 * c[:, 0] + 0.7152 * c[:, 1] + 0.0722 * c[:, 2]
+    np.testing.assert_array_equal(x[:, 3], inten.astype(np.float32))
+
+
+def test_invalid_camera_rejected():
+    cam = small_cam()
+    cam.depth_scale = 0.0
+    with pytest.raises(ValueError):
+        O.cloud_from_rgbd(GOLD["bil_depth"], GOLD["bil_bgr"], cam, O.Bilateral(0, 0, 1, 1), 0.2, 3.5)
+
+
+def test_bounds_keep_reference_max_init():
+    x = f4(-np.abs(np.random.default_rng(0).normal(size=(10, 3))) - 1.0)
+    mn, mx = O.bounds(x)
+    assert np.all(mn == x[:, :3].min(0))
+    assert np.all(mx == np.float32(np.finfo(np.float32).tiny))  # FLT_MIN, geometry_util.cuh:92
+
+
+# ---- voxel --------------------------------------------------------------------------------
+
+def test_voxel_matches_numpy_unique():
+    xyz, rgb, v = GOLD["vox_xyz"], GOLD["vox_rgb"], float(GOLD["vox_size"])
+    ox, orgb, keys = O.voxel_downsample(f4(xyz), f4(rgb), v)
+    mean = GOLD["vox_mean"]
+    assert len(ox) == len(mean)
+    # 2^-30 fixed-point sums: within one float ulp of the FP64 mean plus the 2^-31 per-point
+    # quantisation (dominant only for means near zero)
+    for got, want in ((ox[:, :3], mean[:, :3]), (orgb[:, :3], mean[:, 3:])):
+        tol = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64) + 2.0 ** -30
+        assert np.all(np.abs(got.astype(np.float64) - want) <= tol)
+    assert np.mean(ox[:, :3] == mean[:, :3].astype(np.float32)) > 0.95
+    mn = xyz.min(0)
+    dims = (np.floor((xyz.max(0) - mn) / np.float32(v)) + 1).astype(np.int64)
+    k = GOLD["vox_keys"]
+    np.testing.assert_array_equal(keys, (k[:, 0] * dims[1] + k[:, 1]) * dims[2] + k[:, 2])
+    c = orgb.astype(np.float64)
+    inten = (0.2126 * c[:, 0] + 0.7152 * c[:, 1] + 0.0722 * c[:, 2]).astype(np.float32)
+    np.testing.assert_array_equal(ox[:, 3], inten)
+
+
+def test_voxel_edge_cases():
+    one = f4([[0.1, 0.2, 0.3]])
+    ox, orgb, _ = O.voxel_downsample(one, f4([[0.5, 0.5, 0.5]]), 0.01)
+    assert len(ox) == 1 and np.all(ox[0, :3] == one[0, :3])
+    same = f4(np.tile([[0.5, 0.5, 1.0]], (7, 1)))
+    ox, _, _ = O.voxel_downsample(same, same, 0.02)
+    assert len(ox) == 1
+    with pytest.raises(ValueError):
+        O.voxel_downsample(same, same, 0.0)
+    with pytest.raises(ValueError):
+        O.voxel_downsample(f4([[0, 0, 0], [1e4, 1e4, 1e4]]), f4(np.zeros((2, 3))), 1e-9)
+
+
+# ---- neighbours -----------------------------------------------------------------------------
+
+def test_nn_and_radius_match_brute_force():
+    ref, qry, r = f4(GOLD["nb_ref"]), f4(GOLD["nb_qry"]), float(GOLD["nb_radius"])
+    np.testing.assert_array_equal(O.nn_search(qry, ref, r), GOLD["nb_nn"])
+    off, idx = O.radius_search(qry, ref, r, 0)
+    np.testing.assert_array_equal(off, GOLD["nb_off"])
+    np.testing.assert_array_equal(idx, GOLD["nb_idx"])
+    off, idx = O.radius_search(qry, ref, r, 3)
+    np.testing.assert_array_equal(off, GOLD["nb_offk"])
+    np.testing.assert_array_equal(idx, GOLD["nb_idxk"])
+
+
+def test_radius_sets_agree_with_kdtree():
+    rng = np.random.default_rng(3)
+    p = rng.uniform(0, 1, size=(3000, 3)).astype(np.float32)
+    off, idx = O.radius_search(f4(p), f4(p), 0.05, 0)
+    tree = cKDTree(p.astype(np.float64))
+    ball = tree.query_ball_point(p.astype(np.float64), 0.05 * (1 - 1e-6))
+    mism = 0
+    for i, b in enumerate(ball):
+        mine = set(idx[off[i]:off[i + 1]].tolist())
+        mism += not set(b) <= mine
+    assert mism == 0
+
+
+def test_grid_cells_formula_and_cell_growth():
+    rng = np.random.default_rng(4)
+    p = f4(rng.uniform(-1, 1, size=(500, 3)))
+    cells, origin, cell, dims = O.grid_cells(p, 0.1)
+    mn, mx = O.bounds(p)
+    assert np.all(origin == mn - np.float32(1.5) * np.float32(0.1)) and cell == np.float32(0.1)
+    exp = np.clip(np.floor((p[:, :3].astype(np.float64) - origin.astype(np.float64)) / np.float64(cell)), 0, dims - 1)
+    np.testing.assert_array_equal(cells, exp.astype(np.int32))
+    # tiny radius: the cell edge doubles until the dense grid fits the cap
+    _, _, cell2, dims2 = O.grid_cells(p, 1e-4)
+    assert cell2 > np.float32(1e-4) and np.prod(dims2.astype(np.float64)) <= 2 ** 25
+
+
+# ---- normals / eigen ----------------------------------------------------------------------
+
+def test_eigen3_matches_numpy():
+    rng = np.random.default_rng(5)
+    for k in range(200):
+        A = rng.normal(size=(3, 3))
+        A = A + A.T
+        if k % 10 == 0:
+            A = np.diag(rng.normal(size=3))
+        if k % 17 == 0:
+            A = np.zeros((3, 3))
+        ev, V = O.eigen3(A)
+        np.testing.assert_allclose(np.sort(ev), np.linalg.eigvalsh(A), atol=1e-12 * (1 + np.abs(A).max()))
+        np.testing.assert_allclose(A @ V, V * ev, atol=1e-11 * (1 + np.abs(A).max()))
+        np.testing.assert_allclose(V.T @ V, np.eye(3), atol=1e-12)
+
+
+@pytest.mark.parametrize("K", [0, 15])
+def test_normals_match_numpy_eigh(K):
+    p, r = GOLD["nrm_pts"], float(GOLD["nrm_radius"])
+    n = O.estimate_normals(f4(p), r, K)[:, :3].astype(np.float64)
+    ref = GOLD[f"nrm_K{K}"]
+    ok = np.abs(ref).sum(1) > 0
+    assert angle(n[ok], ref[ok], signed=True).max() < 1e-6
+    assert np.all(n[:, 2] <= 0)  # oriented toward (0,0,-1)
+    assert np.all(n[~ok] == 0)
+
+
+def test_normals_closed_forms():
+    rng = np.random.default_rng(6)
+    a = rng.uniform(-0.5, 0.5, size=(400, 2))
+    nrm = np.array([0.2, -0.3, 0.9])
+    nrm /= np.linalg.norm(nrm)
+    e1 = np.cross(nrm, [1, 0, 0])
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(nrm, e1)
+    plane = np.array([0, 0, 1.5]) + a[:, :1] * e1 + a[:, 1:] * e2
+    n = O.estimate_normals(f4(plane), 0.1, 30)[:, :3]
+    n = n[np.abs(n).sum(1) > 0]  # isolated border points (< 3 neighbours) have no normal
+    assert len(n) > 390
+    # float32 input quantisation bounds the fit: ~eps32 * |p| / r ~ 1e-6 rad
+    assert angle(n, np.tile(nrm, (len(n), 1))).max() < 5e-6 and np.all(n[:, 2] < 0)
+    u = rng.normal(size=(2000, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    u = u[u[:, 2] < -0.3]
+    sph = np.array([0, 0, 2.0]) + 0.3 * u
+    n = O.estimate_normals(f4(sph), 0.06, 30)[:, :3]
+    radial = (sph - [0, 0, 2.0]) / 0.3
+    err = np.degrees(angle(n, radial))  # random sampling; rim points see one-sided patches
+    assert np.median(err) < 2.0 and np.percentile(err, 90) < 5.0
+    two = O.estimate_normals(f4([[0, 0, 1], [0.001, 0, 1]]), 0.1, 0)
+    assert np.all(two == 0)  # count < 3
+
+
+def test_gradient_matches_numpy_solve_and_plane_slope():
+    p, r = GOLD["nrm_pts"], float(GOLD["grad_radius"])
+    g = O.color_gradient(f4(p, GOLD["grad_int"]), f4(GOLD["grad_nrm"]), r, 0)[:, :3]
+    ref = GOLD["grad"]
+    np.testing.assert_allclose(g, ref, rtol=2e-6, atol=2e-6 * np.abs(ref).max())
+    # linear intensity on a plane: gradient = tangential slope (up to the 1e-6 ridge)
+    rng = np.random.default_rng(7)
+    a = rng.uniform(-0.3, 0.3, size=(600, 2))
+    pts = np.stack([a[:, 0], a[:, 1], np.full(600, 1.0)], 1)
+    inten = 0.5 + 0.8 * a[:, 0] - 0.4 * a[:, 1]
+    n = np.tile([[0, 0, -1.0]], (600, 1))
+    g = O.color_gradient(f4(pts, inten), f4(n), 0.06, 0)[:, :3]
+    inner = (np.abs(a) < 0.2).all(1)
+    np.testing.assert_allclose(g[inner], np.tile([0.8, -0.4, 0.0], (inner.sum(), 1)), atol=2e-3)
+
+
+# ---- Gauss-Newton, solve, ICP, adjoint ----------------------------------------------------------
+
+def test_gauss_newton_matches_finite_difference_jacobian():
+    G = GOLD
+    out = O.build_gn(G["gn_src"], G["gn_tgt"], f4(G["gn_tn"]), f4(G["gn_tg"]), G["gn_nn"],
+                     float(G["gn_lam"]), G["gn_T"])
+    JTJ = np.zeros((6, 6))
+    k = 0
+    for a in range(6):
+        for b in range(a, 6):
+            JTJ[a, b] = JTJ[b, a] = out[k]
+            k += 1
+    scale = np.abs(G["gn_JTJ"]).max()
+    np.testing.assert_allclose(JTJ, G["gn_JTJ"], rtol=1e-6, atol=1e-7 * scale)
+    np.testing.assert_allclose(out[21:27], G["gn_JTr"], rtol=1e-6, atol=1e-7 * np.abs(G["gn_JTr"]).max())
+    assert abs(out[27] - G["gn_r2"]) < 1e-9 * max(1, G["gn_r2"])
+    assert out[28] == G["gn_ncorr"]
+
+
+def test_solve6_and_euler():
+    st, xi, T = O.solve6([G for G in [GOLD["sv_A"][a, b] for a in range(6) for b in range(a, 6)]], GOLD["sv_b"])
+    assert st == 0
+    np.testing.assert_allclose(xi, GOLD["sv_xi"], rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(T, GOLD["sv_T"], atol=1e-13)
+    ang = np.array([0.1, -0.2, 0.3, 0, 0, 0])
+    np.testing.assert_allclose(O.euler_to_T(ang)[:3, :3], Rotation.from_euler("xyz", ang[:3]).as_matrix(), atol=1e-15)
+    # not positive definite -> identity step and status 4 (eigen_solver.hpp:60-62)
+    st, xi, T = O.solve6(np.zeros(21), np.ones(6))
+    assert st == 4 and np.all(xi == 0) and np.allclose(T, np.eye(4))
+
+
+def noiseless_pair(seed=8, n=4000):
+    rng = np.random.default_rng(seed)
+    pts = []
+    for nrm, d in (([0, 0, 1], 2.0), ([0, 1, 0], 0.8), ([1, 0, 0], -0.9)):
+        a = rng.uniform(-0.8, 0.8, size=(n, 2))
+        nrm = np.array(nrm, float)
+        e1 = np.roll(nrm, 1)
+        e2 = np.cross(nrm, e1)
+        pts.append(nrm * d + a[:, :1] * e1 + a[:, 1:] * e2 * 0.999)
+    p = np.concatenate(pts)
+    p = p[(p[:, 2] > 0.5) & (np.abs(p[:, 0]) < 1.0) & (np.abs(p[:, 1]) < 0.9)]
+    inten = 0.5 + 0.3 * np.sin(8 * p[:, 0]) * np.cos(6 * p[:, 1]) + 0.2 * np.sin(5 * p[:, 2])
+    return p, inten
+
+
+def test_icp_recovers_known_transform():
+    p, inten = noiseless_pair()
+    T_true = np.eye(4)
+    T_true[:3, :3] = Rotation.from_euler("xyz", [0.03, -0.02, 0.04]).as_matrix()
+    T_true[:3, 3] = [0.02, -0.015, 0.01]
+    src = (p - T_true[:3, 3]) @ T_true[:3, :3]  # T_true maps src -> tgt
+    tgt = f4(p, inten)
+    sx = f4(src, inten)
+    tn = O.estimate_normals(tgt, 0.05, 30)
+    tg = O.color_gradient(tgt, tn, 0.04, 0)
+    T, rep = O.icp_align(sx, tgt, tn, tg, O.icp_params(60, 0.1, 0.968, 0.0, 0.0))
+    ang = np.arccos(np.clip((np.trace(T[:3, :3] @ T_true[:3, :3].T) - 1) / 2, -1, 1))
+    assert ang < 1e-6 and np.linalg.norm(T[:3, 3] - T_true[:3, 3]) < 1e-6
+    assert rep.iterations == 60 and rep.fitness > 0.95
+
+
+def test_objective_gradient_matches_finite_differences():
+    p, inten = noiseless_pair(9, 800)
+    tgt = f4(p, inten)
+    tn = O.estimate_normals(tgt, 0.1, 30)
+    tg = O.color_gradient(tgt, tn, 0.08, 0)
+    rng = np.random.default_rng(10)
+    src = f4(p + rng.normal(0, 0.01, p.shape), inten + rng.normal(0, 0.02, len(p)))
+    T = np.eye(4)
+    T[:3, :3] = Rotation.from_euler("xyz", [0.01, 0.02, -0.01]).as_matrix()
+    nn = O.nn_search(f4((src[:, :3].astype(np.float64) @ T[:3, :3].T) + T[:3, 3]), tgt, 0.1)
+    xi = np.array([0.004, -0.003, 0.002, 0.001, -0.002, 0.003])
+    E, dxi, dp = O.objective_grad(src, tgt, tn, tg, nn, 0.968, T, xi)
+    h = 1e-6
+    fd = np.zeros(6)
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = h
+        fd[k] = (O.objective_grad(src, tgt, tn, tg, nn, 0.968, T, xi + e, False)[0] -
+                 O.objective_grad(src, tgt, tn, tg, nn, 0.968, T, xi - e, False)[0]) / (2 * h)
+    np.testing.assert_allclose(dxi, fd, rtol=1e-5, atol=1e-8 * np.abs(fd).max())
+    # at xi = 0 the pose gradient equals 2 J^T r of the Gauss-Newton build
+    _, dxi0, _ = O.objective_grad(src, tgt, tn, tg, nn, 0.968, T, np.zeros(6))
+    sysv = O.build_gn(src, tgt, tn, tg, nn, 0.968, T)
+    np.testing.assert_allclose(dxi0, 2 * sysv[21:27], rtol=1e-10, atol=1e-12)
+    # per-point gradients vs finite differences on a few points
+    for i in np.nonzero(nn >= 0)[0][:5]:
+        for a in range(4):
+            s2 = src.copy()
+            hp = 1e-4 if a < 3 else 1e-3
+            s2[i, a] += np.float32(hp)
+            ep = O.objective_grad(s2, tgt, tn, tg, nn, 0.968, T, xi, False)[0]
+            s2[i, a] -= np.float32(2 * hp)
+            em = O.objective_grad(s2, tgt, tn, tg, nn, 0.968, T, xi, False)[0]
+            step = float(np.float32(src[i, a] + np.float32(hp)) - np.float32(src[i, a] - np.float32(hp)))
+            assert abs((ep - em) / step - dp[i, a]) < 1e-4 * (1 + abs(dp[i, a]))
+
+
+def test_outlier_removal_counts():
+    rng = np.random.default_rng(11)
+    p = np.concatenate([rng.uniform(0, 0.2, size=(400, 3)), rng.uniform(0.5, 3, size=(30, 3))]).astype(np.float32)
+    ox, _ = O.radius_outlier_removal(f4(p), f4(np.zeros_like(p)), 0.03, 4)
+    d2 = ((p[:, None].astype(np.float64) - p[None].astype(np.float64)) ** 2).sum(-1)
+    keep = (d2 < 0.03 * 0.03).sum(1) > 4
+    np.testing.assert_array_equal(ox[:, :3], p[keep])
+
+
+PLANTS = {
+    1: ("transposed cross product in J_g", lambda: test_gauss_newton_matches_finite_difference_jacobian()),
+    2: ("dropped (k-1)^2 n n^T term", lambda: test_gradient_matches_numpy_solve_and_plane_slope()),
+    3: ("wrong Euler composition order", lambda: test_solve6_and_euler()),
+    4: ("largest instead of smallest eigenvector", lambda: test_normals_match_numpy_eigh(15)),
+    5: ("nearest-neighbour tie toward the higher index", lambda: test_nn_and_radius_match_brute_force()),
+    6: ("Gaussian weight with the 1/2 factor", lambda: test_bilateral_matches_direct_fp64_loop()),
+}
+
+
+@pytest.mark.parametrize("plant", sorted(PLANTS))
+def test_planted_mistakes_are_caught(plant, tmp_path):
+    """Each plausible mistake, planted in the oracle (-DORC_PLANT), fails its pin."""
+    path = O.build_variant(plant, str(tmp_path))
+    with O.using(path):
+        with pytest.raises(AssertionError):
+            PLANTS[plant][1]()
