@@ -29,6 +29,7 @@ GCA_OK, GCA_ERR_INVALID_VALUE, GCA_ERR_OUT_OF_MEMORY, GCA_ERR_DEVICE, GCA_ERR_NO
 BILATERAL_NONE, BILATERAL_FIXED, BILATERAL_REFERENCE_ADAPTIVE = range(3)
 ACCUM_COMPENSATED, ACCUM_NAIVE_FP32, ACCUM_FP64 = range(3)
 RESIDUAL_PAIR, RESIDUAL_ABS, RESIDUAL_RMSE = range(3)
+STAGES = ("frontend", "voxel", "grid", "normals", "gradient", "icp_iter")
 
 
 class GcaError(RuntimeError):
@@ -87,6 +88,8 @@ _SIGS = {
     "gca_ctx_destroy": (i32, [vp]),
     "gca_ctx_synchronize": (i32, [vp]),
     "gca_ctx_get_stream": (i32, [vp, P(vp)]),
+    "gca_ctx_profile": (i32, [vp, i32]),
+    "gca_ctx_profile_read": (i32, [vp, vp, vp]),
     "gca_cloud_create": (i32, [vp, i64, P(vp)]),
     "gca_cloud_destroy": (i32, [vp]),
     "gca_cloud_size": (i32, [vp, P(i64)]),

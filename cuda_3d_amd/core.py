@@ -63,6 +63,17 @@ class Context:
         check(lib.gca_ctx_get_stream(self.handle, C.byref(s)), "gca_ctx_get_stream")
         return s.value or 0
 
+    def profile(self, enable: bool = True):
+        """Bracket every stage launch with hipEvents on the ctx stream (clears old records)."""
+        check(lib.gca_ctx_profile(self.handle, 1 if enable else 0), "gca_ctx_profile")
+
+    def profile_read(self) -> dict:
+        """{stage: (launches, total_ms)} since profiling was enabled (synchronises)."""
+        n = np.zeros(len(L.STAGES), np.int64)
+        ms = np.zeros(len(L.STAGES), np.float64)
+        check(lib.gca_ctx_profile_read(self.handle, _ptr(n), _ptr(ms)), "gca_ctx_profile_read")
+        return {s: (int(n[i]), float(ms[i])) for i, s in enumerate(L.STAGES)}
+
     def close(self):
         if getattr(self, "handle", None):
             lib.gca_ctx_destroy(self.handle)

@@ -325,6 +325,26 @@ int32_t grid_build(gca_ctx ctx, gca_cloud c, float radius, bool exact)
     return GCA_OK;
 }
 
+StageTimer::StageTimer(gca_ctx c, int stage) : ctx(c)
+{
+    if (!ctx->prof_on) return;
+    while (ctx->prof_pool.size() < ctx->prof_used + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        ctx->prof_pool.push_back(e);
+    }
+    gca_ctx_s::ProfRec r{stage, ctx->prof_pool[ctx->prof_used], ctx->prof_pool[ctx->prof_used + 1]};
+    ctx->prof_used += 2;
+    if (hipEventRecord(r.a, ctx->stream) != hipSuccess) return;
+    idx = (int)ctx->prof.size();
+    ctx->prof.push_back(r);
+}
+
+StageTimer::~StageTimer()
+{
+    if (idx >= 0) (void)hipEventRecord(ctx->prof[idx].b, ctx->stream);
+}
+
 }  // namespace gca
 
 using namespace gca;
@@ -377,6 +397,7 @@ int32_t gca_ctx_destroy(gca_ctx c)
     for (int i = 0; i < 16; ++i)
         if (c->level_clouds[i]) gca_cloud_destroy(c->level_clouds[i]);
     for (int i = 0; i < 16; ++i) hipFree(c->scratch[i]);
+    for (hipEvent_t e : c->prof_pool) hipEventDestroy(e);
     hipFree(c->bitmap);
     hipFree(c->lut);
     hipFree(c->icp_state);
@@ -399,6 +420,33 @@ int32_t gca_ctx_get_stream(gca_ctx c, void **stream)
 {
     if (!c || !stream) return GCA_ERR_INVALID_VALUE;
     *stream = (void *)c->stream;
+    return GCA_OK;
+}
+
+int32_t gca_ctx_profile(gca_ctx c, int32_t enable)
+{
+    if (!c) return GCA_ERR_INVALID_VALUE;
+    GCA_HIP(hipStreamSynchronize(c->stream));
+    c->prof.clear();
+    c->prof_used = 0;
+    c->prof_on = enable != 0;
+    return GCA_OK;
+}
+
+int32_t gca_ctx_profile_read(gca_ctx c, int64_t *launches, double *total_ms)
+{
+    if (!c || !launches || !total_ms) return GCA_ERR_INVALID_VALUE;
+    GCA_HIP(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < GCA_STAGE_COUNT; ++s) {
+        launches[s] = 0;
+        total_ms[s] = 0.0;
+    }
+    for (const auto &r : c->prof) {
+        float ms = 0.f;
+        GCA_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        launches[r.stage] += 1;
+        total_ms[r.stage] += ms;
+    }
     return GCA_OK;
 }
 

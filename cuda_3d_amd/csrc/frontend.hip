@@ -299,6 +299,7 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
     uint32_t *boff = (uint32_t *)ctx_scratch(ctx, 4, sizeof(uint32_t) * (nblocks + 1), false);
     if (!st_xyzi || !st_rgb || !st_valid || !bcnt || !boff) return GCA_ERR_OUT_OF_MEMORY;
     dim3 grid(fp.nbx, H);
+    StageTimer timer(ctx, GCA_STAGE_FRONTEND);
     hipLaunchKernelGGL(k_rgbd_pixels, grid, dim3(kRowSeg), 0, ctx->stream, depth, bgr,
                        (const double *)ctx->lut, fp, st_xyzi, st_rgb, st_valid, bcnt, out->hdr);
     GCA_HIP(hipGetLastError());

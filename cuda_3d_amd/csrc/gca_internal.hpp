@@ -100,7 +100,26 @@ struct gca_ctx_s {
     size_t partials_bytes = 0;
     // per-level working clouds of the multi-scale driver (created on first use)
     gca_cloud level_clouds[16] = {nullptr};
+    // stage profiler: event pairs recorded on `stream` while enabled
+    bool prof_on = false;
+    struct ProfRec {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<ProfRec> prof;
+    std::vector<hipEvent_t> prof_pool;
+    size_t prof_used = 0;
 };
+
+namespace gca {
+// Brackets a stage's launches with hipEvents when the ctx profiler is on (RAII).
+struct StageTimer {
+    gca_ctx ctx;
+    int idx = -1;
+    StageTimer(gca_ctx c, int stage);
+    ~StageTimer();
+};
+}  // namespace gca
 
 struct gca_cloud_s {
     gca_ctx ctx = nullptr;

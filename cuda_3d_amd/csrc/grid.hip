@@ -89,6 +89,7 @@ int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d)
         G.cap_pts = c->cap;
     }
     const unsigned nb = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (c->cap + 255) / 256));
+    StageTimer timer(ctx, GCA_STAGE_GRID);
     hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, ctx->stream, c->xyzi, c->hdr, d, G.count,
                        G.cell_of);
     GCA_HIP(hipGetLastError());

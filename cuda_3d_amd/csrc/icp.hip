@@ -580,6 +580,7 @@ static void launch_iter(gca_ctx ctx, int mode, const PairDesc *pairs, const int3
                         float sc)
 {
     dim3 g((unsigned)nblocks), b(kIcpBlock);
+    StageTimer timer(ctx, GCA_STAGE_ICP_ITER);
     if (mode == GCA_ACCUM_NAIVE_FP32)
         hipLaunchKernelGGL(k_icp_iter<GCA_ACCUM_NAIVE_FP32>, g, b, 0, ctx->stream, pairs, bp, sl, sc);
     else if (mode == GCA_ACCUM_FP64)

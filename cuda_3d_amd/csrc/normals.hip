@@ -323,6 +323,7 @@ static unsigned wave_blocks(int64_t n)
 int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
 {
     const Grid &G = c->grid;
+    StageTimer timer(ctx, GCA_STAGE_NORMALS);
     hipLaunchKernelGGL(k_normals, dim3(wave_blocks(c->cap)), dim3(kWave * kWavesPerBlock), 0, ctx->stream,
                        c->xyzi, c->hdr, query_desc(G, radius), G.start, G.order, G.sxyzi, K, c->nrm);
     GCA_HIP(hipGetLastError());
@@ -332,6 +333,7 @@ int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
 int32_t launch_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
 {
     const Grid &G = c->grid;
+    StageTimer timer(ctx, GCA_STAGE_GRADIENT);
     hipLaunchKernelGGL(k_gradient, dim3(wave_blocks(c->cap)), dim3(kWave * kWavesPerBlock), 0, ctx->stream,
                        c->xyzi, c->nrm, c->hdr, query_desc(G, radius), G.start, G.order, G.sxyzi, K, c->grad);
     GCA_HIP(hipGetLastError());

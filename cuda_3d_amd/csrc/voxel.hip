@@ -154,6 +154,7 @@ int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out)
     if (!lin || !word_pre || !acc) return GCA_ERR_OUT_OF_MEMORY;
     const unsigned nb = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (n + 255) / 256));
     ctx->bitmap_dirty = true;  // until finalize has run to completion
+    StageTimer timer(ctx, GCA_STAGE_VOXEL);
     hipLaunchKernelGGL(k_voxel_mark, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->hdr, vp, ctx->bitmap, lin);
     GCA_HIP(hipGetLastError());
     GCA_TRY(exclusive_scan_u32(ctx, ctx->bitmap, word_pre, words, (uint32_t *)&out->hdr->n, true));

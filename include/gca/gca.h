@@ -140,6 +140,22 @@ int32_t gca_ctx_synchronize(gca_ctx ctx);
 /* Borrow the ctx's hipStream_t (e.g. to order caller work or time it). */
 int32_t gca_ctx_get_stream(gca_ctx ctx, void **stream);
 
+/* Stage timing with hipEvents on the ctx stream (no reference counterpart: the reference times
+ * only with std::chrono in its demos, apps/app.cpp:117,354-357).  While enabled, every launch of
+ * a stage is bracketed by an event pair.  Stages: */
+enum {
+    GCA_STAGE_FRONTEND = 0,   /* bilateral + back-projection + compaction (3 kernels) */
+    GCA_STAGE_VOXEL = 1,      /* mark + popcount scan + accumulate + finalize */
+    GCA_STAGE_GRID = 2,       /* count + scan + scatter */
+    GCA_STAGE_NORMALS = 3,    /* one kernel */
+    GCA_STAGE_GRADIENT = 4,   /* one kernel */
+    GCA_STAGE_ICP_ITER = 5,   /* one kernel per Gauss-Newton iteration */
+    GCA_STAGE_COUNT = 6
+};
+int32_t gca_ctx_profile(gca_ctx ctx, int32_t enable);  /* enable != 0 also clears the records */
+/* Synchronises; launches[s] and total_ms[s] (arrays of GCA_STAGE_COUNT) since the last enable. */
+int32_t gca_ctx_profile_read(gca_ctx ctx, int64_t *launches, double *total_ms);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Point clouds                                                                              */
 
