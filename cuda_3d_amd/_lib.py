@@ -30,6 +30,7 @@ BILATERAL_NONE, BILATERAL_FIXED, BILATERAL_REFERENCE_ADAPTIVE = range(3)
 ACCUM_COMPENSATED, ACCUM_NAIVE_FP32, ACCUM_FP64 = range(3)
 RESIDUAL_PAIR, RESIDUAL_ABS, RESIDUAL_RMSE = range(3)
 STAGES = ("frontend", "voxel", "grid", "normals", "gradient", "icp_iter")
+TUNE_KEYS = ("icp_lanes_small", "icp_lanes_medium", "icp_lanes_large", "icp_max_blocks", "normals_lanes")
 
 
 class GcaError(RuntimeError):
@@ -89,6 +90,8 @@ _SIGS = {
     "gca_ctx_synchronize": (i32, [vp]),
     "gca_ctx_get_stream": (i32, [vp, P(vp)]),
     "gca_ctx_profile": (i32, [vp, i32]),
+    "gca_ctx_set_tuning": (i32, [vp, i32, i32]),
+    "gca_ctx_get_tuning": (i32, [vp, i32, P(i32)]),
     "gca_ctx_profile_read": (i32, [vp, vp, vp]),
     "gca_cloud_create": (i32, [vp, i64, P(vp)]),
     "gca_cloud_destroy": (i32, [vp]),

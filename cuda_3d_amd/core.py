@@ -63,6 +63,19 @@ class Context:
         check(lib.gca_ctx_get_stream(self.handle, C.byref(s)), "gca_ctx_get_stream")
         return s.value or 0
 
+    def set_tuning(self, **kw):
+        """Launch-shape knobs (names in _lib.TUNE_KEYS), 0 = built-in heuristic."""
+        for k, v in kw.items():
+            check(lib.gca_ctx_set_tuning(self.handle, L.TUNE_KEYS.index(k), int(v)), f"set_tuning {k}")
+
+    def get_tuning(self) -> dict:
+        out = {}
+        for i, k in enumerate(L.TUNE_KEYS):
+            v = C.c_int32()
+            check(lib.gca_ctx_get_tuning(self.handle, i, C.byref(v)), "get_tuning")
+            out[k] = v.value
+        return out
+
     def profile(self, enable: bool = True):
         """Bracket every stage launch with hipEvents on the ctx stream (clears old records)."""
         check(lib.gca_ctx_profile(self.handle, 1 if enable else 0), "gca_ctx_profile")

@@ -264,6 +264,7 @@ void cloud_points_changed(gca_cloud c)
 {
     c->host_valid = false;
     c->grid.valid = false;
+    c->grid.attrs_valid = false;
 }
 
 int32_t launch_bounds(gca_ctx ctx, gca_cloud c)
@@ -423,6 +424,25 @@ int32_t gca_ctx_get_stream(gca_ctx c, void **stream)
     return GCA_OK;
 }
 
+int32_t gca_ctx_set_tuning(gca_ctx c, int32_t key, int32_t value)
+{
+    if (!c || key < 0 || key >= GCA_TUNE_COUNT || value < 0) return GCA_ERR_INVALID_VALUE;
+    auto pow2 = [](int32_t v, int32_t hi) { return v == 0 || (v <= hi && (v & (v - 1)) == 0); };
+    if (key <= GCA_TUNE_ICP_LANES_LARGE && !pow2(value, 64)) return GCA_ERR_INVALID_VALUE;
+    if (key == GCA_TUNE_ICP_MAX_BLOCKS && value > 2048) return GCA_ERR_INVALID_VALUE;
+    if (key == GCA_TUNE_NORMALS_LANES && (!pow2(value, 64) || (value != 0 && value < 8)))
+        return GCA_ERR_INVALID_VALUE;
+    c->tune[key] = value;
+    return GCA_OK;
+}
+
+int32_t gca_ctx_get_tuning(gca_ctx c, int32_t key, int32_t *value)
+{
+    if (!c || !value || key < 0 || key >= GCA_TUNE_COUNT) return GCA_ERR_INVALID_VALUE;
+    *value = c->tune[key];
+    return GCA_OK;
+}
+
 int32_t gca_ctx_profile(gca_ctx c, int32_t enable)
 {
     if (!c) return GCA_ERR_INVALID_VALUE;
@@ -490,6 +510,8 @@ static void grid_free(Grid &g)
     hipFree(g.order);
     hipFree(g.sxyzi);
     hipFree(g.cell_of);
+    hipFree(g.snrm);
+    hipFree(g.sgrad);
     g = Grid();
 }
 
@@ -593,6 +615,7 @@ int32_t gca_cloud_upload_normals(gca_cloud c, const float *normals, int64_t n)
     GCA_TRY(upload_vec3(c, c->nrm, normals, n));
     c->has_normals = true;
     c->has_gradient = false;
+    c->grid.attrs_valid = false;
     return GCA_OK;
 }
 
@@ -601,6 +624,7 @@ int32_t gca_cloud_upload_gradient(gca_cloud c, const float *gradient, int64_t n)
     if (!c) return GCA_ERR_INVALID_VALUE;
     GCA_TRY(upload_vec3(c, c->grad, gradient, n));
     c->has_gradient = true;
+    c->grid.attrs_valid = false;
     return GCA_OK;
 }
 
@@ -758,6 +782,7 @@ int32_t gca_estimate_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t max
     GCA_TRY(launch_normals(ctx, c, radius, max_nn));
     c->has_normals = true;
     c->has_gradient = false;
+    c->grid.attrs_valid = false;
     return GCA_OK;
 }
 
@@ -769,6 +794,7 @@ int32_t gca_compute_color_gradient(gca_ctx ctx, gca_cloud c, float radius, int32
     GCA_TRY(grid_build(ctx, c, radius));
     GCA_TRY(launch_gradient(ctx, c, radius, max_nn));
     c->has_gradient = true;
+    c->grid.attrs_valid = false;
     return GCA_OK;
 }
 

@@ -10,6 +10,13 @@ namespace gca {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Device-scope ordering for the last-block (ticket) pattern.  __threadfence() is a seq_cst
+// fence, which on gfx950 both writes back AND invalidates the XCD's L2: issued by every block
+// it evicts the data the other blocks are still reading.  Writers only need release
+// (write-back); only the block that wins the ticket needs acquire (invalidate).
+__device__ __forceinline__ void fence_release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void fence_acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
 // ---- wave64 scans / reductions ----------------------------------------------------------
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {

@@ -71,6 +71,9 @@ struct Grid {
     int32_t *order = nullptr;    // original index of each sorted slot
     float4 *sxyzi = nullptr;     // points in cell order
     uint32_t *cell_of = nullptr; // linear cell of each original point (cap_pts), 32-bit
+    float4 *snrm = nullptr;      // normals / colour gradients in cell order (registration
+    float4 *sgrad = nullptr;     // targets), filled by grid_gather_attributes
+    bool attrs_valid = false;
 };
 
 }  // namespace gca
@@ -100,6 +103,8 @@ struct gca_ctx_s {
     size_t partials_bytes = 0;
     // per-level working clouds of the multi-scale driver (created on first use)
     gca_cloud level_clouds[16] = {nullptr};
+    // launch-shape tuning (GCA_TUNE_*), 0 = heuristic
+    int32_t tune[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // stage profiler: event pairs recorded on `stream` while enabled
     bool prof_on = false;
     struct ProfRec {
@@ -166,6 +171,8 @@ int32_t launch_radius(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int
                       int32_t *offsets, int32_t *idx, int64_t idx_cap, int64_t *total);
 int32_t launch_grid_cells(gca_ctx ctx, gca_cloud c, int32_t *cells);
 int32_t launch_outlier(gca_ctx ctx, gca_cloud in, float radius, int32_t min_nn, gca_cloud out);
+// Copies normals and colour gradients into the grid's cell order (c->grid.snrm / sgrad).
+int32_t grid_gather_attributes(gca_ctx ctx, gca_cloud c);
 int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn);
 int32_t launch_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn);
 int32_t launch_transform(gca_ctx ctx, gca_cloud c, const double T[16]);

@@ -79,9 +79,12 @@ int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d)
         hipFree(G.order);
         hipFree(G.sxyzi);
         hipFree(G.cell_of);
+        hipFree(G.snrm);
+        hipFree(G.sgrad);
         G.order = nullptr;
         G.sxyzi = nullptr;
         G.cell_of = nullptr;
+        G.snrm = G.sgrad = nullptr;
         G.cap_pts = 0;
         GCA_HIP(hipMalloc(&G.order, sizeof(int32_t) * c->cap));
         GCA_HIP(hipMalloc(&G.sxyzi, sizeof(float4) * c->cap));
@@ -99,6 +102,38 @@ int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d)
     GCA_HIP(hipGetLastError());
     G.desc = d;
     G.valid = true;
+    G.attrs_valid = false;
+    return GCA_OK;
+}
+
+__global__ __launch_bounds__(256) void k_gather_sorted(const CloudHdr *__restrict__ hdr,
+                                                       const int32_t *__restrict__ order,
+                                                       const float4 *__restrict__ nrm,
+                                                       const float4 *__restrict__ grad,
+                                                       float4 *__restrict__ snrm, float4 *__restrict__ sgrad)
+{
+    const int n = hdr->n;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const int32_t j = order[s];
+        snrm[s] = nrm[j];
+        sgrad[s] = grad[j];
+    }
+}
+
+int32_t grid_gather_attributes(gca_ctx ctx, gca_cloud c)
+{
+    Grid &G = c->grid;
+    if (!G.valid) return GCA_ERR_NOT_READY;
+    if (G.attrs_valid) return GCA_OK;
+    if (!G.snrm) {
+        GCA_HIP(hipMalloc(&G.snrm, sizeof(float4) * G.cap_pts));
+        GCA_HIP(hipMalloc(&G.sgrad, sizeof(float4) * G.cap_pts));
+    }
+    const unsigned nb = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (c->cap + 255) / 256));
+    hipLaunchKernelGGL(k_gather_sorted, dim3(nb), dim3(256), 0, ctx->stream, c->hdr, G.order, c->nrm, c->grad,
+                       G.snrm, G.sgrad);
+    GCA_HIP(hipGetLastError());
+    G.attrs_valid = true;
     return GCA_OK;
 }
 

@@ -36,13 +36,17 @@ constexpr int kIcpBlock = 256;
 constexpr int kNSys = 29;        // 21 JTJ + 6 JTr + r^2 + n_corr
 constexpr int kPartStride = 32;  // doubles per block partial
 
+constexpr int kMaxClusters = 64;  // block partials are reduced in clusters, then clusters
+constexpr int kTicketStride = 64; // uint32 per ticket: one 256-byte line each, so the atomics
+                                  // of different clusters do not serialise on a shared line
+
 struct IcpState {
     double T[16];
     double sys[kNSys];
     double conv_rot, conv_t2;
     int32_t iter, max_iter, done, converged, solver_failures, pad0;
-    uint32_t ticket;
-    int32_t pad1;
+    alignas(256) uint32_t ticket;
+    alignas(256) uint32_t cl_ticket[kMaxClusters * kTicketStride];
 };
 
 struct PairDesc {
@@ -52,12 +56,16 @@ struct PairDesc {
     const uint32_t *start;
     const int32_t *order;
     const float4 *sxyzi;
-    const float4 *nrm;
+    const float4 *nrm;   // target normals / gradients by original index
     const float4 *grad;
+    const float4 *snrm;  // ... and in grid (slot) order
+    const float4 *sgrad;
     IcpState *st;
-    double *partials;
+    double *partials;   // nblk block partials, then nclust cluster partials (kPartStride each)
     int32_t nblk;
     int32_t first_block;
+    int32_t cluster;    // blocks per cluster
+    int32_t nclust;
 };
 
 // ---- per-correspondence residuals and Jacobians (FP32, reference formulas) ------------------
@@ -209,14 +217,17 @@ __device__ __forceinline__ int32_t nn_slot(const PairDesc &P, float qx, float qy
 
 // Block reduction of the accumulator into FP64 partial `out[0..27]`, plus the correspondence
 // count into out[28].  Wave: butterfly merges; block: the 4 wave results merged in wave order.
-template <int MODE>
+// LEAD: only lanes that are multiples of LEAD hold data (group leaders), so the butterfly runs
+// over masks LEAD .. 32 only.
+template <int MODE, int LEAD = 1>
 __device__ __forceinline__ void block_partial(Acc<MODE> &A, int ncorr, double *out)
 {
     __shared__ Acc<MODE> wacc[kIcpBlock / kWave];
     __shared__ int wcnt[kIcpBlock / kWave];
 #pragma unroll
-    for (int m = kWave / 2; m > 0; m >>= 1) A.merge_from_lane(m);
-    ncorr = wave_sum(ncorr);
+    for (int m = kWave / 2; m >= LEAD; m >>= 1) A.merge_from_lane(m);
+#pragma unroll
+    for (int m = kWave / 2; m >= LEAD; m >>= 1) ncorr += __shfl_xor(ncorr, m, kWave);
     const int w = threadIdx.x / kWave;
     if (lane_id() == 0) {
         wacc[w] = A;
@@ -276,7 +287,109 @@ __device__ __forceinline__ void solve_and_update(IcpState *st, const double *sys
     if (it >= st->max_iter) st->done = 1;
 }
 
-template <int MODE>
+// Nearest neighbour searched by a group of G lanes (G divides 64): lane g of the group scans
+// candidates s0 + g, s0 + g + G, ... of every run, then the group reduces (d2, index) by
+// shuffles.  All lanes of the group return the same slot.
+//
+// Latency structure: the bounds of all (ix, iy) runs (at most 3 x 3 when cell >= radius) are
+// loaded up front, then the lane walks one flattened candidate space with kUnroll independent
+// loads in flight, so a query costs a few memory round trips instead of one per candidate.
+constexpr int kMaxRuns = 9;
+constexpr int kUnroll = 4;
+
+template <int G>
+__device__ __forceinline__ int32_t nn_slot_group(const PairDesc &P, float qx, float qy, float qz, int g)
+{
+    int32_t lo[3], hi[3];
+    double bd = DBL_MAX;
+    int32_t bi = INT_MAX, bs = -1;
+    if (grid_range(P.g, qx, qy, qz, lo, hi)) {
+        const double r2 = (double)P.g.radius * (double)P.g.radius;
+        const int nry = hi[1] - lo[1] + 1;
+        const int nruns = (hi[0] - lo[0] + 1) * nry;
+        uint32_t rs[kMaxRuns], cum[kMaxRuns + 1];
+        if (nruns <= kMaxRuns) {
+#pragma unroll
+            for (int r = 0; r < kMaxRuns; ++r) {
+                uint32_t a = 0, b = 0;
+                if (r < nruns) {
+                    const int ix = lo[0] + r / nry, iy = lo[1] + r % nry;
+                    const int64_t row = ((int64_t)ix * P.g.dims[1] + iy) * P.g.dims[2];
+                    a = P.start[row + lo[2]];
+                    b = P.start[row + hi[2] + 1];
+                }
+                rs[r] = a;
+                cum[r + 1] = b - a;
+            }
+            cum[0] = 0;
+#pragma unroll
+            for (int r = 0; r < kMaxRuns; ++r) cum[r + 1] += cum[r];
+            const uint32_t total = cum[kMaxRuns];
+            for (uint32_t f0 = g; f0 < total; f0 += G * kUnroll) {
+                uint32_t sl[kUnroll];
+                float4 c[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    const uint32_t f = f0 + u * G;
+                    uint32_t s = 0xffffffffu;
+                    if (f < total) {
+                        s = rs[0] + f;
+#pragma unroll
+                        for (int r = 1; r < kMaxRuns; ++r)
+                            if (f >= cum[r]) s = rs[r] + (f - cum[r]);
+                        c[u] = P.sxyzi[s];
+                    }
+                    sl[u] = s;
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    if (sl[u] == 0xffffffffu) continue;
+                    const double d2 = dist2(c[u].x, c[u].y, c[u].z, qx, qy, qz);
+                    if (d2 < r2 && d2 <= bd) {
+                        const int32_t idx = P.order[sl[u]];
+                        if (key_less(d2, idx, bd, bi)) {
+                            bd = d2;
+                            bi = idx;
+                            bs = (int32_t)sl[u];
+                        }
+                    }
+                }
+            }
+        } else {
+            // cell smaller than the radius (not produced by grid_build): plain run walk
+            for (int32_t ix = lo[0]; ix <= hi[0]; ++ix)
+                for (int32_t iy = lo[1]; iy <= hi[1]; ++iy) {
+                    const int64_t row = ((int64_t)ix * P.g.dims[1] + iy) * P.g.dims[2];
+                    const uint32_t s0 = P.start[row + lo[2]], s1 = P.start[row + hi[2] + 1];
+                    for (uint32_t s = s0 + g; s < s1; s += G) {
+                        const float4 c = P.sxyzi[s];
+                        const double d2 = dist2(c.x, c.y, c.z, qx, qy, qz);
+                        if (d2 < r2 && d2 <= bd) {
+                            const int32_t idx = P.order[s];
+                            if (key_less(d2, idx, bd, bi)) {
+                                bd = d2;
+                                bi = idx;
+                                bs = (int32_t)s;
+                            }
+                        }
+                    }
+                }
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) {
+        const double od = __shfl_xor(bd, m, kWave);
+        const int32_t oi = __shfl_xor(bi, m, kWave), os = __shfl_xor(bs, m, kWave);
+        if (key_less(od, oi, bd, bi)) {
+            bd = od;
+            bi = oi;
+            bs = os;
+        }
+    }
+    return bs;
+}
+
+template <int MODE, int G>
 __global__ __launch_bounds__(kIcpBlock) void k_icp_iter(const PairDesc *__restrict__ pairs,
                                                         const int32_t *__restrict__ block_pair,
                                                         float sl, float sc)
@@ -293,15 +406,22 @@ __global__ __launch_bounds__(kIcpBlock) void k_icp_iter(const PairDesc *__restri
     Acc<MODE> A;
     A.zero();
     int ncorr = 0;
-    for (int i = lb * kIcpBlock + threadIdx.x; i < n; i += P.nblk * kIcpBlock) {
-        const float4 p = P.src[i];
-        float x, y, z;
-        xform_f32(T, p, x, y, z);
-        const int32_t s = nn_slot(P, x, y, z);
-        if (s >= 0) {
-            const int32_t j = P.order[s];
+    const int g = threadIdx.x % G;
+    constexpr int kGroups = kIcpBlock / G;
+    // one query per group of G lanes; the group leader accumulates the 28 sums
+    for (int i0 = lb * kGroups; i0 < n; i0 += P.nblk * kGroups) {
+        const int i = i0 + (int)threadIdx.x / G;
+        const bool active = i < n;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (active) {
+            p = P.src[i];
+            xform_f32(T, p, x, y, z);
+        }
+        const int32_t s = nn_slot_group<G>(P, active ? x : NAN, y, z, g);
+        if (s >= 0 && g == 0) {
             Terms t;
-            corr_terms(x, y, z, p.w, P.sxyzi[s], P.nrm[j], P.grad[j], sl, sc, t);
+            corr_terms(x, y, z, p.w, P.sxyzi[s], P.snrm[s], P.sgrad[s], sl, sc, t);
             acc_terms(A, t);
             ++ncorr;
         }
@@ -309,19 +429,32 @@ __global__ __launch_bounds__(kIcpBlock) void k_icp_iter(const PairDesc *__restri
     __shared__ double sys[kPartStride];
     __shared__ bool last;
     double *my = P.partials + (int64_t)lb * kPartStride;
-    block_partial(A, ncorr, sys);
+    block_partial<MODE, G>(A, ncorr, sys);
     if (threadIdx.x < kNSys) my[threadIdx.x] = sys[threadIdx.x];
-    __threadfence();
+    // two-level deterministic finish: the last block of each cluster sums the cluster's block
+    // partials, the last cluster sums the cluster partials and solves
+    fence_release_agent();
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&st->ticket, 1u) == (uint32_t)(P.nblk - 1);
+    const int cl = lb / P.cluster;
+    const int csize = min(P.cluster, P.nblk - cl * P.cluster);
+    if (threadIdx.x == 0) last = atomicAdd(&st->cl_ticket[cl * kTicketStride], 1u) == (uint32_t)(csize - 1);
     __syncthreads();
     if (!last) return;
-    __threadfence();
-    sum_partials(P.partials, P.nblk, sys);
+    fence_acquire_agent();
+    double *cpart = P.partials + (int64_t)P.nblk * kPartStride;
+    sum_partials(P.partials + (int64_t)cl * P.cluster * kPartStride, csize, sys);
+    if (threadIdx.x < kNSys) cpart[(int64_t)cl * kPartStride + threadIdx.x] = sys[threadIdx.x];
+    if (threadIdx.x == 0) st->cl_ticket[cl * kTicketStride] = 0u;
+    fence_release_agent();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&st->ticket, 1u) == (uint32_t)(P.nclust - 1);
+    __syncthreads();
+    if (!last) return;
+    fence_acquire_agent();
+    sum_partials(cpart, P.nclust, sys);
     if (threadIdx.x == 0) {
         solve_and_update(st, sys);
         st->ticket = 0u;
-        __threadfence();
     }
 }
 
@@ -354,12 +487,12 @@ __global__ __launch_bounds__(kIcpBlock) void k_build_gn(PairDesc P, const float4
     __shared__ bool last;
     block_partial(A, ncorr, sys);
     if (threadIdx.x < kNSys) P.partials[(int64_t)blockIdx.x * kPartStride + threadIdx.x] = sys[threadIdx.x];
-    __threadfence();
+    fence_release_agent();
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(&st->ticket, 1u) == (uint32_t)(P.nblk - 1);
     __syncthreads();
     if (!last) return;
-    __threadfence();
+    fence_acquire_agent();
     sum_partials(P.partials, P.nblk, sys);
     if (threadIdx.x < kNSys) st->sys[threadIdx.x] = sys[threadIdx.x];
     if (threadIdx.x == 0) st->ticket = 0u;
@@ -382,12 +515,12 @@ __device__ __forceinline__ void sum_finish(double (&v)[NV], double *partials, in
         for (int w = 0; w < kIcpBlock / kWave; ++w) a += wv[w][threadIdx.x];
         partials[(int64_t)blockIdx.x * kPartStride + threadIdx.x] = a;
     }
-    __threadfence();
+    fence_release_agent();
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == (uint32_t)(nblk - 1);
     __syncthreads();
     if (!last) return;
-    __threadfence();
+    fence_acquire_agent();
     if (threadIdx.x < NV) {
         double a = 0.0;
         for (int b = 0; b < nblk; ++b) a += partials[(int64_t)b * kPartStride + threadIdx.x];
@@ -515,12 +648,32 @@ __global__ void k_state_init(IcpState *st, int n_states, const double *__restric
     s.converged = 0;
     s.solver_failures = 0;
     s.ticket = 0u;
+    for (int c = 0; c < kMaxClusters; ++c) s.cl_ticket[c * kTicketStride] = 0u;
 }
 
 // ---------------------------------------------------------------------------------------------
 // host side
 
 static int32_t icp_blocks(int64_t n) { return (int32_t)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kIcpBlock - 1) / kIcpBlock)); }
+
+// Lanes per correspondence query: wide groups shorten each query's dependent load chain,
+// which is what bounds small clouds; narrow groups keep large clouds' wave count reasonable.
+static int icp_group_lanes(gca_ctx ctx, int64_t max_n)
+{
+    const int key = max_n < 16384 ? GCA_TUNE_ICP_LANES_SMALL
+                                  : (max_n < 131072 ? GCA_TUNE_ICP_LANES_MEDIUM : GCA_TUNE_ICP_LANES_LARGE);
+    if (ctx->tune[key]) return ctx->tune[key];
+    if (max_n < 16384) return 16;
+    if (max_n < 131072) return 8;
+    return 4;
+}
+
+static int32_t icp_blocks_g(gca_ctx ctx, int64_t n, int G)
+{
+    const int64_t groups_per_block = kIcpBlock / G;
+    const int64_t cap = ctx->tune[GCA_TUNE_ICP_MAX_BLOCKS] ? ctx->tune[GCA_TUNE_ICP_MAX_BLOCKS] : 1024;
+    return (int32_t)std::min<int64_t>(cap, std::max<int64_t>(1, (n + groups_per_block - 1) / groups_per_block));
+}
 
 struct IcpWork {
     IcpState *states = nullptr;
@@ -530,10 +683,15 @@ struct IcpWork {
     double *aux = nullptr;  // T0 upload / results
 };
 
+constexpr int kIcpCluster = 32;  // block partials per cluster of the two-level finish
+
+static int32_t clusters_of(int32_t nblk) { return (nblk + kIcpCluster - 1) / kIcpCluster; }
+
+// partial slots: every pair's block partials followed by its cluster partials (<= 2 per block)
 static int32_t icp_work(gca_ctx ctx, int n_pairs, int64_t total_blocks, IcpWork &w)
 {
     w.states = (IcpState *)ctx_scratch(ctx, 13, sizeof(IcpState) * n_pairs, false);
-    w.partials = (double *)ctx_scratch(ctx, 14, sizeof(double) * kPartStride * (total_blocks + 1), false);
+    w.partials = (double *)ctx_scratch(ctx, 14, sizeof(double) * kPartStride * (2 * total_blocks + 1), false);
     if (!w.states || !w.partials) return GCA_ERR_OUT_OF_MEMORY;
     const size_t aux_doubles = std::max<size_t>(16 * (size_t)n_pairs, 64);
     const size_t need = sizeof(PairDesc) * n_pairs + sizeof(int32_t) * total_blocks + sizeof(double) * aux_doubles + 1024;
@@ -568,25 +726,44 @@ static PairDesc make_pair(gca_cloud src, gca_cloud tgt, IcpState *st, double *pa
         P.sxyzi = tgt->grid.sxyzi;
         P.nrm = tgt->nrm;
         P.grad = tgt->grad;
+        P.snrm = tgt->grid.snrm;
+        P.sgrad = tgt->grid.sgrad;
     }
     P.st = st;
     P.partials = partials;
     P.nblk = nblk;
     P.first_block = first;
+    P.cluster = kIcpCluster;
+    P.nclust = clusters_of(nblk);
     return P;
 }
 
-static void launch_iter(gca_ctx ctx, int mode, const PairDesc *pairs, const int32_t *bp, int64_t nblocks, float sl,
-                        float sc)
+template <int MODE>
+static void launch_iter_mode(gca_ctx ctx, int G, const PairDesc *pairs, const int32_t *bp, dim3 g, float sl, float sc)
 {
-    dim3 g((unsigned)nblocks), b(kIcpBlock);
+    dim3 b(kIcpBlock);
+    switch (G) {
+    case 64: hipLaunchKernelGGL((k_icp_iter<MODE, 64>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
+    case 32: hipLaunchKernelGGL((k_icp_iter<MODE, 32>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
+    case 16: hipLaunchKernelGGL((k_icp_iter<MODE, 16>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
+    case 8: hipLaunchKernelGGL((k_icp_iter<MODE, 8>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
+    case 4: hipLaunchKernelGGL((k_icp_iter<MODE, 4>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
+    case 2: hipLaunchKernelGGL((k_icp_iter<MODE, 2>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
+    default: hipLaunchKernelGGL((k_icp_iter<MODE, 1>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
+    }
+}
+
+static void launch_iter(gca_ctx ctx, int mode, int G, const PairDesc *pairs, const int32_t *bp, int64_t nblocks,
+                        float sl, float sc)
+{
+    dim3 g((unsigned)nblocks);
     StageTimer timer(ctx, GCA_STAGE_ICP_ITER);
     if (mode == GCA_ACCUM_NAIVE_FP32)
-        hipLaunchKernelGGL(k_icp_iter<GCA_ACCUM_NAIVE_FP32>, g, b, 0, ctx->stream, pairs, bp, sl, sc);
+        launch_iter_mode<GCA_ACCUM_NAIVE_FP32>(ctx, G, pairs, bp, g, sl, sc);
     else if (mode == GCA_ACCUM_FP64)
-        hipLaunchKernelGGL(k_icp_iter<GCA_ACCUM_FP64>, g, b, 0, ctx->stream, pairs, bp, sl, sc);
+        launch_iter_mode<GCA_ACCUM_FP64>(ctx, G, pairs, bp, g, sl, sc);
     else
-        hipLaunchKernelGGL(k_icp_iter<GCA_ACCUM_COMPENSATED>, g, b, 0, ctx->stream, pairs, bp, sl, sc);
+        launch_iter_mode<GCA_ACCUM_COMPENSATED>(ctx, G, pairs, bp, g, sl, sc);
 }
 
 static int32_t prepare_target(gca_ctx ctx, gca_cloud tgt, float corr, float grad_r)
@@ -597,6 +774,7 @@ static int32_t prepare_target(gca_ctx ctx, gca_cloud tgt, float corr, float grad
         GCA_TRY(gca_compute_color_gradient(ctx, tgt, grad_r, 0));
     }
     GCA_TRY(grid_build(ctx, tgt, corr));
+    GCA_TRY(grid_gather_attributes(ctx, tgt));
     return GCA_OK;
 }
 
@@ -616,13 +794,17 @@ static int32_t run_batch(gca_ctx ctx, int n, const gca_cloud *srcs, const gca_cl
     GCA_TRY(validate_params(p));
     std::vector<int64_t> ns(n);
     std::vector<int32_t> nblk(n), first(n);
-    int64_t total = 0;
+    int64_t total = 0, max_n = 0;
     for (int k = 0; k < n; ++k) {
         if (!srcs[k] || !tgts[k]) return GCA_ERR_INVALID_VALUE;
         GCA_TRY(prepare_target(ctx, tgts[k], p->max_correspondence_distance, grad_r));
         GCA_TRY(cloud_sync_header(srcs[k]));
         ns[k] = srcs[k]->host_hdr.n;
-        nblk[k] = icp_blocks(ns[k]);
+        max_n = std::max(max_n, ns[k]);
+    }
+    const int G = icp_group_lanes(ctx, max_n);
+    for (int k = 0; k < n; ++k) {
+        nblk[k] = icp_blocks_g(ctx, ns[k], G);
         first[k] = (int32_t)total;
         total += nblk[k];
     }
@@ -630,9 +812,11 @@ static int32_t run_batch(gca_ctx ctx, int n, const gca_cloud *srcs, const gca_cl
     GCA_TRY(icp_work(ctx, n, total, w));
     std::vector<PairDesc> P(n);
     std::vector<int32_t> bp((size_t)total);
+    int64_t slot = 0;
     for (int k = 0; k < n; ++k) {
-        P[k] = make_pair(srcs[k], tgts[k], w.states + k, w.partials + (int64_t)first[k] * kPartStride, nblk[k], first[k],
+        P[k] = make_pair(srcs[k], tgts[k], w.states + k, w.partials + slot * kPartStride, nblk[k], first[k],
                          p->max_correspondence_distance);
+        slot += nblk[k] + P[k].nclust;
         for (int b = 0; b < nblk[k]; ++b) bp[first[k] + b] = k;
     }
     GCA_HIP(hipMemcpyAsync(w.pairs, P.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice, ctx->stream));
@@ -643,10 +827,12 @@ static int32_t run_batch(gca_ctx ctx, int n, const gca_cloud *srcs, const gca_cl
     GCA_HIP(hipGetLastError());
     const float sl = sqrtf(p->lambda_geometric), sc = sqrtf(1.0f - p->lambda_geometric);
     for (int it = 0; it < p->max_iterations; ++it)
-        launch_iter(ctx, p->accumulate_mode, w.pairs, n > 1 ? w.block_pair : nullptr, total, sl, sc);
+        launch_iter(ctx, p->accumulate_mode, G, w.pairs, n > 1 ? w.block_pair : nullptr, total, sl, sc);
     GCA_HIP(hipGetLastError());
     std::vector<IcpState> S(n);
-    GCA_HIP(hipMemcpyAsync(S.data(), w.states, sizeof(IcpState) * n, hipMemcpyDeviceToHost, ctx->stream));
+    // results only (the ticket lines after them stay on the device)
+    GCA_HIP(hipMemcpy2DAsync(S.data(), sizeof(IcpState), w.states, sizeof(IcpState), offsetof(IcpState, ticket), n,
+                             hipMemcpyDeviceToHost, ctx->stream));
     GCA_HIP(hipStreamSynchronize(ctx->stream));
     int32_t status = GCA_OK;
     for (int k = 0; k < n; ++k) {
@@ -711,7 +897,7 @@ int32_t gca_build_gauss_newton(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const 
                            T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11], sl, sc);
     GCA_HIP(hipGetLastError());
     IcpState S;
-    GCA_HIP(hipMemcpyAsync(&S, w.states, sizeof(IcpState), hipMemcpyDeviceToHost, ctx->stream));
+    GCA_HIP(hipMemcpyAsync(&S, w.states, offsetof(IcpState, ticket), hipMemcpyDeviceToHost, ctx->stream));
     GCA_HIP(hipStreamSynchronize(ctx->stream));
     memcpy(out, S.sys, sizeof(double) * kNSys);
     return GCA_OK;

@@ -25,6 +25,7 @@ namespace gca {
 
 constexpr int kNbCap = 256;          // LDS candidates per wave
 constexpr int kWavesPerBlock = 4;
+constexpr int kNormBatch = 16;       // queries gathered per wavefront before the eigen solves
 
 struct NbBuf {
     double d2[kNbCap];
@@ -209,12 +210,20 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_normals(const float4 
                                                                   int32_t K, float4 *__restrict__ nrm)
 {
     __shared__ NbBuf bufs[kWavesPerBlock];
-    NbBuf &buf = bufs[threadIdx.x / kWave];
+    __shared__ long long msum[kWavesPerBlock][kNormBatch][9];
+    __shared__ int32_t mcnt[kWavesPerBlock][kNormBatch];
+    const int w = threadIdx.x / kWave;
+    NbBuf &buf = bufs[w];
     const GridRefs G{g, start, order, sxyzi};
     const int n = hdr->n;
     const double inv_r = 1.0 / (double)g.radius;
     const int nw = gridDim.x * kWavesPerBlock;
-    for (int qi = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; qi < n; qi += nw) {
+    // a wavefront gathers the moments of kNormBatch queries one after the other, then
+    // kNormBatch lanes diagonalise them in parallel
+    for (int q0 = (blockIdx.x * kWavesPerBlock + w) * kNormBatch; q0 < n; q0 += nw * kNormBatch) {
+        const int nq = min(kNormBatch, n - q0);
+        for (int k = 0; k < nq; ++k) {
+        const int qi = q0 + k;
         const float4 q = xyzi[qi];
         long long S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         const int32_t cnt = for_each_neighbour(G, q, K, buf, [&](uint32_t, int32_t, float4 p) {
@@ -232,8 +241,16 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_normals(const float4 
             S[8] += fx((u2 * u2) * kFx40);
         });
 #pragma unroll
-        for (int k = 0; k < 9; ++k) S[k] = wave_sum(S[k]);
-        if (lane_id() == 0) nrm[qi] = normal_from_moments(cnt, S);
+        for (int c = 0; c < 9; ++c) S[c] = wave_sum(S[c]);
+        if (lane_id() == 0) {
+            for (int c = 0; c < 9; ++c) msum[w][k][c] = S[c];
+            mcnt[w][k] = cnt;
+        }
+        }
+        wave_lds_sync();
+        const int l = lane_id();
+        if (l < nq) nrm[q0 + l] = normal_from_moments(mcnt[w][l], msum[w][l]);
+        wave_lds_sync();
     }
 }
 
@@ -324,7 +341,7 @@ int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
 {
     const Grid &G = c->grid;
     StageTimer timer(ctx, GCA_STAGE_NORMALS);
-    hipLaunchKernelGGL(k_normals, dim3(wave_blocks(c->cap)), dim3(kWave * kWavesPerBlock), 0, ctx->stream,
+    hipLaunchKernelGGL(k_normals, dim3(wave_blocks((c->cap + kNormBatch - 1) / kNormBatch)), dim3(kWave * kWavesPerBlock), 0, ctx->stream,
                        c->xyzi, c->hdr, query_desc(G, radius), G.start, G.order, G.sxyzi, K, c->nrm);
     GCA_HIP(hipGetLastError());
     return GCA_OK;

@@ -47,16 +47,56 @@ __device__ __forceinline__ int64_t voxel_key(const VoxelParams &vp, float4 p)
     return (k[0] * vp.dims[1] + k[1]) * vp.dims[2] + k[2];
 }
 
+// Segment id of each lane: the lane index of the first lane of its run of equal keys (runs
+// are maximal sequences of consecutive lanes with the same key).
+__device__ __forceinline__ int run_head(int64_t key)
+{
+    const int lane = lane_id();
+    const int64_t prev = __shfl_up(key, 1, kWave);
+    int head = (lane == 0 || prev != key) ? lane : 0;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int o = __shfl_up(head, d, kWave);
+        if (lane >= d) head = max(head, o);
+    }
+    return head;
+}
+
+// In-run suffix reduction: afterwards the run head holds the run's total.
+template <typename T, typename Op>
+__device__ __forceinline__ T run_reduce(T v, int head, Op op)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T o = __shfl_down(v, d, kWave);
+        const int oh = __shfl_down(head, d, kWave);
+        if (lane + d < kWave && oh == head) v = op(v, o);
+    }
+    return v;
+}
+
+// Raster-ordered points hit the same voxel in runs, so each wavefront first merges runs and
+// only the first lane of a run issues the atomics.
 __global__ __launch_bounds__(256) void k_voxel_mark(const float4 *__restrict__ xyzi,
                                                     const CloudHdr *__restrict__ hdr, VoxelParams vp,
                                                     uint32_t *__restrict__ bitmap,
                                                     int64_t *__restrict__ lin)
 {
     const int n = hdr->n;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int64_t L = voxel_key(vp, xyzi[i]);
-        lin[i] = L;
-        atomicOr(&bitmap[L >> 5], 1u << (uint32_t)(L & 31));
+    const int stride = gridDim.x * blockDim.x;
+    for (int b = blockIdx.x * blockDim.x; b < n; b += stride) {  // block-uniform trip count
+        const int i = b + threadIdx.x;
+        int64_t L = -1 - (int64_t)lane_id();  // unique keys for inactive lanes
+        if (i < n) {
+            L = voxel_key(vp, xyzi[i]);
+            lin[i] = L;
+        }
+        const int64_t word = L >= 0 ? (L >> 5) : L;
+        const int head = run_head(word);
+        uint32_t bits = L >= 0 ? (1u << (uint32_t)(L & 31)) : 0u;
+        bits = run_reduce(bits, head, [](uint32_t a, uint32_t c) { return a | c; });
+        if (L >= 0 && head == lane_id()) atomicOr(&bitmap[word], bits);
     }
 }
 
@@ -69,21 +109,36 @@ __global__ __launch_bounds__(256) void k_voxel_accum(const float4 *__restrict__ 
                                                      unsigned long long *__restrict__ acc)
 {
     const int n = hdr->n;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int64_t L = lin[i];
-        const uint32_t w = bitmap[L >> 5];
-        const uint32_t below = w & ((1u << (uint32_t)(L & 31)) - 1u);
-        const uint32_t r = word_pre[L >> 5] + (uint32_t)__popc(below);
-        unsigned long long *a = acc + 8 * (int64_t)r;
-        const float4 p = xyzi[i], c = rgb[i];
-        atomicAdd(&a[0], (unsigned long long)fx((double)p.x * kFx30));
-        atomicAdd(&a[1], (unsigned long long)fx((double)p.y * kFx30));
-        atomicAdd(&a[2], (unsigned long long)fx((double)p.z * kFx30));
-        atomicAdd(&a[3], (unsigned long long)fx((double)c.x * kFx30));
-        atomicAdd(&a[4], (unsigned long long)fx((double)c.y * kFx30));
-        atomicAdd(&a[5], (unsigned long long)fx((double)c.z * kFx30));
-        atomicAdd(&a[6], 1ull);
-        a[7] = (unsigned long long)L;  // same value from every point of the voxel
+    const int stride = gridDim.x * blockDim.x;
+    auto add = [](unsigned long long a, unsigned long long c) { return a + c; };
+    for (int b = blockIdx.x * blockDim.x; b < n; b += stride) {  // block-uniform trip count
+        const int i = b + threadIdx.x;
+        int64_t r = -1 - (int64_t)lane_id();  // unique keys for inactive lanes
+        int64_t L = 0;
+        unsigned long long v[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (i < n) {
+            L = lin[i];
+            const uint32_t w = bitmap[L >> 5];
+            const uint32_t below = w & ((1u << (uint32_t)(L & 31)) - 1u);
+            r = (int64_t)(word_pre[L >> 5] + (uint32_t)__popc(below));
+            const float4 p = xyzi[i], c = rgb[i];
+            v[0] = (unsigned long long)fx((double)p.x * kFx30);
+            v[1] = (unsigned long long)fx((double)p.y * kFx30);
+            v[2] = (unsigned long long)fx((double)p.z * kFx30);
+            v[3] = (unsigned long long)fx((double)c.x * kFx30);
+            v[4] = (unsigned long long)fx((double)c.y * kFx30);
+            v[5] = (unsigned long long)fx((double)c.z * kFx30);
+            v[6] = 1ull;
+        }
+        const int head = run_head(r);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) v[k] = run_reduce(v[k], head, add);
+        if (r >= 0 && head == lane_id()) {
+            unsigned long long *a = acc + 8 * r;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) atomicAdd(&a[k], v[k]);
+            a[7] = (unsigned long long)L;  // same value from every point of the voxel
+        }
     }
 }
 

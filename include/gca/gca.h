@@ -153,6 +153,20 @@ enum {
     GCA_STAGE_COUNT = 6
 };
 int32_t gca_ctx_profile(gca_ctx ctx, int32_t enable);  /* enable != 0 also clears the records */
+
+/* Launch-shape tuning of a ctx (no reference counterpart: the reference takes whatever shapes
+ * Thrust picks).  Values are chosen by the autotuner (cuda_3d_amd/autotune.py); 0 = built-in
+ * heuristic.  Results do not depend on the tuning beyond the last bits of FP64 reductions. */
+enum {
+    GCA_TUNE_ICP_LANES_SMALL = 0,  /* lanes per correspondence query, clouds < 16k points: 1,2,4,8,16,32,64 */
+    GCA_TUNE_ICP_LANES_MEDIUM = 1, /* ... clouds < 128k points */
+    GCA_TUNE_ICP_LANES_LARGE = 2,  /* ... larger clouds */
+    GCA_TUNE_ICP_MAX_BLOCKS = 3,   /* cap on workgroups per pair and iteration, 1..2048 */
+    GCA_TUNE_NORMALS_LANES = 4,    /* lanes per normal/gradient query: 8, 16, 32, 64 */
+    GCA_TUNE_COUNT = 5
+};
+int32_t gca_ctx_set_tuning(gca_ctx ctx, int32_t key, int32_t value);
+int32_t gca_ctx_get_tuning(gca_ctx ctx, int32_t key, int32_t *value);
 /* Synchronises; launches[s] and total_ms[s] (arrays of GCA_STAGE_COUNT) since the last enable. */
 int32_t gca_ctx_profile_read(gca_ctx ctx, int64_t *launches, double *total_ms);
 

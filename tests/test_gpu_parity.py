@@ -380,8 +380,13 @@ def test_batch_matches_single_pairs(ctx):
         icp.set_source_point_cloud(srcs[k])
         icp.set_target_point_cloud(tgts[k])
         icp.align()
-        np.testing.assert_array_equal(Tb[k], icp.get_final_transformation_matrix())  # same kernel, same bits
-        assert reps[k].iterations == 30
+        # same kernel; the block partition of the FP64 reduction follows the batch's largest
+        # cloud, so only the last bits may differ
+        np.testing.assert_allclose(Tb[k], icp.get_final_transformation_matrix(), rtol=0, atol=1e-12)
+        # a degenerate pair may stop early (failed solve -> identity step -> converged, as in
+        # the reference); batched and single runs must agree on it
+        assert reps[k].iterations == icp.report.iterations
+        assert reps[k].solver_failures == icp.report.solver_failures
 
 
 def test_determinism(frame):
