@@ -1,19 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark: 640x480 depth -> normals -> 3-level colored ICP, ms/frame on one MI355X.
 
-A step is one new frame through FramePipeline (front end with 5x5 bilateral, voxel 0.04 /
-0.02 / 0.01 with normals and colour gradients for each level, then 3-level colored ICP with 30
-Gauss-Newton iterations per level against the previous frame).  The two frames of a seeded
-synthetic scene alternate, so every step registers a real frame pair.  Depth and colour are
-resident in HBM before the timed region.
+A step is one new frame through FramePipeline: front end with 5x5 bilateral, voxel 0.04 / 0.02
+/ 0.01 with normals (r 2.5 v, K 30) and colour gradients (r 2 v) for each level, then 3-level
+colored ICP against the previous frame, at most 30 Gauss-Newton iterations per level with the
+reference's fixed stopping rule (color_icp.hpp:45-47: cos(theta_step) >= 1 - 1e-6 and
+|t_step|^2 <= 1e-6).  The two frames of a seeded synthetic scene alternate, so every step
+registers a real frame pair.  Depth and colour are resident in HBM before the timed region.
 
 The JSON line also carries:
-  roofline      the dominant kernel (one ICP iteration), algorithmic bytes per launch
-                16 M_src + 40 M_tgt (source x,y,z,I; target x,y,z,I + normal + gradient) over
-                its average launch time, from hipEvents on the library's stream in a second,
-                identical timed pass with per-launch event brackets;
+  roofline      the dominant kernel (the persistent ICP launch, one per frame), algorithmic
+                bytes per launch = sum over levels of iterations x (24 M_src + 48 M_tgt)
+                (source x,y,z,I read + previous match read and written; target x,y,z,I, normal
+                and gradient read once) over its average launch time, from hipEvents on the
+                library's stream in a second, identical pass;
   cpu_baseline  the FP64 oracle (C, one core) on one frame of the same workload;
-  batch         64 uneven pairs registered by the batched kernel vs 64 per-pair launches.
+  batch         64 uneven pairs x 3 levels in one persistent launch vs per-pair, per-level
+                plain calls, same stopping rule (poses must agree bit for bit).
 """
 from __future__ import annotations
 
@@ -35,8 +38,8 @@ def log(msg):
 
 
 def read_traffic():
-    """HBM bytes per ICP-iteration launch measured with rocprofv3 --pmc (committed summary)."""
-    path = os.path.join(ROOT, "profiles", "icp_iter_traffic_640x480.md")
+    """HBM bytes per ICP launch measured with rocprofv3 --pmc (committed summary)."""
+    path = os.path.join(ROOT, "profiles", "icp_persist_traffic_640x480.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -57,7 +60,7 @@ def cpu_baseline(frames, cam, levels, iters):
     camo = O.make_camera(cam.width, cam.height, cam.depth_intrin.fx, cam.depth_intrin.fy,
                          cam.depth_intrin.cx, cam.depth_intrin.cy)
     bo = O.Bilateral(1, 2, 2.0, 30.0)
-    lo = [O.LevelParams(v, 2.5 * v, 30, 2 * v, 0, O.icp_params(iters, 2.5 * v, 0.968, 0.0, 0.0)) for v in levels]
+    lo = [O.LevelParams(v, 2.5 * v, 30, 2 * v, 0, O.icp_params(iters, 2.5 * v, 0.968, 1e-6, 1e-6)) for v in levels]
     tx, tr, _, _ = O.cloud_from_rgbd(td, tc, camo, bo, 0.2, 3.5)
     t0 = time.perf_counter()
     sx, sr, _, _ = O.cloud_from_rgbd(sd, sc, camo, bo, 0.2, 3.5)  # the new frame's front end
@@ -95,14 +98,15 @@ def run_batch(g, ctx, n_pairs, levels, iters, seed=2026):
             t_l.append(t2)
         lv.append((s_l, t_l))
     ctx.synchronize()
-    params = [g.icp_params(iters, 2.5 * v, 0.968, 0, 0.0, 0.0) for v in levels]
+    nl = len(levels)
+    params = [g.icp_params(iters, 2.5 * v, 0.968, 0, 1e-6, 1e-6) for v in levels]
+    grads = [2.0 * v for v in levels]
+    S = [[lv[l][0][k] for l in range(nl)] for k in range(n_pairs)]
+    Tg = [[lv[l][1][k] for l in range(nl)] for k in range(n_pairs)]
     I = np.tile(np.eye(4), (n_pairs, 1, 1))
 
     def batched():
-        T = I.copy()
-        for l in range(len(levels)):
-            T, _ = g.batch_icp(lv[l][0], lv[l][1], params[l], T)
-        return T
+        return g.batch_multiscale_icp(S, Tg, params, I, grads)
 
     def plain():
         T = np.zeros((n_pairs, 4, 4))
@@ -112,8 +116,8 @@ def run_batch(g, ctx, n_pairs, levels, iters, seed=2026):
             Tk = np.ascontiguousarray(np.eye(4).reshape(16))
             rep = L.IcpReport()
             for l, v in enumerate(levels):
-                L.check(L.lib.gca_color_icp_align(ctx.handle, lv[l][0][k].handle, lv[l][1][k].handle,
-                                                  C.byref(params[l]), 2.0 * v, Tk.ctypes.data_as(C.c_void_p),
+                L.check(L.lib.gca_color_icp_align(ctx.handle, S[k][l].handle, Tg[k][l].handle,
+                                                  C.byref(params[l]), grads[l], Tk.ctypes.data_as(C.c_void_p),
                                                   Tk.ctypes.data_as(C.c_void_p), C.byref(rep)), "align")
             lat.append(time.perf_counter() - t1)
             T[k] = Tk.reshape(4, 4)
@@ -123,12 +127,13 @@ def run_batch(g, ctx, n_pairs, levels, iters, seed=2026):
     plain()  # warm both paths
     ctx.synchronize()
     reps = 3
-    tb = []
+    tb, blat = [], None
     for _ in range(reps):
         t1 = time.perf_counter()
-        Tb = batched()
+        Tb, breps = batched()
         ctx.synchronize()
         tb.append(time.perf_counter() - t1)
+        blat = [row[-1].latency_us * 1e-3 for row in breps]
     tp = []
     for _ in range(reps):
         t1 = time.perf_counter()
@@ -138,13 +143,18 @@ def run_batch(g, ctx, n_pairs, levels, iters, seed=2026):
     tb_s, tp_s = min(tb), min(tp)
     diff = float(np.abs(Tb - Tp).max())
     lat = np.array(lat) * 1e3
+    sizes = [[S[k][l].points_number() for k in range(n_pairs)] for l in range(nl)]
+    iters_run = [[r.iterations for r in row] for row in breps]
     return {
-        "pairs": n_pairs, "points_per_side": int(counts.sum()),
-        "points_min": int(counts.min()), "points_max": int(counts.max()),
-        "levels": list(levels), "iters_per_level": iters,
+        "pairs": n_pairs, "points_sampled_per_side": int(counts.sum()),
+        "points_sampled_min": int(counts.min()), "points_sampled_max": int(counts.max()),
+        "points_per_level_min": [int(min(x)) for x in sizes], "points_per_level_max": [int(max(x)) for x in sizes],
+        "points_per_level_total": [int(sum(x)) for x in sizes],
+        "levels": list(levels), "max_iters_per_level": iters, "stopping_rule": "reference (1e-6, 1e-6)",
+        "iterations_total": int(sum(map(sum, iters_run))),
         "batched_pairs_per_s": n_pairs / tb_s, "plain_pairs_per_s": n_pairs / tp_s,
         "speedup": tp_s / tb_s, "batched_ms": tb_s * 1e3, "plain_ms": tp_s * 1e3,
-        "batched_latency_ms_p50": tb_s * 1e3, "batched_latency_ms_p99": tb_s * 1e3,
+        "batched_latency_ms_p50": float(np.percentile(blat, 50)), "batched_latency_ms_p99": float(np.percentile(blat, 99)),
         "plain_latency_ms_p50": float(np.percentile(lat, 50)), "plain_latency_ms_p99": float(np.percentile(lat, 99)),
         "max_abs_pose_diff_batched_vs_plain": diff,
     }
@@ -206,29 +216,30 @@ def main():
     full_n = [pipe.full[s].points_number() for s in (0, 1)]
     roof = None
     stages = None
+    iters_seen = []
     if not args.no_roofline_pass:
         ctx.profile(True)
-        pairs_seen = []
+        algo = 0.0
         for _ in range(args.steps):
             new = pipe.slot
             step()
-            pairs_seen.append((new, 1 - new))
+            it = pipe.iterations()
+            iters_seen.append(it)
+            src, tgt = new, 1 - new
+            algo += sum(it[l] * (24.0 * sizes[src][l] + 48.0 * sizes[tgt][l]) for l in range(len(levels)))
         prof = ctx.profile_read()
         ctx.profile(False)
-        n_icp, icp_ms = prof["icp_iter"]
-        algo = 0.0
-        for src, tgt in pairs_seen:
-            for l in range(len(levels)):
-                algo += iters * (16.0 * sizes[src][l] + 40.0 * sizes[tgt][l])
+        n_icp, icp_ms = prof["icp"]
         per_launch_bytes = algo / max(n_icp, 1)
         avg_launch_ms = icp_ms / max(n_icp, 1)
         achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": "k_icp_iter", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+        roof = {"bound": "hbm", "kernel": "k_icp_persist", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
                 "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_us": avg_launch_ms * 1e3,
                 "launches": n_icp}
         stages = {s: {"launches_per_frame": n / args.steps, "ms_per_frame": t / args.steps} for s, (n, t) in prof.items()}
-        log(f"roofline pass: icp_iter {n_icp} launches avg {avg_launch_ms * 1e3:.2f} us, {achieved:.1f} GB/s")
+        log(f"roofline pass: icp {n_icp} launches avg {avg_launch_ms * 1e3:.2f} us, {achieved:.1f} GB/s; "
+            f"iterations per level {iters_seen[-2:]}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -236,7 +247,7 @@ def main():
         cpu_ms, _ = cpu_baseline([(td, tc), (sd, sc)], cam, levels, iters)
         cpu = {"value": cpu_ms, "unit": "ms/frame", "cores": 1, "kind": "port",
                "sample": "one 640x480 frame: front end + voxel/normals/gradient at 3 levels + 3-level colored ICP "
-                         "(30 it/level), FP64 oracle in C on one host core"}
+                         "(max 30 it/level, reference stopping rule), FP64 oracle in C on one host core"}
         log(f"cpu baseline {cpu_ms:.0f} ms/frame")
 
     batch = None
@@ -251,8 +262,10 @@ def main():
         "ms_per_step": ms, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic seeded RGB-D scene (3 planes + 4 spheres, 2 mm noise, 5% holes), frames resident in HBM",
         "config": {"workload": "640x480 depth+RGB -> bilateral 5x5 -> voxel 0.04/0.02/0.01 -> normals r=2.5v K=30 "
-                               "+ colour gradient r=2v -> 3-level colored ICP 30 it/level",
-                   "width": W, "height": H, "levels": list(levels), "iters_per_level": iters,
+                               "+ colour gradient r=2v -> 3-level colored ICP, max 30 it/level, reference "
+                               "stopping rule",
+                   "width": W, "height": H, "levels": list(levels), "max_iters_per_level": iters,
+                   "iterations_per_level": iters_seen[-2:] if iters_seen else None,
                    "points_full": full_n, "points_per_level": sizes},
         "roofline": roof, "cpu_baseline": cpu, "batch": batch, "stages": stages,
     }

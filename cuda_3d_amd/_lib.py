@@ -29,8 +29,9 @@ GCA_OK, GCA_ERR_INVALID_VALUE, GCA_ERR_OUT_OF_MEMORY, GCA_ERR_DEVICE, GCA_ERR_NO
 BILATERAL_NONE, BILATERAL_FIXED, BILATERAL_REFERENCE_ADAPTIVE = range(3)
 ACCUM_COMPENSATED, ACCUM_NAIVE_FP32, ACCUM_FP64 = range(3)
 RESIDUAL_PAIR, RESIDUAL_ABS, RESIDUAL_RMSE = range(3)
-STAGES = ("frontend", "voxel", "grid", "normals", "gradient", "icp_iter")
-TUNE_KEYS = ("icp_lanes_small", "icp_lanes_medium", "icp_lanes_large", "icp_max_blocks", "normals_lanes")
+STAGES = ("frontend", "voxel", "grid", "normals", "gradient", "icp")
+TUNE_KEYS = ("icp_lanes_small", "icp_lanes_medium", "icp_lanes_large", "icp_max_units", "normals_lanes",
+             "icp_persist_blocks", "icp_launch")
 
 
 class GcaError(RuntimeError):
@@ -68,7 +69,7 @@ class IcpParams(C.Structure):
 class IcpReport(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32),
                 ("solver_failures", C.c_int32), ("n_correspondences", C.c_int32),
-                ("rmse", C.c_double), ("fitness", C.c_double)]
+                ("rmse", C.c_double), ("fitness", C.c_double), ("latency_us", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -121,6 +122,7 @@ _SIGS = {
     "gca_color_icp_align": (i32, [vp, vp, vp, P(IcpParams), f32, vp, vp, P(IcpReport)]),
     "gca_color_icp_objective_grad": (i32, [vp, vp, vp, vp, f32, vp, vp, vp, vp, vp]),
     "gca_color_icp_batch": (i32, [vp, i32, vp, vp, P(IcpParams), f32, vp, vp, vp]),
+    "gca_color_icp_batch_multiscale": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
     "gca_multiscale_icp": (i32, [vp, vp, vp, i32, P(LevelParams), vp, vp, vp]),
 }
 for _name, (_res, _args) in _SIGS.items():

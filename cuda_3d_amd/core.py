@@ -496,7 +496,7 @@ def objective_grad(src: PointCloud, tgt: PointCloud, nn, T, xi, lambda_geometric
 
 
 def batch_icp(srcs, tgts, params: L.IcpParams, T0s, gradient_radius=0.0):
-    """gca_color_icp_batch: all pairs in one launch per iteration.  Returns (T [n,4,4], reports)."""
+    """gca_color_icp_batch: all pairs in one persistent launch.  Returns (T [n,4,4], reports)."""
     n = len(srcs)
     ctx = srcs[0].ctx
     hs = (C.c_void_p * n)(*[s.handle.value for s in srcs])
@@ -507,6 +507,26 @@ def batch_icp(srcs, tgts, params: L.IcpParams, T0s, gradient_radius=0.0):
     check(lib.gca_color_icp_batch(ctx.handle, n, hs, ht, C.byref(params), gradient_radius,
                                   _ptr(T0), _ptr(T), reps), "gca_color_icp_batch")
     return T.reshape(n, 4, 4), list(reps)
+
+
+def batch_multiscale_icp(srcs, tgts, params, T0s, gradient_radii):
+    """gca_color_icp_batch_multiscale: srcs[k][l], tgts[k][l] = pair k at level l (coarse to
+    fine); params / gradient_radii one per level.  All pairs and levels run in one persistent
+    launch.  Returns (T [n,4,4], reports [n][levels])."""
+    n, nl = len(srcs), len(params)
+    if any(len(s) != nl for s in srcs) or any(len(t_) != nl for t_ in tgts) or len(tgts) != n:
+        raise ValueError("srcs / tgts must be n_pairs x n_levels")
+    ctx = srcs[0][0].ctx
+    hs = (C.c_void_p * (n * nl))(*[c.handle.value for row in srcs for c in row])
+    ht = (C.c_void_p * (n * nl))(*[c.handle.value for row in tgts for c in row])
+    prm = (L.IcpParams * nl)(*params)
+    gr = (C.c_float * nl)(*[float(r) for r in gradient_radii])
+    T0 = np.ascontiguousarray(np.asarray(T0s, np.float64).reshape(n, 16))
+    T = np.zeros((n, 16))
+    reps = (L.IcpReport * (n * nl))()
+    check(lib.gca_color_icp_batch_multiscale(ctx.handle, n, nl, hs, ht, prm, gr, _ptr(T0), _ptr(T), reps),
+          "gca_color_icp_batch_multiscale")
+    return T.reshape(n, 4, 4), [list(reps[k * nl:(k + 1) * nl]) for k in range(n)]
 
 
 def level_params(voxel, normal_radius=None, normal_max_nn=30, gradient_radius=None,

@@ -24,6 +24,7 @@ int32_t hip_status(hipError_t e)
 
 void *ctx_scratch(gca_ctx ctx, int slot, size_t bytes, bool zeroed)
 {
+    if (slot < 0 || slot >= gca_ctx_s::kScratchSlots) return nullptr;
     if (bytes == 0) bytes = 16;
     if (ctx->scratch_bytes[slot] < bytes) {
         if (ctx->scratch[slot]) {
@@ -381,6 +382,9 @@ int32_t gca_ctx_create(int32_t device, gca_ctx *out)
     gca_ctx c = new (std::nothrow) gca_ctx_s();
     if (!c) return GCA_ERR_OUT_OF_MEMORY;
     c->device = device;
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->n_cu < 1)
+        c->n_cu = 1;
+    if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->wall_khz = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess) {
         delete c;
@@ -397,7 +401,7 @@ int32_t gca_ctx_destroy(gca_ctx c)
     hipStreamSynchronize(c->stream);
     for (int i = 0; i < 16; ++i)
         if (c->level_clouds[i]) gca_cloud_destroy(c->level_clouds[i]);
-    for (int i = 0; i < 16; ++i) hipFree(c->scratch[i]);
+    for (int i = 0; i < gca_ctx_s::kScratchSlots; ++i) hipFree(c->scratch[i]);
     for (hipEvent_t e : c->prof_pool) hipEventDestroy(e);
     hipFree(c->bitmap);
     hipFree(c->lut);
@@ -429,7 +433,9 @@ int32_t gca_ctx_set_tuning(gca_ctx c, int32_t key, int32_t value)
     if (!c || key < 0 || key >= GCA_TUNE_COUNT || value < 0) return GCA_ERR_INVALID_VALUE;
     auto pow2 = [](int32_t v, int32_t hi) { return v == 0 || (v <= hi && (v & (v - 1)) == 0); };
     if (key <= GCA_TUNE_ICP_LANES_LARGE && !pow2(value, 64)) return GCA_ERR_INVALID_VALUE;
-    if (key == GCA_TUNE_ICP_MAX_BLOCKS && value > 2048) return GCA_ERR_INVALID_VALUE;
+    if (key == GCA_TUNE_ICP_MAX_UNITS && value > 1024) return GCA_ERR_INVALID_VALUE;
+    if (key == GCA_TUNE_ICP_PERSIST_BLOCKS && value > 65536) return GCA_ERR_INVALID_VALUE;
+    if (key == GCA_TUNE_ICP_LAUNCH && value > 2) return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_NORMALS_LANES && (!pow2(value, 64) || (value != 0 && value < 8)))
         return GCA_ERR_INVALID_VALUE;
     c->tune[key] = value;

@@ -102,16 +102,16 @@ __device__ __forceinline__ int32_t grid_cell(double p, float o, float s, int32_t
     return (int32_t)c;
 }
 
-// Cell range overlapped by [q - r, q + r]; false if empty or q not finite.
-__device__ __forceinline__ bool grid_range(const GridDesc &g, float qx, float qy, float qz,
-                                           int32_t lo[3], int32_t hi[3])
+// Cell range overlapped by the box [q - r, q + r] (r in FP64); false if empty or q not finite.
+__device__ __forceinline__ bool grid_range_r(const GridDesc &g, float qx, float qy, float qz, double r,
+                                             int32_t lo[3], int32_t hi[3])
 {
     const float q[3] = {qx, qy, qz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (!isfinite(q[a])) return false;
-        double l = floor(((double)q[a] - (double)g.radius - (double)g.origin[a]) / (double)g.cell);
-        double h = floor(((double)q[a] + (double)g.radius - (double)g.origin[a]) / (double)g.cell);
+        double l = floor(((double)q[a] - r - (double)g.origin[a]) / (double)g.cell);
+        double h = floor(((double)q[a] + r - (double)g.origin[a]) / (double)g.cell);
         l = fmax(l, 0.0);
         h = fmin(h, (double)(g.dims[a] - 1));
         if (l > h) return false;
@@ -119,6 +119,13 @@ __device__ __forceinline__ bool grid_range(const GridDesc &g, float qx, float qy
         hi[a] = (int32_t)h;
     }
     return true;
+}
+
+// Cell range overlapped by [q - g.radius, q + g.radius].
+__device__ __forceinline__ bool grid_range(const GridDesc &g, float qx, float qy, float qz,
+                                           int32_t lo[3], int32_t hi[3])
+{
+    return grid_range_r(g, qx, qy, qz, (double)g.radius, lo, hi);
 }
 
 __device__ __forceinline__ double dist2(float px, float py, float pz, float qx, float qy, float qz)

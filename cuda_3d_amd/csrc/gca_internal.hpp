@@ -83,9 +83,12 @@ struct gca_ctx_s {
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     // scratch buffers owned by the ctx, grown on demand
-    void *scratch[16] = {nullptr};
-    size_t scratch_bytes[16] = {0};
-    bool scratch_zeroed[16] = {false};
+    static constexpr int kScratchSlots = 24;
+    void *scratch[kScratchSlots] = {nullptr};
+    size_t scratch_bytes[kScratchSlots] = {0};
+    bool scratch_zeroed[kScratchSlots] = {false};
+    int n_cu = 0;         // compute units of the device (persistent launch sizing)
+    int wall_khz = 0;     // rate of the device wall clock (wall_clock64)
     // persistent bitmap of the voxel downsampler (kept all-zero between calls)
     uint32_t *bitmap = nullptr;
     int64_t bitmap_words = 0;

@@ -1,23 +1,27 @@
-// icp.hip -- colored ICP on the device (K5 + K9 + K10).
+// icp.hip -- colored ICP on the device.
 //
-// Replaces, per iteration of color_icp::align (src/registration/color_icp.cu:151-234):
+// Replaces color_icp::align (src/registration/color_icp.cu:151-234), whose every iteration is
 //   point_cloud::nn_search (grid rebuild + sort + nn_search_functor, cuda_nn_search.cu:545-619)
 //   cuda_build_gauss_newton_color_icp (thrust::transform_reduce of 172-byte tuples,
 //       cuda_color_icp_build_least_square.cu:134-266,431-474) + device->host copy
 //   solve_JTJ_JTr<true> (host Eigen LDLT, eigen_solver.hpp:44-85)
-//   point_cloud::transform (in-place FP32 transform + bounds reduce + device->host copy)
-// with ONE kernel launch per iteration and no host round trip:
-//   - every thread maps its source points by the current pose (FP64 -> FP32), finds the
-//     nearest target point in the cached target grid, and forms the residuals and 6-D
-//     Jacobians in FP32 (reference formulas);
-//   - the 21 + 6 + 1 sums are accumulated per thread with TwoSum-compensated FP32, reduced
-//     across the wavefront and the block with compensated merges, and written as FP64 block
-//     partials;
-//   - the last block to finish (atomic ticket) sums the partials in a fixed order in FP64,
-//     solves the 6x6 system by LDL^T in FP64, applies the Euler increment to the FP64 pose,
-//     evaluates the stopping rule and re-arms the ticket.  Later launches see `done` and exit.
-// The same kernel runs many (source, target) pairs at once: a block -> pair table gives every
-// pair its own range of blocks, its own partials and its own ticket.
+//   point_cloud::transform (in-place FP32 transform + bounds reduce + device->host copy),
+// by ONE persistent launch that runs every iteration of every registration state (a source /
+// target pair at one pyramid level):
+//   - an iteration of a state is split into `nvb` work units (fixed strided sets of source
+//     points).  Workgroups claim units from the state's queue word (epoch << 32 | units claimed)
+//     with one atomic, so any number of resident workgroups serve any number of states and a
+//     workgroup never waits on work that nobody has claimed;
+//   - a unit maps its source points by the current pose (FP64 -> FP32), finds the nearest
+//     target point in the cached target grid (exact; the search box is shrunk by the previous
+//     iteration's match), forms residuals and 6-D Jacobians in FP32 (reference formulas),
+//     accumulates the 28 sums with TwoSum-compensated FP32 and writes one FP64 partial;
+//   - the workgroup that completes the last unit of an epoch (ticket) sums the partials in a
+//     fixed order in FP64, solves the 6x6 system by LDL^T, applies the Euler increment to the
+//     FP64 pose, evaluates the stopping rule and opens the next epoch -- or, when the state is
+//     done, hands its pose to the next level of its chain and opens that.
+// Units are a fixed partition of the points, so the result does not depend on which workgroup
+// ran which unit: the one-launch-per-iteration variant (k_icp_iter) gives bit-identical poses.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,24 +38,32 @@ namespace gca {
 
 constexpr int kIcpBlock = 256;
 constexpr int kNSys = 29;        // 21 JTJ + 6 JTr + r^2 + n_corr
-constexpr int kPartStride = 32;  // doubles per block partial
+constexpr int kPartStride = 32;  // doubles per partial
+constexpr int kMaxUnits = 1024;  // work units (partials) per state and iteration
 
-constexpr int kMaxClusters = 64;  // block partials are reduced in clusters, then clusters
-constexpr int kTicketStride = 64; // uint32 per ticket: one 256-byte line each, so the atomics
-                                  // of different clusters do not serialise on a shared line
+constexpr uint32_t kEpochWait = 0xfffffffeu;  // chained level whose predecessor still runs
+constexpr uint32_t kEpochDone = 0xffffffffu;
+// Idle polls after which a workgroup of the persistent launch gives up and flags an error.  A
+// poll is at least one device-scope memory round trip (~1 us), so this is seconds: it only
+// ends a launch whose states can no longer make progress.
+constexpr uint32_t kSpinLimit = 1u << 22;
 
 struct IcpState {
     double T[16];
     double sys[kNSys];
     double conv_rot, conv_t2;
-    int32_t iter, max_iter, done, converged, solver_failures, pad0;
-    alignas(256) uint32_t ticket;
-    alignas(256) uint32_t cl_ticket[kMaxClusters * kTicketStride];
+    int32_t iter, max_iter, done, converged, solver_failures, n_src;
+    int32_t G, nvb, n_tgt, next;  // lanes per query, work units, target size, next level (-1)
+    long long t_init, t_done;     // device wall clock at set-up and when the state finished
+    alignas(256) unsigned long long word;  // queue word: (epoch << 32) | units claimed
+    alignas(256) uint32_t ticket;          // units completed in the current epoch
+    alignas(256) uint32_t pad_end;
 };
 
 struct PairDesc {
     const float4 *src;
     const CloudHdr *src_hdr;
+    const CloudHdr *tgt_hdr;
     GridDesc g;
     const uint32_t *start;
     const int32_t *order;
@@ -60,13 +72,30 @@ struct PairDesc {
     const float4 *grad;
     const float4 *snrm;  // ... and in grid (slot) order
     const float4 *sgrad;
-    IcpState *st;
-    double *partials;   // nblk block partials, then nclust cluster partials (kPartStride each)
-    int32_t nblk;
-    int32_t first_block;
-    int32_t cluster;    // blocks per cluster
-    int32_t nclust;
+    double *partials;    // one FP64 partial (kPartStride doubles) per work unit
+    int32_t *nnprev;     // matched target slot of each source point in the previous iteration
+    IcpState *st;        // single-pair kernels (Gauss-Newton build, residuals, objective)
+    float sl, sc;        // sqrt(lambda), sqrt(1 - lambda)
+    int32_t nblk;        // single-pair kernels: workgroups
+    int32_t unit_cap;    // cap on work units per iteration
+    int32_t lanes[3];    // lanes per query for n < 16384, n < 131072, larger
 };
+
+// Launch-shape rule shared by host and device: the unit partition fixes the summation order, so
+// it must come out the same wherever it is computed.
+__host__ __device__ inline int icp_lanes_for(const int32_t lanes[3], int64_t n)
+{
+    return n < 16384 ? lanes[0] : (n < 131072 ? lanes[1] : lanes[2]);
+}
+
+__host__ __device__ inline int32_t icp_units_for(int64_t n, int G, int32_t cap)
+{
+    const int64_t groups = kIcpBlock / G;
+    int64_t u = (n + groups - 1) / groups;
+    if (u < 1) u = 1;
+    if (u > cap) u = cap;
+    return (int32_t)u;
+}
 
 // ---- per-correspondence residuals and Jacobians (FP32, reference formulas) ------------------
 struct Terms {
@@ -187,47 +216,25 @@ __device__ __forceinline__ void xform_f32(const double *T, float4 p, float &x, f
     z = (float)(((T[8] * a + T[9] * b) + T[10] * c) + T[11]);
 }
 
-// NN of (qx,qy,qz): sorted slot of argmin (d2, index) within g.radius, or -1.
-__device__ __forceinline__ int32_t nn_slot(const PairDesc &P, float qx, float qy, float qz)
+// Device-scope coherent load of a double (bypasses caches that may hold a stale copy).
+__device__ __forceinline__ double load_coherent(const double *p)
 {
-    int32_t lo[3], hi[3];
-    if (!grid_range(P.g, qx, qy, qz, lo, hi)) return -1;
-    const double r2 = (double)P.g.radius * (double)P.g.radius;
-    double bd = DBL_MAX;
-    int32_t bi = INT_MAX, bs = -1;
-    for (int32_t ix = lo[0]; ix <= hi[0]; ++ix)
-        for (int32_t iy = lo[1]; iy <= hi[1]; ++iy) {
-            const int64_t row = ((int64_t)ix * P.g.dims[1] + iy) * P.g.dims[2];
-            const uint32_t s0 = P.start[row + lo[2]], s1 = P.start[row + hi[2] + 1];
-            for (uint32_t s = s0; s < s1; ++s) {
-                const float4 c = P.sxyzi[s];
-                const double d2 = dist2(c.x, c.y, c.z, qx, qy, qz);
-                if (d2 < r2 && d2 <= bd) {
-                    const int32_t idx = P.order[s];
-                    if (key_less(d2, idx, bd, bi)) {
-                        bd = d2;
-                        bi = idx;
-                        bs = (int32_t)s;
-                    }
-                }
-            }
-        }
-    return bs;
+    const unsigned long long u =
+        __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)u);
 }
 
 // Block reduction of the accumulator into FP64 partial `out[0..27]`, plus the correspondence
 // count into out[28].  Wave: butterfly merges; block: the 4 wave results merged in wave order.
-// LEAD: only lanes that are multiples of LEAD hold data (group leaders), so the butterfly runs
-// over masks LEAD .. 32 only.
-template <int MODE, int LEAD = 1>
-__device__ __forceinline__ void block_partial(Acc<MODE> &A, int ncorr, double *out)
+// Only lanes that are multiples of `lead` (group leaders) hold data, so the butterfly runs over
+// masks 32 .. lead.
+template <int MODE>
+__device__ __forceinline__ void block_partial(Acc<MODE> &A, int ncorr, int lead, double *out)
 {
     __shared__ Acc<MODE> wacc[kIcpBlock / kWave];
     __shared__ int wcnt[kIcpBlock / kWave];
-#pragma unroll
-    for (int m = kWave / 2; m >= LEAD; m >>= 1) A.merge_from_lane(m);
-#pragma unroll
-    for (int m = kWave / 2; m >= LEAD; m >>= 1) ncorr += __shfl_xor(ncorr, m, kWave);
+    for (int m = kWave / 2; m >= lead; m >>= 1) A.merge_from_lane(m);
+    for (int m = kWave / 2; m >= lead; m >>= 1) ncorr += __shfl_xor(ncorr, m, kWave);
     const int w = threadIdx.x / kWave;
     if (lane_id() == 0) {
         wacc[w] = A;
@@ -246,14 +253,16 @@ __device__ __forceinline__ void block_partial(Acc<MODE> &A, int ncorr, double *o
     }
 }
 
-// Last-block: fixed-order FP64 sum of the pair's partials into sys[29] (in LDS).
-__device__ __forceinline__ void sum_partials(const double *partials, int nblk, double *sys)
+// Fixed-order FP64 sum of `n` partials into sys[29] (in LDS): 8 interleaved groups, then the
+// groups in order.  Ends with a barrier.
+__device__ __forceinline__ void sum_partials(const double *partials, int n, double *sys)
 {
     __shared__ double part[8][kPartStride];
     const int v = threadIdx.x % kPartStride, grp = threadIdx.x / kPartStride;  // 8 groups of 32
     if (v < kNSys) {
         double a = 0.0;
-        for (int b = grp; b < nblk; b += 8) a += partials[(int64_t)b * kPartStride + v];
+#pragma unroll 8
+        for (int b = grp; b < n; b += 8) a += partials[(int64_t)b * kPartStride + v];
         part[grp][v] = a;
     }
     __syncthreads();
@@ -265,50 +274,57 @@ __device__ __forceinline__ void sum_partials(const double *partials, int nblk, d
     __syncthreads();
 }
 
-__device__ __forceinline__ void solve_and_update(IcpState *st, const double *sys)
-{
-    double xi[6], dT[16], T[16];
-    const bool ok = solve6(sys, sys + 21, xi);
-    euler_to_T(xi, dT);
-    for (int k = 0; k < 16; ++k) T[k] = st->T[k];
-    matmul4(dT, T, T);
-    for (int k = 0; k < 16; ++k) st->T[k] = T[k];
-    for (int k = 0; k < kNSys; ++k) st->sys[k] = sys[k];
-    if (!ok) st->solver_failures += 1;
-    const int it = st->iter + 1;
-    st->iter = it;
-    // stopping rule of color_icp.cu:216-228 on the increment
-    const double cos_t = 0.5 * (dT[0] + dT[5] + dT[10] - 1.0);
-    const double tt = (dT[3] * dT[3] + dT[7] * dT[7]) + dT[11] * dT[11];
-    if (cos_t >= 1.0 - st->conv_rot && tt <= st->conv_t2) {
-        st->converged = 1;
-        st->done = 1;
-    }
-    if (it >= st->max_iter) st->done = 1;
-}
-
-// Nearest neighbour searched by a group of G lanes (G divides 64): lane g of the group scans
-// candidates s0 + g, s0 + g + G, ... of every run, then the group reduces (d2, index) by
-// shuffles.  All lanes of the group return the same slot.
+// ---- nearest neighbour ----------------------------------------------------------------------
+// Lanes g = 0..G-1 of a group search one query: lane g scans candidates g, g + G, ... of the
+// flattened candidate space, then the group reduces by shuffles.  The result is the sorted slot
+// of argmin (d2, original index) over target points with d2 < r^2 (FP64 d2), or -1.
 //
-// Latency structure: the bounds of all (ix, iy) runs (at most 3 x 3 when cell >= radius) are
-// loaded up front, then the lane walks one flattened candidate space with kUnroll independent
-// loads in flight, so a query costs a few memory round trips instead of one per candidate.
+// Warm start: `sp` is the slot matched in the previous iteration.  Its distance under the
+// current pose bounds the answer, so only cells overlapping the box of half-width sqrt(d2_sp)
+// around the query can hold it.  The slack kWarmRel / kWarmAbs covers FP64 rounding of d2 and
+// of the cell arithmetic, so the box always contains every point with d2 <= d2_sp and the
+// result is exactly that of the full search.  Ties in d2 compare original indices, which are
+// loaded only then.
 constexpr int kMaxRuns = 9;
 constexpr int kUnroll = 4;
+constexpr double kWarmRel = 1e-9, kWarmAbs = 1e-9;
 
-template <int G>
-__device__ __forceinline__ int32_t nn_slot_group(const PairDesc &P, float qx, float qy, float qz, int g)
+__device__ __forceinline__ void nn_consider(const PairDesc &P, double d2, int32_t s, double r2, double &bd,
+                                            int32_t &bs)
 {
-    int32_t lo[3], hi[3];
+    if (!(d2 < r2)) return;
+    if (d2 < bd) {
+        bd = d2;
+        bs = s;
+    } else if (d2 == bd && s != bs && P.order[s] < P.order[bs]) {
+        bs = s;
+    }
+}
+
+__device__ __forceinline__ int32_t nn_group(const PairDesc &P, float qx, float qy, float qz, int g, int G,
+                                            int32_t sp)
+{
+    const double r2 = (double)P.g.radius * (double)P.g.radius;
     double bd = DBL_MAX;
-    int32_t bi = INT_MAX, bs = -1;
-    if (grid_range(P.g, qx, qy, qz, lo, hi)) {
-        const double r2 = (double)P.g.radius * (double)P.g.radius;
+    int32_t bs = -1;
+    double box = (double)P.g.radius;
+    if (sp >= 0) {
+        const float4 c = P.sxyzi[sp];
+        const double d2 = dist2(c.x, c.y, c.z, qx, qy, qz);
+        if (d2 < r2) {
+            bd = d2;
+            bs = sp;
+            box = fmin(box, sqrt(d2) * (1.0 + kWarmRel) + kWarmAbs);
+        }
+    }
+    int32_t lo[3], hi[3];
+    if (grid_range_r(P.g, qx, qy, qz, box, lo, hi)) {
         const int nry = hi[1] - lo[1] + 1;
         const int nruns = (hi[0] - lo[0] + 1) * nry;
-        uint32_t rs[kMaxRuns], cum[kMaxRuns + 1];
         if (nruns <= kMaxRuns) {
+            // bounds of all (ix, iy) runs up front, then one flattened walk with kUnroll
+            // independent loads in flight per lane
+            uint32_t rs[kMaxRuns], cum[kMaxRuns + 1];
 #pragma unroll
             for (int r = 0; r < kMaxRuns; ++r) {
                 uint32_t a = 0, b = 0;
@@ -344,15 +360,7 @@ __device__ __forceinline__ int32_t nn_slot_group(const PairDesc &P, float qx, fl
 #pragma unroll
                 for (int u = 0; u < kUnroll; ++u) {
                     if (sl[u] == 0xffffffffu) continue;
-                    const double d2 = dist2(c[u].x, c[u].y, c[u].z, qx, qy, qz);
-                    if (d2 < r2 && d2 <= bd) {
-                        const int32_t idx = P.order[sl[u]];
-                        if (key_less(d2, idx, bd, bi)) {
-                            bd = d2;
-                            bi = idx;
-                            bs = (int32_t)sl[u];
-                        }
-                    }
+                    nn_consider(P, dist2(c[u].x, c[u].y, c[u].z, qx, qy, qz), (int32_t)sl[u], r2, bd, bs);
                 }
             }
         } else {
@@ -363,99 +371,270 @@ __device__ __forceinline__ int32_t nn_slot_group(const PairDesc &P, float qx, fl
                     const uint32_t s0 = P.start[row + lo[2]], s1 = P.start[row + hi[2] + 1];
                     for (uint32_t s = s0 + g; s < s1; s += G) {
                         const float4 c = P.sxyzi[s];
-                        const double d2 = dist2(c.x, c.y, c.z, qx, qy, qz);
-                        if (d2 < r2 && d2 <= bd) {
-                            const int32_t idx = P.order[s];
-                            if (key_less(d2, idx, bd, bi)) {
-                                bd = d2;
-                                bi = idx;
-                                bs = (int32_t)s;
-                            }
-                        }
+                        nn_consider(P, dist2(c.x, c.y, c.z, qx, qy, qz), (int32_t)s, r2, bd, bs);
                     }
                 }
         }
     }
-#pragma unroll
     for (int m = 1; m < G; m <<= 1) {
         const double od = __shfl_xor(bd, m, kWave);
-        const int32_t oi = __shfl_xor(bi, m, kWave), os = __shfl_xor(bs, m, kWave);
-        if (key_less(od, oi, bd, bi)) {
+        const int32_t os = __shfl_xor(bs, m, kWave);
+        if (od < bd) {
             bd = od;
-            bi = oi;
+            bs = os;
+        } else if (od == bd && os != bs && os >= 0 && P.order[os] < P.order[bs]) {
             bs = os;
         }
     }
     return bs;
 }
 
-template <int MODE, int G>
-__global__ __launch_bounds__(kIcpBlock) void k_icp_iter(const PairDesc *__restrict__ pairs,
-                                                        const int32_t *__restrict__ block_pair,
-                                                        float sl, float sc)
+// ---- one work unit --------------------------------------------------------------------------
+// Unit `vb` of a state covers source points i0 + k, i0 = (vb + j * nvb) * groups, k < groups
+// (one query per group of G lanes; the group leader accumulates).  Leaves the FP64 partial of
+// the unit in sys[0..28] (LDS; entry k written by thread k).
+template <int MODE>
+__device__ __forceinline__ void icp_unit(const PairDesc &P, int n, int G, int nvb, int ntgt, int vb, bool warm,
+                                         const double *T, double *sys)
 {
-    const int pid = block_pair ? block_pair[blockIdx.x] : 0;
-    const PairDesc P = pairs[pid];
-    IcpState *st = P.st;
-    if (st->done) return;  // converged or out of iterations: the rest of the launches are no-ops
-    const int lb = blockIdx.x - P.first_block;
-    double T[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) T[k] = st->T[k];
-    const int n = P.src_hdr->n;
+    const int lg = __builtin_ctz((unsigned)G);
+    const int g = (int)threadIdx.x & (G - 1);
+    const int groups = kIcpBlock >> lg;
     Acc<MODE> A;
     A.zero();
     int ncorr = 0;
-    const int g = threadIdx.x % G;
-    constexpr int kGroups = kIcpBlock / G;
-    // one query per group of G lanes; the group leader accumulates the 28 sums
-    for (int i0 = lb * kGroups; i0 < n; i0 += P.nblk * kGroups) {
-        const int i = i0 + (int)threadIdx.x / G;
+    for (int i0 = vb * groups; i0 < n; i0 += nvb * groups) {
+        const int i = i0 + ((int)threadIdx.x >> lg);
         const bool active = i < n;
         float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        float x = 0.f, y = 0.f, z = 0.f;
+        float x = NAN, y = 0.f, z = 0.f;
+        int32_t sp = -1;
         if (active) {
             p = P.src[i];
+            if (warm) sp = P.nnprev[i];
             xform_f32(T, p, x, y, z);
         }
-        const int32_t s = nn_slot_group<G>(P, active ? x : NAN, y, z, g);
-        if (s >= 0 && g == 0) {
-            Terms t;
-            corr_terms(x, y, z, p.w, P.sxyzi[s], P.snrm[s], P.sgrad[s], sl, sc, t);
-            acc_terms(A, t);
-            ++ncorr;
+        // the warm-start slot is only a bound: any in-range value is safe, stale or not
+        if ((uint32_t)sp >= (uint32_t)ntgt) sp = -1;
+        const int32_t s = nn_group(P, x, y, z, g, G, sp);
+        if (active && g == 0) {
+            P.nnprev[i] = s;
+            if (s >= 0) {
+                Terms t;
+                corr_terms(x, y, z, p.w, P.sxyzi[s], P.snrm[s], P.sgrad[s], P.sl, P.sc, t);
+                acc_terms(A, t);
+                ++ncorr;
+            }
         }
     }
-    __shared__ double sys[kPartStride];
-    __shared__ bool last;
-    double *my = P.partials + (int64_t)lb * kPartStride;
-    block_partial<MODE, G>(A, ncorr, sys);
-    if (threadIdx.x < kNSys) my[threadIdx.x] = sys[threadIdx.x];
-    // two-level deterministic finish: the last block of each cluster sums the cluster's block
-    // partials, the last cluster sums the cluster partials and solves
-    fence_release_agent();
-    __syncthreads();
-    const int cl = lb / P.cluster;
-    const int csize = min(P.cluster, P.nblk - cl * P.cluster);
-    if (threadIdx.x == 0) last = atomicAdd(&st->cl_ticket[cl * kTicketStride], 1u) == (uint32_t)(csize - 1);
-    __syncthreads();
-    if (!last) return;
-    fence_acquire_agent();
-    double *cpart = P.partials + (int64_t)P.nblk * kPartStride;
-    sum_partials(P.partials + (int64_t)cl * P.cluster * kPartStride, csize, sys);
-    if (threadIdx.x < kNSys) cpart[(int64_t)cl * kPartStride + threadIdx.x] = sys[threadIdx.x];
-    if (threadIdx.x == 0) st->cl_ticket[cl * kTicketStride] = 0u;
-    fence_release_agent();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&st->ticket, 1u) == (uint32_t)(P.nclust - 1);
-    __syncthreads();
-    if (!last) return;
-    fence_acquire_agent();
-    sum_partials(cpart, P.nclust, sys);
-    if (threadIdx.x == 0) {
-        solve_and_update(st, sys);
-        st->ticket = 0u;
+    block_partial<MODE>(A, ncorr, G, sys);
+}
+
+// Element t of the Euler increment Rz(xi2) Ry(xi1) Rx(xi0) | xi3..5 (same expressions as
+// euler_to_T).  trig = sin/cos of xi0, xi1, xi2.
+__device__ __forceinline__ double euler_elem(int t, const double *trig, const double *xi)
+{
+    const double sa = trig[0], ca = trig[1], sb = trig[2], cb = trig[3], sc = trig[4], cc = trig[5];
+    switch (t) {
+    case 0: return cc * cb;
+    case 1: return cc * sb * sa - sc * ca;
+    case 2: return cc * sb * ca + sc * sa;
+    case 3: return xi[3];
+    case 4: return sc * cb;
+    case 5: return sc * sb * sa + cc * ca;
+    case 6: return sc * sb * ca - cc * sa;
+    case 7: return xi[4];
+    case 8: return -sb;
+    case 9: return cb * sa;
+    case 10: return cb * ca;
+    case 11: return xi[5];
+    case 15: return 1.0;
+    default: return 0.0;
     }
+}
+
+// Epoch finish of state p (whole workgroup, after the acquire that follows the last ticket):
+// fixed-order FP64 sum of the unit partials, LDL^T solve, Euler increment, T <- dT T, the
+// stopping rule of color_icp.cu:216-228, then the hand-over: the next epoch of this state, or
+// the pose to the next level of the chain.  A failed solve takes the identity step (which also
+// satisfies the stopping rule), as the reference does.
+__device__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double *partials, int nvb, double *sys)
+{
+    IcpState *st = states + p;
+    __shared__ double Tc[16], Tn[16], dT[16], xi[6], trig[6];
+    __shared__ double f_crot, f_ct2;
+    __shared__ int f_iter, f_max, f_fail, f_next, f_ok, f_done;
+    const int t = threadIdx.x;
+    // state fields load alongside the partial sums
+    if (t >= 64 && t < 80) {
+        Tc[t - 64] = st->T[t - 64];
+    } else if (t == 80) {
+        f_iter = st->iter;
+        f_max = st->max_iter;
+        f_fail = st->solver_failures;
+        f_next = st->next;
+        f_crot = st->conv_rot;
+        f_ct2 = st->conv_t2;
+    }
+    sum_partials(partials, nvb, sys);
+    if (t == 0) f_ok = solve6(sys, sys + 21, xi) ? 1 : 0;
+    __syncthreads();
+    if (t < 3) {
+        double s, c;
+        sincos(xi[t], &s, &c);
+        trig[2 * t] = s;
+        trig[2 * t + 1] = c;
+    }
+    __syncthreads();
+    if (t < 16) dT[t] = euler_elem(t, trig, xi);
+    __syncthreads();
+    if (t < 16) {
+        const int i = t >> 2, j = t & 3;
+        const double v =
+            ((dT[4 * i] * Tc[j] + dT[4 * i + 1] * Tc[4 + j]) + dT[4 * i + 2] * Tc[8 + j]) + dT[4 * i + 3] * Tc[12 + j];
+        Tn[t] = v;
+        st->T[t] = v;
+    } else if (t >= 32 && t < 32 + kNSys) {
+        st->sys[t - 32] = sys[t - 32];
+    } else if (t == 64) {
+        const int it = f_iter + 1;
+        const double cos_t = 0.5 * (dT[0] + dT[5] + dT[10] - 1.0);
+        const double tt = (dT[3] * dT[3] + dT[7] * dT[7]) + dT[11] * dT[11];
+        const bool conv = cos_t >= 1.0 - f_crot && tt <= f_ct2;
+        const bool done = conv || it >= f_max;
+        st->iter = it;
+        if (!f_ok) st->solver_failures = f_fail + 1;
+        if (conv) st->converged = 1;
+        if (done) {
+            st->done = 1;
+            st->t_done = wall_clock64();
+        }
+        __hip_atomic_store(&st->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        f_done = done ? 1 : 0;
+    }
+    __syncthreads();
+    if (f_done && f_next >= 0 && t < 16) states[f_next].T[t] = Tn[t];
+    fence_release_agent();
+    __syncthreads();
+    if (t == 0) {
+        if (!f_done) {
+            __hip_atomic_store(&st->word, (unsigned long long)(epoch + 1) << 32, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&st->word, (unsigned long long)kEpochDone << 32, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            if (f_next >= 0)
+                __hip_atomic_store(&states[f_next].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Unit partial -> global, ticket, and the epoch finish by the workgroup completing the epoch.
+template <int MODE>
+__device__ __forceinline__ void unit_commit(const PairDesc &P, IcpState *states, int p, int vb, int nvb,
+                                            uint32_t epoch, double *sys)
+{
+    __shared__ bool last;
+    if (threadIdx.x < kNSys) P.partials[(int64_t)vb * kPartStride + threadIdx.x] = sys[threadIdx.x];
+    fence_release_agent();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&states[p].ticket, 1u) == (uint32_t)(nvb - 1);
+    __syncthreads();
+    if (!last) return;
+    fence_acquire_agent();
+    icp_finish(states, p, epoch, P.partials, nvb, sys);
+}
+
+// ---- persistent launch: every iteration of every state --------------------------------------
+// Scheduling loop of one workgroup: read all queue words (one coherent load per state, in
+// parallel), pick the first state with unclaimed units in round-robin order from the state
+// last served, claim a unit with one atomic, run it.  A workgroup only waits (sleep + re-poll)
+// when every unit of every open epoch is claimed, i.e. held by running workgroups, so progress
+// never depends on how many workgroups are resident.  Exits when every state is done.
+template <int MODE>
+__global__ __launch_bounds__(kIcpBlock) void k_icp_persist(const PairDesc *__restrict__ pairs, IcpState *states,
+                                                           int n_states, uint32_t *err)
+{
+    __shared__ double T[12];
+    __shared__ double sys[kPartStride];
+    __shared__ int s_pick, s_alive, s_vb;
+    __shared__ uint32_t s_epoch;
+    int home = (int)(blockIdx.x % (unsigned)n_states);
+    uint32_t idle = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            s_pick = INT_MAX;
+            s_alive = 0;
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < n_states; p += kIcpBlock) {
+            const unsigned long long w =
+                __hip_atomic_load(&states[p].word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t e = (uint32_t)(w >> 32), c = (uint32_t)w;
+            if (e != kEpochDone) {
+                s_alive = 1;
+                if (e != kEpochWait && c < (uint32_t)states[p].nvb) atomicMin(&s_pick, (p - home + n_states) % n_states);
+            }
+        }
+        __syncthreads();
+        const int pick = s_pick, alive = s_alive;
+        __syncthreads();
+        if (!alive) break;
+        if (pick == INT_MAX) {
+            if (++idle > kSpinLimit) {
+                if (threadIdx.x == 0) atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        const int p = __builtin_amdgcn_readfirstlane((home + pick) % n_states);
+        IcpState *st = states + p;
+        if (threadIdx.x == 0) {
+            const unsigned long long old =
+                __hip_atomic_fetch_add(&st->word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t e = (uint32_t)(old >> 32), c = (uint32_t)old;
+            const bool ok = e < kEpochWait && c < (uint32_t)st->nvb;
+            s_vb = ok ? (int)c : -1;
+            s_epoch = e;
+        }
+        __syncthreads();
+        const int vb = s_vb;
+        const uint32_t epoch = s_epoch;
+        if (vb < 0) continue;  // lost the race for the last unit: poll again
+        // The claim saw epoch e, so the pose of epoch e is published and no later pose can be
+        // written before this unit completes.
+        if (threadIdx.x < 12) T[threadIdx.x] = load_coherent(&st->T[threadIdx.x]);
+        __syncthreads();
+        idle = 0;
+        home = p;
+        const PairDesc &P = pairs[p];
+        const int nvb = st->nvb;
+        icp_unit<MODE>(P, st->n_src, st->G, nvb, st->n_tgt, vb, epoch > 0, T, sys);
+        unit_commit<MODE>(P, states, p, vb, nvb, epoch, sys);
+    }
+}
+
+// ---- one launch per iteration (same units, same finish) -------------------------------------
+// Workgroup b runs unit b - unit_first[s] of state s = unit_state[b]; states that are done exit.
+template <int MODE>
+__global__ __launch_bounds__(kIcpBlock) void k_icp_iter(const PairDesc *__restrict__ pairs, IcpState *states,
+                                                        const int32_t *__restrict__ unit_state,
+                                                        const int32_t *__restrict__ unit_first)
+{
+    __shared__ double T[12];
+    __shared__ double sys[kPartStride];
+    const int p = __builtin_amdgcn_readfirstlane(unit_state[blockIdx.x]);
+    IcpState *st = states + p;
+    if (st->done) return;
+    const int vb = (int)blockIdx.x - unit_first[p];
+    const uint32_t epoch = (uint32_t)st->iter;
+    if (threadIdx.x < 12) T[threadIdx.x] = st->T[threadIdx.x];
+    __syncthreads();
+    const PairDesc &P = pairs[p];
+    const int nvb = st->nvb;
+    icp_unit<MODE>(P, st->n_src, st->G, nvb, st->n_tgt, vb, epoch > 0, T, sys);
+    unit_commit<MODE>(P, states, p, vb, nvb, epoch, sys);
 }
 
 // Gauss-Newton system for given correspondences (no solve): writes sys into st->sys.
@@ -485,7 +664,7 @@ __global__ __launch_bounds__(kIcpBlock) void k_build_gn(PairDesc P, const float4
     }
     __shared__ double sys[kPartStride];
     __shared__ bool last;
-    block_partial(A, ncorr, sys);
+    block_partial<MODE>(A, ncorr, 1, sys);
     if (threadIdx.x < kNSys) P.partials[(int64_t)blockIdx.x * kPartStride + threadIdx.x] = sys[threadIdx.x];
     fence_release_agent();
     __syncthreads();
@@ -632,23 +811,57 @@ __global__ __launch_bounds__(kIcpBlock) void k_objective_grad(PairDesc P, const 
     sum_finish<7>(v, P.partials, P.nblk, &P.st->ticket, result);
 }
 
-__global__ void k_state_init(IcpState *st, int n_states, const double *__restrict__ T0, int max_iter,
-                             double conv_rot, double conv_t2)
+// ---------------------------------------------------------------------------------------------
+// state set-up
+
+struct StateInit {
+    double conv_rot, conv_t2;
+    int32_t max_iter;
+    int32_t next;  // next level of the chain (-1)
+    int32_t open;  // 1 = first level of its chain (runs from the start)
+    int32_t t0;    // row of the initial pose in the T0 table
+};
+
+// Sizes come from the clouds' device headers, so set-up needs no host round trip.
+__global__ void k_state_init(IcpState *states, const PairDesc *__restrict__ pairs, const StateInit *__restrict__ init,
+                             const double *__restrict__ T0, int n, uint32_t *err)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_states) return;
-    IcpState &s = st[p];
-    for (int k = 0; k < 16; ++k) s.T[k] = T0[16 * p + k];
+    if (p == 0) *err = 0u;
+    if (p >= n) return;
+    IcpState &s = states[p];
+    const PairDesc &P = pairs[p];
+    const StateInit &I = init[p];
+    for (int k = 0; k < 16; ++k) s.T[k] = T0[16 * I.t0 + k];
     for (int k = 0; k < kNSys; ++k) s.sys[k] = 0.0;
-    s.conv_rot = conv_rot;
-    s.conv_t2 = conv_t2;
+    s.conv_rot = I.conv_rot;
+    s.conv_t2 = I.conv_t2;
     s.iter = 0;
-    s.max_iter = max_iter;
-    s.done = max_iter <= 0 ? 1 : 0;
+    s.max_iter = I.max_iter;
     s.converged = 0;
     s.solver_failures = 0;
+    s.n_src = P.src_hdr->n;
+    s.n_tgt = P.tgt_hdr->n;
+    s.G = icp_lanes_for(P.lanes, s.n_src);
+    s.nvb = icp_units_for(s.n_src, s.G, P.unit_cap);
+    s.next = I.next;
+    s.done = I.max_iter <= 0 ? 1 : 0;
+    s.t_init = wall_clock64();
+    s.t_done = s.t_init;
+    s.word = I.max_iter <= 0 ? ((unsigned long long)kEpochDone << 32)
+                             : (I.open ? 0ull : ((unsigned long long)kEpochWait << 32));
     s.ticket = 0u;
-    for (int c = 0; c < kMaxClusters; ++c) s.cl_ticket[c * kTicketStride] = 0u;
+}
+
+// State of the single-pair kernels (Gauss-Newton build, residuals, objective).
+__global__ void k_state_simple(IcpState *st, const double *__restrict__ T0)
+{
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 16; ++k) st->T[k] = T0[k];
+    for (int k = 0; k < kNSys; ++k) st->sys[k] = 0.0;
+    st->iter = 0;
+    st->done = 0;
+    st->ticket = 0u;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -656,45 +869,42 @@ __global__ void k_state_init(IcpState *st, int n_states, const double *__restric
 
 static int32_t icp_blocks(int64_t n) { return (int32_t)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kIcpBlock - 1) / kIcpBlock)); }
 
-// Lanes per correspondence query: wide groups shorten each query's dependent load chain,
-// which is what bounds small clouds; narrow groups keep large clouds' wave count reasonable.
-static int icp_group_lanes(gca_ctx ctx, int64_t max_n)
+// Lanes per correspondence query by source size: wide groups shorten each query's dependent
+// load chain, which is what bounds small clouds; narrow groups keep large clouds' wave count
+// reasonable.
+static void icp_shape(gca_ctx ctx, int32_t lanes[3], int32_t *unit_cap)
 {
-    const int key = max_n < 16384 ? GCA_TUNE_ICP_LANES_SMALL
-                                  : (max_n < 131072 ? GCA_TUNE_ICP_LANES_MEDIUM : GCA_TUNE_ICP_LANES_LARGE);
-    if (ctx->tune[key]) return ctx->tune[key];
-    if (max_n < 16384) return 16;
-    if (max_n < 131072) return 8;
-    return 4;
-}
-
-static int32_t icp_blocks_g(gca_ctx ctx, int64_t n, int G)
-{
-    const int64_t groups_per_block = kIcpBlock / G;
-    const int64_t cap = ctx->tune[GCA_TUNE_ICP_MAX_BLOCKS] ? ctx->tune[GCA_TUNE_ICP_MAX_BLOCKS] : 1024;
-    return (int32_t)std::min<int64_t>(cap, std::max<int64_t>(1, (n + groups_per_block - 1) / groups_per_block));
+    static const int32_t def[3] = {8, 4, 4};
+    for (int k = 0; k < 3; ++k) lanes[k] = ctx->tune[GCA_TUNE_ICP_LANES_SMALL + k] ? ctx->tune[GCA_TUNE_ICP_LANES_SMALL + k] : def[k];
+    *unit_cap = std::min(kMaxUnits, ctx->tune[GCA_TUNE_ICP_MAX_UNITS] ? ctx->tune[GCA_TUNE_ICP_MAX_UNITS] : 512);
 }
 
 struct IcpWork {
     IcpState *states = nullptr;
     double *partials = nullptr;
+    int32_t *nnprev = nullptr;
     PairDesc *pairs = nullptr;
-    int32_t *block_pair = nullptr;
-    double *aux = nullptr;  // T0 upload / results
+    StateInit *init = nullptr;
+    double *T0 = nullptr;
+    int32_t *unit_state = nullptr;
+    int32_t *unit_first = nullptr;
+    uint32_t *err = nullptr;
 };
 
-constexpr int kIcpCluster = 32;  // block partials per cluster of the two-level finish
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-static int32_t clusters_of(int32_t nblk) { return (nblk + kIcpCluster - 1) / kIcpCluster; }
-
-// partial slots: every pair's block partials followed by its cluster partials (<= 2 per block)
-static int32_t icp_work(gca_ctx ctx, int n_pairs, int64_t total_blocks, IcpWork &w)
+static int32_t icp_work(gca_ctx ctx, int n_states, int n_t0, int64_t partial_slots, int64_t nnprev_n,
+                        int64_t table_units, IcpWork &w)
 {
-    w.states = (IcpState *)ctx_scratch(ctx, 13, sizeof(IcpState) * n_pairs, false);
-    w.partials = (double *)ctx_scratch(ctx, 14, sizeof(double) * kPartStride * (2 * total_blocks + 1), false);
-    if (!w.states || !w.partials) return GCA_ERR_OUT_OF_MEMORY;
-    const size_t aux_doubles = std::max<size_t>(16 * (size_t)n_pairs, 64);
-    const size_t need = sizeof(PairDesc) * n_pairs + sizeof(int32_t) * total_blocks + sizeof(double) * aux_doubles + 1024;
+    w.states = (IcpState *)ctx_scratch(ctx, 13, sizeof(IcpState) * n_states, false);
+    w.partials = (double *)ctx_scratch(ctx, 14, sizeof(double) * kPartStride * std::max<int64_t>(partial_slots, 1), false);
+    w.nnprev = (int32_t *)ctx_scratch(ctx, 16, sizeof(int32_t) * std::max<int64_t>(nnprev_n, 1), false);
+    if (!w.states || !w.partials || !w.nnprev) return GCA_ERR_OUT_OF_MEMORY;
+    const size_t b_pairs = align256(sizeof(PairDesc) * n_states), b_init = align256(sizeof(StateInit) * n_states),
+                 b_t0 = align256(sizeof(double) * 16 * std::max(n_t0, 4)),
+                 b_us = align256(sizeof(int32_t) * std::max<int64_t>(table_units, 1)),
+                 b_uf = align256(sizeof(int32_t) * n_states), b_err = 256;
+    const size_t need = b_pairs + b_init + b_t0 + b_us + b_uf + b_err;
     if (!ctx->icp_state || ctx->icp_state_bytes < need) {
         GCA_HIP(hipStreamSynchronize(ctx->stream));
         hipFree(ctx->icp_state);
@@ -705,65 +915,40 @@ static int32_t icp_work(gca_ctx ctx, int n_pairs, int64_t total_blocks, IcpWork 
     }
     char *base = (char *)ctx->icp_state;
     w.pairs = (PairDesc *)base;
-    base += ((sizeof(PairDesc) * n_pairs + 255) & ~size_t(255));
-    w.aux = (double *)base;
-    base += ((sizeof(double) * aux_doubles + 255) & ~size_t(255));
-    w.block_pair = (int32_t *)base;
+    base += b_pairs;
+    w.init = (StateInit *)base;
+    base += b_init;
+    w.T0 = (double *)base;
+    base += b_t0;
+    w.unit_state = (int32_t *)base;
+    base += b_us;
+    w.unit_first = (int32_t *)base;
+    base += b_uf;
+    w.err = (uint32_t *)base;
     return GCA_OK;
 }
 
-static PairDesc make_pair(gca_cloud src, gca_cloud tgt, IcpState *st, double *partials, int32_t nblk,
-                          int32_t first, float radius)
+static PairDesc make_pair(gca_cloud src, gca_cloud tgt, double *partials, int32_t *nnprev, float lambda,
+                          float radius)
 {
     PairDesc P{};
     P.src = src->xyzi;
     P.src_hdr = src->hdr;
-    if (tgt) {
-        P.g = tgt->grid.desc;
-        P.g.radius = radius;
-        P.start = tgt->grid.start;
-        P.order = tgt->grid.order;
-        P.sxyzi = tgt->grid.sxyzi;
-        P.nrm = tgt->nrm;
-        P.grad = tgt->grad;
-        P.snrm = tgt->grid.snrm;
-        P.sgrad = tgt->grid.sgrad;
-    }
-    P.st = st;
+    P.tgt_hdr = tgt->hdr;
+    P.g = tgt->grid.desc;
+    P.g.radius = radius;
+    P.start = tgt->grid.start;
+    P.order = tgt->grid.order;
+    P.sxyzi = tgt->grid.sxyzi;
+    P.nrm = tgt->nrm;
+    P.grad = tgt->grad;
+    P.snrm = tgt->grid.snrm;
+    P.sgrad = tgt->grid.sgrad;
     P.partials = partials;
-    P.nblk = nblk;
-    P.first_block = first;
-    P.cluster = kIcpCluster;
-    P.nclust = clusters_of(nblk);
+    P.nnprev = nnprev;
+    P.sl = sqrtf(lambda);
+    P.sc = sqrtf(1.0f - lambda);
     return P;
-}
-
-template <int MODE>
-static void launch_iter_mode(gca_ctx ctx, int G, const PairDesc *pairs, const int32_t *bp, dim3 g, float sl, float sc)
-{
-    dim3 b(kIcpBlock);
-    switch (G) {
-    case 64: hipLaunchKernelGGL((k_icp_iter<MODE, 64>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
-    case 32: hipLaunchKernelGGL((k_icp_iter<MODE, 32>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
-    case 16: hipLaunchKernelGGL((k_icp_iter<MODE, 16>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
-    case 8: hipLaunchKernelGGL((k_icp_iter<MODE, 8>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
-    case 4: hipLaunchKernelGGL((k_icp_iter<MODE, 4>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
-    case 2: hipLaunchKernelGGL((k_icp_iter<MODE, 2>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
-    default: hipLaunchKernelGGL((k_icp_iter<MODE, 1>), g, b, 0, ctx->stream, pairs, bp, sl, sc); break;
-    }
-}
-
-static void launch_iter(gca_ctx ctx, int mode, int G, const PairDesc *pairs, const int32_t *bp, int64_t nblocks,
-                        float sl, float sc)
-{
-    dim3 g((unsigned)nblocks);
-    StageTimer timer(ctx, GCA_STAGE_ICP_ITER);
-    if (mode == GCA_ACCUM_NAIVE_FP32)
-        launch_iter_mode<GCA_ACCUM_NAIVE_FP32>(ctx, G, pairs, bp, g, sl, sc);
-    else if (mode == GCA_ACCUM_FP64)
-        launch_iter_mode<GCA_ACCUM_FP64>(ctx, G, pairs, bp, g, sl, sc);
-    else
-        launch_iter_mode<GCA_ACCUM_COMPENSATED>(ctx, G, pairs, bp, g, sl, sc);
 }
 
 static int32_t prepare_target(gca_ctx ctx, gca_cloud tgt, float corr, float grad_r)
@@ -773,9 +958,7 @@ static int32_t prepare_target(gca_ctx ctx, gca_cloud tgt, float corr, float grad
         if (!(grad_r > 0.0f)) return GCA_ERR_NOT_READY;
         GCA_TRY(gca_compute_color_gradient(ctx, tgt, grad_r, 0));
     }
-    GCA_TRY(grid_build(ctx, tgt, corr));
-    GCA_TRY(grid_gather_attributes(ctx, tgt));
-    return GCA_OK;
+    return grid_build(ctx, tgt, corr);
 }
 
 static int32_t validate_params(const gca_icp_params *p)
@@ -787,83 +970,170 @@ static int32_t validate_params(const gca_icp_params *p)
     return GCA_OK;
 }
 
-// Runs the batched loop.  T0/T_out: n x 16, reports n.
-static int32_t run_batch(gca_ctx ctx, int n, const gca_cloud *srcs, const gca_cloud *tgts, const gca_icp_params *p,
-                         float grad_r, const double *T0, double *T_out, gca_icp_report *reps)
+template <int MODE>
+static void launch_persist(gca_ctx ctx, const IcpWork &w, int n_states, int blocks)
 {
-    GCA_TRY(validate_params(p));
-    std::vector<int64_t> ns(n);
-    std::vector<int32_t> nblk(n), first(n);
-    int64_t total = 0, max_n = 0;
-    for (int k = 0; k < n; ++k) {
-        if (!srcs[k] || !tgts[k]) return GCA_ERR_INVALID_VALUE;
-        GCA_TRY(prepare_target(ctx, tgts[k], p->max_correspondence_distance, grad_r));
-        GCA_TRY(cloud_sync_header(srcs[k]));
-        ns[k] = srcs[k]->host_hdr.n;
-        max_n = std::max(max_n, ns[k]);
-    }
-    const int G = icp_group_lanes(ctx, max_n);
-    for (int k = 0; k < n; ++k) {
-        nblk[k] = icp_blocks_g(ctx, ns[k], G);
-        first[k] = (int32_t)total;
-        total += nblk[k];
-    }
-    IcpWork w;
-    GCA_TRY(icp_work(ctx, n, total, w));
-    std::vector<PairDesc> P(n);
-    std::vector<int32_t> bp((size_t)total);
-    int64_t slot = 0;
-    for (int k = 0; k < n; ++k) {
-        P[k] = make_pair(srcs[k], tgts[k], w.states + k, w.partials + slot * kPartStride, nblk[k], first[k],
-                         p->max_correspondence_distance);
-        slot += nblk[k] + P[k].nclust;
-        for (int b = 0; b < nblk[k]; ++b) bp[first[k] + b] = k;
-    }
-    GCA_HIP(hipMemcpyAsync(w.pairs, P.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice, ctx->stream));
-    GCA_HIP(hipMemcpyAsync(w.block_pair, bp.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice, ctx->stream));
-    GCA_HIP(hipMemcpyAsync(w.aux, T0, sizeof(double) * 16 * n, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_state_init, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, w.states, n, (const double *)w.aux,
-                       p->max_iterations, p->conv_rotation, p->conv_translation_sq);
-    GCA_HIP(hipGetLastError());
-    const float sl = sqrtf(p->lambda_geometric), sc = sqrtf(1.0f - p->lambda_geometric);
-    for (int it = 0; it < p->max_iterations; ++it)
-        launch_iter(ctx, p->accumulate_mode, G, w.pairs, n > 1 ? w.block_pair : nullptr, total, sl, sc);
-    GCA_HIP(hipGetLastError());
-    std::vector<IcpState> S(n);
-    // results only (the ticket lines after them stay on the device)
-    GCA_HIP(hipMemcpy2DAsync(S.data(), sizeof(IcpState), w.states, sizeof(IcpState), offsetof(IcpState, ticket), n,
-                             hipMemcpyDeviceToHost, ctx->stream));
-    GCA_HIP(hipStreamSynchronize(ctx->stream));
-    int32_t status = GCA_OK;
-    for (int k = 0; k < n; ++k) {
-        memcpy(T_out + 16 * k, S[k].T, sizeof(double) * 16);
-        if (reps) {
-            gca_icp_report &r = reps[k];
-            r.iterations = S[k].iter;
-            r.converged = S[k].converged;
-            r.solver_failures = S[k].solver_failures;
-            r.n_correspondences = (int32_t)S[k].sys[28];
-            r.rmse = ns[k] > 0 ? S[k].sys[27] / (double)ns[k] : 0.0;  // color_icp.cu:231
-            r.fitness = ns[k] > 0 ? S[k].sys[28] / (double)ns[k] : 0.0;
-        }
-    }
-    return status;
+    hipLaunchKernelGGL(k_icp_persist<MODE>, dim3((unsigned)blocks), dim3(kIcpBlock), 0, ctx->stream, w.pairs,
+                       w.states, n_states, w.err);
 }
 
-static int32_t single_pair(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const int32_t *nn, IcpWork &w, PairDesc &P,
-                           int64_t &n)
+template <int MODE>
+static void launch_iter(gca_ctx ctx, const IcpWork &w, int64_t off, int64_t units)
+{
+    hipLaunchKernelGGL(k_icp_iter<MODE>, dim3((unsigned)units), dim3(kIcpBlock), 0, ctx->stream, w.pairs, w.states,
+                       w.unit_state + off, w.unit_first);
+}
+
+// Registers n_pairs chains of n_levels states (state k * n_levels + l = pair k at level l,
+// coarse to fine): level l + 1 of a pair starts from the pose level l ends with.  Levels with
+// max_iterations == 0 are skipped.  T0 / T_out: n_pairs x 16; reports: n_states (may be NULL).
+static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud *srcs, const gca_cloud *tgts,
+                          const gca_icp_params *params, const float *grad_r, const double *T0, double *T_out,
+                          gca_icp_report *reps)
+{
+    const int n_states = n_pairs * n_levels;
+    for (int l = 0; l < n_levels; ++l) {
+        GCA_TRY(validate_params(&params[l]));
+        if (params[l].accumulate_mode != params[0].accumulate_mode) return GCA_ERR_INVALID_VALUE;
+    }
+    for (int s = 0; s < n_states; ++s)
+        if (!srcs[s] || !tgts[s] || srcs[s]->ctx != ctx || tgts[s]->ctx != ctx) return GCA_ERR_INVALID_VALUE;
+    // every target's grid first, then the descriptors: a later build for another state of the
+    // same target may replace arrays an earlier one set up
+    for (int s = 0; s < n_states; ++s)
+        GCA_TRY(prepare_target(ctx, tgts[s], params[s % n_levels].max_correspondence_distance, grad_r[s % n_levels]));
+    for (int s = 0; s < n_states; ++s) GCA_TRY(grid_gather_attributes(ctx, tgts[s]));
+
+    int32_t lanes[3], unit_cap;
+    icp_shape(ctx, lanes, &unit_cap);
+    const bool per_iter = ctx->tune[GCA_TUNE_ICP_LAUNCH] == 2;
+    std::vector<int32_t> nvb(n_states, 0);
+    int64_t table_units = 0;
+    if (per_iter) {  // the launch grids need the sizes on the host
+        for (int s = 0; s < n_states; ++s) {
+            GCA_TRY(cloud_sync_header(srcs[s]));
+            const int64_t n = srcs[s]->host_hdr.n;
+            nvb[s] = icp_units_for(n, icp_lanes_for(lanes, n), unit_cap);
+            table_units += nvb[s];
+        }
+    }
+    std::vector<int64_t> nn_off(n_states + 1, 0);
+    for (int s = 0; s < n_states; ++s) nn_off[s + 1] = nn_off[s] + srcs[s]->cap;
+    IcpWork w;
+    GCA_TRY(icp_work(ctx, n_states, n_pairs, (int64_t)n_states * unit_cap, nn_off[n_states], table_units, w));
+
+    std::vector<PairDesc> P(n_states);
+    std::vector<StateInit> I(n_states);
+    std::vector<int32_t> tail(n_pairs, -1);  // last state of each chain
+    for (int k = 0; k < n_pairs; ++k) {
+        int prev = -1;
+        for (int l = 0; l < n_levels; ++l) {
+            const int s = k * n_levels + l;
+            const gca_icp_params &pl = params[l];
+            P[s] = make_pair(srcs[s], tgts[s], w.partials + (int64_t)s * unit_cap * kPartStride, w.nnprev + nn_off[s],
+                             pl.lambda_geometric, pl.max_correspondence_distance);
+            memcpy(P[s].lanes, lanes, sizeof(lanes));
+            P[s].unit_cap = unit_cap;
+            I[s] = StateInit{pl.conv_rotation, pl.conv_translation_sq, pl.max_iterations, -1, 0, k};
+            if (pl.max_iterations > 0) {
+                if (prev < 0) I[s].open = 1;
+                else I[prev].next = s;
+                prev = s;
+            }
+        }
+        tail[k] = prev;
+    }
+    GCA_HIP(hipMemcpyAsync(w.pairs, P.data(), sizeof(PairDesc) * n_states, hipMemcpyHostToDevice, ctx->stream));
+    GCA_HIP(hipMemcpyAsync(w.init, I.data(), sizeof(StateInit) * n_states, hipMemcpyHostToDevice, ctx->stream));
+    GCA_HIP(hipMemcpyAsync(w.T0, T0, sizeof(double) * 16 * n_pairs, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_state_init, dim3((n_states + 63) / 64), dim3(64), 0, ctx->stream, w.states,
+                       (const PairDesc *)w.pairs, (const StateInit *)w.init, (const double *)w.T0, n_states, w.err);
+    GCA_HIP(hipGetLastError());
+
+    const int mode = params[0].accumulate_mode;
+    std::vector<int32_t> us, uf(n_states, 0);
+    if (!per_iter) {
+        const int blocks = ctx->tune[GCA_TUNE_ICP_PERSIST_BLOCKS] ? ctx->tune[GCA_TUNE_ICP_PERSIST_BLOCKS] : 2 * ctx->n_cu;
+        StageTimer timer(ctx, GCA_STAGE_ICP);
+        if (mode == GCA_ACCUM_NAIVE_FP32) launch_persist<GCA_ACCUM_NAIVE_FP32>(ctx, w, n_states, blocks);
+        else if (mode == GCA_ACCUM_FP64) launch_persist<GCA_ACCUM_FP64>(ctx, w, n_states, blocks);
+        else launch_persist<GCA_ACCUM_COMPENSATED>(ctx, w, n_states, blocks);
+        GCA_HIP(hipGetLastError());
+    } else {
+        // unit tables level by level: level l's launches cover the units of its n_pairs states
+        std::vector<int64_t> loff(n_levels + 1, 0);
+        us.reserve((size_t)table_units);
+        for (int l = 0; l < n_levels; ++l) {
+            loff[l] = (int64_t)us.size();
+            for (int k = 0; k < n_pairs; ++k) {
+                const int s = k * n_levels + l;
+                uf[s] = (int32_t)((int64_t)us.size() - loff[l]);
+                for (int b = 0; b < nvb[s]; ++b) us.push_back(s);
+            }
+        }
+        loff[n_levels] = (int64_t)us.size();
+        GCA_HIP(hipMemcpyAsync(w.unit_state, us.data(), sizeof(int32_t) * us.size(), hipMemcpyHostToDevice, ctx->stream));
+        GCA_HIP(hipMemcpyAsync(w.unit_first, uf.data(), sizeof(int32_t) * n_states, hipMemcpyHostToDevice, ctx->stream));
+        for (int l = 0; l < n_levels; ++l) {
+            const int64_t units = loff[l + 1] - loff[l];
+            for (int it = 0; it < params[l].max_iterations && units > 0; ++it) {
+                StageTimer timer(ctx, GCA_STAGE_ICP);
+                if (mode == GCA_ACCUM_NAIVE_FP32) launch_iter<GCA_ACCUM_NAIVE_FP32>(ctx, w, loff[l], units);
+                else if (mode == GCA_ACCUM_FP64) launch_iter<GCA_ACCUM_FP64>(ctx, w, loff[l], units);
+                else launch_iter<GCA_ACCUM_COMPENSATED>(ctx, w, loff[l], units);
+            }
+        }
+        GCA_HIP(hipGetLastError());
+    }
+
+    std::vector<IcpState> S(n_states);
+    uint32_t err = 0;
+    // results only (the queue and ticket lines after them stay on the device)
+    GCA_HIP(hipMemcpy2DAsync(S.data(), sizeof(IcpState), w.states, sizeof(IcpState), offsetof(IcpState, word),
+                             n_states, hipMemcpyDeviceToHost, ctx->stream));
+    GCA_HIP(hipMemcpyAsync(&err, w.err, sizeof(err), hipMemcpyDeviceToHost, ctx->stream));
+    GCA_HIP(hipStreamSynchronize(ctx->stream));
+    if (err) return GCA_ERR_DEVICE;
+    for (int k = 0; k < n_pairs; ++k)
+        memcpy(T_out + 16 * k, tail[k] >= 0 ? S[tail[k]].T : T0 + 16 * k, sizeof(double) * 16);
+    if (reps) {
+        long long t0 = S[0].t_init;
+        for (int s = 1; s < n_states; ++s) t0 = std::min(t0, S[s].t_init);
+        const double ticks_per_us = ctx->wall_khz > 0 ? ctx->wall_khz * 1e-3 : 0.0;
+        for (int s = 0; s < n_states; ++s) {
+            gca_icp_report &r = reps[s];
+            const double n = (double)S[s].n_src;
+            r.latency_us = ticks_per_us > 0 ? (double)(S[s].t_done - t0) / ticks_per_us : 0.0;
+            r.iterations = S[s].iter;
+            r.converged = S[s].converged;
+            r.solver_failures = S[s].solver_failures;
+            r.n_correspondences = (int32_t)S[s].sys[28];
+            r.rmse = n > 0 ? S[s].sys[27] / n : 0.0;  // color_icp.cu:231
+            r.fitness = n > 0 ? S[s].sys[28] / n : 0.0;
+        }
+    }
+    return GCA_OK;
+}
+
+// Set-up of the single-pair kernels over given correspondences.
+static int32_t single_pair(gca_ctx ctx, gca_cloud src, gca_cloud tgt, IcpWork &w, PairDesc &P, int64_t &n)
 {
     GCA_TRY(cloud_sync_header(src));
     n = src->host_hdr.n;
     const int32_t nb = icp_blocks(n);
-    GCA_TRY(icp_work(ctx, 1, nb, w));
-    P = make_pair(src, nullptr, w.states, w.partials, nb, 0, 0.f);
+    GCA_TRY(icp_work(ctx, 1, 1, nb, 1, 0, w));
+    P = PairDesc{};
+    P.src = src->xyzi;
+    P.src_hdr = src->hdr;
+    P.tgt_hdr = tgt->hdr;
     P.nrm = tgt->nrm;
     P.grad = tgt->grad;
-    (void)nn;
+    P.partials = w.partials;
+    P.st = w.states;
+    P.nblk = nb;
     static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-    GCA_HIP(hipMemcpyAsync(w.aux, I, sizeof(I), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, ctx->stream, w.states, 1, (const double *)w.aux, 0, 0.0, 0.0);
+    GCA_HIP(hipMemcpyAsync(w.T0, I, sizeof(I), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_state_simple, dim3(1), dim3(64), 0, ctx->stream, w.states, (const double *)w.T0);
     GCA_HIP(hipGetLastError());
     return GCA_OK;
 }
@@ -871,6 +1141,7 @@ static int32_t single_pair(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const int3
 }  // namespace gca
 
 using namespace gca;
+
 
 extern "C" {
 
@@ -883,7 +1154,7 @@ int32_t gca_build_gauss_newton(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const 
     IcpWork w;
     PairDesc P;
     int64_t n;
-    GCA_TRY(single_pair(ctx, src, tgt, nn_dev, w, P, n));
+    GCA_TRY(single_pair(ctx, src, tgt, w, P, n));
     const float sl = sqrtf(lambda), sc = sqrtf(1.0f - lambda);
     dim3 g((unsigned)P.nblk), b(kIcpBlock);
     if (mode == GCA_ACCUM_NAIVE_FP32)
@@ -897,7 +1168,7 @@ int32_t gca_build_gauss_newton(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const 
                            T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11], sl, sc);
     GCA_HIP(hipGetLastError());
     IcpState S;
-    GCA_HIP(hipMemcpyAsync(&S, w.states, offsetof(IcpState, ticket), hipMemcpyDeviceToHost, ctx->stream));
+    GCA_HIP(hipMemcpyAsync(&S, w.states, offsetof(IcpState, word), hipMemcpyDeviceToHost, ctx->stream));
     GCA_HIP(hipStreamSynchronize(ctx->stream));
     memcpy(out, S.sys, sizeof(double) * kNSys);
     return GCA_OK;
@@ -912,12 +1183,12 @@ int32_t gca_compute_residuals(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const i
     IcpWork w;
     PairDesc P;
     int64_t n;
-    GCA_TRY(single_pair(ctx, src, tgt, nn_dev, w, P, n));
-    GCA_HIP(hipMemcpyAsync(w.aux + 16, T, sizeof(double) * 16, hipMemcpyHostToDevice, ctx->stream));
-    double *res = w.aux + 32;
+    GCA_TRY(single_pair(ctx, src, tgt, w, P, n));
+    GCA_HIP(hipMemcpyAsync(w.T0 + 16, T, sizeof(double) * 16, hipMemcpyHostToDevice, ctx->stream));
+    double *res = w.T0 + 32;
     const float sl = sqrtf(lambda), sc = sqrtf(1.0f - lambda);
     hipLaunchKernelGGL(k_residuals, dim3((unsigned)P.nblk), dim3(kIcpBlock), 0, ctx->stream, P, tgt->xyzi, nn_dev,
-                       (const double *)(w.aux + 16), lambda, sl, sc, kind, out_dev, res);
+                       (const double *)(w.T0 + 16), lambda, sl, sc, kind, out_dev, res);
     GCA_HIP(hipGetLastError());
     double s = 0.0;
     GCA_HIP(hipMemcpyAsync(&s, res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -980,18 +1251,29 @@ int32_t gca_solve_gauss_newton(const double jtj[21], const double jtr[6], double
 int32_t gca_color_icp_align(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const gca_icp_params *p, float grad_r,
                             const double T0[16], double T_out[16], gca_icp_report *rep)
 {
-    if (!ctx || !src || !tgt || !T_out) return GCA_ERR_INVALID_VALUE;
+    if (!ctx || !src || !tgt || !T_out || !p) return GCA_ERR_INVALID_VALUE;
     const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-    return run_batch(ctx, 1, &src, &tgt, p, grad_r, T0 ? T0 : I, T_out, rep);
+    return run_states(ctx, 1, 1, &src, &tgt, p, &grad_r, T0 ? T0 : I, T_out, rep);
 }
 
 int32_t gca_color_icp_batch(gca_ctx ctx, int32_t n_pairs, const gca_cloud *srcs, const gca_cloud *tgts,
                             const gca_icp_params *p, float grad_r, const double *T0, double *T_out,
                             gca_icp_report *reps)
 {
-    if (!ctx || n_pairs < 0 || (n_pairs > 0 && (!srcs || !tgts || !T0 || !T_out))) return GCA_ERR_INVALID_VALUE;
+    if (!ctx || !p || n_pairs < 0 || (n_pairs > 0 && (!srcs || !tgts || !T0 || !T_out))) return GCA_ERR_INVALID_VALUE;
     if (n_pairs == 0) return GCA_OK;
-    return run_batch(ctx, n_pairs, srcs, tgts, p, grad_r, T0, T_out, reps);
+    return run_states(ctx, n_pairs, 1, srcs, tgts, p, &grad_r, T0, T_out, reps);
+}
+
+int32_t gca_color_icp_batch_multiscale(gca_ctx ctx, int32_t n_pairs, int32_t n_levels, const gca_cloud *srcs,
+                                       const gca_cloud *tgts, const gca_icp_params *params,
+                                       const float *gradient_radius, const double *T0, double *T_out,
+                                       gca_icp_report *reports)
+{
+    if (!ctx || n_pairs < 0 || n_levels < 1 || n_levels > 16 || !params || !gradient_radius) return GCA_ERR_INVALID_VALUE;
+    if (n_pairs > 0 && (!srcs || !tgts || !T0 || !T_out)) return GCA_ERR_INVALID_VALUE;
+    if (n_pairs == 0) return GCA_OK;
+    return run_states(ctx, n_pairs, n_levels, srcs, tgts, params, gradient_radius, T0, T_out, reports);
 }
 
 int32_t gca_multiscale_icp(gca_ctx ctx, gca_cloud src_full, gca_cloud tgt_full, int32_t n_levels,
@@ -1000,22 +1282,31 @@ int32_t gca_multiscale_icp(gca_ctx ctx, gca_cloud src_full, gca_cloud tgt_full, 
 {
     if (!ctx || !src_full || !tgt_full || !levels || !T_out || n_levels < 1 || n_levels > 8)
         return GCA_ERR_INVALID_VALUE;
-    double T[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-    if (T0) memcpy(T, T0, sizeof(T));
+    const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    gca_cloud cs[8], ct[8];
+    gca_icp_params prm[8];
+    float gr[8];
     for (int l = 0; l < n_levels; ++l) {
         const gca_level_params &L = levels[l];
-        gca_cloud *cs = &ctx->level_clouds[2 * l], *ct = &ctx->level_clouds[2 * l + 1];
-        if (!*cs) GCA_TRY(gca_cloud_create(ctx, 1, cs));
-        if (!*ct) GCA_TRY(gca_cloud_create(ctx, 1, ct));
-        GCA_TRY(gca_voxel_downsample(ctx, src_full, L.voxel_size, *cs));
-        GCA_TRY(gca_voxel_downsample(ctx, tgt_full, L.voxel_size, *ct));
-        GCA_TRY(gca_estimate_normals(ctx, *ct, L.normal_radius, L.normal_max_nn));
-        GCA_TRY(gca_compute_color_gradient(ctx, *ct, L.gradient_radius, L.gradient_max_nn));
-        gca_icp_report rep{};
-        GCA_TRY(gca_color_icp_align(ctx, *cs, *ct, &L.icp, L.gradient_radius, T, T, &rep));
-        if (reports) reports[l] = rep;
+        GCA_TRY(validate_params(&L.icp));
+        gca_cloud *a = &ctx->level_clouds[2 * l], *b = &ctx->level_clouds[2 * l + 1];
+        if (!*a) GCA_TRY(gca_cloud_create(ctx, 1, a));
+        if (!*b) GCA_TRY(gca_cloud_create(ctx, 1, b));
+        GCA_TRY(gca_voxel_downsample(ctx, src_full, L.voxel_size, *a));
+        GCA_TRY(gca_voxel_downsample(ctx, tgt_full, L.voxel_size, *b));
+        GCA_TRY(gca_estimate_normals(ctx, *b, L.normal_radius, L.normal_max_nn));
+        GCA_TRY(gca_compute_color_gradient(ctx, *b, L.gradient_radius, L.gradient_max_nn));
+        cs[l] = *a;
+        ct[l] = *b;
+        prm[l] = L.icp;
+        gr[l] = L.gradient_radius;
     }
-    memcpy(T_out, T, sizeof(T));
+    // all levels in one launch, each starting from the pose the previous one ends with
+    std::vector<gca_icp_report> rep(n_levels);
+    std::vector<gca_cloud> S(cs, cs + n_levels), Tg(ct, ct + n_levels);
+    GCA_TRY(run_states(ctx, 1, n_levels, S.data(), Tg.data(), prm, gr, T0 ? T0 : I, T_out, rep.data()));
+    if (reports)
+        for (int l = 0; l < n_levels; ++l) reports[l] = rep[l];
     return GCA_OK;
 }
 
@@ -1029,7 +1320,7 @@ int32_t gca_color_icp_objective_grad(gca_ctx ctx, gca_cloud src, gca_cloud tgt, 
     IcpWork w;
     PairDesc P;
     int64_t n;
-    GCA_TRY(single_pair(ctx, src, tgt, nn_dev, w, P, n));
+    GCA_TRY(single_pair(ctx, src, tgt, w, P, n));
     double prm[59];
     memcpy(prm, T, sizeof(double) * 16);
     const double ca = cos(xi[0]), sa = sin(xi[0]), cb = cos(xi[1]), sb = sin(xi[1]), cc = cos(xi[2]), sc = sin(xi[2]);

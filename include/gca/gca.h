@@ -115,6 +115,8 @@ typedef struct gca_icp_report {
     int32_t n_correspondences; /* correspondences in the last iteration */
     double rmse;             /* reference "RMSE": sum r^2 / N_src of the last build (color_icp.cu:231) */
     double fitness;          /* n_correspondences / N_src */
+    double latency_us;       /* device clock from the call's set-up until this registration
+                                finished (per-pair latency inside a batched launch) */
 } gca_icp_report;
 
 /* Per-level schedule of the multi-scale driver (coarse to fine). */
@@ -149,7 +151,7 @@ enum {
     GCA_STAGE_GRID = 2,       /* count + scan + scatter */
     GCA_STAGE_NORMALS = 3,    /* one kernel */
     GCA_STAGE_GRADIENT = 4,   /* one kernel */
-    GCA_STAGE_ICP_ITER = 5,   /* one kernel per Gauss-Newton iteration */
+    GCA_STAGE_ICP = 5,        /* ICP solve: one persistent launch per call (or one per iteration) */
     GCA_STAGE_COUNT = 6
 };
 int32_t gca_ctx_profile(gca_ctx ctx, int32_t enable);  /* enable != 0 also clears the records */
@@ -161,9 +163,11 @@ enum {
     GCA_TUNE_ICP_LANES_SMALL = 0,  /* lanes per correspondence query, clouds < 16k points: 1,2,4,8,16,32,64 */
     GCA_TUNE_ICP_LANES_MEDIUM = 1, /* ... clouds < 128k points */
     GCA_TUNE_ICP_LANES_LARGE = 2,  /* ... larger clouds */
-    GCA_TUNE_ICP_MAX_BLOCKS = 3,   /* cap on workgroups per pair and iteration, 1..2048 */
+    GCA_TUNE_ICP_MAX_UNITS = 3,    /* cap on work units (FP64 partials) per pair and iteration, 1..1024 */
     GCA_TUNE_NORMALS_LANES = 4,    /* lanes per normal/gradient query: 8, 16, 32, 64 */
-    GCA_TUNE_COUNT = 5
+    GCA_TUNE_ICP_PERSIST_BLOCKS = 5, /* workgroups of the persistent ICP launch (0 = 2 per CU), 1..65536 */
+    GCA_TUNE_ICP_LAUNCH = 6,       /* 0/1 = one persistent launch per call, 2 = one launch per iteration */
+    GCA_TUNE_COUNT = 7
 };
 int32_t gca_ctx_set_tuning(gca_ctx ctx, int32_t key, int32_t value);
 int32_t gca_ctx_get_tuning(gca_ctx ctx, int32_t key, int32_t *value);
@@ -323,12 +327,24 @@ int32_t gca_color_icp_objective_grad(gca_ctx ctx, gca_cloud src, gca_cloud tgt,
 
 /* Batched colored ICP: n_pairs independent (src, tgt) pairs registered together, each with
  * its own T0 (n_pairs x 16) and result (n_pairs x 16).  Every iteration of every pair runs in
- * one launch with per-pair work ranges.  Reference: color_icp::align called once per pair
+ * one persistent launch that serves the pairs' work queues.  Reference: color_icp::align called once per pair
  * (src/registration/color_icp.cu:151-234). */
 int32_t gca_color_icp_batch(gca_ctx ctx, int32_t n_pairs, const gca_cloud *srcs,
                             const gca_cloud *tgts, const gca_icp_params *p,
                             float gradient_radius, const double *T0, double *T_out,
                             gca_icp_report *reports);
+
+/* Batched coarse-to-fine colored ICP: n_pairs chains of n_levels registrations, all in one
+ * persistent launch.  srcs/tgts: n_pairs * n_levels clouds, pair k level l at k * n_levels + l
+ * (coarse to fine); params and gradient_radius: one per level (accumulate_mode equal across
+ * levels).  Level l + 1 of a pair starts from the pose level l ends with; levels with
+ * max_iterations == 0 are skipped.  T0 / T_out: n_pairs x 16; reports: n_pairs * n_levels (may
+ * be NULL).  Reference: color_icp::align per pair and level, as sequenced by
+ * apps/test_color_icp.cpp:378-414. */
+int32_t gca_color_icp_batch_multiscale(gca_ctx ctx, int32_t n_pairs, int32_t n_levels,
+                                       const gca_cloud *srcs, const gca_cloud *tgts,
+                                       const gca_icp_params *params, const float *gradient_radius,
+                                       const double *T0, double *T_out, gca_icp_report *reports);
 
 /* Multi-scale (coarse-to-fine) colored ICP between two full-resolution clouds: per level,
  * voxel downsample both, normals + colour gradient of the target, then align, carrying the

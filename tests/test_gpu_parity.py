@@ -359,11 +359,13 @@ def test_accumulation_modes_pose_error(frame):
     assert max(errs[0]) <= 1e-5 and max(errs[2]) <= 1e-5
 
 
-def test_batch_matches_single_pairs(ctx):
+@pytest.fixture(scope="module")
+def uneven_pairs(ctx):
+    """Four surface-sample pairs of uneven sizes (one degenerate), targets prepared."""
     import cuda_3d_amd as g
     from cuda_3d_amd import scene
 
-    srcs, tgts, T0s, singles = [], [], [], []
+    srcs, tgts = [], []
     for k, n in enumerate((3000, 20000, 7000, 500)):
         sx, sr, tx, tr, G = scene.sample_surface_pair(100 + k, n)
         s = g.PointCloud.from_numpy(sx, sr, ctx=ctx)
@@ -372,21 +374,135 @@ def test_batch_matches_single_pairs(ctx):
         t.compute_color_gradient(0.04, 0)
         srcs.append(s)
         tgts.append(t)
-        T0s.append(np.eye(4))
-    p = icp_p(g, 30, 0.05)
-    Tb, reps = g.batch_icp(srcs, tgts, p, T0s)
+    return srcs, tgts
+
+
+def rep_key(r):
+    """Report fields that are results (latency_us is timing)."""
+    d = r.as_dict()
+    d.pop("latency_us")
+    return d
+
+
+def single_align(src, tgt, iters=30, T0=None):
+    import cuda_3d_amd as g
+
+    icp = g.ColorICP(iters, 0.05, 0.04, conv_rotation=0.0, conv_translation_sq=0.0)
+    icp.set_source_point_cloud(src)
+    icp.set_target_point_cloud(tgt)
+    assert icp.align(T0)
+    return icp.get_final_transformation_matrix(), icp.report
+
+
+def test_batch_matches_single_pairs(uneven_pairs):
+    import cuda_3d_amd as g
+
+    srcs, tgts = uneven_pairs
+    Tb, reps = g.batch_icp(srcs, tgts, icp_p(g, 30, 0.05), [np.eye(4)] * len(srcs))
     for k in range(len(srcs)):
-        icp = g.ColorICP(30, 0.05, 0.04, conv_rotation=0.0, conv_translation_sq=0.0)
-        icp.set_source_point_cloud(srcs[k])
-        icp.set_target_point_cloud(tgts[k])
-        icp.align()
-        # same kernel; the block partition of the FP64 reduction follows the batch's largest
-        # cloud, so only the last bits may differ
-        np.testing.assert_allclose(Tb[k], icp.get_final_transformation_matrix(), rtol=0, atol=1e-12)
+        T, rep = single_align(srcs[k], tgts[k])
+        # each pair's work partition depends on its own size only: bit-identical
+        np.testing.assert_array_equal(Tb[k], T)
         # a degenerate pair may stop early (failed solve -> identity step -> converged, as in
         # the reference); batched and single runs must agree on it
-        assert reps[k].iterations == icp.report.iterations
-        assert reps[k].solver_failures == icp.report.solver_failures
+        assert (reps[k].iterations, reps[k].solver_failures, reps[k].n_correspondences) == \
+               (rep.iterations, rep.solver_failures, rep.n_correspondences)
+        assert reps[k].rmse == rep.rmse
+
+
+def test_launch_strategies_identical(ctx, uneven_pairs):
+    """One persistent launch and one launch per iteration run the same units and finish."""
+    import cuda_3d_amd as g
+
+    srcs, tgts = uneven_pairs
+    p = icp_p(g, 12, 0.05)
+    out = {}
+    for launch in (1, 2):
+        ctx.set_tuning(icp_launch=launch)
+        try:
+            out[launch] = g.batch_icp(srcs, tgts, p, [np.eye(4)] * len(srcs))
+        finally:
+            ctx.set_tuning(icp_launch=0)
+    np.testing.assert_array_equal(out[1][0], out[2][0])
+    assert [rep_key(r) for r in out[1][1]] == [rep_key(r) for r in out[2][1]]
+
+
+def test_batch_multiscale_matches_sequential(ctx, frame):
+    """Chained levels in one launch == level-by-level calls carrying the pose on the host."""
+    import cuda_3d_amd as g
+
+    vs = (0.04, 0.02, 0.01)
+    S, Tg = [], []
+    for v in vs:
+        s = frame.src.voxel_grid_down_sample(v)
+        t = frame.tgt.voxel_grid_down_sample(v)
+        t.estimate_normals(2.5 * v, 30)
+        t.compute_color_gradient(2 * v, 0)
+        S.append(s)
+        Tg.append(t)
+    params = [icp_p(g, 30, 2.5 * v, conv=(1e-6, 1e-6)) for v in vs]
+    for launch in (1, 2):
+        ctx.set_tuning(icp_launch=launch)
+        try:
+            Tb, reps = g.batch_multiscale_icp([S, S], [Tg, Tg], params, [np.eye(4), frame.G], [2 * v for v in vs])
+        finally:
+            ctx.set_tuning(icp_launch=0)
+        for k, T0 in enumerate((np.eye(4), frame.G)):
+            T = T0
+            for l, v in enumerate(vs):
+                icp = g.ColorICP(30, 2.5 * v, 2 * v)  # reference stopping rule
+                icp.set_source_point_cloud(S[l])
+                icp.set_target_point_cloud(Tg[l])
+                assert icp.align(T)
+                T = icp.get_final_transformation_matrix()
+                assert rep_key(reps[k][l]) == rep_key(icp.report)
+            np.testing.assert_array_equal(Tb[k], T)
+        print(f"[gpu-parity] chained levels (launch {launch}) iterations "
+              f"{[[r.iterations for r in row] for row in reps]}", flush=True)
+
+
+def test_warm_start_matches_full_search(level, gn_case):
+    """Iteration k+1 of the device loop (nearest neighbours searched in a box shrunk by the
+    previous match) against an independent full search + Gauss-Newton build + host solve at the
+    pose iteration k ends with: same correspondences, hence the same system and step."""
+    import cuda_3d_amd as g
+
+    src, tgt = gn_case["src"], level["gpu"]
+    ctx = src.ctx
+    P = src.download()[0].astype(np.float64)
+    for k in (1, 2, 5):
+        Tk, _ = single_align(src, tgt, k)
+        Tk1, rep1 = single_align(src, tgt, k + 1)
+        # the source mapped exactly as the device maps it: FP64 affine map, rounded once to FP32
+        m = np.stack([((Tk[a, 0] * P[:, 0] + Tk[a, 1] * P[:, 1]) + Tk[a, 2] * P[:, 2]) + Tk[a, 3]
+                      for a in range(3)], 1).astype(np.float32)
+        moved = g.PointCloud.from_numpy(m, np.zeros_like(m), ctx=ctx)
+        nn = g.PointCloud.nn_search(moved, tgt, 0.05)
+        JTJ, JTr, r2, nc = g.build_gauss_newton(src, tgt, nn, LAM, Tk, 0)
+        assert nc == rep1.n_correspondences
+        assert abs(r2 / src.points_number() - rep1.rmse) <= 1e-12 * rep1.rmse
+        _, _, dT = g.solve_gauss_newton(JTJ, JTr)
+        np.testing.assert_allclose(dT @ Tk, Tk1, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("shape", [dict(icp_lanes_small=1, icp_lanes_medium=1, icp_lanes_large=1),
+                                   dict(icp_lanes_small=64, icp_lanes_medium=32, icp_lanes_large=16),
+                                   dict(icp_max_units=1), dict(icp_max_units=7, icp_persist_blocks=3)])
+def test_shape_tuning_invariance(ctx, uneven_pairs, shape):
+    """Launch shapes change only the FP64 summation order of the partials."""
+    import cuda_3d_amd as g
+
+    srcs, tgts = uneven_pairs
+    p = icp_p(g, 10, 0.05)
+    Tref, rref = g.batch_icp(srcs, tgts, p, [np.eye(4)] * len(srcs))
+    ctx.set_tuning(**shape)
+    try:
+        T, r = g.batch_icp(srcs, tgts, p, [np.eye(4)] * len(srcs))
+    finally:
+        ctx.set_tuning(**{k: 0 for k in shape})
+    np.testing.assert_allclose(T, Tref, rtol=0, atol=1e-10)
+    assert [x.n_correspondences for x in r] == [x.n_correspondences for x in rref]
+    assert [x.iterations for x in r] == [x.iterations for x in rref]
 
 
 def test_determinism(frame):
