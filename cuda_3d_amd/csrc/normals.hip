@@ -1,17 +1,18 @@
-// normals.hip -- PCA normals (K7) and colour gradient (K8), one wavefront per query point.
+// normals.hip -- PCA normals and colour gradient: neighbourhood moments, then per-point solves.
 //
 // Replaces cuda_estimate_normals (src/geometry/cuda_estimate_normals.cu:22-199) and
 // cuda_compute_color_gradient (src/registration/cuda_compute_color_gradient.cu:13-127).
 //
 // The reference materialises every (query, neighbour) pair, reduces 36-byte raw-moment
 // records by key in FP32 and solves the 3x3 eigenproblem with an FP32 trigonometric closed
-// form.  Here a wavefront walks the query's grid runs (64 candidates per step, coalesced
-// float4 reads), and the neighbourhood sums are accumulated in 2^-40 fixed point scaled by the
-// radius, centred on the query point: integer adds are exact, so the wave reduction order does
-// not matter and the result equals the FP64 oracle's.  The covariance is diagonalised with a
+// form.  Here a group of G lanes walks the query's grid runs (all run bounds loaded first, then
+// one flattened candidate walk with several loads in flight per lane), and the neighbourhood
+// sums are accumulated in 2^-40 fixed point scaled by the radius, centred on the query point:
+// integer adds are exact, so neither G nor the reduction order changes the result, which equals
+// the FP64 oracle's.  A query with more than K > 0 points within the radius is finished by one
+// wavefront that buffers the in-radius candidates in LDS and keeps a candidate iff fewer than K
+// others precede it in (d2, index) order.  The covariance is then diagonalised per point with a
 // cyclic Jacobi iteration in FP64 (only +, -, *, /, sqrt: identical on host and device).
-// With a K-nearest cap, in-radius candidates are buffered in LDS and a candidate is kept iff
-// fewer than K others precede it in (d2, index) order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,7 +26,6 @@ namespace gca {
 
 constexpr int kNbCap = 256;          // LDS candidates per wave
 constexpr int kWavesPerBlock = 4;
-constexpr int kNormBatch = 16;       // queries gathered per wavefront before the eigen solves
 
 struct NbBuf {
     double d2[kNbCap];
@@ -202,159 +202,311 @@ __host__ __device__ inline float4 normal_from_moments(int32_t cnt, const long lo
     return out;
 }
 
-__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_normals(const float4 *__restrict__ xyzi,
-                                                                  const CloudHdr *__restrict__ hdr,
-                                                                  GridDesc g, const uint32_t *__restrict__ start,
-                                                                  const int32_t *__restrict__ order,
-                                                                  const float4 *__restrict__ sxyzi,
-                                                                  int32_t K, float4 *__restrict__ nrm)
+// ---- group walk ------------------------------------------------------------------------------
+// The common case (no K cap, or at most K points within the radius) runs G lanes per query:
+// every lane visits its share of the query's flattened candidate space with kNbUnroll loads in
+// flight and accumulates integer moments; the group then sums them with shuffles.  Integer
+// sums are exact, so the result does not depend on G or on the visiting order.
+constexpr int kNbBlock = 256;
+constexpr int kNbRuns = 9;
+constexpr int kNbUnroll = 4;
+
+struct Moments {
+    long long S[9];
+    int32_t cnt;
+    int32_t pad;
+};
+
+// Calls f(p) on the lane for every candidate of its share with d2 < r^2; returns the group's
+// in-radius count (on every lane of the group).
+template <class F>
+__device__ __forceinline__ int32_t group_walk(const GridRefs &G, float4 q, int g, int gs, F &&f)
+{
+    int32_t lo[3], hi[3];
+    int32_t cnt = 0;
+    if (grid_range(G.g, q.x, q.y, q.z, lo, hi)) {
+        const double r2 = (double)G.g.radius * (double)G.g.radius;
+        const int nry = hi[1] - lo[1] + 1;
+        const int nruns = (hi[0] - lo[0] + 1) * nry;
+        if (nruns <= kNbRuns) {
+            uint32_t rs[kNbRuns], cum[kNbRuns + 1];
+#pragma unroll
+            for (int r = 0; r < kNbRuns; ++r) {
+                uint32_t a = 0, b = 0;
+                if (r < nruns) {
+                    const int ix = lo[0] + r / nry, iy = lo[1] + r % nry;
+                    const int64_t row = ((int64_t)ix * G.g.dims[1] + iy) * G.g.dims[2];
+                    a = G.start[row + lo[2]];
+                    b = G.start[row + hi[2] + 1];
+                }
+                rs[r] = a;
+                cum[r + 1] = b - a;
+            }
+            cum[0] = 0;
+#pragma unroll
+            for (int r = 0; r < kNbRuns; ++r) cum[r + 1] += cum[r];
+            const uint32_t total = cum[kNbRuns];
+            for (uint32_t f0 = g; f0 < total; f0 += gs * kNbUnroll) {
+                float4 c[kNbUnroll];
+                bool ok[kNbUnroll];
+#pragma unroll
+                for (int u = 0; u < kNbUnroll; ++u) {
+                    const uint32_t fi = f0 + u * gs;
+                    ok[u] = fi < total;
+                    if (ok[u]) {
+                        uint32_t sl = rs[0] + fi;
+#pragma unroll
+                        for (int r = 1; r < kNbRuns; ++r)
+                            if (fi >= cum[r]) sl = rs[r] + (fi - cum[r]);
+                        c[u] = G.sxyzi[sl];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kNbUnroll; ++u)
+                    if (ok[u] && dist2(c[u].x, c[u].y, c[u].z, q.x, q.y, q.z) < r2) {
+                        f(c[u]);
+                        ++cnt;
+                    }
+            }
+        } else {
+            // cell smaller than the radius (not produced by grid_build): plain run walk
+            for (int32_t ix = lo[0]; ix <= hi[0]; ++ix)
+                for (int32_t iy = lo[1]; iy <= hi[1]; ++iy) {
+                    const int64_t row = ((int64_t)ix * G.g.dims[1] + iy) * G.g.dims[2];
+                    const uint32_t s0 = G.start[row + lo[2]], s1 = G.start[row + hi[2] + 1];
+                    for (uint32_t sl = s0 + g; sl < s1; sl += gs) {
+                        const float4 c = G.sxyzi[sl];
+                        if (dist2(c.x, c.y, c.z, q.x, q.y, q.z) < r2) {
+                            f(c);
+                            ++cnt;
+                        }
+                    }
+                }
+        }
+    }
+    for (int m = 1; m < gs; m <<= 1) cnt += __shfl_xor(cnt, m, kWave);
+    return cnt;
+}
+
+// Neighbourhood moments (2^-40 fixed point) of one neighbour p of query q.
+//   normals:  u = (p - q) / r;  S = sum u, sum u u^T (xx xy xz yy yz zz)
+//   gradient: e = p - q, w = (e - (e.n) n) / r, dI = I_p - I_q;  S = sum w w^T, sum w dI
+// The query itself contributes exactly zero to every sum (and one to the count).
+template <int KIND>
+__device__ __forceinline__ void add_moments(long long S[9], float4 q, float4 p, const double nv[3], double inv_r)
+{
+    if (KIND == 0) {
+        const double u0 = ((double)p.x - (double)q.x) * inv_r;
+        const double u1 = ((double)p.y - (double)q.y) * inv_r;
+        const double u2 = ((double)p.z - (double)q.z) * inv_r;
+        S[0] += fx(u0 * kFx40);
+        S[1] += fx(u1 * kFx40);
+        S[2] += fx(u2 * kFx40);
+        S[3] += fx((u0 * u0) * kFx40);
+        S[4] += fx((u0 * u1) * kFx40);
+        S[5] += fx((u0 * u2) * kFx40);
+        S[6] += fx((u1 * u1) * kFx40);
+        S[7] += fx((u1 * u2) * kFx40);
+        S[8] += fx((u2 * u2) * kFx40);
+    } else {
+        const double e0 = (double)p.x - (double)q.x;
+        const double e1 = (double)p.y - (double)q.y;
+        const double e2 = (double)p.z - (double)q.z;
+        const double tn = (e0 * nv[0] + e1 * nv[1]) + e2 * nv[2];
+        const double w0 = (e0 - tn * nv[0]) * inv_r;
+        const double w1 = (e1 - tn * nv[1]) * inv_r;
+        const double w2 = (e2 - tn * nv[2]) * inv_r;
+        const double dI = (double)p.w - (double)q.w;
+        S[0] += fx((w0 * w0) * kFx40);
+        S[1] += fx((w0 * w1) * kFx40);
+        S[2] += fx((w0 * w2) * kFx40);
+        S[3] += fx((w1 * w1) * kFx40);
+        S[4] += fx((w1 * w2) * kFx40);
+        S[5] += fx((w2 * w2) * kFx40);
+        S[6] += fx((w0 * dI) * kFx40);
+        S[7] += fx((w1 * dI) * kFx40);
+        S[8] += fx((w2 * dI) * kFx40);
+    }
+}
+
+// Pass 1: moments of every query whose neighbourhood needs no K selection; queries with more
+// than K > 0 points within the radius are appended to the overflow list instead.
+template <int KIND>
+__global__ __launch_bounds__(kNbBlock) void k_moments(const float4 *__restrict__ xyzi, const float4 *__restrict__ nrm,
+                                                      const CloudHdr *__restrict__ hdr, GridDesc g,
+                                                      const uint32_t *__restrict__ start,
+                                                      const int32_t *__restrict__ order,
+                                                      const float4 *__restrict__ sxyzi, int32_t K, int gs,
+                                                      Moments *__restrict__ mom, int32_t *__restrict__ ovf,
+                                                      uint32_t *__restrict__ ovf_n)
+{
+    const GridRefs G{g, start, order, sxyzi};
+    const int n = hdr->n;
+    const int lg = __builtin_ctz((unsigned)gs);
+    const int gl = (int)threadIdx.x & (gs - 1);
+    const int per_block = kNbBlock >> lg;
+    const double inv_r = 1.0 / (double)g.radius;
+    for (int base = blockIdx.x * per_block; base < n; base += gridDim.x * per_block) {
+        const int qi = base + ((int)threadIdx.x >> lg);
+        if (qi >= n) break;  // whole groups leave together
+        const float4 q = xyzi[qi];
+        double nv[3] = {0.0, 0.0, 0.0};
+        if (KIND == 1) {
+            const float4 nf = nrm[qi];
+            nv[0] = nf.x;
+            nv[1] = nf.y;
+            nv[2] = nf.z;
+        }
+        long long S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int32_t cnt = group_walk(G, q, gl, gs, [&](float4 p) { add_moments<KIND>(S, q, p, nv, inv_r); });
+        if (K > 0 && cnt > K) {
+            if (gl == 0) ovf[atomicAdd(ovf_n, 1u)] = qi;
+            continue;
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            for (int m = 1; m < gs; m <<= 1) S[c] += __shfl_xor(S[c], m, kWave);
+        if (gl == 0) {
+            Moments &M = mom[qi];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) M.S[c] = S[c];
+            M.cnt = cnt;
+        }
+    }
+}
+
+// Pass 2: the overflow queries, one wavefront each, with exact K-nearest selection.
+template <int KIND>
+__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_moments_overflow(
+    const float4 *__restrict__ xyzi, const float4 *__restrict__ nrm, const CloudHdr *__restrict__ hdr, GridDesc g,
+    const uint32_t *__restrict__ start, const int32_t *__restrict__ order, const float4 *__restrict__ sxyzi,
+    int32_t K, const int32_t *__restrict__ ovf, const uint32_t *__restrict__ ovf_n, Moments *__restrict__ mom)
 {
     __shared__ NbBuf bufs[kWavesPerBlock];
-    __shared__ long long msum[kWavesPerBlock][kNormBatch][9];
-    __shared__ int32_t mcnt[kWavesPerBlock][kNormBatch];
     const int w = threadIdx.x / kWave;
     NbBuf &buf = bufs[w];
     const GridRefs G{g, start, order, sxyzi};
     const int n = hdr->n;
+    const uint32_t m = *ovf_n;
     const double inv_r = 1.0 / (double)g.radius;
-    const int nw = gridDim.x * kWavesPerBlock;
-    // a wavefront gathers the moments of kNormBatch queries one after the other, then
-    // kNormBatch lanes diagonalise them in parallel
-    for (int q0 = (blockIdx.x * kWavesPerBlock + w) * kNormBatch; q0 < n; q0 += nw * kNormBatch) {
-        const int nq = min(kNormBatch, n - q0);
-        for (int k = 0; k < nq; ++k) {
-        const int qi = q0 + k;
+    for (uint32_t e = blockIdx.x * kWavesPerBlock + w; e < m; e += gridDim.x * kWavesPerBlock) {
+        const int qi = ovf[e];
+        if (qi < 0 || qi >= n) continue;
         const float4 q = xyzi[qi];
+        double nv[3] = {0.0, 0.0, 0.0};
+        if (KIND == 1) {
+            const float4 nf = nrm[qi];
+            nv[0] = nf.x;
+            nv[1] = nf.y;
+            nv[2] = nf.z;
+        }
         long long S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         const int32_t cnt = for_each_neighbour(G, q, K, buf, [&](uint32_t, int32_t, float4 p) {
-            const double u0 = ((double)p.x - (double)q.x) * inv_r;
-            const double u1 = ((double)p.y - (double)q.y) * inv_r;
-            const double u2 = ((double)p.z - (double)q.z) * inv_r;
-            S[0] += fx(u0 * kFx40);
-            S[1] += fx(u1 * kFx40);
-            S[2] += fx(u2 * kFx40);
-            S[3] += fx((u0 * u0) * kFx40);
-            S[4] += fx((u0 * u1) * kFx40);
-            S[5] += fx((u0 * u2) * kFx40);
-            S[6] += fx((u1 * u1) * kFx40);
-            S[7] += fx((u1 * u2) * kFx40);
-            S[8] += fx((u2 * u2) * kFx40);
+            add_moments<KIND>(S, q, p, nv, inv_r);
         });
 #pragma unroll
         for (int c = 0; c < 9; ++c) S[c] = wave_sum(S[c]);
         if (lane_id() == 0) {
-            for (int c = 0; c < 9; ++c) msum[w][k][c] = S[c];
-            mcnt[w][k] = cnt;
+            Moments &M = mom[qi];
+            for (int c = 0; c < 9; ++c) M.S[c] = S[c];
+            M.cnt = cnt;
         }
-        }
-        wave_lds_sync();
-        const int l = lane_id();
-        if (l < nq) nrm[q0 + l] = normal_from_moments(mcnt[w][l], msum[w][l]);
-        wave_lds_sync();
     }
 }
 
-__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_gradient(const float4 *__restrict__ xyzi,
-                                                                   const float4 *__restrict__ nrm,
-                                                                   const CloudHdr *__restrict__ hdr,
-                                                                   GridDesc g, const uint32_t *__restrict__ start,
-                                                                   const int32_t *__restrict__ order,
-                                                                   const float4 *__restrict__ sxyzi,
-                                                                   int32_t K, float4 *__restrict__ grad)
+// Colour gradient from the tangent-plane moments: A = sum w w^T r^2 + (k-1)^2 n n^T + 1e-6 I,
+// b = sum w dI r, solved by unpivoted LDL^T (cuda_compute_color_gradient.cu:41-90,
+// little_3x3_ldlt_solve.cuh:11-60); fewer than 5 neighbours (query included) -> zero.
+__device__ __forceinline__ float4 gradient_from_moments(int32_t k, const long long S[9], float4 nf, double radius)
 {
-    __shared__ NbBuf bufs[kWavesPerBlock];
-    NbBuf &buf = bufs[threadIdx.x / kWave];
-    const GridRefs G{g, start, order, sxyzi};
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (k < 5) return out;  // cuda_compute_color_gradient.cu:41-44
+    const double nv[3] = {(double)nf.x, (double)nf.y, (double)nf.z};
+    const double rr = radius * radius;
+    const double kk = (double)(k - 1) * (double)(k - 1);
+    const int IA[6] = {0, 0, 0, 1, 1, 2}, IB[6] = {0, 1, 2, 1, 2, 2};
+    double A[3][3];
+    for (int c = 0; c < 6; ++c) {
+        double v = ((double)S[c] / kFx40) * rr + kk * (nv[IA[c]] * nv[IB[c]]);
+        A[IA[c]][IB[c]] = v;
+        A[IB[c]][IA[c]] = v;
+    }
+    for (int a = 0; a < 3; ++a) A[a][a] += 1.0e-6;
+    double b[3];
+    for (int a = 0; a < 3; ++a) b[a] = ((double)S[6 + a] / kFx40) * radius;
+    double D0 = A[0][0];
+    double L10 = A[1][0] / D0, L20 = A[2][0] / D0;
+    double D1 = A[1][1] - L10 * L10 * D0;
+    double L21 = (A[2][1] - L10 * L20 * D0) / D1;
+    double D2 = A[2][2] - L20 * L20 * D0 - L21 * L21 * D1;
+    double x0 = b[0];
+    double x1 = b[1] - L10 * x0;
+    double x2 = b[2] - L20 * x0 - L21 * x1;
+    x0 /= D0;
+    x1 /= D1;
+    x2 /= D2;
+    x1 -= L21 * x2;
+    x0 -= L10 * x1 + L20 * x2;
+    return make_float4((float)x0, (float)x1, (float)x2, 0.0f);
+}
+
+// Pass 3: one thread per point (FP64 eigen / LDL^T solves keep every lane busy).  Re-arms the
+// overflow counter for the next call.
+template <int KIND>
+__global__ __launch_bounds__(kNbBlock) void k_solve(const CloudHdr *__restrict__ hdr, const Moments *__restrict__ mom,
+                                                    const float4 *__restrict__ nrm, double radius,
+                                                    float4 *__restrict__ out, uint32_t *__restrict__ ovf_n)
+{
     const int n = hdr->n;
-    const double inv_r = 1.0 / (double)g.radius;
-    const double rr = (double)g.radius * (double)g.radius;
-    const int nw = gridDim.x * kWavesPerBlock;
-    for (int qi = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; qi < n; qi += nw) {
-        const float4 q = xyzi[qi];
-        const float4 nf = nrm[qi];
-        const double nv[3] = {(double)nf.x, (double)nf.y, (double)nf.z};
-        const double Ip = (double)q.w;
-        long long S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // A: xx xy xz yy yz zz, b: x y z
-        const int32_t k = for_each_neighbour(G, q, K, buf, [&](uint32_t, int32_t idx, float4 p) {
-            if (idx == qi) return;  // cuda_compute_color_gradient.cu:61-62
-            const double e0 = (double)p.x - (double)q.x;
-            const double e1 = (double)p.y - (double)q.y;
-            const double e2 = (double)p.z - (double)q.z;
-            const double tn = (e0 * nv[0] + e1 * nv[1]) + e2 * nv[2];
-            const double w0 = (e0 - tn * nv[0]) * inv_r;
-            const double w1 = (e1 - tn * nv[1]) * inv_r;
-            const double w2 = (e2 - tn * nv[2]) * inv_r;
-            const double dI = (double)p.w - Ip;
-            S[0] += fx((w0 * w0) * kFx40);
-            S[1] += fx((w0 * w1) * kFx40);
-            S[2] += fx((w0 * w2) * kFx40);
-            S[3] += fx((w1 * w1) * kFx40);
-            S[4] += fx((w1 * w2) * kFx40);
-            S[5] += fx((w2 * w2) * kFx40);
-            S[6] += fx((w0 * dI) * kFx40);
-            S[7] += fx((w1 * dI) * kFx40);
-            S[8] += fx((w2 * dI) * kFx40);
-        });
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ovf_n = 0u;
+    for (int i = blockIdx.x * kNbBlock + threadIdx.x; i < n; i += gridDim.x * kNbBlock) {
+        const Moments &M = mom[i];
+        long long S[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) S[c] = wave_sum(S[c]);
-        if (lane_id() == 0) {
-            float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (k >= 5) {  // cuda_compute_color_gradient.cu:41-44
-                const double kk = (double)(k - 1) * (double)(k - 1);
-                const int IA[6] = {0, 0, 0, 1, 1, 2}, IB[6] = {0, 1, 2, 1, 2, 2};
-                double A[3][3];
-                for (int c = 0; c < 6; ++c) {
-                    double v = ((double)S[c] / kFx40) * rr + kk * (nv[IA[c]] * nv[IB[c]]);
-                    A[IA[c]][IB[c]] = v;
-                    A[IB[c]][IA[c]] = v;
-                }
-                for (int a = 0; a < 3; ++a) A[a][a] += 1.0e-6;
-                double b[3];
-                for (int a = 0; a < 3; ++a) b[a] = ((double)S[6 + a] / kFx40) * (double)g.radius;
-                // unpivoted LDL^T (little_3x3_ldlt_solve.cuh:11-60)
-                double D0 = A[0][0];
-                double L10 = A[1][0] / D0, L20 = A[2][0] / D0;
-                double D1 = A[1][1] - L10 * L10 * D0;
-                double L21 = (A[2][1] - L10 * L20 * D0) / D1;
-                double D2 = A[2][2] - L20 * L20 * D0 - L21 * L21 * D1;
-                double x0 = b[0];
-                double x1 = b[1] - L10 * x0;
-                double x2 = b[2] - L20 * x0 - L21 * x1;
-                x0 /= D0;
-                x1 /= D1;
-                x2 /= D2;
-                x1 -= L21 * x2;
-                x0 -= L10 * x1 + L20 * x2;
-                out = make_float4((float)x0, (float)x1, (float)x2, 0.0f);
-            }
-            grad[qi] = out;
-        }
+        for (int c = 0; c < 9; ++c) S[c] = M.S[c];
+        out[i] = KIND == 0 ? normal_from_moments(M.cnt, S) : gradient_from_moments(M.cnt, S, nrm[i], radius);
     }
 }
 
-static unsigned wave_blocks(int64_t n)
+static unsigned grid_for(int64_t items, int per_block, int64_t cap)
 {
-    return (unsigned)std::min<int64_t>(8192, std::max<int64_t>(1, (n + kWavesPerBlock - 1) / kWavesPerBlock));
+    return (unsigned)std::min<int64_t>(cap, std::max<int64_t>(1, (items + per_block - 1) / per_block));
+}
+
+static int normals_lanes(gca_ctx ctx) { return ctx->tune[GCA_TUNE_NORMALS_LANES] ? ctx->tune[GCA_TUNE_NORMALS_LANES] : 16; }
+
+template <int KIND>
+static int32_t launch_moments_pipeline(gca_ctx ctx, gca_cloud c, float radius, int32_t K, float4 *out)
+{
+    const Grid &G = c->grid;
+    const int gs = normals_lanes(ctx);
+    const int64_t cap = std::max<int64_t>(c->cap, 1);
+    Moments *mom = (Moments *)ctx_scratch(ctx, 17, sizeof(Moments) * cap, false);
+    int32_t *ovf = (int32_t *)ctx_scratch(ctx, 18, sizeof(int32_t) * cap, false);
+    uint32_t *ovf_n = (uint32_t *)ctx_scratch(ctx, 19, 256, true);  // zero, re-armed by k_solve
+    if (!mom || !ovf || !ovf_n) return GCA_ERR_OUT_OF_MEMORY;
+    const GridDesc d = query_desc(G, radius);
+    StageTimer timer(ctx, KIND == 0 ? GCA_STAGE_NORMALS : GCA_STAGE_GRADIENT);
+    hipLaunchKernelGGL(k_moments<KIND>, dim3(grid_for(cap, kNbBlock / gs, 4096)), dim3(kNbBlock), 0, ctx->stream,
+                       c->xyzi, c->nrm, c->hdr, d, G.start, G.order, G.sxyzi, K, gs, mom, ovf, ovf_n);
+    if (K > 0)
+        hipLaunchKernelGGL(k_moments_overflow<KIND>, dim3(grid_for(cap, kWavesPerBlock, 1024)),
+                           dim3(kWave * kWavesPerBlock), 0, ctx->stream, c->xyzi, c->nrm, c->hdr, d, G.start,
+                           G.order, G.sxyzi, K, (const int32_t *)ovf, (const uint32_t *)ovf_n, mom);
+    hipLaunchKernelGGL(k_solve<KIND>, dim3(grid_for(cap, kNbBlock, 2048)), dim3(kNbBlock), 0, ctx->stream, c->hdr,
+                       (const Moments *)mom, (const float4 *)c->nrm, (double)radius, out, ovf_n);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
 }
 
 int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
 {
-    const Grid &G = c->grid;
-    StageTimer timer(ctx, GCA_STAGE_NORMALS);
-    hipLaunchKernelGGL(k_normals, dim3(wave_blocks((c->cap + kNormBatch - 1) / kNormBatch)), dim3(kWave * kWavesPerBlock), 0, ctx->stream,
-                       c->xyzi, c->hdr, query_desc(G, radius), G.start, G.order, G.sxyzi, K, c->nrm);
-    GCA_HIP(hipGetLastError());
-    return GCA_OK;
+    return launch_moments_pipeline<0>(ctx, c, radius, K, c->nrm);
 }
 
 int32_t launch_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
 {
-    const Grid &G = c->grid;
-    StageTimer timer(ctx, GCA_STAGE_GRADIENT);
-    hipLaunchKernelGGL(k_gradient, dim3(wave_blocks(c->cap)), dim3(kWave * kWavesPerBlock), 0, ctx->stream,
-                       c->xyzi, c->nrm, c->hdr, query_desc(G, radius), G.start, G.order, G.sxyzi, K, c->grad);
-    GCA_HIP(hipGetLastError());
-    return GCA_OK;
+    return launch_moments_pipeline<1>(ctx, c, radius, K, c->grad);
 }
 
 }  // namespace gca

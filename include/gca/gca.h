@@ -149,8 +149,8 @@ enum {
     GCA_STAGE_FRONTEND = 0,   /* bilateral + back-projection + compaction (3 kernels) */
     GCA_STAGE_VOXEL = 1,      /* mark + popcount scan + accumulate + finalize */
     GCA_STAGE_GRID = 2,       /* count + scan + scatter */
-    GCA_STAGE_NORMALS = 3,    /* one kernel */
-    GCA_STAGE_GRADIENT = 4,   /* one kernel */
+    GCA_STAGE_NORMALS = 3,    /* moments + capped overflow + per-point eigen solve */
+    GCA_STAGE_GRADIENT = 4,   /* moments + per-point 3x3 solve */
     GCA_STAGE_ICP = 5,        /* ICP solve: one persistent launch per call (or one per iteration) */
     GCA_STAGE_COUNT = 6
 };
@@ -164,7 +164,7 @@ enum {
     GCA_TUNE_ICP_LANES_MEDIUM = 1, /* ... clouds < 128k points */
     GCA_TUNE_ICP_LANES_LARGE = 2,  /* ... larger clouds */
     GCA_TUNE_ICP_MAX_UNITS = 3,    /* cap on work units (FP64 partials) per pair and iteration, 1..1024 */
-    GCA_TUNE_NORMALS_LANES = 4,    /* lanes per normal/gradient query: 8, 16, 32, 64 */
+    GCA_TUNE_NORMALS_LANES = 4,    /* lanes per normal/gradient query: 1,2,4,8,16,32,64 */
     GCA_TUNE_ICP_PERSIST_BLOCKS = 5, /* workgroups of the persistent ICP launch (0 = 2 per CU), 1..65536 */
     GCA_TUNE_ICP_LAUNCH = 6,       /* 0/1 = one persistent launch per call, 2 = one launch per iteration */
     GCA_TUNE_COUNT = 7

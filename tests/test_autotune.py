@@ -1,0 +1,59 @@
+"""Autotuner logic on the CPU: coordinate descent over a fake ctx whose stage times are a known
+function of the launch shape."""
+import json
+
+from cuda_3d_amd import autotune
+
+
+class FakeCtx:
+    """Stage times: normals minimal at lanes 16, ICP separable with minimum at (4, 8, 2, 256)."""
+
+    def __init__(self):
+        self.t = {"normals_lanes": 0, "icp_lanes_small": 0, "icp_lanes_medium": 0, "icp_lanes_large": 0,
+                  "icp_max_units": 0, "icp_persist_blocks": 0, "icp_launch": 0}
+        self.on = False
+        self.calls = 0
+
+    def get_tuning(self):
+        return dict(self.t)
+
+    def set_tuning(self, **kw):
+        for k, v in kw.items():
+            assert k in self.t
+            self.t[k] = v
+
+    def profile(self, on=True):
+        self.on = on
+
+    def profile_read(self):
+        import math
+
+        t = self.t
+        lg = lambda v: math.log2(max(v, 1))  # noqa: E731
+        normals = 1.0 + (lg(t["normals_lanes"]) - 4) ** 2
+        icp = (3.0 + (lg(t["icp_lanes_small"]) - 2) ** 2 + (lg(t["icp_lanes_medium"]) - 3) ** 2
+               + (lg(t["icp_lanes_large"]) - 1) ** 2 + abs(lg(t["icp_max_units"]) - 8) + 0.01 * t["icp_persist_blocks"])
+        return {"normals": (3, normals), "gradient": (3, 0.5), "icp": (1, icp)}
+
+
+def test_coordinate_descent_finds_separable_minimum():
+    ctx = FakeCtx()
+    ran = []
+    best, rows = autotune.tune(ctx, lambda: ran.append(1), autotune.space(256), rounds=3, reps=3, log=lambda m: None)
+    assert best["normals_lanes"] == 16
+    assert (best["icp_lanes_small"], best["icp_lanes_medium"], best["icp_lanes_large"]) == (4, 8, 2)
+    assert best["icp_max_units"] == 256
+    assert best["icp_persist_blocks"] == 128  # smallest candidate: time grows with it
+    # the ctx is left at the best settings, and every measurement is recorded
+    assert all(ctx.t[k] == v for k, v in best.items())
+    assert len(rows) >= sum(len(v) for v in autotune.space(256).values())
+    assert len(ran) == 1 + 3 * len(rows)  # one warm-up run, reps per measurement
+
+
+def test_table_round_trip(tmp_path):
+    ctx = FakeCtx()
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps({"tuning": {"normals_lanes": 8, "icp_max_units": 128, "not_a_knob": 3}}))
+    applied = autotune.apply(ctx, str(path))
+    assert applied == {"normals_lanes": 8, "icp_max_units": 128}
+    assert ctx.t["normals_lanes"] == 8 and ctx.t["icp_max_units"] == 128
