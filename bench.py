@@ -22,7 +22,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -70,84 +69,41 @@ def cpu_baseline(frames, cam, levels, iters):
 
 
 def run_batch(g, ctx, n_pairs, levels, iters, seed=2026):
-    import ctypes as C
-
     import numpy as np
 
-    from cuda_3d_amd import _lib as L
-    from cuda_3d_amd import scene
+    from cuda_3d_amd.workloads import BatchPairs
 
-    rng = np.random.default_rng(np.random.PCG64(seed))
-    counts = np.rint(np.exp(rng.uniform(math.log(5e3), math.log(2e5), n_pairs))).astype(int)
-    pairs = []
-    t0 = time.perf_counter()
-    for k, n in enumerate(counts):
-        sx, sr, tx, tr, G = scene.sample_surface_pair(seed + 17 * k, int(n))
-        pairs.append((g.PointCloud.from_numpy(sx, sr, ctx=ctx), g.PointCloud.from_numpy(tx, tr, ctx=ctx)))
-    log(f"batch: generated {n_pairs} pairs, {int(counts.sum())} points per side ({time.perf_counter() - t0:.1f}s)")
-    # per-level clouds (untimed preparation, identical for both paths)
-    lv = []
-    for v in levels:
-        s_l, t_l = [], []
-        for s, t in pairs:
-            s2 = s.voxel_grid_down_sample(v)
-            t2 = t.voxel_grid_down_sample(v)
-            t2.estimate_normals(2.5 * v, 30)
-            t2.compute_color_gradient(2.0 * v, 0)
-            s_l.append(s2)
-            t_l.append(t2)
-        lv.append((s_l, t_l))
-    ctx.synchronize()
-    nl = len(levels)
-    params = [g.icp_params(iters, 2.5 * v, 0.968, 0, 1e-6, 1e-6) for v in levels]
-    grads = [2.0 * v for v in levels]
-    S = [[lv[l][0][k] for l in range(nl)] for k in range(n_pairs)]
-    Tg = [[lv[l][1][k] for l in range(nl)] for k in range(n_pairs)]
-    I = np.tile(np.eye(4), (n_pairs, 1, 1))
-
-    def batched():
-        return g.batch_multiscale_icp(S, Tg, params, I, grads)
-
-    def plain():
-        T = np.zeros((n_pairs, 4, 4))
-        lat = []
-        for k in range(n_pairs):
-            t1 = time.perf_counter()
-            Tk = np.ascontiguousarray(np.eye(4).reshape(16))
-            rep = L.IcpReport()
-            for l, v in enumerate(levels):
-                L.check(L.lib.gca_color_icp_align(ctx.handle, S[k][l].handle, Tg[k][l].handle,
-                                                  C.byref(params[l]), grads[l], Tk.ctypes.data_as(C.c_void_p),
-                                                  Tk.ctypes.data_as(C.c_void_p), C.byref(rep)), "align")
-            lat.append(time.perf_counter() - t1)
-            T[k] = Tk.reshape(4, 4)
-        return T, lat
-
-    batched()
-    plain()  # warm both paths
+    bp = BatchPairs(ctx, n_pairs, levels, iters, seed, log=log)
+    bp.batched()
+    bp.plain(ctx)  # warm both paths
     ctx.synchronize()
     reps = 3
     tb, blat = [], None
     for _ in range(reps):
         t1 = time.perf_counter()
-        Tb, breps = batched()
+        Tb, breps = bp.batched()
         ctx.synchronize()
         tb.append(time.perf_counter() - t1)
         blat = [row[-1].latency_us * 1e-3 for row in breps]
     tp = []
     for _ in range(reps):
         t1 = time.perf_counter()
-        Tp, lat = plain()
+        Tp, lat = bp.plain(ctx)
         ctx.synchronize()
         tp.append(time.perf_counter() - t1)
     tb_s, tp_s = min(tb), min(tp)
     diff = float(np.abs(Tb - Tp).max())
     lat = np.array(lat) * 1e3
-    sizes = [[S[k][l].points_number() for k in range(n_pairs)] for l in range(nl)]
+    sizes = bp.sizes
     iters_run = [[r.iterations for r in row] for row in breps]
+    # registration quality against the generating transforms (not a parity bar)
+    truth_err = []
+    for k, G in enumerate(bp.truth):
+        dR = Tb[k][:3, :3] @ G[:3, :3].T
+        truth_err.append(float(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1))))
     return {
-        "pairs": n_pairs, "points_sampled_per_side": int(counts.sum()),
-        "points_sampled_min": int(counts.min()), "points_sampled_max": int(counts.max()),
+        "pairs": n_pairs, "points_sampled_per_side": int(bp.counts.sum()),
+        "points_sampled_min": int(bp.counts.min()), "points_sampled_max": int(bp.counts.max()),
         "points_per_level_min": [int(min(x)) for x in sizes], "points_per_level_max": [int(max(x)) for x in sizes],
         "points_per_level_total": [int(sum(x)) for x in sizes],
         "levels": list(levels), "max_iters_per_level": iters, "stopping_rule": "reference (1e-6, 1e-6)",
@@ -157,6 +113,7 @@ def run_batch(g, ctx, n_pairs, levels, iters, seed=2026):
         "batched_latency_ms_p50": float(np.percentile(blat, 50)), "batched_latency_ms_p99": float(np.percentile(blat, 99)),
         "plain_latency_ms_p50": float(np.percentile(lat, 50)), "plain_latency_ms_p99": float(np.percentile(lat, 99)),
         "max_abs_pose_diff_batched_vs_plain": diff,
+        "rotation_err_vs_truth_rad_median": float(np.median(truth_err)),
     }
 
 

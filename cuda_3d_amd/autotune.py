@@ -39,6 +39,7 @@ def space(n_cu: int) -> Dict[str, List[int]]:
         "icp_lanes_large": [1, 2, 4, 8, 16],
         "icp_max_units": [64, 128, 256, 512, 1024],
         "icp_persist_blocks": [n_cu // 2, n_cu, 2 * n_cu, 4 * n_cu, 8 * n_cu],
+        "icp_lanes_batch": [1, 2, 4, 8],
     }
 
 
@@ -49,6 +50,13 @@ STAGES_OF = {
     "icp_lanes_large": ("icp",),
     "icp_max_units": ("icp",),
     "icp_persist_blocks": ("icp",),
+    "icp_lanes_batch": ("icp",),
+}
+# knobs a workload does not exercise (one-pair calls never read the batch lanes, the batch
+# never reads the one-pair lanes or the normals shape)
+UNUSED_BY = {
+    "frame": ("icp_lanes_batch",),
+    "batch": ("normals_lanes", "icp_lanes_small", "icp_lanes_medium", "icp_lanes_large"),
 }
 
 
@@ -103,35 +111,6 @@ def apply(ctx, path: str = DEFAULT_TABLE) -> dict:
     return settings
 
 
-def frame_workload(ctx, steps: int = 6, W: int = 640, H: int = 480, seed: int = 12):
-    """The 640x480 frame pipeline (front end, 3 voxel levels with normals and gradients,
-    3-level colored ICP) over a seeded frame pair, `steps` frames per run."""
-    import numpy as np
-
-    from . import core as g
-    from . import scene
-    from ._lib import BILATERAL_FIXED
-    from .pipeline import FramePipeline
-
-    (td, tc), (sd, sc), _, cm = scene.frame_pair(seed, W, H)
-    cam = g.make_camera(W, H, cm.fx, cm.fy, cm.cx, cm.cy)
-    bil = g.bilateral(BILATERAL_FIXED, 2, 2.0, 30.0)
-    dev = [(g.device_tensor(td.view(np.int16)), g.device_tensor(tc)),
-           (g.device_tensor(sd.view(np.int16)), g.device_tensor(sc))]
-    pipe = FramePipeline(ctx, cam, bil)
-    state = {"k": 0}
-
-    def run():
-        for _ in range(steps):
-            d, c = dev[state["k"] % 2]
-            pipe.step(d.data_ptr(), c.data_ptr())
-            state["k"] += 1
-        ctx.synchronize()
-
-    run._keep = (dev, pipe)  # keep device buffers alive with the closure
-    return run
-
-
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=DEFAULT_TABLE)
@@ -139,26 +118,38 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--steps", type=int, default=6, help="frames per measured run")
     ap.add_argument("--knobs", default="", help="comma-separated subset of knobs")
+    ap.add_argument("--workload", choices=("frame", "batch"), default="frame",
+                    help="frame: 640x480 odometry step; batch: 64 uneven pairs x 3 levels, one launch")
     args = ap.parse_args(argv)
 
     from . import core as g
+    from . import workloads
 
     ctx = g.Context(0)
     import torch  # noqa: E402  (device properties)
 
     n_cu = torch.cuda.get_device_properties(0).multi_processor_count
     knobs = space(n_cu)
+    knobs = {k: v for k, v in knobs.items() if k not in UNUSED_BY[args.workload]}
     if args.knobs:
         keep = set(args.knobs.split(","))
         knobs = {k: v for k, v in knobs.items() if k in keep}
-    wl = frame_workload(ctx, steps=args.steps)
+    say = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.workload == "frame":
+        wl = workloads.frame_workload(ctx, steps=args.steps)
+        what = (f"640x480 frame pipeline (3 levels, normals K30, 3-level colored ICP), {args.steps} frames "
+                f"per run, median of {args.reps}")
+    else:
+        bp = workloads.BatchPairs(ctx, log=say)
+
+        def wl():
+            bp.batched()
+            ctx.synchronize()
+        what = f"64 uneven pairs x 3 levels in one persistent launch, median of {args.reps}"
     t0 = time.perf_counter()
-    best, rows = tune(ctx, wl, knobs, rounds=args.rounds, reps=args.reps,
-                      log=lambda m: print(m, file=sys.stderr, flush=True))
+    best, rows = tune(ctx, wl, knobs, rounds=args.rounds, reps=args.reps, log=say)
     out = {"tuning": best, "device": torch.cuda.get_device_name(0), "compute_units": n_cu,
-           "workload": "640x480 frame pipeline (3 levels, normals K30, 3-level colored ICP), "
-                       f"{args.steps} frames per run, median of {args.reps}",
-           "seconds": time.perf_counter() - t0, "measurements": rows}
+           "workload": what, "seconds": time.perf_counter() - t0, "measurements": rows}
     with open(args.out, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps({"tuning": best}), flush=True)

@@ -271,7 +271,8 @@ void cloud_points_changed(gca_cloud c)
 int32_t launch_bounds(gca_ctx ctx, gca_cloud c)
 {
     hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(1), 0, ctx->stream, c->hdr);
-    int64_t nb = std::min<int64_t>(1024, std::max<int64_t>(1, (c->cap + 255) / 256));
+    // few blocks: each issues six atomics on the same header words
+    int64_t nb = std::min<int64_t>(64, std::max<int64_t>(1, (c->cap + 255) / 256));
     hipLaunchKernelGGL(k_bounds_reduce, dim3((unsigned)nb), dim3(256), 0, ctx->stream, c->xyzi, c->hdr);
     GCA_HIP(hipGetLastError());
     return GCA_OK;
@@ -432,7 +433,8 @@ int32_t gca_ctx_set_tuning(gca_ctx c, int32_t key, int32_t value)
 {
     if (!c || key < 0 || key >= GCA_TUNE_COUNT || value < 0) return GCA_ERR_INVALID_VALUE;
     auto pow2 = [](int32_t v, int32_t hi) { return v == 0 || (v <= hi && (v & (v - 1)) == 0); };
-    if (key <= GCA_TUNE_ICP_LANES_LARGE && !pow2(value, 64)) return GCA_ERR_INVALID_VALUE;
+    if ((key <= GCA_TUNE_ICP_LANES_LARGE || key == GCA_TUNE_ICP_LANES_BATCH) && !pow2(value, 64))
+        return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_ICP_MAX_UNITS && value > 1024) return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_ICP_PERSIST_BLOCKS && value > 65536) return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_ICP_LAUNCH && value > 2) return GCA_ERR_INVALID_VALUE;

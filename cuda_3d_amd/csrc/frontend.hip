@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_compact(const float4 *__restri
                                                           int32_t W, int32_t nbx,
                                                           float4 *__restrict__ out_xyzi,
                                                           float4 *__restrict__ out_rgb,
-                                                          CloudHdr *__restrict__ hdr)
+                                                          float *__restrict__ bpart)
 {
     __shared__ uint32_t woff[kRowSeg / kWave];
     __shared__ float smin[3][kRowSeg / kWave], smax[3][kRowSeg / kWave];
@@ -206,14 +206,46 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_compact(const float4 *__restri
         if (lane_id() == 0) smin[a][w] = v0, smax[a][w] = v1;
     }
     __syncthreads();
+    // the block's bounds go to its own slot (FLT_MAX / -FLT_MAX if it has no valid pixel); one
+    // small kernel folds them, instead of every block hitting the same six words with atomics
     if (threadIdx.x < 3) {
         int a = threadIdx.x;
         float v0 = smin[a][0], v1 = smax[a][0];
         for (int k = 1; k < kRowSeg / kWave; ++k) v0 = fminf(v0, smin[a][k]), v1 = fmaxf(v1, smax[a][k]);
-        if (v0 <= v1) {
-            atomicMin(&hdr->bmin_key[a], float_to_key(v0));
-            atomicMax(&hdr->bmax_key[a], float_to_key(v1));
+        float *bp = bpart + ((int64_t)y * nbx + blockIdx.x) * 6;
+        bp[a] = v0;
+        bp[3 + a] = v1;
+    }
+}
+
+// Folds the per-block bounds into the header keys: min over blocks (initial FLT_MAX), max over
+// blocks with the reference's initial FLT_MIN (geometry_util.cuh:92).
+__global__ __launch_bounds__(256) void k_bounds_fold(const float *__restrict__ bpart, int64_t nblk,
+                                                     CloudHdr *__restrict__ hdr)
+{
+    __shared__ float smin[3][4], smax[3][4];
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {FLT_MIN, FLT_MIN, FLT_MIN};
+    for (int64_t b = threadIdx.x; b < nblk; b += 256) {
+        const float *bp = bpart + b * 6;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], bp[a]);
+            mx[a] = fmaxf(mx[a], bp[3 + a]);
         }
+    }
+    const int w = threadIdx.x / kWave;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float v0 = wave_min(mn[a]), v1 = wave_max(mx[a]);
+        if (lane_id() == 0) smin[a][w] = v0, smax[a][w] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int a = threadIdx.x;
+        float v0 = smin[a][0], v1 = smax[a][0];
+        for (int k = 1; k < 4; ++k) v0 = fminf(v0, smin[a][k]), v1 = fmaxf(v1, smax[a][k]);
+        hdr->bmin_key[a] = float_to_key(v0);
+        hdr->bmax_key[a] = float_to_key(v1);
     }
 }
 
@@ -297,7 +329,8 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
     uint8_t *st_valid = (uint8_t *)ctx_scratch(ctx, 2, (size_t)P, false);
     uint32_t *bcnt = (uint32_t *)ctx_scratch(ctx, 3, sizeof(uint32_t) * (nblocks + 1), false);
     uint32_t *boff = (uint32_t *)ctx_scratch(ctx, 4, sizeof(uint32_t) * (nblocks + 1), false);
-    if (!st_xyzi || !st_rgb || !st_valid || !bcnt || !boff) return GCA_ERR_OUT_OF_MEMORY;
+    float *bpart = (float *)ctx_scratch(ctx, 20, sizeof(float) * 6 * nblocks, false);
+    if (!st_xyzi || !st_rgb || !st_valid || !bcnt || !boff || !bpart) return GCA_ERR_OUT_OF_MEMORY;
     dim3 grid(fp.nbx, H);
     StageTimer timer(ctx, GCA_STAGE_FRONTEND);
     hipLaunchKernelGGL(k_rgbd_pixels, grid, dim3(kRowSeg), 0, ctx->stream, depth, bgr,
@@ -305,7 +338,8 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
     GCA_HIP(hipGetLastError());
     GCA_TRY(exclusive_scan_u32(ctx, bcnt, boff, nblocks, (uint32_t *)&out->hdr->n));
     hipLaunchKernelGGL(k_rgbd_compact, grid, dim3(kRowSeg), 0, ctx->stream, st_xyzi, st_rgb, st_valid,
-                       boff, W, fp.nbx, out->xyzi, out->rgb, out->hdr);
+                       boff, W, fp.nbx, out->xyzi, out->rgb, bpart);
+    hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, ctx->stream, (const float *)bpart, nblocks, out->hdr);
     GCA_HIP(hipGetLastError());
     return GCA_OK;
 }

@@ -40,6 +40,7 @@ constexpr int kIcpBlock = 256;
 constexpr int kNSys = 29;        // 21 JTJ + 6 JTr + r^2 + n_corr
 constexpr int kPartStride = 32;  // doubles per partial
 constexpr int kMaxUnits = 1024;  // work units (partials) per state and iteration
+static_assert(kNSys <= kWave, "a partial is written by wavefront 0, which also takes the ticket");
 
 constexpr uint32_t kEpochWait = 0xfffffffeu;  // chained level whose predecessor still runs
 constexpr uint32_t kEpochDone = 0xffffffffu;
@@ -81,7 +82,7 @@ struct PairDesc {
     int32_t lanes[3];    // lanes per query for n < 16384, n < 131072, larger
 };
 
-// Launch-shape rule shared by host and device: the unit partition fixes the summation order, so
+// Launch-shape rules shared by host and device: the unit partition fixes the summation order, so
 // it must come out the same wherever it is computed.
 __host__ __device__ inline int icp_lanes_for(const int32_t lanes[3], int64_t n)
 {
@@ -90,8 +91,8 @@ __host__ __device__ inline int icp_lanes_for(const int32_t lanes[3], int64_t n)
 
 __host__ __device__ inline int32_t icp_units_for(int64_t n, int G, int32_t cap)
 {
-    const int64_t groups = kIcpBlock / G;
-    int64_t u = (n + groups - 1) / groups;
+    const int64_t q = kIcpBlock / G;  // queries per pass
+    int64_t u = (n + q - 1) / q;
     if (u < 1) u = 1;
     if (u > cap) u = cap;
     return (int32_t)u;
@@ -216,7 +217,8 @@ __device__ __forceinline__ void xform_f32(const double *T, float4 p, float &x, f
     z = (float)(((T[8] * a + T[9] * b) + T[10] * c) + T[11]);
 }
 
-// Device-scope coherent load of a double (bypasses caches that may hold a stale copy).
+// Device-scope coherent accesses: loads that cannot hit a stale cached copy, and stores that
+// write through to the device coherence point (no L2 write-back of unrelated data needed).
 __device__ __forceinline__ double load_coherent(const double *p)
 {
     const unsigned long long u =
@@ -224,31 +226,46 @@ __device__ __forceinline__ double load_coherent(const double *p)
     return __longlong_as_double((long long)u);
 }
 
+__device__ __forceinline__ int32_t load_coherent(const int32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_coherent(double *p, double v)
+{
+    __hip_atomic_store((unsigned long long *)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Block reduction of the accumulator into FP64 partial `out[0..27]`, plus the correspondence
-// count into out[28].  Wave: butterfly merges; block: the 4 wave results merged in wave order.
-// Only lanes that are multiples of `lead` (group leaders) hold data, so the butterfly runs over
-// masks 32 .. lead.
+// count into out[28].  Only lanes whose thread index is below `span` and a multiple of `lead`
+// hold data: each such wavefront merges them by butterflies over masks min(32, span/2) .. lead,
+// then the wave results are merged in wave order.
 template <int MODE>
-__device__ __forceinline__ void block_partial(Acc<MODE> &A, int ncorr, int lead, double *out)
+__device__ __forceinline__ void block_partial(Acc<MODE> &A, int ncorr, int lead, int span, double *out)
 {
     __shared__ Acc<MODE> wacc[kIcpBlock / kWave];
     __shared__ int wcnt[kIcpBlock / kWave];
-    for (int m = kWave / 2; m >= lead; m >>= 1) A.merge_from_lane(m);
-    for (int m = kWave / 2; m >= lead; m >>= 1) ncorr += __shfl_xor(ncorr, m, kWave);
     const int w = threadIdx.x / kWave;
-    if (lane_id() == 0) {
-        wacc[w] = A;
-        wcnt[w] = ncorr;
+    const int nw = (span + kWave - 1) / kWave;
+    if (w < nw) {
+        const int top = span >= kWave ? kWave / 2 : span / 2;
+        for (int m = top; m >= lead; m >>= 1) A.merge_from_lane(m);
+        for (int m = top; m >= lead; m >>= 1) ncorr += __shfl_xor(ncorr, m, kWave);
+        if (lane_id() == 0) {
+            wacc[w] = A;
+            wcnt[w] = ncorr;
+        }
     }
     __syncthreads();
     if (threadIdx.x < 28) {
         const int k = threadIdx.x;
         double v = 0.0;
-        for (int i = 0; i < kIcpBlock / kWave; ++i) v += wacc[i].value(k);
+        for (int i = 0; i < nw; ++i) v += wacc[i].value(k);
         out[k] = v;
     } else if (threadIdx.x == 28) {
         int c = 0;
-        for (int i = 0; i < kIcpBlock / kWave; ++i) c += wcnt[i];
+        for (int i = 0; i < nw; ++i) c += wcnt[i];
         out[28] = (double)c;
     }
 }
@@ -263,6 +280,27 @@ __device__ __forceinline__ void sum_partials(const double *partials, int n, doub
         double a = 0.0;
 #pragma unroll 8
         for (int b = grp; b < n; b += 8) a += partials[(int64_t)b * kPartStride + v];
+        part[grp][v] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNSys) {
+        double a = 0.0;
+        for (int g = 0; g < 8; ++g) a += part[g][threadIdx.x];
+        sys[threadIdx.x] = a;
+    }
+    __syncthreads();
+}
+
+// The same fixed-order sum over partials written with store_coherent (coherent loads, so the
+// reader needs no cache invalidation).
+__device__ __forceinline__ void sum_partials_coherent(const double *partials, int n, double *sys)
+{
+    __shared__ double part[8][kPartStride];
+    const int v = threadIdx.x % kPartStride, grp = threadIdx.x / kPartStride;
+    if (v < kNSys) {
+        double a = 0.0;
+#pragma unroll 8
+        for (int b = grp; b < n; b += 8) a += load_coherent(partials + (int64_t)b * kPartStride + v);
         part[grp][v] = a;
     }
     __syncthreads();
@@ -390,44 +428,63 @@ __device__ __forceinline__ int32_t nn_group(const PairDesc &P, float qx, float q
 }
 
 // ---- one work unit --------------------------------------------------------------------------
-// Unit `vb` of a state covers source points i0 + k, i0 = (vb + j * nvb) * groups, k < groups
-// (one query per group of G lanes; the group leader accumulates).  Leaves the FP64 partial of
-// the unit in sys[0..28] (LDS; entry k written by thread k).
+// With Q = 256 / G queries per pass, unit `vb` of a state covers the passes of source points
+// [b * Q, b * Q + Q) for b = vb, vb + nvb, ...  Per pass:
+//   1. nearest neighbours: G lanes per query; the group leader parks the matched slot in LDS;
+//   2. lanes 0 .. Q-1 form the residuals and Jacobians of one query each and accumulate them;
+//   3. the pass's block partial is added, in pass order and FP64, to the unit's.
+// The accumulator is never live during the search, which keeps register use (and so the
+// number of resident wavefronts hiding the search's load latency) in check.  Leaves the unit's
+// partial in sys[0..28] (LDS; entry k written by thread k).
 template <int MODE>
 __device__ __forceinline__ void icp_unit(const PairDesc &P, int n, int G, int nvb, int ntgt, int vb, bool warm,
                                          const double *T, double *sys)
 {
+    __shared__ int32_t slot[kIcpBlock];
+    __shared__ double bsys[kPartStride];
     const int lg = __builtin_ctz((unsigned)G);
     const int g = (int)threadIdx.x & (G - 1);
-    const int groups = kIcpBlock >> lg;
-    Acc<MODE> A;
-    A.zero();
-    int ncorr = 0;
-    for (int i0 = vb * groups; i0 < n; i0 += nvb * groups) {
-        const int i = i0 + ((int)threadIdx.x >> lg);
-        const bool active = i < n;
-        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        float x = NAN, y = 0.f, z = 0.f;
-        int32_t sp = -1;
-        if (active) {
-            p = P.src[i];
-            if (warm) sp = P.nnprev[i];
-            xform_f32(T, p, x, y, z);
+    const int Q = kIcpBlock >> lg;
+    bool first = true;
+    for (int b0 = vb * Q; b0 < n; b0 += nvb * Q) {
+        {
+            const int j = (int)threadIdx.x >> lg;
+            const int i = b0 + j;
+            float x = NAN, y = 0.f, z = 0.f;
+            int32_t sp = -1;
+            if (i < n) {
+                if (warm) sp = P.nnprev[i];
+                xform_f32(T, P.src[i], x, y, z);
+            }
+            // the warm-start slot is only a bound: any in-range value is safe, stale or not
+            if ((uint32_t)sp >= (uint32_t)ntgt) sp = -1;
+            const int32_t s = nn_group(P, x, y, z, g, G, sp);
+            if (g == 0) slot[j] = s;
         }
-        // the warm-start slot is only a bound: any in-range value is safe, stale or not
-        if ((uint32_t)sp >= (uint32_t)ntgt) sp = -1;
-        const int32_t s = nn_group(P, x, y, z, g, G, sp);
-        if (active && g == 0) {
+        __syncthreads();
+        Acc<MODE> A;
+        A.zero();
+        int ncorr = 0;
+        const int i = b0 + (int)threadIdx.x;
+        if ((int)threadIdx.x < Q && i < n) {
+            const int32_t s = slot[threadIdx.x];
             P.nnprev[i] = s;
             if (s >= 0) {
+                const float4 p = P.src[i];
+                float x, y, z;
+                xform_f32(T, p, x, y, z);
                 Terms t;
                 corr_terms(x, y, z, p.w, P.sxyzi[s], P.snrm[s], P.sgrad[s], P.sl, P.sc, t);
                 acc_terms(A, t);
-                ++ncorr;
+                ncorr = 1;
             }
         }
+        block_partial<MODE>(A, ncorr, 1, Q, first ? sys : bsys);
+        if (!first && threadIdx.x < kNSys) sys[threadIdx.x] += bsys[threadIdx.x];
+        first = false;
+        __syncthreads();  // slot[] and the block-partial scratch are reused by the next pass
     }
-    block_partial<MODE>(A, ncorr, G, sys);
+    if (first && threadIdx.x < kNSys) sys[threadIdx.x] = 0.0;  // no point in this unit
 }
 
 // Element t of the Euler increment Rz(xi2) Ry(xi1) Rx(xi0) | xi3..5 (same expressions as
@@ -453,7 +510,7 @@ __device__ __forceinline__ double euler_elem(int t, const double *trig, const do
     }
 }
 
-// Epoch finish of state p (whole workgroup, after the acquire that follows the last ticket):
+// Epoch finish of state p (whole workgroup, run by the workgroup that took the last ticket):
 // fixed-order FP64 sum of the unit partials, LDL^T solve, Euler increment, T <- dT T, the
 // stopping rule of color_icp.cu:216-228, then the hand-over: the next epoch of this state, or
 // the pose to the next level of the chain.  A failed solve takes the identity step (which also
@@ -465,18 +522,19 @@ __device__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double
     __shared__ double f_crot, f_ct2;
     __shared__ int f_iter, f_max, f_fail, f_next, f_ok, f_done;
     const int t = threadIdx.x;
-    // state fields load alongside the partial sums
+    // state fields load alongside the partial sums (coherent: the previous epoch's finisher may
+    // have run on another XCD)
     if (t >= 64 && t < 80) {
-        Tc[t - 64] = st->T[t - 64];
+        Tc[t - 64] = load_coherent(&st->T[t - 64]);
     } else if (t == 80) {
-        f_iter = st->iter;
-        f_max = st->max_iter;
-        f_fail = st->solver_failures;
-        f_next = st->next;
-        f_crot = st->conv_rot;
-        f_ct2 = st->conv_t2;
+        f_iter = load_coherent(&st->iter);
+        f_max = load_coherent(&st->max_iter);
+        f_fail = load_coherent(&st->solver_failures);
+        f_next = load_coherent(&st->next);
+        f_crot = load_coherent(&st->conv_rot);
+        f_ct2 = load_coherent(&st->conv_t2);
     }
-    sum_partials(partials, nvb, sys);
+    sum_partials_coherent(partials, nvb, sys);
     if (t == 0) f_ok = solve6(sys, sys + 21, xi) ? 1 : 0;
     __syncthreads();
     if (t < 3) {
@@ -530,18 +588,20 @@ __device__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double
 }
 
 // Unit partial -> global, ticket, and the epoch finish by the workgroup completing the epoch.
+// The partial is written through to the coherence point and is complete (vmcnt 0: threads
+// 0..28 and the ticket-taking thread 0 are all in wavefront 0) before the ticket, which is all
+// the finisher needs; no L2 write-back or invalidation per unit.
 template <int MODE>
 __device__ __forceinline__ void unit_commit(const PairDesc &P, IcpState *states, int p, int vb, int nvb,
                                             uint32_t epoch, double *sys)
 {
     __shared__ bool last;
-    if (threadIdx.x < kNSys) P.partials[(int64_t)vb * kPartStride + threadIdx.x] = sys[threadIdx.x];
-    fence_release_agent();
+    if (threadIdx.x < kNSys) store_coherent(&P.partials[(int64_t)vb * kPartStride + threadIdx.x], sys[threadIdx.x]);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(&states[p].ticket, 1u) == (uint32_t)(nvb - 1);
     __syncthreads();
     if (!last) return;
-    fence_acquire_agent();
     icp_finish(states, p, epoch, P.partials, nvb, sys);
 }
 
@@ -551,8 +611,14 @@ __device__ __forceinline__ void unit_commit(const PairDesc &P, IcpState *states,
 // last served, claim a unit with one atomic, run it.  A workgroup only waits (sleep + re-poll)
 // when every unit of every open epoch is claimed, i.e. held by running workgroups, so progress
 // never depends on how many workgroups are resident.  Exits when every state is done.
+// (Claiming first and scanning only on failure was measured slower: every workgroup that ends
+// a unit after the epoch is drained adds a failing atomic on the hot queue word.)
+// 4 wavefronts per SIMD (<= 128 VGPRs): resident wavefronts are what hides the latency of the
+// neighbour search's dependent loads.
+#define GCA_ICP_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 8)))
+
 template <int MODE>
-__global__ __launch_bounds__(kIcpBlock) void k_icp_persist(const PairDesc *__restrict__ pairs, IcpState *states,
+__global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(const PairDesc *__restrict__ pairs, IcpState *states,
                                                            int n_states, uint32_t *err)
 {
     __shared__ double T[12];
@@ -618,7 +684,7 @@ __global__ __launch_bounds__(kIcpBlock) void k_icp_persist(const PairDesc *__res
 // ---- one launch per iteration (same units, same finish) -------------------------------------
 // Workgroup b runs unit b - unit_first[s] of state s = unit_state[b]; states that are done exit.
 template <int MODE>
-__global__ __launch_bounds__(kIcpBlock) void k_icp_iter(const PairDesc *__restrict__ pairs, IcpState *states,
+__global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_iter(const PairDesc *__restrict__ pairs, IcpState *states,
                                                         const int32_t *__restrict__ unit_state,
                                                         const int32_t *__restrict__ unit_first)
 {
@@ -664,7 +730,7 @@ __global__ __launch_bounds__(kIcpBlock) void k_build_gn(PairDesc P, const float4
     }
     __shared__ double sys[kPartStride];
     __shared__ bool last;
-    block_partial<MODE>(A, ncorr, 1, sys);
+    block_partial<MODE>(A, ncorr, 1, kIcpBlock, sys);
     if (threadIdx.x < kNSys) P.partials[(int64_t)blockIdx.x * kPartStride + threadIdx.x] = sys[threadIdx.x];
     fence_release_agent();
     __syncthreads();
@@ -872,10 +938,17 @@ static int32_t icp_blocks(int64_t n) { return (int32_t)std::min<int64_t>(1024, s
 // Lanes per correspondence query by source size: wide groups shorten each query's dependent
 // load chain, which is what bounds small clouds; narrow groups keep large clouds' wave count
 // reasonable.
-static void icp_shape(gca_ctx ctx, int32_t lanes[3], int32_t *unit_cap)
+// A call with one pair is latency-bound (the GPU is wider than a small cloud's queries): wide
+// lane groups shorten each query's search.  A call with many pairs is throughput-bound: one
+// lane per query wastes no lane on short candidate lists.  (Measured defaults; the autotuner
+// overrides them.)
+static void icp_shape(gca_ctx ctx, int n_pairs, int32_t lanes[3], int32_t *unit_cap)
 {
-    static const int32_t def[3] = {8, 4, 4};
-    for (int k = 0; k < 3; ++k) lanes[k] = ctx->tune[GCA_TUNE_ICP_LANES_SMALL + k] ? ctx->tune[GCA_TUNE_ICP_LANES_SMALL + k] : def[k];
+    static const int32_t def[3] = {16, 2, 4};
+    for (int k = 0; k < 3; ++k) {
+        if (n_pairs > 1) lanes[k] = ctx->tune[GCA_TUNE_ICP_LANES_BATCH] ? ctx->tune[GCA_TUNE_ICP_LANES_BATCH] : 1;
+        else lanes[k] = ctx->tune[GCA_TUNE_ICP_LANES_SMALL + k] ? ctx->tune[GCA_TUNE_ICP_LANES_SMALL + k] : def[k];
+    }
     *unit_cap = std::min(kMaxUnits, ctx->tune[GCA_TUNE_ICP_MAX_UNITS] ? ctx->tune[GCA_TUNE_ICP_MAX_UNITS] : 512);
 }
 
@@ -1005,7 +1078,7 @@ static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_clou
     for (int s = 0; s < n_states; ++s) GCA_TRY(grid_gather_attributes(ctx, tgts[s]));
 
     int32_t lanes[3], unit_cap;
-    icp_shape(ctx, lanes, &unit_cap);
+    icp_shape(ctx, n_pairs, lanes, &unit_cap);
     const bool per_iter = ctx->tune[GCA_TUNE_ICP_LAUNCH] == 2;
     std::vector<int32_t> nvb(n_states, 0);
     int64_t table_units = 0;
@@ -1088,9 +1161,8 @@ static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_clou
 
     std::vector<IcpState> S(n_states);
     uint32_t err = 0;
-    // results only (the queue and ticket lines after them stay on the device)
-    GCA_HIP(hipMemcpy2DAsync(S.data(), sizeof(IcpState), w.states, sizeof(IcpState), offsetof(IcpState, word),
-                             n_states, hipMemcpyDeviceToHost, ctx->stream));
+    // one contiguous copy (a strided copy into pageable memory costs a transfer per row)
+    GCA_HIP(hipMemcpyAsync(S.data(), w.states, sizeof(IcpState) * n_states, hipMemcpyDeviceToHost, ctx->stream));
     GCA_HIP(hipMemcpyAsync(&err, w.err, sizeof(err), hipMemcpyDeviceToHost, ctx->stream));
     GCA_HIP(hipStreamSynchronize(ctx->stream));
     if (err) return GCA_ERR_DEVICE;

@@ -473,7 +473,7 @@ static unsigned grid_for(int64_t items, int per_block, int64_t cap)
     return (unsigned)std::min<int64_t>(cap, std::max<int64_t>(1, (items + per_block - 1) / per_block));
 }
 
-static int normals_lanes(gca_ctx ctx) { return ctx->tune[GCA_TUNE_NORMALS_LANES] ? ctx->tune[GCA_TUNE_NORMALS_LANES] : 16; }
+static int normals_lanes(gca_ctx ctx) { return ctx->tune[GCA_TUNE_NORMALS_LANES] ? ctx->tune[GCA_TUNE_NORMALS_LANES] : 8; }
 
 template <int KIND>
 static int32_t launch_moments_pipeline(gca_ctx ctx, gca_cloud c, float radius, int32_t K, float4 *out)

@@ -44,7 +44,9 @@ class Scene:
     background: tuple
 
 
-def make_scene(seed: int) -> Scene:
+def make_scene(seed: int, scale: float = 1.0) -> Scene:
+    """Seeded scene; `scale` multiplies every distance (a camera at the origin sees the same
+    image, `scale` times farther away, over `scale`^2 times the surface area)."""
     rng = np.random.default_rng(np.random.PCG64(seed))
 
     def colour():
@@ -64,6 +66,9 @@ def make_scene(seed: int) -> Scene:
         z = rng.uniform(1.0, 2.2)
         c = np.array([rng.uniform(-0.45, 0.45) * z, rng.uniform(-0.3, 0.3) * z, z])
         spheres.append((c, rng.uniform(0.12, 0.3), colour()))
+    if scale != 1.0:
+        planes = [(n, d * scale, col) for n, d, col in planes]
+        spheres = [(c * scale, r * scale, col) for c, r, col in spheres]
     return Scene(planes, spheres, colour())
 
 
@@ -150,23 +155,24 @@ def frame_pair(seed: int, width: int, height: int, max_angle_deg=10.0, max_trans
 
 
 def sample_surface_pair(seed: int, n_points: int, max_angle_deg=15.0, max_trans=0.2,
-                        noise_m=0.002, colour_noise=10):
+                        noise_m=0.002, colour_noise=10, scale=1.0):
     """Two independent samplings (n_points each) of one scene's visible surfaces.
 
     Returns (src_xyz, src_rgb, tgt_xyz, tgt_rgb, G) with G mapping source to target.  Used
     for batched-registration workloads where each pair has its own point count.  Points are
-    pixels of a virtual camera whose resolution gives about n_points valid pixels.
+    pixels of a virtual camera whose resolution gives about n_points valid pixels; `scale`
+    scales the scene (make_scene).
     """
     cam_w = int(max(64, math.sqrt(n_points * 4.0 / 3.0 / 0.9)))
     cam_h = int(cam_w * 3 // 4)
     cam = CameraModel(cam_w, cam_h, 0.9609375 * cam_w, 0.9609375 * cam_w, cam_w / 2.0, cam_h / 2.0)
-    scene = make_scene(seed)
+    scene = make_scene(seed, scale)
     G = random_rigid(seed + 104729, max_angle_deg, max_trans)
     out = []
     for k, pose in enumerate((G, np.eye(4))):
         rng = np.random.default_rng(np.random.PCG64(seed * 31 + k))
         z, obj = _raycast(scene, cam, pose)
-        valid = np.isfinite(z) & (z >= 0.5) & (z <= 3.0)
+        valid = np.isfinite(z) & (z >= 0.5 * scale) & (z <= 3.0 * scale)
         ys, xs = np.nonzero(valid)
         sel = rng.permutation(len(xs))[:n_points]
         ys, xs = ys[sel], xs[sel]

@@ -160,14 +160,16 @@ int32_t gca_ctx_profile(gca_ctx ctx, int32_t enable);  /* enable != 0 also clear
  * Thrust picks).  Values are chosen by the autotuner (cuda_3d_amd/autotune.py); 0 = built-in
  * heuristic.  Results do not depend on the tuning beyond the last bits of FP64 reductions. */
 enum {
-    GCA_TUNE_ICP_LANES_SMALL = 0,  /* lanes per correspondence query, clouds < 16k points: 1,2,4,8,16,32,64 */
+    GCA_TUNE_ICP_LANES_SMALL = 0,  /* lanes per correspondence query in one-pair calls, clouds < 16k
+                                      points: 1,2,4,8,16,32,64 */
     GCA_TUNE_ICP_LANES_MEDIUM = 1, /* ... clouds < 128k points */
     GCA_TUNE_ICP_LANES_LARGE = 2,  /* ... larger clouds */
     GCA_TUNE_ICP_MAX_UNITS = 3,    /* cap on work units (FP64 partials) per pair and iteration, 1..1024 */
     GCA_TUNE_NORMALS_LANES = 4,    /* lanes per normal/gradient query: 1,2,4,8,16,32,64 */
     GCA_TUNE_ICP_PERSIST_BLOCKS = 5, /* workgroups of the persistent ICP launch (0 = 2 per CU), 1..65536 */
     GCA_TUNE_ICP_LAUNCH = 6,       /* 0/1 = one persistent launch per call, 2 = one launch per iteration */
-    GCA_TUNE_COUNT = 7
+    GCA_TUNE_ICP_LANES_BATCH = 7,  /* lanes per correspondence query in calls with several pairs (0 = 1) */
+    GCA_TUNE_COUNT = 8
 };
 int32_t gca_ctx_set_tuning(gca_ctx ctx, int32_t key, int32_t value);
 int32_t gca_ctx_get_tuning(gca_ctx ctx, int32_t key, int32_t *value);

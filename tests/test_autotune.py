@@ -6,11 +6,12 @@ from cuda_3d_amd import autotune
 
 
 class FakeCtx:
-    """Stage times: normals minimal at lanes 16, ICP separable with minimum at (4, 8, 2, 256)."""
+    """Stage times: normals minimal at lanes 16, ICP separable with minimum at lanes (4, 8, 2),
+    256 units, batch lanes 2."""
 
     def __init__(self):
         self.t = {"normals_lanes": 0, "icp_lanes_small": 0, "icp_lanes_medium": 0, "icp_lanes_large": 0,
-                  "icp_max_units": 0, "icp_persist_blocks": 0, "icp_launch": 0}
+                  "icp_max_units": 0, "icp_persist_blocks": 0, "icp_launch": 0, "icp_lanes_batch": 0}
         self.on = False
         self.calls = 0
 
@@ -32,7 +33,8 @@ class FakeCtx:
         lg = lambda v: math.log2(max(v, 1))  # noqa: E731
         normals = 1.0 + (lg(t["normals_lanes"]) - 4) ** 2
         icp = (3.0 + (lg(t["icp_lanes_small"]) - 2) ** 2 + (lg(t["icp_lanes_medium"]) - 3) ** 2
-               + (lg(t["icp_lanes_large"]) - 1) ** 2 + abs(lg(t["icp_max_units"]) - 8) + 0.01 * t["icp_persist_blocks"])
+               + (lg(t["icp_lanes_large"]) - 1) ** 2 + abs(lg(t["icp_max_units"]) - 8) + 0.01 * t["icp_persist_blocks"]
+               + (lg(t["icp_lanes_batch"]) - 1) ** 2)
         return {"normals": (3, normals), "gradient": (3, 0.5), "icp": (1, icp)}
 
 
@@ -44,6 +46,7 @@ def test_coordinate_descent_finds_separable_minimum():
     assert (best["icp_lanes_small"], best["icp_lanes_medium"], best["icp_lanes_large"]) == (4, 8, 2)
     assert best["icp_max_units"] == 256
     assert best["icp_persist_blocks"] == 128  # smallest candidate: time grows with it
+    assert best["icp_lanes_batch"] == 2
     # the ctx is left at the best settings, and every measurement is recorded
     assert all(ctx.t[k] == v for k, v in best.items())
     assert len(rows) >= sum(len(v) for v in autotune.space(256).values())

@@ -394,20 +394,47 @@ def single_align(src, tgt, iters=30, T0=None):
     return icp.get_final_transformation_matrix(), icp.report
 
 
-def test_batch_matches_single_pairs(uneven_pairs):
+class tuned:
+    """Context manager: ctx tuning knobs set for the block, back to the heuristics after."""
+
+    def __init__(self, ctx, **kw):
+        self.ctx, self.kw = ctx, kw
+
+    def __enter__(self):
+        self.ctx.set_tuning(**self.kw)
+
+    def __exit__(self, *a):
+        self.ctx.set_tuning(**{k: 0 for k in self.kw})
+
+
+SAME_LANES = dict(icp_lanes_small=4, icp_lanes_medium=4, icp_lanes_large=4, icp_lanes_batch=4)
+
+
+def test_batch_matches_single_pairs(ctx, uneven_pairs):
     import cuda_3d_amd as g
 
     srcs, tgts = uneven_pairs
-    Tb, reps = g.batch_icp(srcs, tgts, icp_p(g, 30, 0.05), [np.eye(4)] * len(srcs))
-    for k in range(len(srcs)):
-        T, rep = single_align(srcs[k], tgts[k])
-        # each pair's work partition depends on its own size only: bit-identical
+    n = len(srcs)
+    # one-pair and batched calls with the same lanes per query: each pair's work partition
+    # depends on its own size only, so the poses are bit-identical
+    with tuned(ctx, **SAME_LANES):
+        Tb, reps = g.batch_icp(srcs, tgts, icp_p(g, 30, 0.05), [np.eye(4)] * n)
+        singles = [single_align(srcs[k], tgts[k]) for k in range(n)]
+    for k in range(n):
+        T, rep = singles[k]
         np.testing.assert_array_equal(Tb[k], T)
         # a degenerate pair may stop early (failed solve -> identity step -> converged, as in
         # the reference); batched and single runs must agree on it
         assert (reps[k].iterations, reps[k].solver_failures, reps[k].n_correspondences) == \
                (rep.iterations, rep.solver_failures, rep.n_correspondences)
         assert reps[k].rmse == rep.rmse
+    # default shapes (one lane per query when batched, wide groups for one pair): only the
+    # order of the FP64 partial sums differs
+    Tb, reps = g.batch_icp(srcs, tgts, icp_p(g, 30, 0.05), [np.eye(4)] * n)
+    for k in range(n):
+        T, rep = single_align(srcs[k], tgts[k])
+        np.testing.assert_allclose(Tb[k], T, rtol=0, atol=1e-11)
+        assert (reps[k].iterations, reps[k].n_correspondences) == (rep.iterations, rep.n_correspondences)
 
 
 def test_launch_strategies_identical(ctx, uneven_pairs):
@@ -418,11 +445,8 @@ def test_launch_strategies_identical(ctx, uneven_pairs):
     p = icp_p(g, 12, 0.05)
     out = {}
     for launch in (1, 2):
-        ctx.set_tuning(icp_launch=launch)
-        try:
+        with tuned(ctx, icp_launch=launch):
             out[launch] = g.batch_icp(srcs, tgts, p, [np.eye(4)] * len(srcs))
-        finally:
-            ctx.set_tuning(icp_launch=0)
     np.testing.assert_array_equal(out[1][0], out[2][0])
     assert [rep_key(r) for r in out[1][1]] == [rep_key(r) for r in out[2][1]]
 
@@ -442,21 +466,18 @@ def test_batch_multiscale_matches_sequential(ctx, frame):
         Tg.append(t)
     params = [icp_p(g, 30, 2.5 * v, conv=(1e-6, 1e-6)) for v in vs]
     for launch in (1, 2):
-        ctx.set_tuning(icp_launch=launch)
-        try:
+        with tuned(ctx, icp_launch=launch, **SAME_LANES):
             Tb, reps = g.batch_multiscale_icp([S, S], [Tg, Tg], params, [np.eye(4), frame.G], [2 * v for v in vs])
-        finally:
-            ctx.set_tuning(icp_launch=0)
-        for k, T0 in enumerate((np.eye(4), frame.G)):
-            T = T0
-            for l, v in enumerate(vs):
-                icp = g.ColorICP(30, 2.5 * v, 2 * v)  # reference stopping rule
-                icp.set_source_point_cloud(S[l])
-                icp.set_target_point_cloud(Tg[l])
-                assert icp.align(T)
-                T = icp.get_final_transformation_matrix()
-                assert rep_key(reps[k][l]) == rep_key(icp.report)
-            np.testing.assert_array_equal(Tb[k], T)
+            for k, T0 in enumerate((np.eye(4), frame.G)):
+                T = T0
+                for l, v in enumerate(vs):
+                    icp = g.ColorICP(30, 2.5 * v, 2 * v)  # reference stopping rule
+                    icp.set_source_point_cloud(S[l])
+                    icp.set_target_point_cloud(Tg[l])
+                    assert icp.align(T)
+                    T = icp.get_final_transformation_matrix()
+                    assert rep_key(reps[k][l]) == rep_key(icp.report)
+                np.testing.assert_array_equal(Tb[k], T)
         print(f"[gpu-parity] chained levels (launch {launch}) iterations "
               f"{[[r.iterations for r in row] for row in reps]}", flush=True)
 
@@ -485,8 +506,7 @@ def test_warm_start_matches_full_search(level, gn_case):
         np.testing.assert_allclose(dT @ Tk, Tk1, rtol=0, atol=1e-11)
 
 
-@pytest.mark.parametrize("shape", [dict(icp_lanes_small=1, icp_lanes_medium=1, icp_lanes_large=1),
-                                   dict(icp_lanes_small=64, icp_lanes_medium=32, icp_lanes_large=16),
+@pytest.mark.parametrize("shape", [dict(icp_lanes_batch=16), dict(icp_lanes_batch=64),
                                    dict(icp_max_units=1), dict(icp_max_units=7, icp_persist_blocks=3)])
 def test_shape_tuning_invariance(ctx, uneven_pairs, shape):
     """Launch shapes change only the FP64 summation order of the partials."""
@@ -495,14 +515,21 @@ def test_shape_tuning_invariance(ctx, uneven_pairs, shape):
     srcs, tgts = uneven_pairs
     p = icp_p(g, 10, 0.05)
     Tref, rref = g.batch_icp(srcs, tgts, p, [np.eye(4)] * len(srcs))
-    ctx.set_tuning(**shape)
-    try:
+    with tuned(ctx, **shape):
         T, r = g.batch_icp(srcs, tgts, p, [np.eye(4)] * len(srcs))
-    finally:
-        ctx.set_tuning(**{k: 0 for k in shape})
     np.testing.assert_allclose(T, Tref, rtol=0, atol=1e-10)
     assert [x.n_correspondences for x in r] == [x.n_correspondences for x in rref]
     assert [x.iterations for x in r] == [x.iterations for x in rref]
+
+
+@pytest.mark.parametrize("lanes", [1, 64])
+def test_single_pair_lanes_invariance(ctx, uneven_pairs, lanes):
+    srcs, tgts = uneven_pairs
+    Tref, rref = single_align(srcs[1], tgts[1], 10)
+    with tuned(ctx, icp_lanes_small=lanes, icp_lanes_medium=lanes, icp_lanes_large=lanes):
+        T, r = single_align(srcs[1], tgts[1], 10)
+    np.testing.assert_allclose(T, Tref, rtol=0, atol=1e-10)
+    assert (r.iterations, r.n_correspondences) == (rref.iterations, rref.n_correspondences)
 
 
 def test_determinism(frame):
