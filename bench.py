@@ -117,6 +117,23 @@ def run_batch(g, ctx, n_pairs, levels, iters, seed=2026):
     }
 
 
+def copy_calibration(nbytes: int, reps: int = 5):
+    """Plain device copies of known size: what the HBM counters read per copy is the scale for
+    the ICP kernel's counter readings."""
+    import torch
+
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes": nbytes, "copies": reps, "ms_per_copy": dt * 1e3,
+                      "GB_per_s": 2 * nbytes / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,10 +143,16 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-pairs", type=int, default=64)
     ap.add_argument("--no-roofline-pass", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of BYTES (reference traffic for rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
 
     import numpy as np
     import torch
