@@ -755,6 +755,25 @@ int32_t gca_nn_search(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int
     return launch_nn(ctx, q, ref, radius, out_dev);
 }
 
+int32_t gca_nn_search_host(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t *out_host,
+                           int64_t capacity, int64_t *n_out)
+{
+    if (!ctx || !q || !ref || !n_out || capacity < 0) return GCA_ERR_INVALID_VALUE;
+    if (!(radius > 0.0f)) return GCA_ERR_INVALID_VALUE;  // point_cloud.cu:512-516
+    GCA_TRY(cloud_sync_header(q));
+    const int64_t n = q->host_hdr.n;
+    *n_out = n;
+    if (n > capacity || (n > 0 && !out_host)) return GCA_ERR_INVALID_VALUE;
+    if (n == 0) return GCA_OK;
+    int32_t *dev = (int32_t *)ctx_scratch(ctx, 21, sizeof(int32_t) * n, false);
+    if (!dev) return GCA_ERR_OUT_OF_MEMORY;
+    GCA_TRY(grid_build(ctx, ref, radius));
+    GCA_TRY(launch_nn(ctx, q, ref, radius, dev));
+    GCA_HIP(hipMemcpyAsync(out_host, dev, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+    GCA_HIP(hipStreamSynchronize(ctx->stream));
+    return GCA_OK;
+}
+
 int32_t gca_radius_search(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t max_nn,
                           int32_t *offsets_dev, int32_t *idx_dev, int64_t idx_capacity,
                           int64_t *total)

@@ -10,8 +10,9 @@
 // reference's FP32 operation order.  Results go to a per-pixel staging buffer and each block
 // records its count of valid pixels.
 // Kernel B: exclusive scan of the block counts (raster order == block order).
-// Kernel C: stable compaction (wave ballot + in-block prefix) into the SoA cloud, plus the
-// bounding box (block min/max, one atomic per block and axis).
+// Kernel C: stable compaction (wave ballot + in-block prefix) into the SoA cloud, plus each
+// block's min/max into its own slot.
+// Kernel D: one workgroup folds the block bounds into the cloud header (no contended atomics).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

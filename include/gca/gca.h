@@ -244,6 +244,13 @@ int32_t gca_radius_outlier_removal(gca_ctx ctx, gca_cloud in, float radius,
 int32_t gca_nn_search(gca_ctx ctx, gca_cloud query, gca_cloud reference, float radius,
                       int32_t *out_dev);
 
+/* The same, into host memory (capacity int32 entries); *n_query receives the query count
+ * (GCA_ERR_INVALID_VALUE if it exceeds capacity).  Synchronises.  Reference: the host-vector
+ * overload point_cloud::nn_search(std::vector<index_t>&, ...) (point_cloud.hpp:112-114,
+ * point_cloud.cu:540-557). */
+int32_t gca_nn_search_host(gca_ctx ctx, gca_cloud query, gca_cloud reference, float radius,
+                           int32_t *out_host, int64_t capacity, int64_t *n_query);
+
 /* All neighbours within radius (max_nn = 0) or the max_nn nearest, each query's list sorted
  * by (squared distance, index).  offsets_dev: int32[n_query+1] device (CSR), idx_dev:
  * int32[idx_capacity] device.  *total receives the number of indices; if it exceeds

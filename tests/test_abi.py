@@ -73,3 +73,14 @@ def test_host_solver_matches_closed_form():
 def test_context_creation_fails_cleanly_without_device():
     h = C.c_void_p()
     assert L.lib.gca_ctx_create(0, C.byref(h)) == L.GCA_ERR_DEVICE
+
+
+def test_cpp_facade_builds_and_links():
+    """The C++ facade (include/gca/gca.hpp) compiles warning-free against gca.h and links
+    with libgca.so; the example prints its usage without touching a device."""
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    r = subprocess.run(["make", "-s", "-B", "-C", cpp], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert "warning" not in r.stderr
+    r = subprocess.run([os.path.join(cpp, "facade_icp")], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr

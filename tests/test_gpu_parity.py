@@ -589,3 +589,35 @@ def test_autograd_function_fd(level, gn_case):
         src_xyzi.grad = None
     print(f"[gpu-parity] autograd vs central FD (h=1e-4) worst relative error {worst:.2e}", flush=True)
     assert worst < 1e-5
+
+
+def test_cpp_facade_matches_python(ctx, tmp_path):
+    """gca::color_icp through the C++ facade == ColorICP through the Python mirror (same ABI
+    calls, same defaults, so the same bits), run as a separate process."""
+    import json
+    import os
+    import subprocess
+
+    import cuda_3d_amd as g
+    from cuda_3d_amd import scene
+
+    sx, sr, tx, tr, G = scene.sample_surface_pair(7, 6000)
+    path = tmp_path / "clouds.bin"
+    with open(path, "wb") as fh:
+        np.array([len(sx), len(tx)], np.int64).tofile(fh)
+        for a in (sx, sr, tx, tr):
+            np.ascontiguousarray(a, np.float32).tofile(fh)
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "facade_icp")
+    r = subprocess.run([exe, str(path), "0.05", "30", "0.05", "0.04", "30"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    s = g.PointCloud.from_numpy(sx, sr, ctx=ctx)
+    t = g.PointCloud.from_numpy(tx, tr, ctx=ctx)
+    t.estimate_normals(0.05, 30)
+    icp = g.ColorICP(30, 0.05, 0.04)
+    icp.set_source_point_cloud(s)
+    icp.set_target_point_cloud(t)
+    assert icp.align()
+    np.testing.assert_array_equal(np.array(out["T"]).reshape(4, 4), icp.get_final_transformation_matrix())
+    assert out["iterations"] == icp.report.iterations and out["n_src"] == len(sx) == out["moved_src"]
