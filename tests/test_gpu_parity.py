@@ -621,3 +621,104 @@ def test_cpp_facade_matches_python(ctx, tmp_path):
     assert icp.align()
     np.testing.assert_array_equal(np.array(out["T"]).reshape(4, 4), icp.get_final_transformation_matrix())
     assert out["iterations"] == icp.report.iterations and out["n_src"] == len(sx) == out["moved_src"]
+
+
+# ---- precision variant and differentiation at their stated sizes -----------------------------------
+
+def test_precision_variant_1280x720(ctx):
+    """1280x720 (~9e5 points), voxel pyramid down to 0.01 m (~2.5e5 points), 3-level colored
+    ICP with 30 iterations per level: pose error against the FP64 oracle for compensated FP32
+    partials (default), the naive-FP32 ablation, and FP64 partials."""
+    import cuda_3d_amd as g
+    from cuda_3d_amd import scene
+
+    W, H = 1280, 720
+    (td, tc), (sd, sc), G, cam = scene.frame_pair(13, W, H)
+    camg = g.make_camera(W, H, cam.fx, cam.fy, cam.cx, cam.cy)
+    camo = O.make_camera(W, H, cam.fx, cam.fy, cam.cx, cam.cy)
+    bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
+    tgt = g.PointCloud.create_from_rgbd(td, tc, camg, bil, 0.2, 3.5, ctx=ctx)
+    src = g.PointCloud.create_from_rgbd(sd, sc, camg, bil, 0.2, 3.5, ctx=ctx)
+    vs = (0.04, 0.02, 0.01)
+    lo = [O.LevelParams(v, 2.5 * v, 30, 2 * v, 0, O.icp_params(30, 2.5 * v, LAM, 0.0, 0.0)) for v in vs]
+    otx, otr, _, _ = O.cloud_from_rgbd(td, tc, camo, O.Bilateral(1, 2, 2.0, 30.0), 0.2, 3.5)
+    osx, osr, _, _ = O.cloud_from_rgbd(sd, sc, camo, O.Bilateral(1, 2, 2.0, 30.0), 0.2, 3.5)
+    print(f"[gpu-parity] 1280x720: {src.points_number()} / {tgt.points_number()} points; oracle running", flush=True)
+    To, _ = O.multiscale_icp(osx, osr, otx, otr, lo)
+    errs = {}
+    for mode, name in ((0, "compensated"), (1, "naive_fp32"), (2, "fp64")):
+        lv = [g.level_params(v, icp=icp_p(g, 30, 2.5 * v, mode)) for v in vs]
+        T, reps = g.multiscale_icp(src, tgt, lv)
+        errs[name] = pose_err(T, To)
+    fine = tgt.voxel_grid_down_sample(0.01).points_number()
+    print(f"[gpu-parity] 1280x720 finest level {fine} points; pose error vs FP64 oracle (rad, m): {errs}", flush=True)
+    assert max(errs["compensated"]) <= 1e-5 and max(errs["fp64"]) <= 1e-5
+    assert all(np.isfinite(errs["naive_fp32"]))
+
+
+def test_differentiation_100k_32_poses(ctx):
+    """Adjoint of the colored-ICP objective on a 1e5-point pair at 32 seeded poses (rotation
+    <= 10 deg, translation <= 0.1 m), through the torch.autograd.Function: dE/dxi against central
+    differences (h = 1e-4) and the FP64 analytic gradient of the oracle; dE/d(x,y,z,I) against
+    central differences on sampled points."""
+    import torch
+
+    import cuda_3d_amd as g
+    from cuda_3d_amd import scene
+    from cuda_3d_amd.autograd import colored_icp_objective
+
+    sx, sr, tx, tr, G = scene.sample_surface_pair(31, 100000)
+    src = g.PointCloud.from_numpy(sx, sr, ctx=ctx)
+    tgt = g.PointCloud.from_numpy(tx, tr, ctx=ctx)
+    tgt.estimate_normals(0.03, 30)
+    tgt.compute_color_gradient(0.03, 0)
+    pad = lambda a: np.concatenate([a, np.zeros((len(a), 1), np.float32)], 1)  # noqa: E731  (float4 rows)
+    ox, on, og = tgt.download_xyzi(), pad(tgt.download_normals()), pad(tgt.download_gradient())
+    osx = src.download_xyzi()
+    pts = torch.as_tensor(osx, device="cuda")
+    work = g.PointCloud(ctx, len(osx))
+    rng = np.random.default_rng(2031)
+    worst = {"xi_fd": 0.0, "xi_fp64": 0.0, "pts_fd": 0.0}
+    h = 1e-4
+    for k in range(32):
+        T = scene.random_rigid(5000 + k, 10.0, 0.1) @ G
+        moved = src.copy()
+        moved.transform(T)
+        nn = g.PointCloud.nn_search(moved, tgt, 0.1)
+        nn_dev = g.device_tensor(nn)
+        xi = torch.zeros(6, dtype=torch.float64, device="cuda", requires_grad=True)
+        p = pts.clone().requires_grad_(True)
+        E = colored_icp_objective(xi, p, tgt, nn_dev, T, LAM, work)
+        E.backward()
+        gxi, gp = xi.grad.cpu().numpy(), p.grad.cpu().numpy()
+        # FP64 analytic gradient of the oracle at the same pose and correspondences
+        oE, odxi, _ = O.objective_grad(osx, ox, on, og, nn, LAM, T, np.zeros(6))
+        worst["xi_fp64"] = max(worst["xi_fp64"], np.abs(gxi - odxi).max() / np.abs(odxi).max())
+        fd = np.zeros(6)
+        for a in range(6):
+            e = np.zeros(6)
+            e[a] = h
+            Ep = colored_icp_objective(torch.tensor(e, device="cuda"), pts, tgt, nn_dev, T, LAM, work).item()
+            Em = colored_icp_objective(torch.tensor(-e, device="cuda"), pts, tgt, nn_dev, T, LAM, work).item()
+            fd[a] = (Ep - Em) / (2 * h)
+        worst["xi_fd"] = max(worst["xi_fd"], np.abs(gxi - fd).max() / np.abs(fd).max())
+        # per-point gradients: central differences on 4 matched points, each coordinate
+        matched = np.nonzero(np.asarray(nn) >= 0)[0]
+        for i in rng.choice(matched, 4, replace=False):
+            for c in range(4):
+                q = pts.clone()
+                base = float(osx[i, c])
+                up, dn = np.float32(base + h), np.float32(base - h)
+                q[i, c] = float(up)
+                Ep = colored_icp_objective(torch.zeros(6, dtype=torch.float64, device="cuda"), q, tgt, nn_dev, T,
+                                           LAM, work).item()
+                q[i, c] = float(dn)
+                Em = colored_icp_objective(torch.zeros(6, dtype=torch.float64, device="cuda"), q, tgt, nn_dev, T,
+                                           LAM, work).item()
+                fdp = (Ep - Em) / (float(up) - float(dn))
+                scale = max(abs(fdp), 1e-3 * np.abs(gp[i]).max(), 1e-12)
+                worst["pts_fd"] = max(worst["pts_fd"], abs(gp[i, c] - fdp) / scale)
+    print(f"[gpu-parity] adjoint on 1e5 points, 32 poses: worst relative error {worst}", flush=True)
+    assert worst["xi_fp64"] < 1e-8
+    assert worst["xi_fd"] < 1e-5
+    assert worst["pts_fd"] < 1e-3
