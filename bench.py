@@ -143,6 +143,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-pairs", type=int, default=64)
     ap.add_argument("--no-roofline-pass", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly (no hipGraph replay)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (reference traffic for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -168,7 +169,8 @@ def main():
     bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
     dev = [(g.device_tensor(td.view(np.int16)), g.device_tensor(tc)), (g.device_tensor(sd.view(np.int16)), g.device_tensor(sc))]
     ptrs = [(d.data_ptr(), c.data_ptr()) for d, c in dev]
-    pipe = FramePipeline(ctx, cam, bil, levels, iters)
+    pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=not args.no_graph)
+    pctx = pipe.ctx  # the pipeline's own stream, profiler and tuning
 
     k = 0
 
@@ -178,27 +180,37 @@ def main():
         pipe.step(d, c)
         k += 1
 
-    for _ in range(max(args.warmup, 2)):
+    for _ in range(max(args.warmup, 5)):  # eager steps of both frame parities, then graph capture
         step()
-    ctx.synchronize()
+    pctx.synchronize()
     torch.cuda.synchronize()
-    log(f"warmup done; timing {args.steps} steps")
+    log(f"warmup done; timing {args.steps} steps ({'graph replay' if not args.no_graph else 'eager'})")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-    ctx.synchronize()
-    torch.cuda.synchronize()
+    pctx.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     log(f"{ms:.3f} ms/frame")
+    # latency of one frame when the host waits for each pose
+    lat = []
+    for _ in range(min(args.steps, 20)):
+        t2 = time.perf_counter()
+        step()
+        pipe.result()
+        lat.append((time.perf_counter() - t2) * 1e3)
+    lat_ms = float(np.median(lat))
+    log(f"{lat_ms:.3f} ms per frame with a pose readback after each frame")
 
     sizes = [pipe.level_sizes(s) for s in (0, 1)]
-    full_n = [pipe.full[s].points_number() for s in (0, 1)]
+    full_n = [pipe.full_size(s) for s in (0, 1)]
     roof = None
     stages = None
     iters_seen = []
     if not args.no_roofline_pass:
-        ctx.profile(True)
+        # the same kernels launched eagerly, each stage bracketed by hipEvents on the stream
+        pipe.set_graph(False)
+        pctx.profile(True)
         algo = 0.0
         for _ in range(args.steps):
             new = pipe.slot
@@ -207,8 +219,9 @@ def main():
             iters_seen.append(it)
             src, tgt = new, 1 - new
             algo += sum(it[l] * (24.0 * sizes[src][l] + 48.0 * sizes[tgt][l]) for l in range(len(levels)))
-        prof = ctx.profile_read()
-        ctx.profile(False)
+        prof = pctx.profile_read()
+        pctx.profile(False)
+        pipe.set_graph(not args.no_graph)
         n_icp, icp_ms = prof["icp"]
         per_launch_bytes = algo / max(n_icp, 1)
         avg_launch_ms = icp_ms / max(n_icp, 1)
@@ -246,6 +259,7 @@ def main():
                                "stopping rule",
                    "width": W, "height": H, "levels": list(levels), "max_iters_per_level": iters,
                    "iterations_per_level": iters_seen[-2:] if iters_seen else None,
+                   "hip_graph": not args.no_graph, "latency_ms_per_frame_with_readback": lat_ms,
                    "points_full": full_n, "points_per_level": sizes},
         "roofline": roof, "cpu_baseline": cpu, "batch": batch, "stages": stages,
     }

@@ -125,6 +125,13 @@ _SIGS = {
     "gca_color_icp_batch": (i32, [vp, i32, vp, vp, P(IcpParams), f32, vp, vp, vp]),
     "gca_color_icp_batch_multiscale": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
     "gca_multiscale_icp": (i32, [vp, vp, vp, i32, P(LevelParams), vp, vp, vp]),
+    "gca_pipeline_create": (i32, [vp, P(Camera), P(Bilateral), f32, f32, i32, vp, P(vp)]),
+    "gca_pipeline_destroy": (i32, [vp]),
+    "gca_pipeline_step": (i32, [vp, vp, vp]),
+    "gca_pipeline_result": (i32, [vp, vp, vp, P(i32)]),
+    "gca_pipeline_set_graph": (i32, [vp, i32]),
+    "gca_pipeline_ctx": (i32, [vp, P(vp)]),
+    "gca_pipeline_sizes": (i32, [vp, i32, P(i64), vp]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)

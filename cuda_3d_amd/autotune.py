@@ -137,6 +137,7 @@ def main(argv=None):
     say = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if args.workload == "frame":
         wl = workloads.frame_workload(ctx, steps=args.steps)
+        ctx = wl.ctx  # the pipeline runs (and is tuned) on its own ctx
         what = (f"640x480 frame pipeline (3 levels, normals K30, 3-level colored ICP), {args.steps} frames "
                 f"per run, median of {args.reps}")
     else:

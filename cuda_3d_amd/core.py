@@ -87,10 +87,19 @@ class Context:
         check(lib.gca_ctx_profile_read(self.handle, _ptr(n), _ptr(ms)), "gca_ctx_profile_read")
         return {s: (int(n[i]), float(ms[i])) for i, s in enumerate(L.STAGES)}
 
+    @classmethod
+    def borrow(cls, handle, device: int = 0) -> "Context":
+        """A non-owning view of a ctx owned elsewhere (e.g. a frame pipeline's own ctx)."""
+        c = cls.__new__(cls)
+        c.handle = handle
+        c.device = device
+        c._borrowed = True
+        return c
+
     def close(self):
-        if getattr(self, "handle", None):
+        if getattr(self, "handle", None) and not getattr(self, "_borrowed", False):
             lib.gca_ctx_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

@@ -330,7 +330,7 @@ int32_t grid_build(gca_ctx ctx, gca_cloud c, float radius, bool exact)
 
 StageTimer::StageTimer(gca_ctx c, int stage) : ctx(c)
 {
-    if (!ctx->prof_on) return;
+    if (!ctx->prof_on || ctx->capturing) return;
     while (ctx->prof_pool.size() < ctx->prof_used + 2) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return;
@@ -439,6 +439,7 @@ int32_t gca_ctx_set_tuning(gca_ctx c, int32_t key, int32_t value)
     if (key == GCA_TUNE_ICP_PERSIST_BLOCKS && value > 65536) return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_ICP_LAUNCH && value > 2) return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_NORMALS_LANES && !pow2(value, 64)) return GCA_ERR_INVALID_VALUE;
+    if (c->tune[key] != value) ++c->tune_version;
     c->tune[key] = value;
     return GCA_OK;
 }

@@ -106,10 +106,14 @@ struct gca_ctx_s {
     size_t partials_bytes = 0;
     // per-level working clouds of the multi-scale driver (created on first use)
     gca_cloud level_clouds[16] = {nullptr};
-    // launch-shape tuning (GCA_TUNE_*), 0 = heuristic
+    // launch-shape tuning (GCA_TUNE_*), 0 = heuristic; the version counts changes (objects that
+    // bake shapes in, e.g. a frame pipeline's plans and graphs, rebuild when it moves)
     int32_t tune[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    // stage profiler: event pairs recorded on `stream` while enabled
+    uint64_t tune_version = 0;
+    // stage profiler: event pairs recorded on `stream` while enabled (not while the stream is
+    // being captured into a graph)
     bool prof_on = false;
+    bool capturing = false;
     struct ProfRec {
         int stage;
         hipEvent_t a, b;
@@ -168,6 +172,10 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
                          const gca_camera *cam, const gca_bilateral *bil, float tmin, float tmax,
                          gca_cloud out);
 int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out);
+// No host round trip: the lattice is computed on the device; the bitmap scan covers max_bits
+// cells (a lattice beyond that flags *err and gives an empty output); scratch by capacity.
+int32_t launch_voxel_planned(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, double max_bits,
+                             uint32_t *err);
 int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d);
 int32_t launch_nn(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t *out);
 int32_t launch_radius(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t max_nn,
@@ -182,6 +190,17 @@ int32_t launch_transform(gca_ctx ctx, gca_cloud c, const double T[16]);
 int32_t launch_bounds(gca_ctx ctx, gca_cloud c);
 int32_t launch_copy_cloud(gca_ctx ctx, gca_cloud src, gca_cloud dst);
 int32_t launch_upload_finish(gca_ctx ctx, gca_cloud c, int64_t n);
+
+// Registration plans (icp.hip): n_pairs chains of n_levels states set up once (descriptors,
+// buffers, pinned result copies); each launch enqueues set-up + one persistent launch + the
+// result copy without a host round trip, so it can be captured in a graph.  Results are read
+// after the ctx stream has been synchronised.
+struct IcpPlan;
+int32_t icp_plan_create(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud *srcs, const gca_cloud *tgts,
+                        const gca_icp_params *params, const float *grad_r, const double *T0, IcpPlan **out);
+int32_t icp_plan_launch(gca_ctx ctx, IcpPlan *plan);
+int32_t icp_plan_results(gca_ctx ctx, const IcpPlan *plan, double *T_out, gca_icp_report *reports);
+void icp_plan_destroy(IcpPlan *plan);
 
 // Host-side canonical grid geometry (same FP32 formulas as the oracle)
 void grid_params(const float bmin[3], const float bmax[3], float r, GridDesc *d);

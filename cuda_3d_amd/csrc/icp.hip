@@ -1024,13 +1024,16 @@ static PairDesc make_pair(gca_cloud src, gca_cloud tgt, double *partials, int32_
     return P;
 }
 
-static int32_t prepare_target(gca_ctx ctx, gca_cloud tgt, float corr, float grad_r)
+// keep_grid: any valid grid with cell >= corr is used as it is (the neighbour results do not
+// depend on it); otherwise a grid too coarse for corr is rebuilt for speed.
+static int32_t prepare_target(gca_ctx ctx, gca_cloud tgt, float corr, float grad_r, bool keep_grid)
 {
     if (!tgt->has_normals) return GCA_ERR_NOT_READY;
     if (!tgt->has_gradient) {
         if (!(grad_r > 0.0f)) return GCA_ERR_NOT_READY;
         GCA_TRY(gca_compute_color_gradient(ctx, tgt, grad_r, 0));
     }
+    if (keep_grid && tgt->grid.valid && tgt->grid.desc.cell >= corr) return GCA_OK;
     return grid_build(ctx, tgt, corr);
 }
 
@@ -1044,25 +1047,22 @@ static int32_t validate_params(const gca_icp_params *p)
 }
 
 template <int MODE>
-static void launch_persist(gca_ctx ctx, const IcpWork &w, int n_states, int blocks)
-{
-    hipLaunchKernelGGL(k_icp_persist<MODE>, dim3((unsigned)blocks), dim3(kIcpBlock), 0, ctx->stream, w.pairs,
-                       w.states, n_states, w.err);
-}
-
-template <int MODE>
 static void launch_iter(gca_ctx ctx, const IcpWork &w, int64_t off, int64_t units)
 {
     hipLaunchKernelGGL(k_icp_iter<MODE>, dim3((unsigned)units), dim3(kIcpBlock), 0, ctx->stream, w.pairs, w.states,
                        w.unit_state + off, w.unit_first);
 }
 
-// Registers n_pairs chains of n_levels states (state k * n_levels + l = pair k at level l,
-// coarse to fine): level l + 1 of a pair starts from the pose level l ends with.  Levels with
-// max_iterations == 0 are skipped.  T0 / T_out: n_pairs x 16; reports: n_states (may be NULL).
-static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud *srcs, const gca_cloud *tgts,
-                          const gca_icp_params *params, const float *grad_r, const double *T0, double *T_out,
-                          gca_icp_report *reps)
+// ---- registration states: shared set-up ----------------------------------------------------
+// State k * n_levels + l = pair k at level l (coarse to fine); level l + 1 of a pair starts from
+// the pose level l ends with; levels with max_iterations == 0 are skipped.
+
+// Validation, target preparation (gradient if absent, grid, grid-ordered attributes) and the
+// launch shape.  Every target's grid is built before any descriptor is made: a later build for
+// another state of the same target may replace arrays an earlier one set up.
+static int32_t states_prepare(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud *srcs, const gca_cloud *tgts,
+                              const gca_icp_params *params, const float *grad_r, bool keep_grid, int32_t lanes[3],
+                              int32_t *unit_cap, std::vector<int64_t> &nn_off)
 {
     const int n_states = n_pairs * n_levels;
     for (int l = 0; l < n_levels; ++l) {
@@ -1071,14 +1071,96 @@ static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_clou
     }
     for (int s = 0; s < n_states; ++s)
         if (!srcs[s] || !tgts[s] || srcs[s]->ctx != ctx || tgts[s]->ctx != ctx) return GCA_ERR_INVALID_VALUE;
-    // every target's grid first, then the descriptors: a later build for another state of the
-    // same target may replace arrays an earlier one set up
     for (int s = 0; s < n_states; ++s)
-        GCA_TRY(prepare_target(ctx, tgts[s], params[s % n_levels].max_correspondence_distance, grad_r[s % n_levels]));
+        GCA_TRY(prepare_target(ctx, tgts[s], params[s % n_levels].max_correspondence_distance, grad_r[s % n_levels],
+                               keep_grid));
     for (int s = 0; s < n_states; ++s) GCA_TRY(grid_gather_attributes(ctx, tgts[s]));
+    icp_shape(ctx, n_pairs, lanes, unit_cap);
+    nn_off.assign(n_states + 1, 0);
+    for (int s = 0; s < n_states; ++s) nn_off[s + 1] = nn_off[s] + srcs[s]->cap;
+    return GCA_OK;
+}
 
+// Descriptors and chain links of the states over the given partial / previous-match buffers.
+static void states_describe(int n_pairs, int n_levels, const gca_cloud *srcs, const gca_cloud *tgts,
+                            const gca_icp_params *params, const int32_t lanes[3], int32_t unit_cap, double *partials,
+                            int32_t *nnprev, const std::vector<int64_t> &nn_off, std::vector<PairDesc> &P,
+                            std::vector<StateInit> &I, std::vector<int32_t> &tail)
+{
+    const int n_states = n_pairs * n_levels;
+    P.assign(n_states, PairDesc{});
+    I.assign(n_states, StateInit{});
+    tail.assign(n_pairs, -1);  // last state of each chain
+    for (int k = 0; k < n_pairs; ++k) {
+        int prev = -1;
+        for (int l = 0; l < n_levels; ++l) {
+            const int s = k * n_levels + l;
+            const gca_icp_params &pl = params[l];
+            P[s] = make_pair(srcs[s], tgts[s], partials + (int64_t)s * unit_cap * kPartStride, nnprev + nn_off[s],
+                             pl.lambda_geometric, pl.max_correspondence_distance);
+            memcpy(P[s].lanes, lanes, sizeof(int32_t) * 3);
+            P[s].unit_cap = unit_cap;
+            I[s] = StateInit{pl.conv_rotation, pl.conv_translation_sq, pl.max_iterations, -1, 0, k};
+            if (pl.max_iterations > 0) {
+                if (prev < 0) I[s].open = 1;
+                else I[prev].next = s;
+                prev = s;
+            }
+        }
+        tail[k] = prev;
+    }
+}
+
+template <int MODE>
+static void launch_persist_mode(gca_ctx ctx, const PairDesc *pairs, IcpState *states, int n_states, uint32_t *err)
+{
+    const int blocks = ctx->tune[GCA_TUNE_ICP_PERSIST_BLOCKS] ? ctx->tune[GCA_TUNE_ICP_PERSIST_BLOCKS] : 2 * ctx->n_cu;
+    hipLaunchKernelGGL(k_icp_persist<MODE>, dim3((unsigned)blocks), dim3(kIcpBlock), 0, ctx->stream, pairs, states,
+                       n_states, err);
+}
+
+static void launch_persist_any(gca_ctx ctx, int mode, const PairDesc *pairs, IcpState *states, int n_states,
+                               uint32_t *err)
+{
+    StageTimer timer(ctx, GCA_STAGE_ICP);
+    if (mode == GCA_ACCUM_NAIVE_FP32) launch_persist_mode<GCA_ACCUM_NAIVE_FP32>(ctx, pairs, states, n_states, err);
+    else if (mode == GCA_ACCUM_FP64) launch_persist_mode<GCA_ACCUM_FP64>(ctx, pairs, states, n_states, err);
+    else launch_persist_mode<GCA_ACCUM_COMPENSATED>(ctx, pairs, states, n_states, err);
+}
+
+// Poses and reports from a host copy of the states.
+static void states_report(gca_ctx ctx, const IcpState *S, int n_pairs, int n_states, const std::vector<int32_t> &tail,
+                          const double *T0, double *T_out, gca_icp_report *reps)
+{
+    for (int k = 0; k < n_pairs; ++k)
+        memcpy(T_out + 16 * k, tail[k] >= 0 ? S[tail[k]].T : T0 + 16 * k, sizeof(double) * 16);
+    if (!reps) return;
+    long long t0 = S[0].t_init;
+    for (int s = 1; s < n_states; ++s) t0 = std::min(t0, S[s].t_init);
+    const double ticks_per_us = ctx->wall_khz > 0 ? ctx->wall_khz * 1e-3 : 0.0;
+    for (int s = 0; s < n_states; ++s) {
+        gca_icp_report &r = reps[s];
+        const double n = (double)S[s].n_src;
+        r.iterations = S[s].iter;
+        r.converged = S[s].converged;
+        r.solver_failures = S[s].solver_failures;
+        r.n_correspondences = (int32_t)S[s].sys[28];
+        r.rmse = n > 0 ? S[s].sys[27] / n : 0.0;  // color_icp.cu:231
+        r.fitness = n > 0 ? S[s].sys[28] / n : 0.0;
+        r.latency_us = ticks_per_us > 0 ? (double)(S[s].t_done - t0) / ticks_per_us : 0.0;
+    }
+}
+
+// One-shot registration (every call sets up, runs and reads back).  T0 / T_out: n_pairs x 16;
+// reports: n_states (may be NULL).
+static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud *srcs, const gca_cloud *tgts,
+                          const gca_icp_params *params, const float *grad_r, const double *T0, double *T_out,
+                          gca_icp_report *reps)
+{
+    const int n_states = n_pairs * n_levels;
     int32_t lanes[3], unit_cap;
-    icp_shape(ctx, n_pairs, lanes, &unit_cap);
+    std::vector<int64_t> nn_off;
+    GCA_TRY(states_prepare(ctx, n_pairs, n_levels, srcs, tgts, params, grad_r, false, lanes, &unit_cap, nn_off));
     const bool per_iter = ctx->tune[GCA_TUNE_ICP_LAUNCH] == 2;
     std::vector<int32_t> nvb(n_states, 0);
     int64_t table_units = 0;
@@ -1090,32 +1172,12 @@ static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_clou
             table_units += nvb[s];
         }
     }
-    std::vector<int64_t> nn_off(n_states + 1, 0);
-    for (int s = 0; s < n_states; ++s) nn_off[s + 1] = nn_off[s] + srcs[s]->cap;
     IcpWork w;
     GCA_TRY(icp_work(ctx, n_states, n_pairs, (int64_t)n_states * unit_cap, nn_off[n_states], table_units, w));
-
-    std::vector<PairDesc> P(n_states);
-    std::vector<StateInit> I(n_states);
-    std::vector<int32_t> tail(n_pairs, -1);  // last state of each chain
-    for (int k = 0; k < n_pairs; ++k) {
-        int prev = -1;
-        for (int l = 0; l < n_levels; ++l) {
-            const int s = k * n_levels + l;
-            const gca_icp_params &pl = params[l];
-            P[s] = make_pair(srcs[s], tgts[s], w.partials + (int64_t)s * unit_cap * kPartStride, w.nnprev + nn_off[s],
-                             pl.lambda_geometric, pl.max_correspondence_distance);
-            memcpy(P[s].lanes, lanes, sizeof(lanes));
-            P[s].unit_cap = unit_cap;
-            I[s] = StateInit{pl.conv_rotation, pl.conv_translation_sq, pl.max_iterations, -1, 0, k};
-            if (pl.max_iterations > 0) {
-                if (prev < 0) I[s].open = 1;
-                else I[prev].next = s;
-                prev = s;
-            }
-        }
-        tail[k] = prev;
-    }
+    std::vector<PairDesc> P;
+    std::vector<StateInit> I;
+    std::vector<int32_t> tail;
+    states_describe(n_pairs, n_levels, srcs, tgts, params, lanes, unit_cap, w.partials, w.nnprev, nn_off, P, I, tail);
     GCA_HIP(hipMemcpyAsync(w.pairs, P.data(), sizeof(PairDesc) * n_states, hipMemcpyHostToDevice, ctx->stream));
     GCA_HIP(hipMemcpyAsync(w.init, I.data(), sizeof(StateInit) * n_states, hipMemcpyHostToDevice, ctx->stream));
     GCA_HIP(hipMemcpyAsync(w.T0, T0, sizeof(double) * 16 * n_pairs, hipMemcpyHostToDevice, ctx->stream));
@@ -1126,11 +1188,7 @@ static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_clou
     const int mode = params[0].accumulate_mode;
     std::vector<int32_t> us, uf(n_states, 0);
     if (!per_iter) {
-        const int blocks = ctx->tune[GCA_TUNE_ICP_PERSIST_BLOCKS] ? ctx->tune[GCA_TUNE_ICP_PERSIST_BLOCKS] : 2 * ctx->n_cu;
-        StageTimer timer(ctx, GCA_STAGE_ICP);
-        if (mode == GCA_ACCUM_NAIVE_FP32) launch_persist<GCA_ACCUM_NAIVE_FP32>(ctx, w, n_states, blocks);
-        else if (mode == GCA_ACCUM_FP64) launch_persist<GCA_ACCUM_FP64>(ctx, w, n_states, blocks);
-        else launch_persist<GCA_ACCUM_COMPENSATED>(ctx, w, n_states, blocks);
+        launch_persist_any(ctx, mode, w.pairs, w.states, n_states, w.err);
         GCA_HIP(hipGetLastError());
     } else {
         // unit tables level by level: level l's launches cover the units of its n_pairs states
@@ -1166,24 +1224,103 @@ static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_clou
     GCA_HIP(hipMemcpyAsync(&err, w.err, sizeof(err), hipMemcpyDeviceToHost, ctx->stream));
     GCA_HIP(hipStreamSynchronize(ctx->stream));
     if (err) return GCA_ERR_DEVICE;
-    for (int k = 0; k < n_pairs; ++k)
-        memcpy(T_out + 16 * k, tail[k] >= 0 ? S[tail[k]].T : T0 + 16 * k, sizeof(double) * 16);
-    if (reps) {
-        long long t0 = S[0].t_init;
-        for (int s = 1; s < n_states; ++s) t0 = std::min(t0, S[s].t_init);
-        const double ticks_per_us = ctx->wall_khz > 0 ? ctx->wall_khz * 1e-3 : 0.0;
-        for (int s = 0; s < n_states; ++s) {
-            gca_icp_report &r = reps[s];
-            const double n = (double)S[s].n_src;
-            r.latency_us = ticks_per_us > 0 ? (double)(S[s].t_done - t0) / ticks_per_us : 0.0;
-            r.iterations = S[s].iter;
-            r.converged = S[s].converged;
-            r.solver_failures = S[s].solver_failures;
-            r.n_correspondences = (int32_t)S[s].sys[28];
-            r.rmse = n > 0 ? S[s].sys[27] / n : 0.0;  // color_icp.cu:231
-            r.fitness = n > 0 ? S[s].sys[28] / n : 0.0;
-        }
-    }
+    states_report(ctx, S.data(), n_pairs, n_states, tail, T0, T_out, reps);
+    return GCA_OK;
+}
+
+// ---- registration plans: set up once, launched many times without host round trips --------
+struct IcpPlan {
+    int n_pairs = 0, n_levels = 0, n_states = 0, mode = 0;
+    IcpState *states = nullptr;
+    double *partials = nullptr;
+    int32_t *nnprev = nullptr;
+    PairDesc *pairs = nullptr;
+    StateInit *init = nullptr;
+    double *T0 = nullptr;
+    uint32_t *err = nullptr;
+    IcpState *host_states = nullptr;  // pinned: written by the launch's copy, read after a sync
+    uint32_t *host_err = nullptr;
+    std::vector<int32_t> tail;
+    std::vector<double> T0h;
+};
+
+void icp_plan_destroy(IcpPlan *p)
+{
+    if (!p) return;
+    hipFree(p->states);
+    hipFree(p->partials);
+    hipFree(p->nnprev);
+    hipFree(p->pairs);
+    hipFree(p->init);
+    hipFree(p->T0);
+    hipFree(p->err);
+    hipHostFree(p->host_states);
+    hipHostFree(p->host_err);
+    delete p;
+}
+
+int32_t icp_plan_create(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud *srcs, const gca_cloud *tgts,
+                        const gca_icp_params *params, const float *grad_r, const double *T0, IcpPlan **out)
+{
+    *out = nullptr;
+    const int n_states = n_pairs * n_levels;
+    int32_t lanes[3], unit_cap;
+    std::vector<int64_t> nn_off;
+    // a plan keeps the targets' grids: later steps rebuild them in place with the same geometry
+    GCA_TRY(states_prepare(ctx, n_pairs, n_levels, srcs, tgts, params, grad_r, true, lanes, &unit_cap, nn_off));
+    IcpPlan *p = new (std::nothrow) IcpPlan();
+    if (!p) return GCA_ERR_OUT_OF_MEMORY;
+    p->n_pairs = n_pairs;
+    p->n_levels = n_levels;
+    p->n_states = n_states;
+    p->mode = params[0].accumulate_mode;
+    p->T0h.assign(T0, T0 + 16 * n_pairs);
+    auto fail = [&](hipError_t e) {
+        icp_plan_destroy(p);
+        return hip_status(e);
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&p->states, sizeof(IcpState) * n_states)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->partials, sizeof(double) * kPartStride * n_states * unit_cap)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->nnprev, sizeof(int32_t) * std::max<int64_t>(nn_off[n_states], 1))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->pairs, sizeof(PairDesc) * n_states)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->init, sizeof(StateInit) * n_states)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->T0, sizeof(double) * 16 * n_pairs)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->err, 256)) != hipSuccess) return fail(e);
+    if ((e = hipHostMalloc(&p->host_states, sizeof(IcpState) * n_states)) != hipSuccess) return fail(e);
+    if ((e = hipHostMalloc(&p->host_err, 256)) != hipSuccess) return fail(e);
+    std::vector<PairDesc> P;
+    std::vector<StateInit> I;
+    states_describe(n_pairs, n_levels, srcs, tgts, params, lanes, unit_cap, p->partials, p->nnprev, nn_off, P, I,
+                    p->tail);
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess ||
+        (e = hipMemcpy(p->pairs, P.data(), sizeof(PairDesc) * n_states, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(p->init, I.data(), sizeof(StateInit) * n_states, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(p->T0, T0, sizeof(double) * 16 * n_pairs, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e);
+    *out = p;
+    return GCA_OK;
+}
+
+// Set-up + persistent launch + results into pinned memory, all on the ctx stream.
+int32_t icp_plan_launch(gca_ctx ctx, IcpPlan *p)
+{
+    hipLaunchKernelGGL(k_state_init, dim3((p->n_states + 63) / 64), dim3(64), 0, ctx->stream, p->states,
+                       (const PairDesc *)p->pairs, (const StateInit *)p->init, (const double *)p->T0, p->n_states,
+                       p->err);
+    launch_persist_any(ctx, p->mode, p->pairs, p->states, p->n_states, p->err);
+    GCA_HIP(hipGetLastError());
+    GCA_HIP(hipMemcpyAsync(p->host_states, p->states, sizeof(IcpState) * p->n_states, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    GCA_HIP(hipMemcpyAsync(p->host_err, p->err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    return GCA_OK;
+}
+
+// After the ctx stream has been synchronised.
+int32_t icp_plan_results(gca_ctx ctx, const IcpPlan *p, double *T_out, gca_icp_report *reps)
+{
+    if (*p->host_err) return GCA_ERR_DEVICE;
+    states_report(ctx, p->host_states, p->n_pairs, p->n_states, p->tail, p->T0h.data(), T_out, reps);
     return GCA_OK;
 }
 

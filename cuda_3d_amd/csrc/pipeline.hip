@@ -1,0 +1,332 @@
+// pipeline.hip -- the per-frame RGB-D odometry step as one native object, replayable as a graph.
+//
+// Replaces the loop of the reference's demo (apps/app.cpp:111-372): create_from_rgbd ->
+// voxel_grid_down_sample -> estimate_normals -> color_icp.align against the previous frame,
+// with the coarse-to-fine schedule of apps/test_color_icp.cpp:378-414.  A step enqueues, on
+// the pipeline's own stream and without any host round trip:
+//   front end (bilateral, back-projection, compaction, bounds)
+//   per level: voxel lattice from the device header + downsample, neighbour grid of fixed
+//              geometry, normals, colour gradient, grid-ordered attributes
+//   registration of the new frame onto the previous one: all levels chained in one persistent
+//              launch (a registration plan built once per frame parity), results copied to
+//              pinned host memory
+// Nothing in a step depends on host-side sizes: the voxel lattice is computed on the device
+// (its scan covers an upper bound from the camera frustum), and each level's grid uses one
+// geometry over the frustum box (neighbour results do not depend on the grid geometry: points
+// and query ranges are clamped alike).  So after one eager step per frame parity, each parity's
+// step is captured into a hipGraph and replayed.
+//
+// The pipeline owns a private ctx: the scratch buffers a captured graph points into are used by
+// nothing else, so no other call can move them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "gca_device.hpp"
+#include "gca_internal.hpp"
+
+using namespace gca;
+
+constexpr int kPipeMaxLevels = 8;
+
+struct gca_pipeline_s {
+    gca_ctx ctx = nullptr;  // private
+    gca_camera cam{};
+    gca_bilateral bil{};
+    float tmin = 0.f, tmax = 0.f;
+    int nl = 0;
+    gca_level_params lv[kPipeMaxLevels]{};
+    gca_cloud full[2] = {nullptr, nullptr};
+    gca_cloud lvc[2][kPipeMaxLevels] = {};
+    GridDesc gdesc[kPipeMaxLevels]{};
+    double max_bits[kPipeMaxLevels] = {};
+    IcpPlan *plan[2] = {nullptr, nullptr};   // plan[s]: frame in slot s onto the frame in 1 - s
+    uint32_t *err_dev = nullptr;              // voxel lattice refusals during a step
+    uint32_t *err_host = nullptr;             // pinned copy
+    int64_t frames = 0;
+    int last = -1;
+    bool use_graph = false;
+    int eager_since_reset = 0;  // eager steps since plans were (re)built
+    uint64_t tune_version = 0;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    const void *cap_depth[2] = {nullptr, nullptr}, *cap_bgr[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+__global__ void k_zero_word(uint32_t *w)
+{
+    if (threadIdx.x == 0) *w = 0u;
+}
+
+void drop_graphs(gca_pipeline p)
+{
+    for (int s = 0; s < 2; ++s) {
+        if (p->exec[s]) hipGraphExecDestroy(p->exec[s]);
+        p->exec[s] = nullptr;
+        p->cap_depth[s] = p->cap_bgr[s] = nullptr;
+    }
+}
+
+void drop_plans(gca_pipeline p)
+{
+    drop_graphs(p);
+    for (int s = 0; s < 2; ++s) {
+        icp_plan_destroy(p->plan[s]);
+        p->plan[s] = nullptr;
+    }
+    p->eager_since_reset = 0;
+}
+
+int32_t make_plan(gca_pipeline p, int slot)
+{
+    gca_cloud srcs[kPipeMaxLevels], tgts[kPipeMaxLevels];
+    gca_icp_params prm[kPipeMaxLevels];
+    float gr[kPipeMaxLevels];
+    for (int l = 0; l < p->nl; ++l) {
+        srcs[l] = p->lvc[slot][l];
+        tgts[l] = p->lvc[1 - slot][l];
+        prm[l] = p->lv[l].icp;
+        gr[l] = p->lv[l].gradient_radius;
+    }
+    static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    return icp_plan_create(p->ctx, 1, p->nl, srcs, tgts, prm, gr, I, &p->plan[slot]);
+}
+
+// Everything a step enqueues (no host round trip, no allocation once warm).
+int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *bgr)
+{
+    gca_ctx ctx = p->ctx;
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(64), 0, ctx->stream, p->err_dev);
+    GCA_TRY(launch_from_rgbd(ctx, depth, bgr, &p->cam, &p->bil, p->tmin, p->tmax, p->full[slot]));
+    for (int l = 0; l < p->nl; ++l) {
+        gca_cloud c = p->lvc[slot][l];
+        const gca_level_params &L = p->lv[l];
+        GCA_TRY(launch_voxel_planned(ctx, p->full[slot], L.voxel_size, c, p->max_bits[l], p->err_dev));
+        GCA_TRY(launch_grid_build(ctx, c, p->gdesc[l]));
+        GCA_TRY(launch_normals(ctx, c, L.normal_radius, L.normal_max_nn));
+        GCA_TRY(launch_gradient(ctx, c, L.gradient_radius, L.gradient_max_nn));
+        c->has_normals = c->has_gradient = true;
+        c->grid.attrs_valid = false;
+        GCA_TRY(grid_gather_attributes(ctx, c));
+    }
+    if (p->frames >= 1) GCA_TRY(icp_plan_launch(ctx, p->plan[slot]));
+    GCA_HIP(hipMemcpyAsync(p->err_host, p->err_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    return GCA_OK;
+}
+
+int32_t capture(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *bgr)
+{
+    gca_ctx ctx = p->ctx;
+    hipGraph_t g = nullptr;
+    ctx->capturing = true;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    int32_t st = e == hipSuccess ? enqueue(p, slot, depth, bgr) : hip_status(e);
+    if (e == hipSuccess) {
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        if (st == GCA_OK && e2 != hipSuccess) st = hip_status(e2);
+    }
+    ctx->capturing = false;
+    if (st == GCA_OK && !g) st = GCA_ERR_DEVICE;
+    hipGraphExec_t ex = nullptr;
+    if (st == GCA_OK) {
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) st = hip_status(e);
+    }
+    if (g) hipGraphDestroy(g);
+    if (st != GCA_OK) return st;
+    if (p->exec[slot]) hipGraphExecDestroy(p->exec[slot]);
+    p->exec[slot] = ex;
+    p->cap_depth[slot] = depth;
+    p->cap_bgr[slot] = bgr;
+    return GCA_OK;
+}
+
+// Axis-aligned box holding every back-projected point: x = ((u - 0.5 - cx) z) / fx over the
+// image corners and z in [tmin, tmax] (cuda_point_cloud_factory.cu:232), padded slightly.
+void frustum_box(const gca_camera &c, float tmin, float tmax, float mn[3], float mx[3])
+{
+    const gca_intrinsics &k = c.depth_intrin;
+    double lo[3] = {DBL_MAX, DBL_MAX, (double)tmin}, hi[3] = {-DBL_MAX, -DBL_MAX, (double)tmax};
+    const double us[2] = {0.0, (double)c.width - 1.0}, vs[2] = {0.0, (double)c.height - 1.0};
+    const double zs[2] = {(double)tmin, (double)tmax};
+    for (double z : zs) {
+        for (double u : us) {
+            const double x = ((u - 0.5 - k.cx) * z) / k.fx;
+            lo[0] = std::min(lo[0], x);
+            hi[0] = std::max(hi[0], x);
+        }
+        for (double v : vs) {
+            const double y = ((v - 0.5 - k.cy) * z) / k.fy;
+            lo[1] = std::min(lo[1], y);
+            hi[1] = std::max(hi[1], y);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        const double pad = 1e-3 * (hi[a] - lo[a]) + 1e-3;
+        mn[a] = (float)(lo[a] - pad);
+        mx[a] = (float)(hi[a] + pad);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gca_pipeline_destroy(gca_pipeline p)
+{
+    if (!p) return GCA_ERR_INVALID_VALUE;
+    if (p->ctx) hipStreamSynchronize(p->ctx->stream);
+    drop_plans(p);
+    for (int s = 0; s < 2; ++s) {
+        if (p->full[s]) gca_cloud_destroy(p->full[s]);
+        for (int l = 0; l < p->nl; ++l)
+            if (p->lvc[s][l]) gca_cloud_destroy(p->lvc[s][l]);
+    }
+    hipFree(p->err_dev);
+    hipHostFree(p->err_host);
+    if (p->ctx) gca_ctx_destroy(p->ctx);
+    delete p;
+    return GCA_OK;
+}
+
+int32_t gca_pipeline_create(gca_ctx ctx, const gca_camera *cam, const gca_bilateral *filter, float tmin, float tmax,
+                            int32_t n_levels, const gca_level_params *levels, gca_pipeline *out)
+{
+    if (!out) return GCA_ERR_INVALID_VALUE;
+    *out = nullptr;
+    if (!ctx || !cam || !levels || n_levels < 1 || n_levels > kPipeMaxLevels) return GCA_ERR_INVALID_VALUE;
+    // cuda_point_cloud_factory.cu:286-290
+    if (cam->width <= 0 || cam->height <= 0 || !(cam->depth_scale > 0.0f)) return GCA_ERR_INVALID_VALUE;
+    if ((int64_t)cam->width * cam->height > (int64_t)INT_MAX / 4) return GCA_ERR_INVALID_VALUE;
+    if (!(cam->depth_intrin.fx > 0.0f) || !(cam->depth_intrin.fy > 0.0f)) return GCA_ERR_INVALID_VALUE;
+    if (!(tmax > tmin) || !(tmin >= 0.0f)) return GCA_ERR_INVALID_VALUE;
+    const gca_bilateral none{GCA_BILATERAL_NONE, 0, 1.0f, 1.0f};
+    const gca_bilateral &f = filter ? *filter : none;
+    if (f.mode < 0 || f.mode > 2) return GCA_ERR_INVALID_VALUE;
+    if (f.mode == GCA_BILATERAL_FIXED &&
+        (f.radius < 0 || f.radius > 15 || !(f.sigma_space > 0.0f) || !(f.sigma_depth > 0.0f)))
+        return GCA_ERR_INVALID_VALUE;
+    for (int l = 0; l < n_levels; ++l) {
+        const gca_level_params &L = levels[l];
+        if (!(L.voxel_size > 0.0f) || !(L.normal_radius > 0.0f) || !(L.gradient_radius > 0.0f) ||
+            L.normal_max_nn < 0 || L.gradient_max_nn < 0 || L.icp.max_iterations < 0 ||
+            !(L.icp.max_correspondence_distance > 0.0f) || !(L.icp.lambda_geometric >= 0.0f) ||
+            L.icp.lambda_geometric > 1.0f || L.icp.accumulate_mode != levels[0].icp.accumulate_mode)
+            return GCA_ERR_INVALID_VALUE;
+    }
+    gca_pipeline p = new (std::nothrow) gca_pipeline_s();
+    if (!p) return GCA_ERR_OUT_OF_MEMORY;
+    p->cam = *cam;
+    p->bil = f;
+    p->tmin = tmin;
+    p->tmax = tmax;
+    p->nl = n_levels;
+    memcpy(p->lv, levels, sizeof(gca_level_params) * n_levels);
+    int32_t st = gca_ctx_create(ctx->device, &p->ctx);
+    const int64_t cap = (int64_t)cam->width * cam->height;
+    for (int s = 0; s < 2 && st == GCA_OK; ++s) {
+        st = gca_cloud_create(p->ctx, cap, &p->full[s]);
+        for (int l = 0; l < n_levels && st == GCA_OK; ++l) st = gca_cloud_create(p->ctx, cap, &p->lvc[s][l]);
+    }
+    if (st == GCA_OK && hipMalloc(&p->err_dev, 256) != hipSuccess) st = GCA_ERR_OUT_OF_MEMORY;
+    if (st == GCA_OK && hipHostMalloc(&p->err_host, 256) != hipSuccess) st = GCA_ERR_OUT_OF_MEMORY;
+    if (st == GCA_OK) *p->err_host = 0u;
+    if (st != GCA_OK) {
+        gca_pipeline_destroy(p);
+        return st;
+    }
+    // per level: one grid geometry over the frustum box (cell = the normals / correspondence
+    // radius, the largest of the level), and the voxel lattice bound of the box
+    float bmn[3], bmx[3];
+    frustum_box(*cam, tmin, tmax, bmn, bmx);
+    for (int l = 0; l < n_levels; ++l) {
+        const gca_level_params &L = levels[l];
+        const float r = std::max(std::max(L.normal_radius, L.icp.max_correspondence_distance), L.gradient_radius);
+        grid_params(bmn, bmx, r, &p->gdesc[l]);
+        double bits = 1.0;
+        for (int a = 0; a < 3; ++a) bits *= std::floor((double)(bmx[a] - bmn[a]) / (double)L.voxel_size) + 2.0;
+        p->max_bits[l] = bits;
+    }
+    p->tune_version = p->ctx->tune_version;
+    *out = p;
+    return GCA_OK;
+}
+
+int32_t gca_pipeline_set_graph(gca_pipeline p, int32_t enable)
+{
+    if (!p) return GCA_ERR_INVALID_VALUE;
+    p->use_graph = enable != 0;
+    if (!p->use_graph) drop_graphs(p);
+    return GCA_OK;
+}
+
+int32_t gca_pipeline_ctx(gca_pipeline p, gca_ctx *ctx)
+{
+    if (!p || !ctx) return GCA_ERR_INVALID_VALUE;
+    *ctx = p->ctx;
+    return GCA_OK;
+}
+
+int32_t gca_pipeline_step(gca_pipeline p, const uint16_t *depth_dev, const uint8_t *bgr_dev)
+{
+    if (!p || !depth_dev || !bgr_dev) return GCA_ERR_INVALID_VALUE;
+    const int slot = (int)(p->frames & 1);
+    if (p->ctx->tune_version != p->tune_version) {  // plans and graphs bake the launch shapes in
+        GCA_HIP(hipStreamSynchronize(p->ctx->stream));
+        drop_plans(p);
+        p->tune_version = p->ctx->tune_version;
+    }
+    if (p->frames >= 1 && !p->plan[slot]) GCA_TRY(make_plan(p, slot));
+    // a graph is captured only once every buffer it will point into exists: after an eager
+    // step of each frame parity with its plan in place
+    const bool graph = p->use_graph && p->frames >= 1 && p->eager_since_reset >= 2 && p->plan[0] && p->plan[1];
+    if (graph) {
+        if (!p->exec[slot] || p->cap_depth[slot] != depth_dev || p->cap_bgr[slot] != bgr_dev)
+            GCA_TRY(capture(p, slot, depth_dev, bgr_dev));
+        GCA_HIP(hipGraphLaunch(p->exec[slot], p->ctx->stream));
+    } else {
+        GCA_TRY(enqueue(p, slot, depth_dev, bgr_dev));
+        if (p->frames >= 1) ++p->eager_since_reset;
+    }
+    // host bookkeeping of a replayed step: the slot's clouds changed on the device
+    p->full[slot]->host_valid = false;
+    for (int l = 0; l < p->nl; ++l) p->lvc[slot][l]->host_valid = false;
+    ++p->frames;
+    p->last = slot;
+    return GCA_OK;
+}
+
+int32_t gca_pipeline_result(gca_pipeline p, double T[16], gca_icp_report *reports, int32_t *valid)
+{
+    if (!p || !T || !valid) return GCA_ERR_INVALID_VALUE;
+    *valid = 0;
+    GCA_HIP(hipStreamSynchronize(p->ctx->stream));
+    if (*p->err_host) return GCA_ERR_INVALID_VALUE;  // a voxel lattice was refused
+    if (p->frames < 2) {
+        static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        memcpy(T, I, sizeof(I));
+        return GCA_OK;
+    }
+    GCA_TRY(icp_plan_results(p->ctx, p->plan[p->last], T, reports));
+    *valid = 1;
+    return GCA_OK;
+}
+
+int32_t gca_pipeline_sizes(gca_pipeline p, int32_t slot, int64_t *full_n, int64_t *level_n)
+{
+    if (!p || slot < 0 || slot > 1 || !full_n) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(cloud_sync_header(p->full[slot]));
+    *full_n = p->full[slot]->host_hdr.n;
+    if (level_n)
+        for (int l = 0; l < p->nl; ++l) {
+            GCA_TRY(cloud_sync_header(p->lvc[slot][l]));
+            level_n[l] = p->lvc[slot][l]->host_hdr.n;
+        }
+    return GCA_OK;
+}
+
+}  // extern "C"

@@ -26,11 +26,44 @@
 
 namespace gca {
 
+constexpr double kVoxelMaxBits = 17179869184.0;  // 2^34 lattice cells
+
 struct VoxelParams {
     float bmin[3];
     float voxel;
     int64_t dims[3];
+    int32_t bad;  // lattice refused (extent / voxel beyond INT_MAX, or more bits than planned)
+    int32_t pad;
 };
+
+// Lattice of a voxel call from the input's device header (one thread), so a launch sequence
+// needs no host round trip: origin = unpadded min bound, dims = floor(extent / v) + 1 per axis
+// in FP32 (point_cloud.cu:205-218, cuda_voxel_grid_down_sample.cu:37-47).  A lattice the
+// reference would refuse, or one with more than max_bits cells, is flagged (bad, *err) and
+// yields an empty output.
+__global__ void k_voxel_params(const CloudHdr *__restrict__ in, float voxel, double max_bits,
+                               VoxelParams *__restrict__ vp, uint32_t *__restrict__ err)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int n = in->n;
+    VoxelParams p{};
+    p.voxel = voxel;
+    double nbits = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        const float mn = key_to_float(in->bmin_key[a]), mx = key_to_float(in->bmax_key[a]);
+        p.bmin[a] = mn;
+        const float ext = __fsub_rn(mx, mn);
+        if (n > 0 && (double)voxel * (double)INT_MAX < (double)ext) p.bad = 1;
+        p.dims[a] = n > 0 ? (int64_t)floorf(__fdiv_rn(ext, voxel)) + 1 : 1;
+        nbits *= (double)p.dims[a];
+    }
+    if (nbits > max_bits) p.bad = 1;
+    if (p.bad) {
+        p.dims[0] = p.dims[1] = p.dims[2] = 1;
+        if (err) atomicOr(err, 1u);
+    }
+    *vp = p;
+}
 
 __device__ __forceinline__ int64_t voxel_key(const VoxelParams &vp, float4 p)
 {
@@ -79,11 +112,13 @@ __device__ __forceinline__ T run_reduce(T v, int head, Op op)
 // Raster-ordered points hit the same voxel in runs, so each wavefront first merges runs and
 // only the first lane of a run issues the atomics.
 __global__ __launch_bounds__(256) void k_voxel_mark(const float4 *__restrict__ xyzi,
-                                                    const CloudHdr *__restrict__ hdr, VoxelParams vp,
+                                                    const CloudHdr *__restrict__ hdr,
+                                                    const VoxelParams *__restrict__ vpp,
                                                     uint32_t *__restrict__ bitmap,
                                                     int64_t *__restrict__ lin)
 {
-    const int n = hdr->n;
+    const VoxelParams vp = *vpp;
+    const int n = vp.bad ? 0 : hdr->n;
     const int stride = gridDim.x * blockDim.x;
     for (int b = blockIdx.x * blockDim.x; b < n; b += stride) {  // block-uniform trip count
         const int i = b + threadIdx.x;
@@ -103,12 +138,13 @@ __global__ __launch_bounds__(256) void k_voxel_mark(const float4 *__restrict__ x
 __global__ __launch_bounds__(256) void k_voxel_accum(const float4 *__restrict__ xyzi,
                                                      const float4 *__restrict__ rgb,
                                                      const CloudHdr *__restrict__ hdr,
+                                                     const VoxelParams *__restrict__ vpp,
                                                      const uint32_t *__restrict__ bitmap,
                                                      const uint32_t *__restrict__ word_pre,
                                                      const int64_t *__restrict__ lin,
                                                      unsigned long long *__restrict__ acc)
 {
-    const int n = hdr->n;
+    const int n = vpp->bad ? 0 : hdr->n;
     const int stride = gridDim.x * blockDim.x;
     auto add = [](unsigned long long a, unsigned long long c) { return a + c; };
     for (int b = blockIdx.x * blockDim.x; b < n; b += stride) {  // block-uniform trip count
@@ -169,26 +205,11 @@ __global__ __launch_bounds__(256) void k_voxel_finalize(unsigned long long *__re
     }
 }
 
-int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out)
+// Kernel sequence of one voxel call: device lattice, mark, popcount scan over `words` bitmap
+// words (>= the lattice's), accumulate, finalize.  Scratch is sized for `n_cap` input points.
+static int32_t voxel_enqueue(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, int64_t n_cap, int64_t words,
+                             double max_bits, uint32_t *err)
 {
-    GCA_TRY(cloud_sync_header(in));
-    const int64_t n = in->host_hdr.n;
-    float mn[3], mx[3];
-    cloud_host_bounds(in, mn, mx);
-    VoxelParams vp{};
-    vp.voxel = voxel;
-    double nbits = 1.0;
-    for (int a = 0; a < 3; ++a) {
-        vp.bmin[a] = mn[a];
-        float ext = mx[a] - mn[a];
-        if (n > 0 && (double)voxel * (double)INT_MAX < (double)ext)  // point_cloud.cu:213-218
-            return GCA_ERR_INVALID_VALUE;
-        vp.dims[a] = n > 0 ? (int64_t)floorf(ext / voxel) + 1 : 1;
-        nbits *= (double)vp.dims[a];
-    }
-    if (nbits > 17179869184.0) return GCA_ERR_INVALID_VALUE;  // > 2^34 voxels: voxel too small
-    GCA_TRY(cloud_reserve(out, std::max<int64_t>(n, 1)));
-    const int64_t words = ((int64_t)nbits + 31) / 32;
     if (ctx->bitmap_words < words || ctx->bitmap_dirty) {
         GCA_HIP(hipStreamSynchronize(ctx->stream));
         if (ctx->bitmap_words < words) {
@@ -202,25 +223,53 @@ int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out)
         GCA_HIP(hipMemsetAsync(ctx->bitmap, 0, sizeof(uint32_t) * ctx->bitmap_words, ctx->stream));
         ctx->bitmap_dirty = false;
     }
-    int64_t *lin = (int64_t *)ctx_scratch(ctx, 5, sizeof(int64_t) * std::max<int64_t>(n, 1), false);
+    const int64_t nc = std::max<int64_t>(n_cap, 1);
+    int64_t *lin = (int64_t *)ctx_scratch(ctx, 5, sizeof(int64_t) * nc, false);
     uint32_t *word_pre = (uint32_t *)ctx_scratch(ctx, 6, sizeof(uint32_t) * (words + 1), false);
-    unsigned long long *acc =
-        (unsigned long long *)ctx_scratch(ctx, 7, sizeof(unsigned long long) * 8 * std::max<int64_t>(n, 1), true);
-    if (!lin || !word_pre || !acc) return GCA_ERR_OUT_OF_MEMORY;
-    const unsigned nb = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (n + 255) / 256));
-    ctx->bitmap_dirty = true;  // until finalize has run to completion
+    unsigned long long *acc = (unsigned long long *)ctx_scratch(ctx, 7, sizeof(unsigned long long) * 8 * nc, true);
+    VoxelParams *vp = (VoxelParams *)ctx_scratch(ctx, 22, sizeof(VoxelParams), false);
+    if (!lin || !word_pre || !acc || !vp) return GCA_ERR_OUT_OF_MEMORY;
+    const unsigned nb = (unsigned)std::min<int64_t>(4096, (nc + 255) / 256);
+    ctx->bitmap_dirty = true;  // until the whole sequence is enqueued
     StageTimer timer(ctx, GCA_STAGE_VOXEL);
-    hipLaunchKernelGGL(k_voxel_mark, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->hdr, vp, ctx->bitmap, lin);
+    hipLaunchKernelGGL(k_voxel_params, dim3(1), dim3(64), 0, ctx->stream, in->hdr, voxel, max_bits, vp, err);
+    hipLaunchKernelGGL(k_voxel_mark, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->hdr, (const VoxelParams *)vp,
+                       ctx->bitmap, lin);
     GCA_HIP(hipGetLastError());
     GCA_TRY(exclusive_scan_u32(ctx, ctx->bitmap, word_pre, words, (uint32_t *)&out->hdr->n, true));
     hipLaunchKernelGGL(k_voxel_accum, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->rgb, in->hdr,
-                       ctx->bitmap, word_pre, lin, acc);
+                       (const VoxelParams *)vp, ctx->bitmap, word_pre, lin, acc);
     hipLaunchKernelGGL(k_voxel_finalize, dim3(nb), dim3(256), 0, ctx->stream, acc, in->hdr, out->hdr,
                        ctx->bitmap, out->xyzi, out->rgb);
     GCA_HIP(hipGetLastError());
     ctx->bitmap_dirty = false;
     cloud_points_changed(out);
     return GCA_OK;
+}
+
+int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out)
+{
+    GCA_TRY(cloud_sync_header(in));
+    const int64_t n = in->host_hdr.n;
+    float mn[3], mx[3];
+    cloud_host_bounds(in, mn, mx);
+    double nbits = 1.0;
+    for (int a = 0; a < 3; ++a) {  // the device lattice below uses the same FP32 formulas
+        const float ext = mx[a] - mn[a];
+        if (n > 0 && (double)voxel * (double)INT_MAX < (double)ext)  // point_cloud.cu:213-218
+            return GCA_ERR_INVALID_VALUE;
+        nbits *= n > 0 ? (double)((int64_t)floorf(ext / voxel) + 1) : 1.0;
+    }
+    if (nbits > kVoxelMaxBits) return GCA_ERR_INVALID_VALUE;  // voxel too small for the lattice
+    GCA_TRY(cloud_reserve(out, std::max<int64_t>(n, 1)));
+    return voxel_enqueue(ctx, in, voxel, out, n, ((int64_t)nbits + 31) / 32, kVoxelMaxBits, nullptr);
+}
+
+int32_t launch_voxel_planned(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, double max_bits, uint32_t *err)
+{
+    const int64_t words = ((int64_t)max_bits + 31) / 32;
+    GCA_TRY(cloud_reserve(out, std::max<int64_t>(in->cap, 1)));
+    return voxel_enqueue(ctx, in, voxel, out, in->cap, words, max_bits, err);
 }
 
 }  // namespace gca

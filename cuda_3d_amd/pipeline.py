@@ -1,16 +1,17 @@
-"""Per-frame RGB-D odometry pipeline over libgca.so.
+"""Per-frame RGB-D odometry pipeline: the native gca_pipeline (include/gca/gca.h).
 
-One step takes a new depth + colour frame that is already resident in device memory and runs:
-  front end     bilateral filter, back-projection, compaction (gca_cloud_from_rgbd)
-  per level     voxel downsample, PCA normals (2.5 v, K nearest), colour gradient (2 v):
-                the target-side data this frame needs when the next frame registers to it
+One step takes a new depth + colour frame that is already resident in device memory and runs,
+without a host round trip:
+  front end     bilateral filter, back-projection, compaction
+  per level     voxel downsample, PCA normals (2.5 v, K nearest), colour gradient (2 v): the
+                target-side data this frame needs when the next frame registers to it
   registration  coarse-to-fine colored ICP of the new frame onto the previous frame: all
                 levels in one persistent launch, each level starting from the pose the
-                previous one ends with (gca_color_icp_batch_multiscale, one pair)
+                previous one ends with
 This is the loop of the reference's demo (apps/app.cpp:111-372: create_from_rgbd ->
 voxel_grid_down_sample -> estimate_normals -> color_icp.align against the previous cloud) with
-the coarse-to-fine schedule of apps/test_color_icp.cpp:378-414.  All working clouds are
-allocated once (two frame slots) and reused, so a step allocates nothing.
+the coarse-to-fine schedule of apps/test_color_icp.cpp:378-414.  Every buffer is allocated
+once (two frame slots); with graph=True each frame parity's step is replayed as a hipGraph.
 """
 from __future__ import annotations
 
@@ -19,69 +20,79 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .core import Context, PointCloud, _ptr, icp_params
+from .core import Context, _ptr, icp_params, level_params
 
 
 class FramePipeline:
     def __init__(self, ctx: Context, camera: L.Camera, bilateral: L.Bilateral,
                  levels=(0.04, 0.02, 0.01), iters=30, normal_max_nn=30, tmin=0.2, tmax=3.5,
-                 conv_rotation=1e-6, conv_translation_sq=1e-6, accumulate_mode=L.ACCUM_COMPENSATED):
+                 conv_rotation=1e-6, conv_translation_sq=1e-6, accumulate_mode=L.ACCUM_COMPENSATED,
+                 graph=True):
         """conv_*: the stopping rule; the defaults are the reference's fixed thresholds
         (color_icp.hpp:45-47), 0, 0 runs every level to `iters`."""
-        self.ctx, self.cam, self.bil = ctx, camera, bilateral
-        self.levels, self.normal_max_nn, self.tmin, self.tmax = tuple(levels), normal_max_nn, tmin, tmax
-        cap = camera.width * camera.height
-        self.full = [PointCloud(ctx, cap) for _ in range(2)]
-        self.lv = [[PointCloud(ctx, cap) for _ in self.levels] for _ in range(2)]
+        self.levels = tuple(levels)
         nl = len(self.levels)
-        self.params = (L.IcpParams * nl)(*[icp_params(iters, 2.5 * v, 0.968, accumulate_mode, conv_rotation,
-                                                      conv_translation_sq) for v in self.levels])
-        self.grad_r = (C.c_float * nl)(*[2.0 * v for v in self.levels])
+        lv = (L.LevelParams * nl)(*[
+            level_params(v, normal_max_nn=normal_max_nn,
+                         icp=icp_params(iters, 2.5 * v, 0.968, accumulate_mode, conv_rotation, conv_translation_sq))
+            for v in self.levels])
+        h = C.c_void_p()
+        L.check(L.lib.gca_pipeline_create(ctx.handle, C.byref(camera), C.byref(bilateral), tmin, tmax, nl, lv,
+                                          C.byref(h)), "gca_pipeline_create")
+        self.handle = h
+        c = C.c_void_p()
+        L.check(L.lib.gca_pipeline_ctx(h, C.byref(c)), "gca_pipeline_ctx")
+        self.ctx = Context.borrow(c, ctx.device)  # the pipeline's own stream, profiler and tuning
+        self.set_graph(graph)
+        self.frames = 0
         self.reports = (L.IcpReport * nl)()
-        # level-cloud handle tables for (source slot, target slot) = (0, 1) and (1, 0)
-        self._tables = {}
-        for a in (0, 1):
-            b = 1 - a
-            self._tables[a] = ((C.c_void_p * nl)(*[c.handle.value for c in self.lv[a]]),
-                               (C.c_void_p * nl)(*[c.handle.value for c in self.lv[b]]))
-        self.slot = 0
-        self.have_prev = False
         self._T = np.zeros(16)
-        self._I = np.ascontiguousarray(np.eye(4).reshape(16))
 
-    def frontend(self, slot: int, depth_ptr: int, bgr_ptr: int):
-        lib, h = L.lib, self.ctx.handle
-        full = self.full[slot]
-        L.check(lib.gca_cloud_from_rgbd(h, C.c_void_p(depth_ptr), C.c_void_p(bgr_ptr), C.byref(self.cam),
-                                        C.byref(self.bil), self.tmin, self.tmax, full.handle), "from_rgbd")
-        for l, v in enumerate(self.levels):
-            c = self.lv[slot][l]
-            L.check(lib.gca_voxel_downsample(h, full.handle, v, c.handle), "voxel")
-            L.check(lib.gca_estimate_normals(h, c.handle, 2.5 * v, self.normal_max_nn), "normals")
-            L.check(lib.gca_compute_color_gradient(h, c.handle, 2.0 * v, 0), "gradient")
+    def set_graph(self, enable: bool):
+        L.check(L.lib.gca_pipeline_set_graph(self.handle, 1 if enable else 0), "gca_pipeline_set_graph")
 
-    def register(self, src_slot: int, tgt_slot: int, T0=None) -> np.ndarray:
-        if tgt_slot != 1 - src_slot:
-            raise ValueError("register() pairs the two frame slots")
-        T = self._T
-        T[:] = self._I if T0 is None else np.asarray(T0, np.float64).reshape(16)
-        hs, ht = self._tables[src_slot]
-        L.check(L.lib.gca_color_icp_batch_multiscale(self.ctx.handle, 1, len(self.levels), hs, ht, self.params,
-                                                     self.grad_r, _ptr(T), _ptr(T), self.reports), "register")
-        return T.reshape(4, 4).copy()
+    @property
+    def slot(self) -> int:
+        """Frame slot the next step uses."""
+        return self.frames % 2
+
+    def step(self, depth_ptr: int, bgr_ptr: int):
+        """Enqueue a new frame (asynchronous)."""
+        L.check(L.lib.gca_pipeline_step(self.handle, C.c_void_p(depth_ptr), C.c_void_p(bgr_ptr)),
+                "gca_pipeline_step")
+        self.frames += 1
+
+    def result(self):
+        """Waits for the last step: the pose mapping the last frame onto the one before (None for
+        the first frame) and the per-level reports."""
+        valid = C.c_int32()
+        L.check(L.lib.gca_pipeline_result(self.handle, _ptr(self._T), self.reports, C.byref(valid)),
+                "gca_pipeline_result")
+        return (self._T.reshape(4, 4).copy() if valid.value else None), list(self.reports)
 
     def iterations(self):
         """Gauss-Newton iterations of each level in the last registration."""
-        return [r.iterations for r in self.reports]
+        return [r.iterations for r in self.result()[1]]
 
-    def step(self, depth_ptr: int, bgr_ptr: int):
-        """Process a new frame; returns the pose mapping it onto the previous frame (or None)."""
-        new = self.slot
-        self.frontend(new, depth_ptr, bgr_ptr)
-        T = self.register(new, 1 - new) if self.have_prev else None
-        self.have_prev = True
-        self.slot = 1 - new
-        return T
+    def sizes(self, slot: int):
+        full = C.c_int64()
+        lv = np.zeros(len(self.levels), np.int64)
+        L.check(L.lib.gca_pipeline_sizes(self.handle, slot, C.byref(full), _ptr(lv)), "gca_pipeline_sizes")
+        return int(full.value), [int(x) for x in lv]
 
     def level_sizes(self, slot: int):
-        return [c.points_number() for c in self.lv[slot]]
+        return self.sizes(slot)[1]
+
+    def full_size(self, slot: int) -> int:
+        return self.sizes(slot)[0]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib.gca_pipeline_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -21,8 +21,9 @@ LAMBDA = 0.968
 
 
 def frame_workload(ctx, steps: int = 6, W: int = 640, H: int = 480, seed: int = 12, levels=LEVELS, iters=30):
-    """Returns run() -> None that pushes `steps` frames through a FramePipeline and syncs.
-    run.pipe is the pipeline."""
+    """Returns run() -> None that pushes `steps` frames through a FramePipeline (eager launches:
+    the stage profiler brackets each launch) and syncs.  run.pipe is the pipeline, run.ctx its
+    own ctx (where its tuning and profiling live)."""
     from . import core as g
     from . import scene
     from ._lib import BILATERAL_FIXED
@@ -34,7 +35,7 @@ def frame_workload(ctx, steps: int = 6, W: int = 640, H: int = 480, seed: int = 
     dev = [(g.device_tensor(td.view(np.int16)), g.device_tensor(tc)),
            (g.device_tensor(sd.view(np.int16)), g.device_tensor(sc))]
     ptrs = [(d.data_ptr(), c.data_ptr()) for d, c in dev]
-    pipe = FramePipeline(ctx, cam, bil, levels, iters)
+    pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=False)
     state = {"k": 0}
 
     def run():
@@ -42,9 +43,10 @@ def frame_workload(ctx, steps: int = 6, W: int = 640, H: int = 480, seed: int = 
             d, c = ptrs[state["k"] % 2]
             pipe.step(d, c)
             state["k"] += 1
-        ctx.synchronize()
+        pipe.ctx.synchronize()
 
     run.pipe = pipe
+    run.ctx = pipe.ctx
     run.frames = ((td, tc), (sd, sc))
     run.camera = cam
     run._keep = dev  # device buffers live as long as the closure

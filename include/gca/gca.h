@@ -362,6 +362,38 @@ int32_t gca_multiscale_icp(gca_ctx ctx, gca_cloud src_full, gca_cloud tgt_full, 
                            const gca_level_params *levels, const double T0[16],
                            double T_out[16], gca_icp_report *reports);
 
+/* ------------------------------------------------------------------------------------------ */
+/* Frame pipeline                                                                            */
+
+/* The per-frame odometry step of the reference's demo (apps/app.cpp:111-372: create_from_rgbd
+ * -> voxel_grid_down_sample -> estimate_normals -> color_icp.align against the previous frame,
+ * coarse to fine as in apps/test_color_icp.cpp:378-414) as one object.  A step runs the front
+ * end, every level's voxel downsample, normals and colour gradient, and the chained
+ * registration of the new frame onto the previous one, all enqueued without a host round trip
+ * on the pipeline's own stream (a private ctx on ctx's device).  Levels: up to 8, coarse to
+ * fine (voxel_size, normal radius / max_nn, gradient radius / max_nn, ICP parameters). */
+typedef struct gca_pipeline_s *gca_pipeline;
+int32_t gca_pipeline_create(gca_ctx ctx, const gca_camera *camera, const gca_bilateral *filter,
+                            float threshold_min_m, float threshold_max_m, int32_t n_levels,
+                            const gca_level_params *levels, gca_pipeline *out);
+int32_t gca_pipeline_destroy(gca_pipeline pipeline);
+/* Enqueue one frame (device Z16 depth and BGR8 colour of camera size; the caller orders its
+ * writes to them before the call).  Asynchronous. */
+int32_t gca_pipeline_step(gca_pipeline pipeline, const uint16_t *depth_dev, const uint8_t *bgr_dev);
+/* Wait for the last step; T = pose mapping the last frame onto the one before (row-major),
+ * reports[n_levels] (may be NULL); *valid = 0 for the first frame (T = identity).
+ * GCA_ERR_INVALID_VALUE if a level's voxel lattice was refused (point_cloud.cu:213-218). */
+int32_t gca_pipeline_result(gca_pipeline pipeline, double T[16], gca_icp_report *reports,
+                            int32_t *valid);
+/* Replay each frame parity's step as a hipGraph (captured after one eager step per parity, and
+ * again when the input pointers or the ctx tuning change). */
+int32_t gca_pipeline_set_graph(gca_pipeline pipeline, int32_t enable);
+/* The pipeline's own ctx (its stream, stage profiler and tuning). */
+int32_t gca_pipeline_ctx(gca_pipeline pipeline, gca_ctx *ctx);
+/* Point counts of a frame slot (0 / 1; the last step used slot (frames - 1) % 2): the full
+ * cloud and level_n[n_levels] (may be NULL).  Synchronises. */
+int32_t gca_pipeline_sizes(gca_pipeline pipeline, int32_t slot, int64_t *full_n, int64_t *level_n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -722,3 +722,127 @@ def test_differentiation_100k_32_poses(ctx):
     assert worst["xi_fp64"] < 1e-8
     assert worst["xi_fd"] < 1e-5
     assert worst["pts_fd"] < 1e-3
+
+
+# ---- native frame pipeline and its graph replay ---------------------------------------------------
+
+def _pipeline_frames(g, W, H, seed):
+    from cuda_3d_amd import scene
+
+    (td, tc), (sd, sc), G, cm = scene.frame_pair(seed, W, H)
+    cam = g.make_camera(W, H, cm.fx, cm.fy, cm.cx, cm.cy)
+    dev = [(g.device_tensor(td.view(np.int16)), g.device_tensor(tc)),
+           (g.device_tensor(sd.view(np.int16)), g.device_tensor(sc))]
+    return cam, dev, ((td, tc), (sd, sc)), G
+
+
+def test_pipeline_matches_composed_calls(ctx, frame):
+    """gca_pipeline (device-side voxel lattice, frustum grids, one chained launch) gives the
+    poses of the same frames registered through gca_multiscale_icp, bit for bit, and matches the
+    oracle."""
+    import cuda_3d_amd as g
+    from cuda_3d_amd.pipeline import FramePipeline
+
+    W, H = (320, 240) if frame.camg.width == 320 else (640, 480)
+    bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
+    pipe = FramePipeline(ctx, frame.camg, bil, graph=False)
+    dt = g.device_tensor(frame.td.view(np.int16))
+    ct = g.device_tensor(frame.tc)
+    ds = g.device_tensor(frame.sd.view(np.int16))
+    cs = g.device_tensor(frame.sc)
+    pipe.step(dt.data_ptr(), ct.data_ptr())
+    assert pipe.result()[0] is None  # first frame: nothing to register against
+    pipe.step(ds.data_ptr(), cs.data_ptr())  # source onto target
+    T, reps = pipe.result()
+    lv = [g.level_params(v, icp=g.icp_params(30, 2.5 * v)) for v in (0.04, 0.02, 0.01)]
+    Tm, mreps = g.multiscale_icp(frame.src, frame.tgt, lv)
+    np.testing.assert_array_equal(T, Tm)
+    assert [rep_key(r) for r in reps] == [rep_key(r) for r in mreps]
+    lo = [O.LevelParams(v, 2.5 * v, 30, 2 * v, 0, O.icp_params(30, 2.5 * v, LAM, 1e-6, 1e-6)) for v in (0.04, 0.02, 0.01)]
+    To, oreps = O.multiscale_icp(frame.osx, frame.osr, frame.otx, frame.otr, lo)
+    er, et = pose_err(T, To)
+    print(f"[gpu-parity] pipeline {W}x{H} vs oracle {er:.3e} rad {et:.3e} m; iterations "
+          f"{[r.iterations for r in reps]} (oracle {[r.iterations for r in oreps]})", flush=True)
+    assert er <= 1e-5 and et <= 1e-5
+    assert [r.iterations for r in reps] == [r.iterations for r in oreps]
+    full_n, lv_n = pipe.sizes(1)
+    assert full_n == frame.src.points_number()
+    pipe.close()
+
+
+def test_pipeline_graph_replays_match_eager(ctx):
+    """Each frame parity's step captured once and replayed: every replay's pose equals the
+    eager pipeline's, with unrelated library work and synchronisations between replays."""
+    import cuda_3d_amd as g
+    from cuda_3d_amd.pipeline import FramePipeline
+
+    cam, dev, _, _ = _pipeline_frames(g, 320, 240, 21)
+    bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
+    eager = FramePipeline(ctx, cam, bil, graph=False)
+    graph = FramePipeline(ctx, cam, bil, graph=True)
+    other = g.PointCloud.from_numpy(np.random.default_rng(5).uniform(0, 1, (2000, 3)), np.full((2000, 3), 0.5), ctx=ctx)
+    for k in range(12):
+        d, c = dev[k % 2]
+        eager.step(d.data_ptr(), c.data_ptr())
+        Te, re = eager.result()
+        graph.step(d.data_ptr(), c.data_ptr())
+        Tg, rg = graph.result()
+        if k == 0:
+            assert Te is None and Tg is None
+            continue
+        np.testing.assert_array_equal(Tg, Te)
+        assert [rep_key(r) for r in rg] == [rep_key(r) for r in re]
+        # unrelated work on another ctx and stream between replays
+        other.voxel_grid_down_sample(0.05)
+        ctx.synchronize()
+    eager.close()
+    graph.close()
+
+
+def test_pipeline_tuning_change_rebuilds(ctx):
+    """Changing the pipeline ctx's tuning drops the plans and graphs; the following steps run
+    eagerly, capture again, and keep agreeing with an untouched pipeline to the last bits."""
+    import cuda_3d_amd as g
+    from cuda_3d_amd.pipeline import FramePipeline
+
+    cam, dev, _, _ = _pipeline_frames(g, 320, 240, 22)
+    bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
+    ref = FramePipeline(ctx, cam, bil, graph=False)
+    pipe = FramePipeline(ctx, cam, bil, graph=True)
+    for k in range(10):
+        if k == 5:
+            pipe.ctx.set_tuning(icp_lanes_small=4, icp_lanes_medium=4, icp_lanes_large=4)
+        d, c = dev[k % 2]
+        ref.step(d.data_ptr(), c.data_ptr())
+        Tr, rr = ref.result()
+        pipe.step(d.data_ptr(), c.data_ptr())
+        Tp, rp = pipe.result()
+        if k == 0:
+            continue
+        np.testing.assert_allclose(Tp, Tr, rtol=0, atol=1e-10)
+        assert [r.iterations for r in rp] == [r.iterations for r in rr]
+    ref.close()
+    pipe.close()
+
+
+def test_pipeline_rejects_bad_arguments(ctx):
+    import ctypes as C
+
+    import cuda_3d_amd as g
+    from cuda_3d_amd import _lib as L
+
+    cam = g.make_camera(64, 48, 60.0, 60.0, 32.0, 24.0)
+    bil = g.bilateral()
+    lv = (L.LevelParams * 1)(g.level_params(0.02))
+    h = C.c_void_p()
+    assert L.lib.gca_pipeline_create(ctx.handle, C.byref(cam), C.byref(bil), 0.2, 3.5, 0, lv, C.byref(h)) == L.GCA_ERR_INVALID_VALUE
+    assert L.lib.gca_pipeline_create(ctx.handle, C.byref(cam), C.byref(bil), 3.5, 0.2, 1, lv, C.byref(h)) == L.GCA_ERR_INVALID_VALUE
+    bad = (L.LevelParams * 1)(g.level_params(0.0))
+    assert L.lib.gca_pipeline_create(ctx.handle, C.byref(cam), C.byref(bil), 0.2, 3.5, 1, bad, C.byref(h)) == L.GCA_ERR_INVALID_VALUE
+    assert L.lib.gca_pipeline_create(ctx.handle, C.byref(cam), C.byref(bil), 0.2, 3.5, 1, lv, C.byref(h)) == 0
+    assert L.lib.gca_pipeline_step(h, None, None) == L.GCA_ERR_INVALID_VALUE
+    T = np.zeros(16)
+    valid = C.c_int32(7)
+    assert L.lib.gca_pipeline_result(h, T.ctypes.data_as(C.c_void_p), None, C.byref(valid)) == 0
+    assert valid.value == 0 and np.array_equal(T.reshape(4, 4), np.eye(4))
+    assert L.lib.gca_pipeline_destroy(h) == 0
