@@ -15,7 +15,8 @@
 //   - a unit maps its source points by the current pose (FP64 -> FP32), finds the nearest
 //     target point in the cached target grid (exact; the search box is shrunk by the previous
 //     iteration's match), forms residuals and 6-D Jacobians in FP32 (reference formulas),
-//     accumulates the 28 sums with TwoSum-compensated FP32 and writes one FP64 partial;
+//     sums the 28 products over its points (TwoSum-compensated FP32 within a pass of up to 256
+//     points, FP64 across passes) and writes one FP64 partial;
 //   - the workgroup that completes the last unit of an epoch (ticket) sums the partials in a
 //     fixed order in FP64, solves the 6x6 system by LDL^T, applies the Euler increment to the
 //     FP64 pose, evaluates the stopping rule and opens the next epoch -- or, when the state is
@@ -196,18 +197,72 @@ struct Acc<GCA_ACCUM_FP64> {
     __device__ __forceinline__ double value(int k) const { return s[k]; }
 };
 
-template <int MODE>
-__device__ __forceinline__ void acc_terms(Acc<MODE> &A, const Terms &t)
+// The 28 FP32 products one correspondence adds: JTJ upper triangle (21), JTr (6), r^2.
+__device__ __forceinline__ void term_products(const Terms &t, float v[28])
 {
     int k = 0;
 #pragma unroll
     for (int a = 0; a < 6; ++a)
 #pragma unroll
-        for (int b = a; b < 6; ++b) A.add(k++, t.Jg[a] * t.Jg[b] + t.Jc[a] * t.Jc[b]);
+        for (int b = a; b < 6; ++b) v[k++] = t.Jg[a] * t.Jg[b] + t.Jc[a] * t.Jc[b];
 #pragma unroll
-    for (int a = 0; a < 6; ++a) A.add(21 + a, t.Jg[a] * t.rg + t.Jc[a] * t.rc);
-    A.add(27, t.rg * t.rg + t.rc * t.rc);
+    for (int a = 0; a < 6; ++a) v[21 + a] = t.Jg[a] * t.rg + t.Jc[a] * t.rc;
+    v[27] = t.rg * t.rg + t.rc * t.rc;
 }
+
+template <int MODE>
+__device__ __forceinline__ void acc_terms(Acc<MODE> &A, const Terms &t)
+{
+    float v[28];
+    term_products(t, v);
+#pragma unroll
+    for (int k = 0; k < 28; ++k) A.add(k, v[k]);
+}
+
+// One sum of the accumulator kind MODE (the scalar form of Acc<MODE>).  An accumulator holding
+// a single term is that term exactly, in every mode.
+template <int MODE>
+struct Sum1;
+
+template <>
+struct Sum1<GCA_ACCUM_COMPENSATED> {
+    float s, c;
+    __device__ void zero() { s = c = 0.f; }
+    __device__ __forceinline__ void add(float x)
+    {
+        const float t = s + x;
+        const float bp = t - s;
+        c += (s - (t - bp)) + (x - bp);
+        s = t;
+    }
+    __device__ __forceinline__ void merge(const Sum1 &o)
+    {
+        const float t = s + o.s;
+        const float bp = t - s;
+        const float e = (s - (t - bp)) + (o.s - bp);
+        s = t;
+        c = (c + o.c) + e;
+    }
+    __device__ double value() const { return (double)s + (double)c; }
+};
+
+template <>
+struct Sum1<GCA_ACCUM_NAIVE_FP32> {
+    float s;
+    __device__ void zero() { s = 0.f; }
+    __device__ __forceinline__ void add(float x) { s += x; }
+    __device__ __forceinline__ void merge(const Sum1 &o) { s += o.s; }
+    __device__ double value() const { return (double)s; }
+};
+
+template <>
+struct Sum1<GCA_ACCUM_FP64> {
+    double s;
+    __device__ void zero() { s = 0.0; }
+    __device__ __forceinline__ void add(float x) { s += (double)x; }
+    __device__ __forceinline__ void merge(const Sum1 &o) { s += o.s; }
+    __device__ double value() const { return s; }
+};
 
 __device__ __forceinline__ void xform_f32(const double *T, float4 p, float &x, float &y, float &z)
 {
@@ -235,6 +290,11 @@ __device__ __forceinline__ void store_coherent(double *p, double v)
 {
     __hip_atomic_store((unsigned long long *)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_coherent(int32_t *p, int32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Block reduction of the accumulator into FP64 partial `out[0..27]`, plus the correspondence
@@ -270,6 +330,39 @@ __device__ __forceinline__ void block_partial(Acc<MODE> &A, int ncorr, int lead,
     }
 }
 
+// Sum over the lanes of one pass of a work unit, where every lane j < Q holds at most one
+// correspondence: rows[k][j] is value k of lane j (k < 28: the products of term_products, zero
+// without a correspondence; k = 28: 1 or 0).  Thread (k = t % 32, chunk = t / 32) sums lanes
+// [chunk Q / 8, (chunk + 1) Q / 8) of value k in lane order with the MODE accumulator; thread k
+// then merges the 8 chunk sums in chunk order into out[k] (FP64).  Starts with the barrier that
+// completes rows[]; out[k] is written by thread k (the caller's next barrier publishes it).
+// Transposing through LDS costs one write and one read per value and lane, where butterflies
+// over the 28 accumulators would take log2(Q) rounds of 28 (or 56) cross-lane moves per lane.
+constexpr int kRowStride = kIcpBlock + 1;  // odd: a wavefront reading 32 rows hits 32 banks
+
+template <int MODE>
+__device__ __forceinline__ void pass_partial(const float (*rows)[kRowStride], int Q, double *out)
+{
+    __shared__ Sum1<MODE> chunk[8][32];
+    const int k = (int)threadIdx.x & 31, ch = (int)threadIdx.x >> 5;
+    __syncthreads();
+    if (k < kNSys) {
+        const int lo = (ch * Q) >> 3, hi = ((ch + 1) * Q) >> 3;
+        Sum1<MODE> a;
+        a.zero();
+#pragma unroll 4
+        for (int j = lo; j < hi; ++j) a.add(rows[k][j]);
+        chunk[ch][k] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNSys) {
+        Sum1<MODE> a = chunk[0][threadIdx.x];
+#pragma unroll
+        for (int c = 1; c < 8; ++c) a.merge(chunk[c][threadIdx.x]);
+        out[threadIdx.x] = a.value();
+    }
+}
+
 // Fixed-order FP64 sum of `n` partials into sys[29] (in LDS): 8 interleaved groups, then the
 // groups in order.  Ends with a barrier.
 __device__ __forceinline__ void sum_partials(const double *partials, int n, double *sys)
@@ -292,15 +385,27 @@ __device__ __forceinline__ void sum_partials(const double *partials, int n, doub
 }
 
 // The same fixed-order sum over partials written with store_coherent (coherent loads, so the
-// reader needs no cache invalidation).
+// reader needs no cache invalidation).  The loads go out in batches of kSumBatch per thread
+// before any add (the coherent loads are served beyond L2; a batch costs one round trip), and
+// padding a batch with +0.0 leaves every sum as the plain loop's.
+constexpr int kSumBatch = 32;
+
 __device__ __forceinline__ void sum_partials_coherent(const double *partials, int n, double *sys)
 {
     __shared__ double part[8][kPartStride];
     const int v = threadIdx.x % kPartStride, grp = threadIdx.x / kPartStride;
     if (v < kNSys) {
         double a = 0.0;
-#pragma unroll 8
-        for (int b = grp; b < n; b += 8) a += load_coherent(partials + (int64_t)b * kPartStride + v);
+        for (int b0 = grp; b0 < n; b0 += 8 * kSumBatch) {
+            double x[kSumBatch];
+#pragma unroll
+            for (int u = 0; u < kSumBatch; ++u) {
+                const int b = b0 + 8 * u;
+                x[u] = b < n ? load_coherent(partials + (int64_t)b * kPartStride + v) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kSumBatch; ++u) a += x[u];
+        }
         part[grp][v] = a;
     }
     __syncthreads();
@@ -431,17 +536,19 @@ __device__ __forceinline__ int32_t nn_group(const PairDesc &P, float qx, float q
 // With Q = 256 / G queries per pass, unit `vb` of a state covers the passes of source points
 // [b * Q, b * Q + Q) for b = vb, vb + nvb, ...  Per pass:
 //   1. nearest neighbours: G lanes per query; the group leader parks the matched slot in LDS;
-//   2. lanes 0 .. Q-1 form the residuals and Jacobians of one query each and accumulate them;
+//   2. lanes 0 .. Q-1 form the residuals, Jacobians and the 28 products of one query each and
+//      park them in LDS; pass_partial sums them (MODE accumulator within the pass);
 //   3. the pass's block partial is added, in pass order and FP64, to the unit's.
-// The accumulator is never live during the search, which keeps register use (and so the
-// number of resident wavefronts hiding the search's load latency) in check.  Leaves the unit's
-// partial in sys[0..28] (LDS; entry k written by thread k).
+// No accumulator is live during the search, which keeps register use (and so the number of
+// resident wavefronts hiding the search's load latency) in check.  Leaves the unit's partial in
+// sys[0..28] (LDS; entry k written by thread k).
 template <int MODE>
 __device__ __forceinline__ void icp_unit(const PairDesc &P, int n, int G, int nvb, int ntgt, int vb, bool warm,
                                          const double *T, double *sys)
 {
     __shared__ int32_t slot[kIcpBlock];
     __shared__ double bsys[kPartStride];
+    __shared__ float rows[kNSys][kRowStride];
     const int lg = __builtin_ctz((unsigned)G);
     const int g = (int)threadIdx.x & (G - 1);
     const int Q = kIcpBlock >> lg;
@@ -462,24 +569,30 @@ __device__ __forceinline__ void icp_unit(const PairDesc &P, int n, int G, int nv
             if (g == 0) slot[j] = s;
         }
         __syncthreads();
-        Acc<MODE> A;
-        A.zero();
-        int ncorr = 0;
         const int i = b0 + (int)threadIdx.x;
-        if ((int)threadIdx.x < Q && i < n) {
-            const int32_t s = slot[threadIdx.x];
-            P.nnprev[i] = s;
-            if (s >= 0) {
-                const float4 p = P.src[i];
-                float x, y, z;
-                xform_f32(T, p, x, y, z);
-                Terms t;
-                corr_terms(x, y, z, p.w, P.sxyzi[s], P.snrm[s], P.sgrad[s], P.sl, P.sc, t);
-                acc_terms(A, t);
-                ncorr = 1;
+        if ((int)threadIdx.x < Q) {
+            float v[28];
+#pragma unroll
+            for (int k = 0; k < 28; ++k) v[k] = 0.f;
+            float c = 0.f;
+            if (i < n) {
+                const int32_t s = slot[threadIdx.x];
+                store_coherent(&P.nnprev[i], s);
+                if (s >= 0) {
+                    const float4 p = P.src[i];
+                    float x, y, z;
+                    xform_f32(T, p, x, y, z);
+                    Terms t;
+                    corr_terms(x, y, z, p.w, P.sxyzi[s], P.snrm[s], P.sgrad[s], P.sl, P.sc, t);
+                    term_products(t, v);
+                    c = 1.f;
+                }
             }
+#pragma unroll
+            for (int k = 0; k < 28; ++k) rows[k][threadIdx.x] = v[k];
+            rows[28][threadIdx.x] = c;
         }
-        block_partial<MODE>(A, ncorr, 1, Q, first ? sys : bsys);
+        pass_partial<MODE>(rows, Q, first ? sys : bsys);
         if (!first && threadIdx.x < kNSys) sys[threadIdx.x] += bsys[threadIdx.x];
         first = false;
         __syncthreads();  // slot[] and the block-partial scratch are reused by the next pass
@@ -515,7 +628,7 @@ __device__ __forceinline__ double euler_elem(int t, const double *trig, const do
 // stopping rule of color_icp.cu:216-228, then the hand-over: the next epoch of this state, or
 // the pose to the next level of the chain.  A failed solve takes the identity step (which also
 // satisfies the stopping rule), as the reference does.
-__device__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double *partials, int nvb, double *sys)
+__device__ __noinline__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double *partials, int nvb, double *sys)
 {
     IcpState *st = states + p;
     __shared__ double Tc[16], Tn[16], dT[16], xi[6], trig[6];
@@ -551,7 +664,7 @@ __device__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double
         const double v =
             ((dT[4 * i] * Tc[j] + dT[4 * i + 1] * Tc[4 + j]) + dT[4 * i + 2] * Tc[8 + j]) + dT[4 * i + 3] * Tc[12 + j];
         Tn[t] = v;
-        st->T[t] = v;
+        store_coherent(&st->T[t], v);
     } else if (t >= 32 && t < 32 + kNSys) {
         st->sys[t - 32] = sys[t - 32];
     } else if (t == 64) {
@@ -560,8 +673,8 @@ __device__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double
         const double tt = (dT[3] * dT[3] + dT[7] * dT[7]) + dT[11] * dT[11];
         const bool conv = cos_t >= 1.0 - f_crot && tt <= f_ct2;
         const bool done = conv || it >= f_max;
-        st->iter = it;
-        if (!f_ok) st->solver_failures = f_fail + 1;
+        store_coherent(&st->iter, it);
+        if (!f_ok) store_coherent(&st->solver_failures, f_fail + 1);
         if (conv) st->converged = 1;
         if (done) {
             st->done = 1;
@@ -571,7 +684,10 @@ __device__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double
         f_done = done ? 1 : 0;
     }
     __syncthreads();
-    if (f_done && f_next >= 0 && t < 16) states[f_next].T[t] = Tn[t];
+    if (f_done && f_next >= 0 && t < 16) store_coherent(&states[f_next].T[t], Tn[t]);
+    // Release before the queue word opens the next epoch or level: readers on other XCDs load
+    // the pose with coherent loads once they see the word.  (A vmcnt wait after the stores is
+    // not enough: it does not order them before the word for readers on other XCDs.)
     fence_release_agent();
     __syncthreads();
     if (t == 0) {
@@ -588,20 +704,23 @@ __device__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double
 }
 
 // Unit partial -> global, ticket, and the epoch finish by the workgroup completing the epoch.
-// The partial is written through to the coherence point and is complete (vmcnt 0: threads
-// 0..28 and the ticket-taking thread 0 are all in wavefront 0) before the ticket, which is all
-// the finisher needs; no L2 write-back or invalidation per unit.
+// The ticket increment is a release (threads 0..28, which store the partial, and the
+// ticket-taking thread 0 are all in wavefront 0) and the finisher acquires after the last
+// ticket: the canonical hand-over, which makes every unit's partial visible to the finisher on
+// whichever XCD it runs.  The partial and the matched slots are written through, so the
+// release has little left to write back.
 template <int MODE>
 __device__ __forceinline__ void unit_commit(const PairDesc &P, IcpState *states, int p, int vb, int nvb,
                                             uint32_t epoch, double *sys)
 {
     __shared__ bool last;
     if (threadIdx.x < kNSys) store_coherent(&P.partials[(int64_t)vb * kPartStride + threadIdx.x], sys[threadIdx.x]);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&states[p].ticket, 1u) == (uint32_t)(nvb - 1);
+    if (threadIdx.x == 0)
+        last = __hip_atomic_fetch_add(&states[p].ticket, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) ==
+               (uint32_t)(nvb - 1);
     __syncthreads();
     if (!last) return;
+    fence_acquire_agent();
     icp_finish(states, p, epoch, P.partials, nvb, sys);
 }
 
@@ -624,7 +743,7 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
     __shared__ double T[12];
     __shared__ double sys[kPartStride];
     __shared__ int s_pick, s_alive, s_vb;
-    __shared__ uint32_t s_epoch;
+    __shared__ uint32_t s_epoch, s_pick_epoch;
     int home = (int)(blockIdx.x % (unsigned)n_states);
     uint32_t idle = 0;
     for (;;) {
@@ -633,17 +752,27 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
             s_alive = 0;
         }
         __syncthreads();
+        int my_rot = INT_MAX;
+        uint32_t my_epoch = 0;
         for (int p = threadIdx.x; p < n_states; p += kIcpBlock) {
             const unsigned long long w =
                 __hip_atomic_load(&states[p].word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const uint32_t e = (uint32_t)(w >> 32), c = (uint32_t)w;
             if (e != kEpochDone) {
                 s_alive = 1;
-                if (e != kEpochWait && c < (uint32_t)states[p].nvb) atomicMin(&s_pick, (p - home + n_states) % n_states);
+                if (e != kEpochWait && c < (uint32_t)states[p].nvb) {
+                    const int rot = (p - home + n_states) % n_states;
+                    if (rot < my_rot) {
+                        my_rot = rot;
+                        my_epoch = e;
+                    }
+                    atomicMin(&s_pick, rot);
+                }
             }
         }
         __syncthreads();
         const int pick = s_pick, alive = s_alive;
+        if (my_rot == pick) s_pick_epoch = my_epoch;  // the one thread that saw the picked state
         __syncthreads();
         if (!alive) break;
         if (pick == INT_MAX) {
@@ -656,6 +785,14 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
         }
         const int p = __builtin_amdgcn_readfirstlane((home + pick) % n_states);
         IcpState *st = states + p;
+        // The claim (wavefront 0) and the pose load (wavefront 1) go out together.  The scan saw
+        // epoch e open, so the pose read now is epoch e's or later; if the claim lands in epoch
+        // e, epoch e cannot finish (and write a later pose) before this unit does, so it is
+        // exactly epoch e's.  A claim in a later epoch reloads the pose after the claim.  The pose
+        // is the only data a unit reads that this launch writes (besides the matched slots,
+        // which only bound the search), and its loads are coherent ones behind the finisher's
+        // release; an acquire fence per claim would also invalidate the target data cached in
+        // L2, which costs about a third of the ICP time.
         if (threadIdx.x == 0) {
             const unsigned long long old =
                 __hip_atomic_fetch_add(&st->word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -663,15 +800,17 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
             const bool ok = e < kEpochWait && c < (uint32_t)st->nvb;
             s_vb = ok ? (int)c : -1;
             s_epoch = e;
+        } else if (threadIdx.x >= kWave && threadIdx.x < kWave + 12) {
+            T[threadIdx.x - kWave] = load_coherent(&st->T[threadIdx.x - kWave]);
         }
         __syncthreads();
         const int vb = s_vb;
         const uint32_t epoch = s_epoch;
         if (vb < 0) continue;  // lost the race for the last unit: poll again
-        // The claim saw epoch e, so the pose of epoch e is published and no later pose can be
-        // written before this unit completes.
-        if (threadIdx.x < 12) T[threadIdx.x] = load_coherent(&st->T[threadIdx.x]);
-        __syncthreads();
+        if (epoch != s_pick_epoch) {
+            if (threadIdx.x < 12) T[threadIdx.x] = load_coherent(&st->T[threadIdx.x]);
+            __syncthreads();
+        }
         idle = 0;
         home = p;
         const PairDesc &P = pairs[p];

@@ -491,7 +491,7 @@ def test_warm_start_matches_full_search(level, gn_case):
     src, tgt = gn_case["src"], level["gpu"]
     ctx = src.ctx
     P = src.download()[0].astype(np.float64)
-    for k in (1, 2, 5):
+    for k in (0, 1, 2, 5):  # k = 0: the first iteration, searched without a previous match
         Tk, _ = single_align(src, tgt, k)
         Tk1, rep1 = single_align(src, tgt, k + 1)
         # the source mapped exactly as the device maps it: FP64 affine map, rounded once to FP32
