@@ -22,6 +22,13 @@ int32_t hip_status(hipError_t e)
     return GCA_ERR_DEVICE;
 }
 
+int32_t ctx_take_device_error(gca_ctx ctx)
+{
+    if (!ctx->dev_err || __atomic_load_n(ctx->dev_err, __ATOMIC_ACQUIRE) == 0u) return GCA_OK;
+    __atomic_store_n(ctx->dev_err, 0u, __ATOMIC_RELEASE);
+    return GCA_ERR_DEVICE;
+}
+
 void *ctx_scratch(gca_ctx ctx, int slot, size_t bytes, bool zeroed)
 {
     if (slot < 0 || slot >= gca_ctx_s::kScratchSlots) return nullptr;
@@ -124,75 +131,140 @@ __global__ __launch_bounds__(256) void k_transform(float4 *__restrict__ xyzi, fl
     }
 }
 
-// ---- exclusive scan of uint32 (reduce-then-scan, 4096 elements per chunk) -----------------
+// ---- exclusive scan of uint32: one pass with decoupled look-back ---------------------------
+// Chunks of kScanChunk elements are handed out in start order by a counter, so a chunk only
+// ever waits on chunks whose workgroups are already running.  Each chunk publishes one 64-bit
+// word per call: (call sequence << 34) | (state << 32) | value, first with its own total (state
+// 1), then, once the totals before it are known, with its inclusive prefix (state 2).  Wavefront
+// 0 looks back over 64 predecessor words at a time and stops at the nearest inclusive prefix.
+// Flag, value and sequence travel in one word, so no ordering between separate stores is
+// needed, and words left by earlier calls carry an older sequence.  The workgroup that takes
+// the last chunk re-arms the counter with the next sequence: nothing is cleared between calls
+// (graph-capture safe).  A wait that outlasts kScanSpin polls stops and raises the ctx's
+// device error (reported by gca_ctx_synchronize); with chunks taken in start order it cannot
+// happen unless the device stops making progress.
 constexpr int kScanBlock = 256;
 constexpr int kScanPer = 16;
 constexpr int kScanChunk = kScanBlock * kScanPer;
+constexpr uint32_t kScanSpin = 1u << 22;
+constexpr unsigned long long kScanAgg = 1ull << 32, kScanInc = 2ull << 32;
+constexpr uint32_t kScanSeqMask = (1u << 30) - 1u;
+
+struct ScanCtl {
+    unsigned long long ctr;  // (sequence << 32) | chunks taken
+};
 
 template <bool POPC>
 __device__ __forceinline__ uint32_t scan_val(uint32_t v) { return POPC ? (uint32_t)__popc(v) : v; }
 
 template <bool POPC>
-__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n,
-                                                            uint32_t *__restrict__ chunk_tot)
+__global__ __launch_bounds__(kScanBlock) void k_scan_onepass(const uint32_t *__restrict__ in, int64_t n,
+                                                             int nchunks, unsigned long long *flags, ScanCtl *ctl,
+                                                             uint32_t *__restrict__ out,
+                                                             uint32_t *__restrict__ total_dev,
+                                                             uint32_t *__restrict__ out_end, uint32_t *dev_err)
 {
     __shared__ uint32_t lds[kScanBlock / kWave + 1];
-    int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanPer; ++k)
-        if (base + k < n) s += scan_val<POPC>(in[base + k]);
-    uint32_t tot;
-    block_excl_scan<kScanBlock>(s, lds, &tot);
-    if (threadIdx.x == 0) chunk_tot[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_chunks(uint32_t *__restrict__ chunk, int nchunks,
-                                                      uint32_t *__restrict__ total_dev,
-                                                      uint32_t *__restrict__ out_end)
-{
-    __shared__ uint32_t lds[1024 / kWave + 1];
-    const int per = (nchunks + 1023) / 1024;
-    const int b = threadIdx.x * per;
-    uint32_t s = 0;
-    for (int k = 0; k < per; ++k)
-        if (b + k < nchunks) s += chunk[b + k];
-    uint32_t tot;
-    uint32_t run = block_excl_scan<1024>(s, lds, &tot);
-    for (int k = 0; k < per; ++k)
-        if (b + k < nchunks) {
-            uint32_t t = chunk[b + k];
-            chunk[b + k] = run;
-            run += t;
-        }
+    __shared__ uint32_t s_chunk, s_seq, s_excl;
     if (threadIdx.x == 0) {
-        if (total_dev) *total_dev = tot;
-        if (out_end) *out_end = tot;
+        const unsigned long long w = __hip_atomic_fetch_add(&ctl->ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_chunk = (uint32_t)w;
+        s_seq = (uint32_t)(w >> 32);
     }
-}
-
-template <bool POPC>
-__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t *__restrict__ in, int64_t n,
-                                                           const uint32_t *__restrict__ chunk_pre,
-                                                           uint32_t *__restrict__ out)
-{
-    __shared__ uint32_t lds[kScanBlock / kWave + 1];
-    int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+    __syncthreads();
+    const int c = (int)s_chunk;
+    const uint32_t seq = s_seq;
+    if (c >= nchunks) {  // only after an interrupted call left the counter behind
+        if (threadIdx.x == 0) __hip_atomic_store(dev_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return;
+    }
+    const int64_t base = (int64_t)c * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+    const bool full = base + kScanPer <= n && ((reinterpret_cast<uintptr_t>(in) & 15u) == 0);
     uint32_t v[kScanPer];
+    if (full) {
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+        for (int q = 0; q < kScanPer / 4; ++q) {
+            const uint4 u = p4[q];
+            v[4 * q] = u.x;
+            v[4 * q + 1] = u.y;
+            v[4 * q + 2] = u.z;
+            v[4 * q + 3] = u.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) v[k] = base + k < n ? in[base + k] : 0u;
+    }
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < kScanPer; ++k) {
-        v[k] = (base + k < n) ? scan_val<POPC>(in[base + k]) : 0u;
+        v[k] = scan_val<POPC>(v[k]);
         s += v[k];
     }
     uint32_t tot;
-    uint32_t run = block_excl_scan<kScanBlock>(s, lds, &tot) + chunk_pre[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanPer; ++k)
-        if (base + k < n) {
-            out[base + k] = run;
-            run += v[k];
+    uint32_t run = block_excl_scan<kScanBlock>(s, lds, &tot);
+    if (threadIdx.x < kWave) {
+        const int lane = lane_id();
+        const unsigned long long tag = (unsigned long long)(seq & kScanSeqMask) << 34;
+        uint32_t excl = 0;
+        if (c == 0) {
+            if (lane == 0) __hip_atomic_store(&flags[0], tag | kScanInc | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0) __hip_atomic_store(&flags[c], tag | kScanAgg | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int end = c;  // window: chunks end - 64 .. end - 1, lane j reads chunk end - 1 - j
+            uint32_t spins = 0;
+            for (;;) {
+                const int k = end - 1 - lane;
+                const unsigned long long w =
+                    k >= 0 ? __hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag | kScanInc);
+                const uint32_t st = (uint32_t)(w >> 32) & 3u;
+                const bool valid = (w >> 34) == (seq & kScanSeqMask) && st != 0u;
+                const unsigned long long inc = __ballot(valid && st == 2u);
+                const int first = inc ? __builtin_ctzll(inc) : kWave;  // nearest inclusive prefix
+                const unsigned long long need = first >= kWave - 1 ? ~0ull : ((2ull << first) - 1ull);
+                if (__ballot(!valid) & need) {
+                    if (++spins > kScanSpin) {
+                        if (lane == 0) __hip_atomic_store(dev_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += wave_sum(lane <= first ? (uint32_t)w : 0u);
+                if (first < kWave) break;
+                end -= kWave;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&flags[c], tag | kScanInc | (uint32_t)(excl + tot), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
+        if (lane == 0) {
+            s_excl = excl;
+            if (c == nchunks - 1) {
+                if (total_dev) *total_dev = excl + tot;
+                if (out_end) *out_end = excl + tot;
+                __hip_atomic_store(&ctl->ctr, (unsigned long long)(seq + 1u) << 32, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    __syncthreads();
+    run += s_excl;
+    uint32_t o[kScanPer];
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        o[k] = run;
+        run += v[k];
+    }
+    if (base + kScanPer <= n && ((reinterpret_cast<uintptr_t>(out) & 15u) == 0)) {
+        uint4 *p4 = reinterpret_cast<uint4 *>(out + base);
+#pragma unroll
+        for (int q = 0; q < kScanPer / 4; ++q) p4[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k)
+            if (base + k < n) out[base + k] = o[k];
+    }
 }
 
 int32_t exclusive_scan_u32(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n,
@@ -201,22 +273,15 @@ int32_t exclusive_scan_u32(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64
     int64_t nchunks = (n + kScanChunk - 1) / kScanChunk;
     if (nchunks < 1) nchunks = 1;
     if (nchunks > 1024 * 64) return GCA_ERR_INVALID_VALUE;
-    uint32_t *chunk = (uint32_t *)ctx_scratch(ctx, 15, sizeof(uint32_t) * nchunks, false);
-    if (!chunk) return GCA_ERR_OUT_OF_MEMORY;
+    unsigned long long *flags = (unsigned long long *)ctx_scratch(ctx, 15, sizeof(unsigned long long) * nchunks, true);
+    ScanCtl *ctl = (ScanCtl *)ctx_scratch(ctx, 23, sizeof(ScanCtl), true);
+    if (!flags || !ctl) return GCA_ERR_OUT_OF_MEMORY;
     if (popcount)
-        hipLaunchKernelGGL(k_scan_reduce<true>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream,
-                           in, n, chunk);
+        hipLaunchKernelGGL(k_scan_onepass<true>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream, in, n,
+                           (int)nchunks, flags, ctl, out, total_dev, out + n, ctx->dev_err);
     else
-        hipLaunchKernelGGL(k_scan_reduce<false>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream,
-                           in, n, chunk);
-    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, ctx->stream, chunk, (int)nchunks,
-                       total_dev, out + n);
-    if (popcount)
-        hipLaunchKernelGGL(k_scan_apply<true>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream,
-                           in, n, chunk, out);
-    else
-        hipLaunchKernelGGL(k_scan_apply<false>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream,
-                           in, n, chunk, out);
+        hipLaunchKernelGGL(k_scan_onepass<false>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream, in, n,
+                           (int)nchunks, flags, ctl, out, total_dev, out + n, ctx->dev_err);
     GCA_HIP(hipGetLastError());
     return GCA_OK;
 }
@@ -249,6 +314,7 @@ int32_t cloud_sync_header(gca_cloud c)
     GCA_HIP(hipMemcpyAsync(&c->host_hdr, c->hdr, sizeof(CloudHdr), hipMemcpyDeviceToHost,
                            c->ctx->stream));
     GCA_HIP(hipStreamSynchronize(c->ctx->stream));
+    GCA_TRY(ctx_take_device_error(c->ctx));  // a header behind a failed scan is not valid
     c->host_valid = true;
     return GCA_OK;
 }
@@ -387,10 +453,15 @@ int32_t gca_ctx_create(int32_t device, gca_ctx *out)
         c->n_cu = 1;
     if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->wall_khz = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess) {
+        hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess ||
+        hipHostMalloc((void **)&c->dev_err, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        if (c->stream) hipStreamDestroy(c->stream);
+        if (c->ev_a) hipEventDestroy(c->ev_a);
+        if (c->ev_b) hipEventDestroy(c->ev_b);
         delete c;
         return GCA_ERR_DEVICE;
     }
+    *c->dev_err = 0u;
     *out = c;
     return GCA_OK;
 }
@@ -411,6 +482,7 @@ int32_t gca_ctx_destroy(gca_ctx c)
     hipEventDestroy(c->ev_a);
     hipEventDestroy(c->ev_b);
     hipStreamDestroy(c->stream);
+    hipHostFree(c->dev_err);
     delete c;
     return GCA_OK;
 }
@@ -419,7 +491,7 @@ int32_t gca_ctx_synchronize(gca_ctx c)
 {
     if (!c) return GCA_ERR_INVALID_VALUE;
     GCA_HIP(hipStreamSynchronize(c->stream));
-    return GCA_OK;
+    return ctx_take_device_error(c);
 }
 
 int32_t gca_ctx_get_stream(gca_ctx c, void **stream)

@@ -87,6 +87,9 @@ struct gca_ctx_s {
     void *scratch[kScratchSlots] = {nullptr};
     size_t scratch_bytes[kScratchSlots] = {0};
     bool scratch_zeroed[kScratchSlots] = {false};
+    // device-raised error word in mapped host memory (a bounded wait that gave up); read and
+    // cleared at synchronisation points
+    uint32_t *dev_err = nullptr;
     int n_cu = 0;         // compute units of the device (persistent launch sizing)
     int wall_khz = 0;     // rate of the device wall clock (wall_clock64)
     // persistent bitmap of the voxel downsampler (kept all-zero between calls)
@@ -151,6 +154,8 @@ namespace gca {
 // Internal helpers implemented in api.hip -------------------------------------------------
 int32_t hip_status(hipError_t e);
 void *ctx_scratch(gca_ctx ctx, int slot, size_t bytes, bool zeroed);
+// GCA_ERR_DEVICE (and clears it) when a kernel raised the ctx's device error word.
+int32_t ctx_take_device_error(gca_ctx ctx);
 int32_t cloud_reserve(gca_cloud c, int64_t cap);
 int32_t cloud_sync_header(gca_cloud c);  // copies hdr to host (synchronises the stream)
 void cloud_host_bounds(gca_cloud c, float mn[3], float mx[3]);

@@ -305,6 +305,7 @@ int32_t gca_pipeline_result(gca_pipeline p, double T[16], gca_icp_report *report
     if (!p || !T || !valid) return GCA_ERR_INVALID_VALUE;
     *valid = 0;
     GCA_HIP(hipStreamSynchronize(p->ctx->stream));
+    GCA_TRY(ctx_take_device_error(p->ctx));
     if (*p->err_host) return GCA_ERR_INVALID_VALUE;  // a voxel lattice was refused
     if (p->frames < 2) {
         static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
