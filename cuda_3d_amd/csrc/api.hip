@@ -394,24 +394,26 @@ int32_t grid_build(gca_ctx ctx, gca_cloud c, float radius, bool exact)
     return GCA_OK;
 }
 
-StageTimer::StageTimer(gca_ctx c, int stage) : ctx(c)
+// Events are recorded on the launching ctx's stream; the record is kept by the owner's profiler
+// (a frame pipeline's level ctxs report to the pipeline's ctx).
+StageTimer::StageTimer(gca_ctx c, int stage) : ctx(c), owner(c->prof_owner ? c->prof_owner : c)
 {
-    if (!ctx->prof_on || ctx->capturing) return;
-    while (ctx->prof_pool.size() < ctx->prof_used + 2) {
+    if (!owner->prof_on || ctx->capturing) return;
+    while (owner->prof_pool.size() < owner->prof_used + 2) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return;
-        ctx->prof_pool.push_back(e);
+        owner->prof_pool.push_back(e);
     }
-    gca_ctx_s::ProfRec r{stage, ctx->prof_pool[ctx->prof_used], ctx->prof_pool[ctx->prof_used + 1]};
-    ctx->prof_used += 2;
+    gca_ctx_s::ProfRec r{stage, owner->prof_pool[owner->prof_used], owner->prof_pool[owner->prof_used + 1]};
+    owner->prof_used += 2;
     if (hipEventRecord(r.a, ctx->stream) != hipSuccess) return;
-    idx = (int)ctx->prof.size();
-    ctx->prof.push_back(r);
+    idx = (int)owner->prof.size();
+    owner->prof.push_back(r);
 }
 
 StageTimer::~StageTimer()
 {
-    if (idx >= 0) (void)hipEventRecord(ctx->prof[idx].b, ctx->stream);
+    if (idx >= 0) (void)hipEventRecord(owner->prof[idx].b, ctx->stream);
 }
 
 }  // namespace gca

@@ -117,6 +117,7 @@ struct gca_ctx_s {
     // being captured into a graph)
     bool prof_on = false;
     bool capturing = false;
+    gca_ctx_s *prof_owner = nullptr;  // stage records go to this ctx's profiler when set
     struct ProfRec {
         int stage;
         hipEvent_t a, b;
@@ -130,6 +131,7 @@ namespace gca {
 // Brackets a stage's launches with hipEvents when the ctx profiler is on (RAII).
 struct StageTimer {
     gca_ctx ctx;
+    gca_ctx owner;  // the ctx whose profiler keeps the record
     int idx = -1;
     StageTimer(gca_ctx c, int stage);
     ~StageTimer();

@@ -2,22 +2,26 @@
 //
 // Replaces the loop of the reference's demo (apps/app.cpp:111-372): create_from_rgbd ->
 // voxel_grid_down_sample -> estimate_normals -> color_icp.align against the previous frame,
-// with the coarse-to-fine schedule of apps/test_color_icp.cpp:378-414.  A step enqueues, on
-// the pipeline's own stream and without any host round trip:
-//   front end (bilateral, back-projection, compaction, bounds)
-//   per level: voxel lattice from the device header + downsample, neighbour grid of fixed
-//              geometry, normals, colour gradient, grid-ordered attributes
-//   registration of the new frame onto the previous one: all levels chained in one persistent
-//              launch (a registration plan built once per frame parity), results copied to
-//              pinned host memory
+// with the coarse-to-fine schedule of apps/test_color_icp.cpp:378-414.  A step enqueues,
+// without any host round trip:
+//   front end (bilateral, back-projection, compaction, bounds), on the pipeline's stream;
+//   per level, on the level's own stream (the levels only read the front end's cloud, so they
+//              run side by side: the small coarse levels fill the device next to the fine one):
+//              voxel lattice from the device header + downsample, neighbour grid of fixed
+//              geometry, normals, colour gradient, grid-ordered attributes;
+//   once every level has joined back: registration of the new frame onto the previous one,
+//              all levels chained in one persistent launch (a registration plan built once per
+//              frame parity), results copied to pinned host memory.
 // Nothing in a step depends on host-side sizes: the voxel lattice is computed on the device
 // (its scan covers an upper bound from the camera frustum), and each level's grid uses one
 // geometry over the frustum box (neighbour results do not depend on the grid geometry: points
 // and query ranges are clamped alike).  So after one eager step per frame parity, each parity's
 // step is captured into a hipGraph and replayed.
 //
-// The pipeline owns a private ctx: the scratch buffers a captured graph points into are used by
-// nothing else, so no other call can move them.
+// The pipeline owns private ctxs (one for the front end and registration, one per level, each
+// with its stream and scratch): the scratch buffers a captured graph points into are used by
+// nothing else, so no other call can move them.  Every cloud belongs to the main ctx; a level's
+// kernels run on its level ctx, ordered by the fork / join events.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,7 +37,9 @@ using namespace gca;
 constexpr int kPipeMaxLevels = 8;
 
 struct gca_pipeline_s {
-    gca_ctx ctx = nullptr;  // private
+    gca_ctx ctx = nullptr;  // private: front end, registration, every cloud
+    gca_ctx lctx[kPipeMaxLevels] = {};  // private, one per level (stream + scratch)
+    hipEvent_t ev_fork = nullptr, ev_join[kPipeMaxLevels] = {};
     gca_camera cam{};
     gca_bilateral bil{};
     float tmin = 0.f, tmax = 0.f;
@@ -102,16 +108,21 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
     gca_ctx ctx = p->ctx;
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(64), 0, ctx->stream, p->err_dev);
     GCA_TRY(launch_from_rgbd(ctx, depth, bgr, &p->cam, &p->bil, p->tmin, p->tmax, p->full[slot]));
+    GCA_HIP(hipEventRecord(p->ev_fork, ctx->stream));
     for (int l = 0; l < p->nl; ++l) {
+        gca_ctx lc = p->lctx[l];
         gca_cloud c = p->lvc[slot][l];
         const gca_level_params &L = p->lv[l];
-        GCA_TRY(launch_voxel_planned(ctx, p->full[slot], L.voxel_size, c, p->max_bits[l], p->err_dev));
-        GCA_TRY(launch_grid_build(ctx, c, p->gdesc[l]));
-        GCA_TRY(launch_normals(ctx, c, L.normal_radius, L.normal_max_nn));
-        GCA_TRY(launch_gradient(ctx, c, L.gradient_radius, L.gradient_max_nn));
+        GCA_HIP(hipStreamWaitEvent(lc->stream, p->ev_fork, 0));
+        GCA_TRY(launch_voxel_planned(lc, p->full[slot], L.voxel_size, c, p->max_bits[l], p->err_dev));
+        GCA_TRY(launch_grid_build(lc, c, p->gdesc[l]));
+        GCA_TRY(launch_normals(lc, c, L.normal_radius, L.normal_max_nn));
+        GCA_TRY(launch_gradient(lc, c, L.gradient_radius, L.gradient_max_nn));
         c->has_normals = c->has_gradient = true;
         c->grid.attrs_valid = false;
-        GCA_TRY(grid_gather_attributes(ctx, c));
+        GCA_TRY(grid_gather_attributes(lc, c));
+        GCA_HIP(hipEventRecord(p->ev_join[l], lc->stream));
+        GCA_HIP(hipStreamWaitEvent(ctx->stream, p->ev_join[l], 0));
     }
     if (p->frames >= 1) GCA_TRY(icp_plan_launch(ctx, p->plan[slot]));
     GCA_HIP(hipMemcpyAsync(p->err_host, p->err_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -123,6 +134,7 @@ int32_t capture(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
     gca_ctx ctx = p->ctx;
     hipGraph_t g = nullptr;
     ctx->capturing = true;
+    for (int l = 0; l < p->nl; ++l) p->lctx[l]->capturing = true;
     hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
     int32_t st = e == hipSuccess ? enqueue(p, slot, depth, bgr) : hip_status(e);
     if (e == hipSuccess) {
@@ -130,6 +142,7 @@ int32_t capture(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         if (st == GCA_OK && e2 != hipSuccess) st = hip_status(e2);
     }
     ctx->capturing = false;
+    for (int l = 0; l < p->nl; ++l) p->lctx[l]->capturing = false;
     if (st == GCA_OK && !g) st = GCA_ERR_DEVICE;
     hipGraphExec_t ex = nullptr;
     if (st == GCA_OK) {
@@ -180,6 +193,8 @@ int32_t gca_pipeline_destroy(gca_pipeline p)
 {
     if (!p) return GCA_ERR_INVALID_VALUE;
     if (p->ctx) hipStreamSynchronize(p->ctx->stream);
+    for (int l = 0; l < kPipeMaxLevels; ++l)
+        if (p->lctx[l]) hipStreamSynchronize(p->lctx[l]->stream);
     drop_plans(p);
     for (int s = 0; s < 2; ++s) {
         if (p->full[s]) gca_cloud_destroy(p->full[s]);
@@ -188,6 +203,11 @@ int32_t gca_pipeline_destroy(gca_pipeline p)
     }
     hipFree(p->err_dev);
     hipHostFree(p->err_host);
+    if (p->ev_fork) hipEventDestroy(p->ev_fork);
+    for (int l = 0; l < kPipeMaxLevels; ++l) {
+        if (p->ev_join[l]) hipEventDestroy(p->ev_join[l]);
+        if (p->lctx[l]) gca_ctx_destroy(p->lctx[l]);
+    }
     if (p->ctx) gca_ctx_destroy(p->ctx);
     delete p;
     return GCA_OK;
@@ -227,6 +247,13 @@ int32_t gca_pipeline_create(gca_ctx ctx, const gca_camera *cam, const gca_bilate
     p->nl = n_levels;
     memcpy(p->lv, levels, sizeof(gca_level_params) * n_levels);
     int32_t st = gca_ctx_create(ctx->device, &p->ctx);
+    for (int l = 0; l < n_levels && st == GCA_OK; ++l) {
+        st = gca_ctx_create(ctx->device, &p->lctx[l]);
+        if (st == GCA_OK) p->lctx[l]->prof_owner = p->ctx;
+        if (st == GCA_OK && hipEventCreateWithFlags(&p->ev_join[l], hipEventDisableTiming) != hipSuccess)
+            st = GCA_ERR_DEVICE;
+    }
+    if (st == GCA_OK && hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess) st = GCA_ERR_DEVICE;
     const int64_t cap = (int64_t)cam->width * cam->height;
     for (int s = 0; s < 2 && st == GCA_OK; ++s) {
         st = gca_cloud_create(p->ctx, cap, &p->full[s]);
@@ -280,6 +307,8 @@ int32_t gca_pipeline_step(gca_pipeline p, const uint16_t *depth_dev, const uint8
         drop_plans(p);
         p->tune_version = p->ctx->tune_version;
     }
+    for (int l = 0; l < p->nl; ++l)  // the level ctxs launch with the pipeline ctx's shapes
+        memcpy(p->lctx[l]->tune, p->ctx->tune, sizeof(p->ctx->tune));
     if (p->frames >= 1 && !p->plan[slot]) GCA_TRY(make_plan(p, slot));
     // a graph is captured only once every buffer it will point into exists: after an eager
     // step of each frame parity with its plan in place
@@ -306,6 +335,7 @@ int32_t gca_pipeline_result(gca_pipeline p, double T[16], gca_icp_report *report
     *valid = 0;
     GCA_HIP(hipStreamSynchronize(p->ctx->stream));
     GCA_TRY(ctx_take_device_error(p->ctx));
+    for (int l = 0; l < p->nl; ++l) GCA_TRY(ctx_take_device_error(p->lctx[l]));
     if (*p->err_host) return GCA_ERR_INVALID_VALUE;  // a voxel lattice was refused
     if (p->frames < 2) {
         static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
