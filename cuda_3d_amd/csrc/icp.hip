@@ -364,36 +364,16 @@ __device__ __forceinline__ void pass_partial(const float (*rows)[kRowStride], in
 }
 
 // Fixed-order FP64 sum of `n` partials into sys[29] (in LDS): 8 interleaved groups, then the
-// groups in order.  Ends with a barrier.
+// groups in order.  The loads go out in batches of kSumBatch per thread before any add (a batch
+// costs one memory round trip), and padding a batch with +0.0 leaves every sum as the plain
+// loop's.  The caller has made the partials visible (kernel boundary, or release / acquire).
+// Ends with a barrier.
+constexpr int kSumBatch = 32;
+
 __device__ __forceinline__ void sum_partials(const double *partials, int n, double *sys)
 {
     __shared__ double part[8][kPartStride];
     const int v = threadIdx.x % kPartStride, grp = threadIdx.x / kPartStride;  // 8 groups of 32
-    if (v < kNSys) {
-        double a = 0.0;
-#pragma unroll 8
-        for (int b = grp; b < n; b += 8) a += partials[(int64_t)b * kPartStride + v];
-        part[grp][v] = a;
-    }
-    __syncthreads();
-    if (threadIdx.x < kNSys) {
-        double a = 0.0;
-        for (int g = 0; g < 8; ++g) a += part[g][threadIdx.x];
-        sys[threadIdx.x] = a;
-    }
-    __syncthreads();
-}
-
-// The same fixed-order sum over partials written with store_coherent (coherent loads, so the
-// reader needs no cache invalidation).  The loads go out in batches of kSumBatch per thread
-// before any add (the coherent loads are served beyond L2; a batch costs one round trip), and
-// padding a batch with +0.0 leaves every sum as the plain loop's.
-constexpr int kSumBatch = 32;
-
-__device__ __forceinline__ void sum_partials_coherent(const double *partials, int n, double *sys)
-{
-    __shared__ double part[8][kPartStride];
-    const int v = threadIdx.x % kPartStride, grp = threadIdx.x / kPartStride;
     if (v < kNSys) {
         double a = 0.0;
         for (int b0 = grp; b0 < n; b0 += 8 * kSumBatch) {
@@ -401,7 +381,7 @@ __device__ __forceinline__ void sum_partials_coherent(const double *partials, in
 #pragma unroll
             for (int u = 0; u < kSumBatch; ++u) {
                 const int b = b0 + 8 * u;
-                x[u] = b < n ? load_coherent(partials + (int64_t)b * kPartStride + v) : 0.0;
+                x[u] = b < n ? partials[(int64_t)b * kPartStride + v] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < kSumBatch; ++u) a += x[u];
@@ -647,7 +627,7 @@ __device__ __noinline__ void icp_finish(IcpState *states, int p, uint32_t epoch,
         f_crot = load_coherent(&st->conv_rot);
         f_ct2 = load_coherent(&st->conv_t2);
     }
-    sum_partials_coherent(partials, nvb, sys);
+    sum_partials(partials, nvb, sys);  // after the acquire in unit_commit
     if (t == 0) f_ok = solve6(sys, sys + 21, xi) ? 1 : 0;
     __syncthreads();
     if (t < 3) {
@@ -1074,21 +1054,23 @@ __global__ void k_state_simple(IcpState *st, const double *__restrict__ T0)
 
 static int32_t icp_blocks(int64_t n) { return (int32_t)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kIcpBlock - 1) / kIcpBlock)); }
 
-// Lanes per correspondence query by source size: wide groups shorten each query's dependent
-// load chain, which is what bounds small clouds; narrow groups keep large clouds' wave count
-// reasonable.
-// A call with one pair is latency-bound (the GPU is wider than a small cloud's queries): wide
-// lane groups shorten each query's search.  A call with many pairs is throughput-bound: one
-// lane per query wastes no lane on short candidate lists.  (Measured defaults; the autotuner
-// overrides them.)
+// Lanes per correspondence query by source size.  A call with one pair is latency-bound (the
+// GPU is wider than a small cloud's queries): wide lane groups shorten each query's dependent
+// load chain.  A call with many pairs is throughput-bound: one lane per query wastes no lane on
+// short candidate lists.  Defaults: the autotuner's choice on MI355X for the 640x480 frame
+// pipeline (levels of ~5k, ~20k and ~74k points) and for 64 uneven pairs; a ctx's tuning
+// overrides them.
 static void icp_shape(gca_ctx ctx, int n_pairs, int32_t lanes[3], int32_t *unit_cap)
 {
-    static const int32_t def[3] = {16, 2, 4};
+    static const int32_t def[3] = {8, 4, 4};
     for (int k = 0; k < 3; ++k) {
         if (n_pairs > 1) lanes[k] = ctx->tune[GCA_TUNE_ICP_LANES_BATCH] ? ctx->tune[GCA_TUNE_ICP_LANES_BATCH] : 1;
         else lanes[k] = ctx->tune[GCA_TUNE_ICP_LANES_SMALL + k] ? ctx->tune[GCA_TUNE_ICP_LANES_SMALL + k] : def[k];
     }
-    *unit_cap = std::min(kMaxUnits, ctx->tune[GCA_TUNE_ICP_MAX_UNITS] ? ctx->tune[GCA_TUNE_ICP_MAX_UNITS] : 512);
+    // units per iteration: one pair needs them to spread over the device, many pairs already
+    // do, and fewer units mean fewer partials to sum per iteration
+    const int32_t def_units = n_pairs > 1 ? 128 : 512;
+    *unit_cap = std::min(kMaxUnits, ctx->tune[GCA_TUNE_ICP_MAX_UNITS] ? ctx->tune[GCA_TUNE_ICP_MAX_UNITS] : def_units);
 }
 
 struct IcpWork {

@@ -407,7 +407,9 @@ class tuned:
         self.ctx.set_tuning(**{k: 0 for k in self.kw})
 
 
-SAME_LANES = dict(icp_lanes_small=4, icp_lanes_medium=4, icp_lanes_large=4, icp_lanes_batch=4)
+# One launch shape for one-pair and batched calls (their defaults differ: lanes per query and the
+# cap on work units, which fixes the partition and so the FP64 summation order).
+SAME_SHAPE = dict(icp_lanes_small=4, icp_lanes_medium=4, icp_lanes_large=4, icp_lanes_batch=4, icp_max_units=512)
 
 
 def test_batch_matches_single_pairs(ctx, uneven_pairs):
@@ -417,7 +419,7 @@ def test_batch_matches_single_pairs(ctx, uneven_pairs):
     n = len(srcs)
     # one-pair and batched calls with the same lanes per query: each pair's work partition
     # depends on its own size only, so the poses are bit-identical
-    with tuned(ctx, **SAME_LANES):
+    with tuned(ctx, **SAME_SHAPE):
         Tb, reps = g.batch_icp(srcs, tgts, icp_p(g, 30, 0.05), [np.eye(4)] * n)
         singles = [single_align(srcs[k], tgts[k]) for k in range(n)]
     for k in range(n):
@@ -466,7 +468,7 @@ def test_batch_multiscale_matches_sequential(ctx, frame):
         Tg.append(t)
     params = [icp_p(g, 30, 2.5 * v, conv=(1e-6, 1e-6)) for v in vs]
     for launch in (1, 2):
-        with tuned(ctx, icp_launch=launch, **SAME_LANES):
+        with tuned(ctx, icp_launch=launch, **SAME_SHAPE):
             Tb, reps = g.batch_multiscale_icp([S, S], [Tg, Tg], params, [np.eye(4), frame.G], [2 * v for v in vs])
             for k, T0 in enumerate((np.eye(4), frame.G)):
                 T = T0
