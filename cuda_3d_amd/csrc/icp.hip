@@ -9,9 +9,10 @@
 // by ONE persistent launch that runs every iteration of every registration state (a source /
 // target pair at one pyramid level):
 //   - an iteration of a state is split into `nvb` work units (fixed strided sets of source
-//     points).  Workgroups claim units from the state's queue word (epoch << 32 | units claimed)
-//     with one atomic, so any number of resident workgroups serve any number of states and a
-//     workgroup never waits on work that nobody has claimed;
+//     points), dealt round-robin into kSubQ sub-queues.  Workgroups claim units from a
+//     sub-queue's word (epoch << 32 | units claimed) with one atomic, so any number of resident
+//     workgroups serve any number of states and a workgroup never waits on work that nobody has
+//     claimed;
 //   - a unit maps its source points by the current pose (FP64 -> FP32), finds the nearest
 //     target point in the cached target grid (exact; the search box is shrunk by the previous
 //     iteration's match), forms residuals and 6-D Jacobians in FP32 (reference formulas),
@@ -50,6 +51,21 @@ constexpr uint32_t kEpochDone = 0xffffffffu;
 // ends a launch whose states can no longer make progress.
 constexpr uint32_t kSpinLimit = 1u << 22;
 
+// Claim words of a state: (epoch << 32) | units claimed, one per sub-queue, each on its own
+// 256-byte line.  Every workgroup that finds an epoch open attempts a claim; with one word those
+// atomics serialise (~11 ns each, ~6 us for 512 workgroups), with kSubQ words they spread.
+constexpr int kSubQ = 8;
+
+struct alignas(256) QWord {
+    unsigned long long w;
+};
+
+// Units of sub-queue x of a state with nvb units.
+__host__ __device__ inline uint32_t subq_units(int nvb, int x)
+{
+    return nvb > x ? (uint32_t)((nvb - x + kSubQ - 1) / kSubQ) : 0u;
+}
+
 struct IcpState {
     double T[16];
     double sys[kNSys];
@@ -57,7 +73,7 @@ struct IcpState {
     int32_t iter, max_iter, done, converged, solver_failures, n_src;
     int32_t G, nvb, n_tgt, next;  // lanes per query, work units, target size, next level (-1)
     long long t_init, t_done;     // device wall clock at set-up and when the state finished
-    alignas(256) unsigned long long word;  // queue word: (epoch << 32) | units claimed
+    QWord q[kSubQ];                        // sub-queue x holds units x, x + kSubQ, ...
     alignas(256) uint32_t ticket;          // units completed in the current epoch
     alignas(256) uint32_t pad_end;
 };
@@ -670,15 +686,15 @@ __device__ __noinline__ void icp_finish(IcpState *states, int p, uint32_t epoch,
     // not enough: it does not order them before the word for readers on other XCDs.)
     fence_release_agent();
     __syncthreads();
-    if (t == 0) {
+    if (t < kSubQ) {
         if (!f_done) {
-            __hip_atomic_store(&st->word, (unsigned long long)(epoch + 1) << 32, __ATOMIC_RELAXED,
+            __hip_atomic_store(&st->q[t].w, (unsigned long long)(epoch + 1) << 32, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(&st->word, (unsigned long long)kEpochDone << 32, __ATOMIC_RELAXED,
+            __hip_atomic_store(&st->q[t].w, (unsigned long long)kEpochDone << 32, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
             if (f_next >= 0)
-                __hip_atomic_store(&states[f_next].word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&states[f_next].q[t].w, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -705,13 +721,14 @@ __device__ __forceinline__ void unit_commit(const PairDesc &P, IcpState *states,
 }
 
 // ---- persistent launch: every iteration of every state --------------------------------------
-// Scheduling loop of one workgroup: read all queue words (one coherent load per state, in
-// parallel), pick the first state with unclaimed units in round-robin order from the state
-// last served, claim a unit with one atomic, run it.  A workgroup only waits (sleep + re-poll)
+// Scheduling loop of one workgroup: read all claim words (one coherent load per state and
+// sub-queue, in parallel), pick the first state with unclaimed units in round-robin order from
+// the state last served and, in it, the sub-queue nearest to the workgroup's own, claim a unit
+// with one atomic, run it.  A workgroup only waits (sleep + re-poll)
 // when every unit of every open epoch is claimed, i.e. held by running workgroups, so progress
 // never depends on how many workgroups are resident.  Exits when every state is done.
 // (Claiming first and scanning only on failure was measured slower: every workgroup that ends
-// a unit after the epoch is drained adds a failing atomic on the hot queue word.)
+// a unit after the epoch is drained adds a failing atomic on a hot claim word.)
 // 4 wavefronts per SIMD (<= 128 VGPRs): resident wavefronts are what hides the latency of the
 // neighbour search's dependent loads.
 #define GCA_ICP_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 8)))
@@ -725,6 +742,7 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
     __shared__ int s_pick, s_alive, s_vb;
     __shared__ uint32_t s_epoch, s_pick_epoch;
     int home = (int)(blockIdx.x % (unsigned)n_states);
+    const int myq = (int)(blockIdx.x % (unsigned)kSubQ);  // workgroups are dealt to XCDs round-robin
     uint32_t idle = 0;
     for (;;) {
         if (threadIdx.x == 0) {
@@ -734,14 +752,17 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
         __syncthreads();
         int my_rot = INT_MAX;
         uint32_t my_epoch = 0;
-        for (int p = threadIdx.x; p < n_states; p += kIcpBlock) {
+        for (int k = threadIdx.x; k < n_states * kSubQ; k += kIcpBlock) {
+            const int p = k / kSubQ, x = k % kSubQ;
             const unsigned long long w =
-                __hip_atomic_load(&states[p].word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_load(&states[p].q[x].w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const uint32_t e = (uint32_t)(w >> 32), c = (uint32_t)w;
             if (e != kEpochDone) {
                 s_alive = 1;
-                if (e != kEpochWait && c < (uint32_t)states[p].nvb) {
-                    const int rot = (p - home + n_states) % n_states;
+                if (e != kEpochWait && c < subq_units(states[p].nvb, x)) {
+                    // first state in round-robin order from the last one served, then the
+                    // sub-queue nearest to this workgroup's own
+                    const int rot = ((p - home + n_states) % n_states) * kSubQ + (x - myq + kSubQ) % kSubQ;
                     if (rot < my_rot) {
                         my_rot = rot;
                         my_epoch = e;
@@ -763,7 +784,8 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
             __builtin_amdgcn_s_sleep(2);
             continue;
         }
-        const int p = __builtin_amdgcn_readfirstlane((home + pick) % n_states);
+        const int p = __builtin_amdgcn_readfirstlane((home + pick / kSubQ) % n_states);
+        const int x = __builtin_amdgcn_readfirstlane((myq + pick % kSubQ) % kSubQ);
         IcpState *st = states + p;
         // The claim (wavefront 0) and the pose load (wavefront 1) go out together.  The scan saw
         // epoch e open, so the pose read now is epoch e's or later; if the claim lands in epoch
@@ -775,10 +797,10 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
         // L2, which costs about a third of the ICP time.
         if (threadIdx.x == 0) {
             const unsigned long long old =
-                __hip_atomic_fetch_add(&st->word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&st->q[x].w, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const uint32_t e = (uint32_t)(old >> 32), c = (uint32_t)old;
-            const bool ok = e < kEpochWait && c < (uint32_t)st->nvb;
-            s_vb = ok ? (int)c : -1;
+            const bool ok = e < kEpochWait && c < subq_units(st->nvb, x);
+            s_vb = ok ? x + kSubQ * (int)c : -1;
             s_epoch = e;
         } else if (threadIdx.x >= kWave && threadIdx.x < kWave + 12) {
             T[threadIdx.x - kWave] = load_coherent(&st->T[threadIdx.x - kWave]);
@@ -1033,8 +1055,9 @@ __global__ void k_state_init(IcpState *states, const PairDesc *__restrict__ pair
     s.done = I.max_iter <= 0 ? 1 : 0;
     s.t_init = wall_clock64();
     s.t_done = s.t_init;
-    s.word = I.max_iter <= 0 ? ((unsigned long long)kEpochDone << 32)
-                             : (I.open ? 0ull : ((unsigned long long)kEpochWait << 32));
+    const unsigned long long w0 = I.max_iter <= 0 ? ((unsigned long long)kEpochDone << 32)
+                                                  : (I.open ? 0ull : ((unsigned long long)kEpochWait << 32));
+    for (int x = 0; x < kSubQ; ++x) s.q[x].w = w0;
     s.ticket = 0u;
 }
 
@@ -1498,7 +1521,7 @@ int32_t gca_build_gauss_newton(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const 
                            T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11], sl, sc);
     GCA_HIP(hipGetLastError());
     IcpState S;
-    GCA_HIP(hipMemcpyAsync(&S, w.states, offsetof(IcpState, word), hipMemcpyDeviceToHost, ctx->stream));
+    GCA_HIP(hipMemcpyAsync(&S, w.states, offsetof(IcpState, q), hipMemcpyDeviceToHost, ctx->stream));
     GCA_HIP(hipStreamSynchronize(ctx->stream));
     memcpy(out, S.sys, sizeof(double) * kNSys);
     return GCA_OK;
