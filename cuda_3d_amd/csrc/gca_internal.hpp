@@ -83,7 +83,7 @@ struct gca_ctx_s {
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     // scratch buffers owned by the ctx, grown on demand
-    static constexpr int kScratchSlots = 24;
+    static constexpr int kScratchSlots = 26;
     void *scratch[kScratchSlots] = {nullptr};
     size_t scratch_bytes[kScratchSlots] = {0};
     bool scratch_zeroed[kScratchSlots] = {false};
@@ -193,6 +193,9 @@ int32_t launch_outlier(gca_ctx ctx, gca_cloud in, float radius, int32_t min_nn, 
 int32_t grid_gather_attributes(gca_ctx ctx, gca_cloud c);
 int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn);
 int32_t launch_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn);
+// Normals (radius, max_nn) and colour gradient (gradient_radius <= radius, all neighbours) from
+// one neighbourhood walk; results equal launch_normals followed by launch_gradient.
+int32_t launch_normals_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn, float gradient_radius);
 int32_t launch_transform(gca_ctx ctx, gca_cloud c, const double T[16]);
 int32_t launch_bounds(gca_ctx ctx, gca_cloud c);
 int32_t launch_copy_cloud(gca_ctx ctx, gca_cloud src, gca_cloud dst);

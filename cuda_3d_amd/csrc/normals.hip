@@ -9,7 +9,10 @@
 // one flattened candidate walk with several loads in flight per lane), and the neighbourhood
 // sums are accumulated in 2^-40 fixed point scaled by the radius, centred on the query point:
 // integer adds are exact, so neither G nor the reduction order changes the result, which equals
-// the FP64 oracle's.  A query with more than K > 0 points within the radius is finished by one
+// the FP64 oracle's.  The colour gradient's sums are the raw ones (sum u u^T, sum u dI); its
+// tangent projection is applied to the sums in the solve, so they need no normal and can be
+// gathered in the same walk as the normals' (k_moments_pair) when the gradient radius is the
+// smaller one.  A query with more than K > 0 points within the radius is finished by one
 // wavefront that buffers the in-radius candidates in LDS and keeps a candidate iff fewer than K
 // others precede it in (d2, index) order.  The covariance is then diagonalised per point with a
 // cyclic Jacobi iteration in FP64 (only +, -, *, /, sqrt: identical on host and device).
@@ -217,8 +220,8 @@ struct Moments {
     int32_t pad;
 };
 
-// Calls f(p) on the lane for every candidate of its share with d2 < r^2; returns the group's
-// in-radius count (on every lane of the group).
+// Calls f(p, d2) on the lane for every candidate of its share with d2 < r^2; returns the
+// group's in-radius count (on every lane of the group).
 template <class F>
 __device__ __forceinline__ int32_t group_walk(const GridRefs &G, float4 q, int g, int gs, F &&f)
 {
@@ -262,11 +265,14 @@ __device__ __forceinline__ int32_t group_walk(const GridRefs &G, float4 q, int g
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < kNbUnroll; ++u)
-                    if (ok[u] && dist2(c[u].x, c[u].y, c[u].z, q.x, q.y, q.z) < r2) {
-                        f(c[u]);
+                for (int u = 0; u < kNbUnroll; ++u) {
+                    if (!ok[u]) continue;
+                    const double d2 = dist2(c[u].x, c[u].y, c[u].z, q.x, q.y, q.z);
+                    if (d2 < r2) {
+                        f(c[u], d2);
                         ++cnt;
                     }
+                }
             }
         } else {
             // cell smaller than the radius (not produced by grid_build): plain run walk
@@ -276,8 +282,9 @@ __device__ __forceinline__ int32_t group_walk(const GridRefs &G, float4 q, int g
                     const uint32_t s0 = G.start[row + lo[2]], s1 = G.start[row + hi[2] + 1];
                     for (uint32_t sl = s0 + g; sl < s1; sl += gs) {
                         const float4 c = G.sxyzi[sl];
-                        if (dist2(c.x, c.y, c.z, q.x, q.y, q.z) < r2) {
-                            f(c);
+                        const double d2 = dist2(c.x, c.y, c.z, q.x, q.y, q.z);
+                        if (d2 < r2) {
+                            f(c, d2);
                             ++cnt;
                         }
                     }
@@ -288,12 +295,13 @@ __device__ __forceinline__ int32_t group_walk(const GridRefs &G, float4 q, int g
     return cnt;
 }
 
-// Neighbourhood moments (2^-40 fixed point) of one neighbour p of query q.
-//   normals:  u = (p - q) / r;  S = sum u, sum u u^T (xx xy xz yy yz zz)
-//   gradient: e = p - q, w = (e - (e.n) n) / r, dI = I_p - I_q;  S = sum w w^T, sum w dI
+// Neighbourhood moments (2^-40 fixed point) of one neighbour p of query q, u = (p - q) / r:
+//   normals:  S = sum u, sum u u^T (xx xy xz yy yz zz)
+//   gradient: S = sum u u^T, sum u dI with dI = I_p - I_q (raw: the projection onto the query's
+//             tangent plane is applied to the sums, gradient_from_moments)
 // The query itself contributes exactly zero to every sum (and one to the count).
 template <int KIND>
-__device__ __forceinline__ void add_moments(long long S[9], float4 q, float4 p, const double nv[3], double inv_r)
+__device__ __forceinline__ void add_moments(long long S[9], float4 q, float4 p, double inv_r)
 {
     if (KIND == 0) {
         const double u0 = ((double)p.x - (double)q.x) * inv_r;
@@ -309,23 +317,19 @@ __device__ __forceinline__ void add_moments(long long S[9], float4 q, float4 p, 
         S[7] += fx((u1 * u2) * kFx40);
         S[8] += fx((u2 * u2) * kFx40);
     } else {
-        const double e0 = (double)p.x - (double)q.x;
-        const double e1 = (double)p.y - (double)q.y;
-        const double e2 = (double)p.z - (double)q.z;
-        const double tn = (e0 * nv[0] + e1 * nv[1]) + e2 * nv[2];
-        const double w0 = (e0 - tn * nv[0]) * inv_r;
-        const double w1 = (e1 - tn * nv[1]) * inv_r;
-        const double w2 = (e2 - tn * nv[2]) * inv_r;
+        const double u0 = ((double)p.x - (double)q.x) * inv_r;
+        const double u1 = ((double)p.y - (double)q.y) * inv_r;
+        const double u2 = ((double)p.z - (double)q.z) * inv_r;
         const double dI = (double)p.w - (double)q.w;
-        S[0] += fx((w0 * w0) * kFx40);
-        S[1] += fx((w0 * w1) * kFx40);
-        S[2] += fx((w0 * w2) * kFx40);
-        S[3] += fx((w1 * w1) * kFx40);
-        S[4] += fx((w1 * w2) * kFx40);
-        S[5] += fx((w2 * w2) * kFx40);
-        S[6] += fx((w0 * dI) * kFx40);
-        S[7] += fx((w1 * dI) * kFx40);
-        S[8] += fx((w2 * dI) * kFx40);
+        S[0] += fx((u0 * u0) * kFx40);
+        S[1] += fx((u0 * u1) * kFx40);
+        S[2] += fx((u0 * u2) * kFx40);
+        S[3] += fx((u1 * u1) * kFx40);
+        S[4] += fx((u1 * u2) * kFx40);
+        S[5] += fx((u2 * u2) * kFx40);
+        S[6] += fx((u0 * dI) * kFx40);
+        S[7] += fx((u1 * dI) * kFx40);
+        S[8] += fx((u2 * dI) * kFx40);
     }
 }
 
@@ -350,15 +354,70 @@ __global__ __launch_bounds__(kNbBlock) void k_moments(const float4 *__restrict__
         const int qi = base + ((int)threadIdx.x >> lg);
         if (qi >= n) break;  // whole groups leave together
         const float4 q = xyzi[qi];
-        double nv[3] = {0.0, 0.0, 0.0};
-        if (KIND == 1) {
-            const float4 nf = nrm[qi];
-            nv[0] = nf.x;
-            nv[1] = nf.y;
-            nv[2] = nf.z;
-        }
         long long S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        const int32_t cnt = group_walk(G, q, gl, gs, [&](float4 p) { add_moments<KIND>(S, q, p, nv, inv_r); });
+        const int32_t cnt = group_walk(G, q, gl, gs, [&](float4 p, double) { add_moments<KIND>(S, q, p, inv_r); });
+        if (K > 0 && cnt > K) {
+            if (gl == 0) ovf[atomicAdd(ovf_n, 1u)] = qi;
+            continue;
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            for (int m = 1; m < gs; m <<= 1) S[c] += __shfl_xor(S[c], m, kWave);
+        if (gl == 0) {
+            Moments &M = mom[qi];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) M.S[c] = S[c];
+            M.cnt = cnt;
+        }
+    }
+}
+
+// Pass 1, normals and colour gradient together (gradient radius rg <= the normals' radius, no K
+// cap for the gradient): the walk covers the normals' radius and the candidates within rg also
+// add the gradient's raw sums, with the same d2 < rg^2 test as the gradient's own walk, so both
+// moment sets equal those of separate walks.  A query with more than K > 0 points within the
+// normals' radius goes to the overflow list for its normal moments; its gradient sums are
+// complete here.
+__global__ __launch_bounds__(kNbBlock) void k_moments_pair(const float4 *__restrict__ xyzi,
+                                                           const CloudHdr *__restrict__ hdr, GridDesc g,
+                                                           const uint32_t *__restrict__ start,
+                                                           const int32_t *__restrict__ order,
+                                                           const float4 *__restrict__ sxyzi, int32_t K, int gs,
+                                                           double rg, Moments *__restrict__ mom,
+                                                           Moments *__restrict__ mom_g, int32_t *__restrict__ ovf,
+                                                           uint32_t *__restrict__ ovf_n)
+{
+    const GridRefs G{g, start, order, sxyzi};
+    const int n = hdr->n;
+    const int lg = __builtin_ctz((unsigned)gs);
+    const int gl = (int)threadIdx.x & (gs - 1);
+    const int per_block = kNbBlock >> lg;
+    const double inv_r = 1.0 / (double)g.radius;
+    const double inv_rg = 1.0 / rg;
+    const double rg2 = rg * rg;
+    for (int base = blockIdx.x * per_block; base < n; base += gridDim.x * per_block) {
+        const int qi = base + ((int)threadIdx.x >> lg);
+        if (qi >= n) break;  // whole groups leave together
+        const float4 q = xyzi[qi];
+        long long S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Sg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        int32_t cg = 0;
+        const int32_t cnt = group_walk(G, q, gl, gs, [&](float4 p, double d2) {
+            add_moments<0>(S, q, p, inv_r);
+            if (d2 < rg2) {
+                add_moments<1>(Sg, q, p, inv_rg);
+                ++cg;
+            }
+        });
+        for (int m = 1; m < gs; m <<= 1) cg += __shfl_xor(cg, m, kWave);
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            for (int m = 1; m < gs; m <<= 1) Sg[c] += __shfl_xor(Sg[c], m, kWave);
+        if (gl == 0) {
+            Moments &M = mom_g[qi];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) M.S[c] = Sg[c];
+            M.cnt = cg;
+        }
         if (K > 0 && cnt > K) {
             if (gl == 0) ovf[atomicAdd(ovf_n, 1u)] = qi;
             continue;
@@ -393,16 +452,9 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_moments_overflow(
         const int qi = ovf[e];
         if (qi < 0 || qi >= n) continue;
         const float4 q = xyzi[qi];
-        double nv[3] = {0.0, 0.0, 0.0};
-        if (KIND == 1) {
-            const float4 nf = nrm[qi];
-            nv[0] = nf.x;
-            nv[1] = nf.y;
-            nv[2] = nf.z;
-        }
         long long S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         const int32_t cnt = for_each_neighbour(G, q, K, buf, [&](uint32_t, int32_t, float4 p) {
-            add_moments<KIND>(S, q, p, nv, inv_r);
+            add_moments<KIND>(S, q, p, inv_r);
         });
 #pragma unroll
         for (int c = 0; c < 9; ++c) S[c] = wave_sum(S[c]);
@@ -425,15 +477,29 @@ __device__ __forceinline__ float4 gradient_from_moments(int32_t k, const long lo
     const double rr = radius * radius;
     const double kk = (double)(k - 1) * (double)(k - 1);
     const int IA[6] = {0, 0, 0, 1, 1, 2}, IB[6] = {0, 1, 2, 1, 2, 2};
+    // sum w w^T = P E P and sum w dI = P F for w = P u, P = I - n n^T (:48-60), E and F the raw
+    // sums; the same expressions as the oracle's project_tangent
+    double M[3][3], f[3], Mn[3];
+    for (int c = 0; c < 6; ++c) {
+        const double v = (double)S[c] / kFx40;
+        M[IA[c]][IB[c]] = v;
+        M[IB[c]][IA[c]] = v;
+    }
+    for (int a = 0; a < 3; ++a) f[a] = (double)S[6 + a] / kFx40;
+    for (int a = 0; a < 3; ++a) Mn[a] = (M[a][0] * nv[0] + M[a][1] * nv[1]) + M[a][2] * nv[2];
+    const double nMn = (nv[0] * Mn[0] + nv[1] * Mn[1]) + nv[2] * Mn[2];
     double A[3][3];
     for (int c = 0; c < 6; ++c) {
-        double v = ((double)S[c] / kFx40) * rr + kk * (nv[IA[c]] * nv[IB[c]]);
-        A[IA[c]][IB[c]] = v;
-        A[IB[c]][IA[c]] = v;
+        const int a = IA[c], bb = IB[c];
+        const double pmp = ((M[a][bb] - nv[a] * Mn[bb]) - Mn[a] * nv[bb]) + (nv[a] * nv[bb]) * nMn;
+        double v = pmp * rr + kk * (nv[a] * nv[bb]);
+        A[a][bb] = v;
+        A[bb][a] = v;
     }
     for (int a = 0; a < 3; ++a) A[a][a] += 1.0e-6;
+    const double fn = (f[0] * nv[0] + f[1] * nv[1]) + f[2] * nv[2];
     double b[3];
-    for (int a = 0; a < 3; ++a) b[a] = ((double)S[6 + a] / kFx40) * radius;
+    for (int a = 0; a < 3; ++a) b[a] = (f[a] - fn * nv[a]) * radius;
     double D0 = A[0][0];
     double L10 = A[1][0] / D0, L20 = A[2][0] / D0;
     double D1 = A[1][1] - L10 * L10 * D0;
@@ -465,6 +531,30 @@ __global__ __launch_bounds__(kNbBlock) void k_solve(const CloudHdr *__restrict__
 #pragma unroll
         for (int c = 0; c < 9; ++c) S[c] = M.S[c];
         out[i] = KIND == 0 ? normal_from_moments(M.cnt, S) : gradient_from_moments(M.cnt, S, nrm[i], radius);
+    }
+}
+
+// Pass 3 of the joint walk: the normal, then the colour gradient on that normal (the value
+// stored, as a separate gradient call would read it back).
+__global__ __launch_bounds__(kNbBlock) void k_solve_pair(const CloudHdr *__restrict__ hdr,
+                                                         const Moments *__restrict__ mom,
+                                                         const Moments *__restrict__ mom_g, double rg,
+                                                         float4 *__restrict__ nrm, float4 *__restrict__ grad,
+                                                         uint32_t *__restrict__ ovf_n)
+{
+    const int n = hdr->n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ovf_n = 0u;
+    for (int i = blockIdx.x * kNbBlock + threadIdx.x; i < n; i += gridDim.x * kNbBlock) {
+        long long S[9];
+        const Moments &M = mom[i];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) S[c] = M.S[c];
+        const float4 nf = normal_from_moments(M.cnt, S);
+        nrm[i] = nf;
+        const Moments &Mg = mom_g[i];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) S[c] = Mg.S[c];
+        grad[i] = gradient_from_moments(Mg.cnt, S, nf, rg);
     }
 }
 
@@ -507,6 +597,32 @@ int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
 int32_t launch_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
 {
     return launch_moments_pipeline<1>(ctx, c, radius, K, c->grad);
+}
+
+int32_t launch_normals_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K, float gradient_radius)
+{
+    if (!(gradient_radius <= radius)) return GCA_ERR_INVALID_VALUE;
+    const Grid &G = c->grid;
+    const int gs = normals_lanes(ctx);
+    const int64_t cap = std::max<int64_t>(c->cap, 1);
+    Moments *mom = (Moments *)ctx_scratch(ctx, 17, sizeof(Moments) * cap, false);
+    int32_t *ovf = (int32_t *)ctx_scratch(ctx, 18, sizeof(int32_t) * cap, false);
+    uint32_t *ovf_n = (uint32_t *)ctx_scratch(ctx, 19, 256, true);  // zero, re-armed by k_solve_pair
+    Moments *mom_g = (Moments *)ctx_scratch(ctx, 24, sizeof(Moments) * cap, false);
+    if (!mom || !ovf || !ovf_n || !mom_g) return GCA_ERR_OUT_OF_MEMORY;
+    const GridDesc d = query_desc(G, radius);
+    const double rg = (double)gradient_radius;
+    StageTimer timer(ctx, GCA_STAGE_NORMALS);
+    hipLaunchKernelGGL(k_moments_pair, dim3(grid_for(cap, kNbBlock / gs, 4096)), dim3(kNbBlock), 0, ctx->stream,
+                       c->xyzi, c->hdr, d, G.start, G.order, G.sxyzi, K, gs, rg, mom, mom_g, ovf, ovf_n);
+    if (K > 0)
+        hipLaunchKernelGGL(k_moments_overflow<0>, dim3(grid_for(cap, kWavesPerBlock, 1024)),
+                           dim3(kWave * kWavesPerBlock), 0, ctx->stream, c->xyzi, c->nrm, c->hdr, d, G.start,
+                           G.order, G.sxyzi, K, (const int32_t *)ovf, (const uint32_t *)ovf_n, mom);
+    hipLaunchKernelGGL(k_solve_pair, dim3(grid_for(cap, kNbBlock, 2048)), dim3(kNbBlock), 0, ctx->stream, c->hdr,
+                       (const Moments *)mom, (const Moments *)mom_g, rg, c->nrm, c->grad, ovf_n);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
 }
 
 }  // namespace gca

@@ -8,7 +8,8 @@
 //   per level, on the level's own stream (the levels only read the front end's cloud, so they
 //              run side by side: the small coarse levels fill the device next to the fine one):
 //              voxel lattice from the device header + downsample, neighbour grid of fixed
-//              geometry, normals, colour gradient, grid-ordered attributes;
+//              geometry, normals and colour gradient (one neighbourhood walk when the gradient's
+//              radius is the smaller and it has no K cap), grid-ordered attributes;
 //   once every level has joined back: registration of the new frame onto the previous one,
 //              all levels chained in one persistent launch (a registration plan built once per
 //              frame parity), results copied to pinned host memory.
@@ -116,8 +117,12 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         GCA_HIP(hipStreamWaitEvent(lc->stream, p->ev_fork, 0));
         GCA_TRY(launch_voxel_planned(lc, p->full[slot], L.voxel_size, c, p->max_bits[l], p->err_dev));
         GCA_TRY(launch_grid_build(lc, c, p->gdesc[l]));
-        GCA_TRY(launch_normals(lc, c, L.normal_radius, L.normal_max_nn));
-        GCA_TRY(launch_gradient(lc, c, L.gradient_radius, L.gradient_max_nn));
+        if (L.gradient_max_nn == 0 && L.gradient_radius <= L.normal_radius) {
+            GCA_TRY(launch_normals_gradient(lc, c, L.normal_radius, L.normal_max_nn, L.gradient_radius));
+        } else {
+            GCA_TRY(launch_normals(lc, c, L.normal_radius, L.normal_max_nn));
+            GCA_TRY(launch_gradient(lc, c, L.gradient_radius, L.gradient_max_nn));
+        }
         c->has_normals = c->has_gradient = true;
         c->grid.attrs_valid = false;
         GCA_TRY(grid_gather_attributes(lc, c));

@@ -55,9 +55,10 @@ class IcpParams(C.Structure):
 
 
 class IcpReport(C.Structure):
+    """gca_icp_report (include/gca/gca.h), which the oracle fills; latency_us stays 0 here."""
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32),
                 ("solver_failures", C.c_int32), ("n_correspondences", C.c_int32),
-                ("rmse", C.c_double), ("fitness", C.c_double)]
+                ("rmse", C.c_double), ("fitness", C.c_double), ("latency_us", C.c_double)]
 
 
 class LevelParams(C.Structure):
@@ -116,6 +117,7 @@ def lib():
                                         C.c_int32, vp, vp]
         L.orc_estimate_normals.argtypes = [f32p, C.c_int64, f32p, f32p, C.c_float, C.c_int32, f32p]
         L.orc_eigen3.argtypes = [f64p, f64p, f64p]
+        L.orc_abi_sizes.argtypes = [i64p]
         L.orc_color_gradient.argtypes = [f32p, f32p, C.c_int64, f32p, f32p, C.c_float, C.c_int32,
                                          f32p]
         L.orc_build_gn.argtypes = [f32p, C.c_int64, f32p, f32p, f32p, i32p, C.c_float, f64p, f64p]
@@ -210,6 +212,14 @@ class _Grid(C.Structure):
     _fields_ = [("origin", C.c_float * 3), ("cell", C.c_float), ("dims", C.c_int32 * 3),
                 ("ncells", C.c_int64), ("start", C.POINTER(C.c_int32)),
                 ("order", C.POINTER(C.c_int32)), ("n", C.c_int32)]
+
+
+def abi_sizes():
+    """sizeof of (gca_camera, gca_bilateral, gca_icp_params, gca_icp_report, gca_level_params,
+    orc_grid) as compiled into the oracle."""
+    out = np.zeros(6, np.int64)
+    lib().orc_abi_sizes(out)
+    return [int(v) for v in out]
 
 
 def grid_cells(xyzi, radius, bmin=None, bmax=None):

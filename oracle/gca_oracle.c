@@ -517,6 +517,29 @@ void orc_estimate_normals(const float *xyzi, int64_t n, const float bmin[3], con
 /* ------------------------------------------------------------------------------------------ */
 /* Colour gradient                                                                           */
 
+/* P E P and P F for P = I - n n^T, with E (xx xy xz yy yz zz) and F the fixed-point sums: the
+ * canonical FP64 operation order (the device's gradient solve uses the same expressions). */
+static void project_tangent(const int64_t SE[6], const int64_t SF[3], const double n[3], double PMP[6],
+                            double Pf[3])
+{
+    static const int IA[6] = {0, 0, 0, 1, 1, 2}, IB[6] = {0, 1, 2, 1, 2, 2};
+    double M[3][3], f[3], Mn[3];
+    for (int c = 0; c < 6; ++c) {
+        double v = (double)SE[c] / FX40;
+        M[IA[c]][IB[c]] = v;
+        M[IB[c]][IA[c]] = v;
+    }
+    for (int a = 0; a < 3; ++a) f[a] = (double)SF[a] / FX40;
+    for (int a = 0; a < 3; ++a) Mn[a] = (M[a][0] * n[0] + M[a][1] * n[1]) + M[a][2] * n[2];
+    double nMn = (n[0] * Mn[0] + n[1] * Mn[1]) + n[2] * Mn[2];
+    for (int c = 0; c < 6; ++c) {
+        int a = IA[c], b = IB[c];
+        PMP[c] = ((M[a][b] - n[a] * Mn[b]) - Mn[a] * n[b]) + (n[a] * n[b]) * nMn;
+    }
+    double fn = (f[0] * n[0] + f[1] * n[1]) + f[2] * n[2];
+    for (int a = 0; a < 3; ++a) Pf[a] = f[a] - fn * n[a];
+}
+
 void orc_color_gradient(const float *xyzi, const float *normals4, int64_t n, const float bmin[3],
                         const float bmax[3], float radius, int32_t max_nn, float *grad4)
 {
@@ -534,24 +557,26 @@ void orc_color_gradient(const float *xyzi, const float *normals4, int64_t n, con
         if (k < 5) continue; /* cuda_compute_color_gradient.cu:41-44 */
         double nv[3] = {normals4[4 * i], normals4[4 * i + 1], normals4[4 * i + 2]};
         double Ip = (double)xyzi[4 * i + 3];
-        int64_t SA[6] = {0, 0, 0, 0, 0, 0}, Sb[3] = {0, 0, 0};
+        /* Raw neighbourhood sums of u = (p - q) / r: E = sum u u^T, F = sum u dI (exact in fixed
+         * point); the tangent projection P = I - n n^T of :48-60 is applied to the sums below
+         * (sum (P u)(P u)^T = P E P, sum (P u) dI = P F), so the sums do not need the normal. */
+        int64_t SE[6] = {0, 0, 0, 0, 0, 0}, SF[3] = {0, 0, 0};
         for (int32_t t = 0; t < k; ++t) {
             int32_t j = buf[t].idx;
             if (j == (int32_t)i) continue; /* :61-62 */
             const float *p = xyzi + 4 * (int64_t)j;
-            double e[3] = {(double)p[0] - q[0], (double)p[1] - q[1], (double)p[2] - q[2]};
-            double tn = (e[0] * nv[0] + e[1] * nv[1]) + e[2] * nv[2];
-            double w[3];
-            for (int a = 0; a < 3; ++a) w[a] = (e[a] - tn * nv[a]) * inv_r;
+            double u[3] = {((double)p[0] - q[0]) * inv_r, ((double)p[1] - q[1]) * inv_r, ((double)p[2] - q[2]) * inv_r};
             double dI = (double)p[3] - Ip;
-            SA[0] += llrint((w[0] * w[0]) * FX40);
-            SA[1] += llrint((w[0] * w[1]) * FX40);
-            SA[2] += llrint((w[0] * w[2]) * FX40);
-            SA[3] += llrint((w[1] * w[1]) * FX40);
-            SA[4] += llrint((w[1] * w[2]) * FX40);
-            SA[5] += llrint((w[2] * w[2]) * FX40);
-            for (int a = 0; a < 3; ++a) Sb[a] += llrint((w[a] * dI) * FX40);
+            SE[0] += llrint((u[0] * u[0]) * FX40);
+            SE[1] += llrint((u[0] * u[1]) * FX40);
+            SE[2] += llrint((u[0] * u[2]) * FX40);
+            SE[3] += llrint((u[1] * u[1]) * FX40);
+            SE[4] += llrint((u[1] * u[2]) * FX40);
+            SE[5] += llrint((u[2] * u[2]) * FX40);
+            for (int a = 0; a < 3; ++a) SF[a] += llrint((u[a] * dI) * FX40);
         }
+        double PMP[6], Pf[3];
+        project_tangent(SE, SF, nv, PMP, Pf);
         /* :75-84: + (k-1)^2 n n^T + 1e-6 I */
 #if ORC_PLANT == 2 /* orthogonality constraint term dropped */
         double kk = 0.0;
@@ -561,13 +586,13 @@ void orc_color_gradient(const float *xyzi, const float *normals4, int64_t n, con
         double A[3][3];
         static const int IA[6] = {0, 0, 0, 1, 1, 2}, IB[6] = {0, 1, 2, 1, 2, 2};
         for (int c = 0; c < 6; ++c) {
-            double v = ((double)SA[c] / FX40) * rr + kk * (nv[IA[c]] * nv[IB[c]]);
+            double v = PMP[c] * rr + kk * (nv[IA[c]] * nv[IB[c]]);
             A[IA[c]][IB[c]] = v;
             A[IB[c]][IA[c]] = v;
         }
         for (int a = 0; a < 3; ++a) A[a][a] += 1.0e-6;
         double b[3];
-        for (int a = 0; a < 3; ++a) b[a] = ((double)Sb[a] / FX40) * (double)radius;
+        for (int a = 0; a < 3; ++a) b[a] = Pf[a] * (double)radius;
         /* unpivoted LDL^T (little_3x3_ldlt_solve.cuh:11-60) */
         double D0 = A[0][0];
         double L10 = A[1][0] / D0, L20 = A[2][0] / D0;
@@ -947,4 +972,14 @@ int orc_multiscale_icp(const float *src_xyzi, const float *src_rgb, int64_t ns,
     free(tn);
     free(tg);
     return rc;
+}
+
+void orc_abi_sizes(int64_t out[6])
+{
+    out[0] = (int64_t)sizeof(gca_camera);
+    out[1] = (int64_t)sizeof(gca_bilateral);
+    out[2] = (int64_t)sizeof(gca_icp_params);
+    out[3] = (int64_t)sizeof(gca_icp_report);
+    out[4] = (int64_t)sizeof(gca_level_params);
+    out[5] = (int64_t)sizeof(orc_grid);
 }

@@ -33,6 +33,11 @@ typedef struct orc_grid {
     int32_t n;
 } orc_grid;
 
+/* sizeof of the structs the Python binding mirrors, in this order: gca_camera, gca_bilateral,
+ * gca_icp_params, gca_icp_report, gca_level_params, orc_grid (a mirror that drifts from the
+ * header would let the library write past the caller's arrays). */
+void orc_abi_sizes(int64_t out[6]);
+
 /* cuda_point_cloud_factory.cu:189-267 + geometry_util.cuh:124-130 (compaction).
  * xyzi/rgb: float4 per output point (capacity W*H).  pix: source pixel index (may be NULL).
  * Returns the number of points, or -1 on invalid arguments (.cu:286-290). */

@@ -37,6 +37,16 @@ def small_cam():
     return O.make_camera(16, 12, 20.0, 20.0, 8.0, 6.0)
 
 
+def test_binding_structs_match_compiled_oracle():
+    """The ctypes mirrors have the sizes the oracle was compiled with (a shorter mirror would let
+    the oracle write past the caller's report arrays)."""
+    import ctypes as C
+
+    want = [C.sizeof(O.Camera), C.sizeof(O.Bilateral), C.sizeof(O.IcpParams), C.sizeof(O.IcpReport),
+            C.sizeof(O.LevelParams), C.sizeof(O._Grid)]
+    assert O.abi_sizes() == want
+
+
 def test_bilateral_matches_direct_fp64_loop():
     d, bgr = GOLD["bil_depth"], GOLD["bil_bgr"]
     xyzi, rgb, pix, filt = O.cloud_from_rgbd(d, bgr, small_cam(), O.Bilateral(1, 2, 2.0, 30.0), 0.2, 3.5)
