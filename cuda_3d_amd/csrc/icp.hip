@@ -721,10 +721,10 @@ __device__ __forceinline__ void unit_commit(const PairDesc &P, IcpState *states,
 }
 
 // ---- persistent launch: every iteration of every state --------------------------------------
-// Scheduling loop of one workgroup: read all claim words (one coherent load per state and
-// sub-queue, in parallel), pick the first state with unclaimed units in round-robin order from
-// the state last served and, in it, the sub-queue nearest to the workgroup's own, claim a unit
-// with one atomic, run it.  A workgroup only waits (sleep + re-poll)
+// Scheduling loop of one workgroup: read the claim words (one coherent load per state and
+// sub-queue, a window at a time), pick the first state with unclaimed units in round-robin order
+// from the state last served and, in it, the sub-queue nearest to the workgroup's own, claim a
+// unit with one atomic, run it.  A workgroup only waits (sleep + re-poll)
 // when every unit of every open epoch is claimed, i.e. held by running workgroups, so progress
 // never depends on how many workgroups are resident.  Exits when every state is done.
 // (Claiming first and scanning only on failure was measured slower: every workgroup that ends
@@ -744,35 +744,44 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
     int home = (int)(blockIdx.x % (unsigned)n_states);
     const int myq = (int)(blockIdx.x % (unsigned)kSubQ);  // workgroups are dealt to XCDs round-robin
     uint32_t idle = 0;
+    const int n_words = n_states * kSubQ;
     for (;;) {
-        if (threadIdx.x == 0) {
-            s_pick = INT_MAX;
-            s_alive = 0;
-        }
-        __syncthreads();
+        // The claim words are read a window of kIcpBlock words at a time, from the state last
+        // served on; the next window is read only when this one has no unclaimed unit.  (Every
+        // claim used to read all words: with many states that polling traffic on a few hot lines
+        // bounded the batched launch.)
         int my_rot = INT_MAX;
         uint32_t my_epoch = 0;
-        for (int k = threadIdx.x; k < n_states * kSubQ; k += kIcpBlock) {
-            const int p = k / kSubQ, x = k % kSubQ;
-            const unsigned long long w =
-                __hip_atomic_load(&states[p].q[x].w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t e = (uint32_t)(w >> 32), c = (uint32_t)w;
-            if (e != kEpochDone) {
-                s_alive = 1;
-                if (e != kEpochWait && c < subq_units(states[p].nvb, x)) {
+        int pick = INT_MAX, alive = 0;
+        for (int w0 = 0; w0 < n_words; w0 += kIcpBlock) {
+            if (threadIdx.x == 0) {
+                s_pick = INT_MAX;
+                s_alive = 0;
+            }
+            __syncthreads();
+            const int r = w0 + (int)threadIdx.x;  // rotation index: state (home + r / kSubQ), slot r % kSubQ
+            if (r < n_words) {
+                const int p = (home + r / kSubQ) % n_states, x = (myq + r % kSubQ) % kSubQ;
+                const unsigned long long w =
+                    __hip_atomic_load(&states[p].q[x].w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t e = (uint32_t)(w >> 32), c = (uint32_t)w;
+                if (e != kEpochDone) {
+                    s_alive = 1;
                     // first state in round-robin order from the last one served, then the
                     // sub-queue nearest to this workgroup's own
-                    const int rot = ((p - home + n_states) % n_states) * kSubQ + (x - myq + kSubQ) % kSubQ;
-                    if (rot < my_rot) {
-                        my_rot = rot;
+                    if (e != kEpochWait && c < subq_units(states[p].nvb, x)) {
+                        my_rot = r;
                         my_epoch = e;
+                        atomicMin(&s_pick, r);
                     }
-                    atomicMin(&s_pick, rot);
                 }
             }
+            __syncthreads();
+            pick = s_pick;
+            alive |= s_alive;
+            __syncthreads();
+            if (pick != INT_MAX) break;
         }
-        __syncthreads();
-        const int pick = s_pick, alive = s_alive;
         if (my_rot == pick) s_pick_epoch = my_epoch;  // the one thread that saw the picked state
         __syncthreads();
         if (!alive) break;
