@@ -736,12 +736,13 @@ constexpr int kScanWindow = 64;  // claim words read per scan round (one wavefro
 
 template <int MODE>
 __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(const PairDesc *__restrict__ pairs, IcpState *states,
-                                                           int n_states, uint32_t *err)
+                                                           int s_first, int n_states, uint32_t *err)
 {
     __shared__ double T[12];
     __shared__ double sys[kPartStride];
     __shared__ int s_pick, s_alive, s_vb;
     __shared__ uint32_t s_epoch, s_pick_epoch;
+    // serves states [s_first, s_first + n_states); `home` (the state last served) is relative
     int home = (int)(blockIdx.x % (unsigned)n_states);
     const int myq = (int)(blockIdx.x % (unsigned)kSubQ);  // workgroups are dealt to XCDs round-robin
     uint32_t idle = 0;
@@ -763,7 +764,7 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
             __syncthreads();
             const int r = w0 + (int)threadIdx.x;  // rotation index: state (home + r / kSubQ), slot r % kSubQ
             if ((int)threadIdx.x < kScanWindow && r < n_words) {
-                const int p = (home + r / kSubQ) % n_states, x = (myq + r % kSubQ) % kSubQ;
+                const int p = s_first + (home + r / kSubQ) % n_states, x = (myq + r % kSubQ) % kSubQ;
                 const unsigned long long w =
                     __hip_atomic_load(&states[p].q[x].w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const uint32_t e = (uint32_t)(w >> 32), c = (uint32_t)w;
@@ -795,7 +796,7 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
             __builtin_amdgcn_s_sleep(2);
             continue;
         }
-        const int p = __builtin_amdgcn_readfirstlane((home + pick / kSubQ) % n_states);
+        const int p = __builtin_amdgcn_readfirstlane(s_first + (home + pick / kSubQ) % n_states);
         const int x = __builtin_amdgcn_readfirstlane((myq + pick % kSubQ) % kSubQ);
         IcpState *st = states + p;
         // The claim (wavefront 0) and the pose load (wavefront 1) go out together.  The scan saw
@@ -825,7 +826,7 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
             __syncthreads();
         }
         idle = 0;
-        home = p;
+        home = p - s_first;
         const PairDesc &P = pairs[p];
         const int nvb = st->nvb;
         icp_unit<MODE>(P, st->n_src, st->G, nvb, st->n_tgt, vb, epoch > 0, T, sys);
@@ -1072,6 +1073,20 @@ __global__ void k_state_init(IcpState *states, const PairDesc *__restrict__ pair
     s.ticket = 0u;
 }
 
+// Sizes and launch shape of states [s0, s0 + n) from their clouds' headers, for states whose
+// source was still being built when k_state_init ran (a frame pipeline's pyramid levels).
+__global__ void k_state_sizes(IcpState *states, const PairDesc *__restrict__ pairs, int s0, int n)
+{
+    const int p = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= s0 + n) return;
+    IcpState &s = states[p];
+    const PairDesc &P = pairs[p];
+    s.n_src = P.src_hdr->n;
+    s.n_tgt = P.tgt_hdr->n;
+    s.G = icp_lanes_for(P.lanes, s.n_src);
+    s.nvb = icp_units_for(s.n_src, s.G, P.unit_cap);
+}
+
 // State of the single-pair kernels (Gauss-Newton build, residuals, objective).
 __global__ void k_state_simple(IcpState *st, const double *__restrict__ T0)
 {
@@ -1267,20 +1282,26 @@ static void states_describe(int n_pairs, int n_levels, const gca_cloud *srcs, co
 }
 
 template <int MODE>
-static void launch_persist_mode(gca_ctx ctx, const PairDesc *pairs, IcpState *states, int n_states, uint32_t *err)
+static void launch_persist_mode(gca_ctx ctx, const PairDesc *pairs, IcpState *states, int s_first, int n_states,
+                                uint32_t *err)
 {
     const int blocks = ctx->tune[GCA_TUNE_ICP_PERSIST_BLOCKS] ? ctx->tune[GCA_TUNE_ICP_PERSIST_BLOCKS] : 2 * ctx->n_cu;
     hipLaunchKernelGGL(k_icp_persist<MODE>, dim3((unsigned)blocks), dim3(kIcpBlock), 0, ctx->stream, pairs, states,
-                       n_states, err);
+                       s_first, n_states, err);
 }
 
+// The persistent launch over states [s_first, s_first + n_states) (the chain links of their
+// finishers may open states after the range: those run in a later launch).
 static void launch_persist_any(gca_ctx ctx, int mode, const PairDesc *pairs, IcpState *states, int n_states,
-                               uint32_t *err)
+                               uint32_t *err, int s_first = 0)
 {
     StageTimer timer(ctx, GCA_STAGE_ICP);
-    if (mode == GCA_ACCUM_NAIVE_FP32) launch_persist_mode<GCA_ACCUM_NAIVE_FP32>(ctx, pairs, states, n_states, err);
-    else if (mode == GCA_ACCUM_FP64) launch_persist_mode<GCA_ACCUM_FP64>(ctx, pairs, states, n_states, err);
-    else launch_persist_mode<GCA_ACCUM_COMPENSATED>(ctx, pairs, states, n_states, err);
+    if (mode == GCA_ACCUM_NAIVE_FP32)
+        launch_persist_mode<GCA_ACCUM_NAIVE_FP32>(ctx, pairs, states, s_first, n_states, err);
+    else if (mode == GCA_ACCUM_FP64)
+        launch_persist_mode<GCA_ACCUM_FP64>(ctx, pairs, states, s_first, n_states, err);
+    else
+        launch_persist_mode<GCA_ACCUM_COMPENSATED>(ctx, pairs, states, s_first, n_states, err);
 }
 
 // Poses and reports from a host copy of the states.
@@ -1454,6 +1475,36 @@ int32_t icp_plan_create(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud 
         (e = hipMemcpy(p->T0, T0, sizeof(double) * 16 * n_pairs, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e);
     *out = p;
+    return GCA_OK;
+}
+
+// Plan launch in parts, for a caller that overlaps each state's registration with the
+// building of the next state's source: states set up (sizes are provisional), then per state
+// its sizes and one persistent launch over it alone (each on the stream that built its source,
+// after the launch of the state chained before it), then the results.
+int32_t icp_plan_init(gca_ctx ctx, IcpPlan *p)
+{
+    hipLaunchKernelGGL(k_state_init, dim3((p->n_states + 63) / 64), dim3(64), 0, ctx->stream, p->states,
+                       (const PairDesc *)p->pairs, (const StateInit *)p->init, (const double *)p->T0, p->n_states,
+                       p->err);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
+}
+
+int32_t icp_plan_run_state(gca_ctx ctx, IcpPlan *p, int s)
+{
+    if (s < 0 || s >= p->n_states) return GCA_ERR_INVALID_VALUE;
+    hipLaunchKernelGGL(k_state_sizes, dim3(1), dim3(64), 0, ctx->stream, p->states, (const PairDesc *)p->pairs, s, 1);
+    launch_persist_any(ctx, p->mode, p->pairs, p->states, 1, p->err, s);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
+}
+
+int32_t icp_plan_copy_results(gca_ctx ctx, IcpPlan *p)
+{
+    GCA_HIP(hipMemcpyAsync(p->host_states, p->states, sizeof(IcpState) * p->n_states, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    GCA_HIP(hipMemcpyAsync(p->host_err, p->err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     return GCA_OK;
 }
 

@@ -10,9 +10,12 @@
 //              voxel lattice from the device header + downsample, neighbour grid of fixed
 //              geometry, normals and colour gradient (one neighbourhood walk when the gradient's
 //              radius is the smaller and it has no K cap), grid-ordered attributes;
-//   once every level has joined back: registration of the new frame onto the previous one,
-//              all levels chained in one persistent launch (a registration plan built once per
-//              frame parity), results copied to pinned host memory.
+//   registration of the new frame onto the previous one, level by level (coarse to fine): a
+//              registration plan built once per frame parity is set up before the fork, and each
+//              level's persistent launch runs on that level's stream once its source is built
+//              and the coarser level's launch has handed it the pose, so the coarse levels
+//              register while the fine ones are still being built; results are copied to pinned
+//              host memory once every level has joined back.
 // Nothing in a step depends on host-side sizes: the voxel lattice is computed on the device
 // (its scan covers an upper bound from the camera frustum), and each level's grid uses one
 // geometry over the frustum box (neighbour results do not depend on the grid geometry: points
@@ -41,6 +44,7 @@ struct gca_pipeline_s {
     gca_ctx ctx = nullptr;  // private: front end, registration, every cloud
     gca_ctx lctx[kPipeMaxLevels] = {};  // private, one per level (stream + scratch)
     hipEvent_t ev_fork = nullptr, ev_join[kPipeMaxLevels] = {};
+    hipEvent_t ev_icp[kPipeMaxLevels] = {};  // level l's registration launched (chain order)
     gca_camera cam{};
     gca_bilateral bil{};
     float tmin = 0.f, tmax = 0.f;
@@ -109,6 +113,8 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
     gca_ctx ctx = p->ctx;
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(64), 0, ctx->stream, p->err_dev);
     GCA_TRY(launch_from_rgbd(ctx, depth, bgr, &p->cam, &p->bil, p->tmin, p->tmax, p->full[slot]));
+    IcpPlan *plan = p->frames >= 1 ? p->plan[slot] : nullptr;
+    if (plan) GCA_TRY(icp_plan_init(ctx, plan));
     GCA_HIP(hipEventRecord(p->ev_fork, ctx->stream));
     for (int l = 0; l < p->nl; ++l) {
         gca_ctx lc = p->lctx[l];
@@ -126,10 +132,17 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         c->has_normals = c->has_gradient = true;
         c->grid.attrs_valid = false;
         GCA_TRY(grid_gather_attributes(lc, c));
+        if (plan) {
+            // this level's registration: once its source is built and the coarser level's
+            // registration (which hands it the pose) has run
+            if (l > 0) GCA_HIP(hipStreamWaitEvent(lc->stream, p->ev_icp[l - 1], 0));
+            GCA_TRY(icp_plan_run_state(lc, plan, l));
+            if (l + 1 < p->nl) GCA_HIP(hipEventRecord(p->ev_icp[l], lc->stream));
+        }
         GCA_HIP(hipEventRecord(p->ev_join[l], lc->stream));
         GCA_HIP(hipStreamWaitEvent(ctx->stream, p->ev_join[l], 0));
     }
-    if (p->frames >= 1) GCA_TRY(icp_plan_launch(ctx, p->plan[slot]));
+    if (plan) GCA_TRY(icp_plan_copy_results(ctx, plan));
     GCA_HIP(hipMemcpyAsync(p->err_host, p->err_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     return GCA_OK;
 }
@@ -211,6 +224,7 @@ int32_t gca_pipeline_destroy(gca_pipeline p)
     if (p->ev_fork) hipEventDestroy(p->ev_fork);
     for (int l = 0; l < kPipeMaxLevels; ++l) {
         if (p->ev_join[l]) hipEventDestroy(p->ev_join[l]);
+        if (p->ev_icp[l]) hipEventDestroy(p->ev_icp[l]);
         if (p->lctx[l]) gca_ctx_destroy(p->lctx[l]);
     }
     if (p->ctx) gca_ctx_destroy(p->ctx);
@@ -255,7 +269,8 @@ int32_t gca_pipeline_create(gca_ctx ctx, const gca_camera *cam, const gca_bilate
     for (int l = 0; l < n_levels && st == GCA_OK; ++l) {
         st = gca_ctx_create(ctx->device, &p->lctx[l]);
         if (st == GCA_OK) p->lctx[l]->prof_owner = p->ctx;
-        if (st == GCA_OK && hipEventCreateWithFlags(&p->ev_join[l], hipEventDisableTiming) != hipSuccess)
+        if (st == GCA_OK && (hipEventCreateWithFlags(&p->ev_join[l], hipEventDisableTiming) != hipSuccess ||
+                             hipEventCreateWithFlags(&p->ev_icp[l], hipEventDisableTiming) != hipSuccess))
             st = GCA_ERR_DEVICE;
     }
     if (st == GCA_OK && hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess) st = GCA_ERR_DEVICE;
