@@ -135,6 +135,18 @@ __global__ __launch_bounds__(256) void k_voxel_mark(const float4 *__restrict__ x
     }
 }
 
+// Accumulation with workgroup pre-aggregation.  A workgroup takes kVoxChunk consecutive points
+// (raster order: a few image rows, whose points share few voxels) and sums their fixed-point
+// values per voxel in an LDS hash table keyed by the voxel's rank (open addressing, linear
+// probing); it then adds each distinct voxel's sums to global memory once.  A point whose probe
+// sequence finds no slot adds to global memory directly.  Integer sums are exact, so the order
+// in which they arrive does not change the result.  (Device-scope 64-bit atomics are what bound
+// the per-run version: ~8-14 G/s.)
+constexpr int kVoxChunk = 2048;
+constexpr int kVoxTableBits = 10;
+constexpr int kVoxTable = 1 << kVoxTableBits;
+constexpr int kVoxProbe = 16;
+
 __global__ __launch_bounds__(256) void k_voxel_accum(const float4 *__restrict__ xyzi,
                                                      const float4 *__restrict__ rgb,
                                                      const CloudHdr *__restrict__ hdr,
@@ -144,20 +156,25 @@ __global__ __launch_bounds__(256) void k_voxel_accum(const float4 *__restrict__ 
                                                      const int64_t *__restrict__ lin,
                                                      unsigned long long *__restrict__ acc)
 {
+    __shared__ int32_t t_rank[kVoxTable];
+    __shared__ long long t_key[kVoxTable];
+    __shared__ unsigned long long t_sum[7][kVoxTable];
     const int n = vpp->bad ? 0 : hdr->n;
-    const int stride = gridDim.x * blockDim.x;
-    auto add = [](unsigned long long a, unsigned long long c) { return a + c; };
-    for (int b = blockIdx.x * blockDim.x; b < n; b += stride) {  // block-uniform trip count
-        const int i = b + threadIdx.x;
-        int64_t r = -1 - (int64_t)lane_id();  // unique keys for inactive lanes
-        int64_t L = 0;
-        unsigned long long v[7] = {0, 0, 0, 0, 0, 0, 0};
-        if (i < n) {
-            L = lin[i];
+    for (int h = threadIdx.x; h < kVoxTable; h += blockDim.x) {
+        t_rank[h] = -1;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) t_sum[k][h] = 0ull;
+    }
+    for (int base = blockIdx.x * kVoxChunk; base < n; base += gridDim.x * kVoxChunk) {  // block-uniform
+        __syncthreads();  // table clear (initial, or by the previous chunk's flush)
+        const int end = min(base + kVoxChunk, n);
+        for (int i = base + (int)threadIdx.x; i < end; i += blockDim.x) {
+            const int64_t L = lin[i];
             const uint32_t w = bitmap[L >> 5];
             const uint32_t below = w & ((1u << (uint32_t)(L & 31)) - 1u);
-            r = (int64_t)(word_pre[L >> 5] + (uint32_t)__popc(below));
+            const int32_t r = (int32_t)(word_pre[L >> 5] + (uint32_t)__popc(below));
             const float4 p = xyzi[i], c = rgb[i];
+            unsigned long long v[7];
             v[0] = (unsigned long long)fx((double)p.x * kFx30);
             v[1] = (unsigned long long)fx((double)p.y * kFx30);
             v[2] = (unsigned long long)fx((double)p.z * kFx30);
@@ -165,15 +182,39 @@ __global__ __launch_bounds__(256) void k_voxel_accum(const float4 *__restrict__ 
             v[4] = (unsigned long long)fx((double)c.y * kFx30);
             v[5] = (unsigned long long)fx((double)c.z * kFx30);
             v[6] = 1ull;
+            uint32_t h = ((uint32_t)r * 2654435761u) >> (32 - kVoxTableBits);  // multiplicative hash
+            int slot = -1;
+            for (int t = 0; t < kVoxProbe; ++t) {
+                const int32_t old = atomicCAS(&t_rank[h], -1, r);
+                if (old == -1) t_key[h] = L;  // same value from every point of the voxel
+                if (old == -1 || old == r) {
+                    slot = (int)h;
+                    break;
+                }
+                h = (h + 1) & (kVoxTable - 1);
+            }
+            if (slot >= 0) {
+#pragma unroll
+                for (int k = 0; k < 7; ++k) atomicAdd(&t_sum[k][slot], v[k]);
+            } else {
+                unsigned long long *a = acc + 8 * (int64_t)r;
+#pragma unroll
+                for (int k = 0; k < 7; ++k) atomicAdd(&a[k], v[k]);
+                a[7] = (unsigned long long)L;
+            }
         }
-        const int head = run_head(r);
+        __syncthreads();
+        for (int h = threadIdx.x; h < kVoxTable; h += blockDim.x) {
+            const int32_t r = t_rank[h];
+            if (r < 0) continue;
+            unsigned long long *a = acc + 8 * (int64_t)r;
 #pragma unroll
-        for (int k = 0; k < 7; ++k) v[k] = run_reduce(v[k], head, add);
-        if (r >= 0 && head == lane_id()) {
-            unsigned long long *a = acc + 8 * r;
-#pragma unroll
-            for (int k = 0; k < 7; ++k) atomicAdd(&a[k], v[k]);
-            a[7] = (unsigned long long)L;  // same value from every point of the voxel
+            for (int k = 0; k < 7; ++k) {
+                atomicAdd(&a[k], t_sum[k][h]);
+                t_sum[k][h] = 0ull;
+            }
+            a[7] = (unsigned long long)t_key[h];
+            t_rank[h] = -1;
         }
     }
 }
@@ -237,7 +278,8 @@ static int32_t voxel_enqueue(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud o
                        ctx->bitmap, lin);
     GCA_HIP(hipGetLastError());
     GCA_TRY(exclusive_scan_u32(ctx, ctx->bitmap, word_pre, words, (uint32_t *)&out->hdr->n, true));
-    hipLaunchKernelGGL(k_voxel_accum, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->rgb, in->hdr,
+    const unsigned nb_acc = (unsigned)std::min<int64_t>(4096, (nc + kVoxChunk - 1) / kVoxChunk);
+    hipLaunchKernelGGL(k_voxel_accum, dim3(nb_acc), dim3(256), 0, ctx->stream, in->xyzi, in->rgb, in->hdr,
                        (const VoxelParams *)vp, ctx->bitmap, word_pre, lin, acc);
     hipLaunchKernelGGL(k_voxel_finalize, dim3(nb), dim3(256), 0, ctx->stream, acc, in->hdr, out->hdr,
                        ctx->bitmap, out->xyzi, out->rgb);
