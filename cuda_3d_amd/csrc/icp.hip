@@ -742,6 +742,9 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
     __shared__ double sys[kPartStride];
     __shared__ int s_pick, s_alive, s_vb;
     __shared__ uint32_t s_epoch, s_pick_epoch;
+    // A launch over one state needs no more workgroups than the state has units per iteration:
+    // the others could only poll (and their polling competes with whatever else runs).
+    if (n_states == 1 && (int)blockIdx.x >= max(states[s_first].nvb, 1)) return;
     // serves states [s_first, s_first + n_states); `home` (the state last served) is relative
     int home = (int)(blockIdx.x % (unsigned)n_states);
     const int myq = (int)(blockIdx.x % (unsigned)kSubQ);  // workgroups are dealt to XCDs round-robin
