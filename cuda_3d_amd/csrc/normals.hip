@@ -34,6 +34,7 @@ struct NbBuf {
     double d2[kNbCap];
     int32_t idx[kNbCap];
     int32_t slot[kNbCap];
+    float4 p[kNbCap];
 };
 
 __device__ __forceinline__ void wave_lds_sync()
@@ -50,8 +51,15 @@ struct GridRefs {
     const float4 *sxyzi;
 };
 
+constexpr int kNbRuns = 9;      // grid runs per query when the cell is at least the radius
+constexpr int kOvfUnroll = 2;   // 64-candidate steps in flight per wave in the K-selection walk
+
 // Calls f(slot, idx, point) on the lanes for every selected neighbour of q; returns the
-// number of selected neighbours (wave-uniform).  K == 0: all within radius.
+// number of selected neighbours (wave-uniform).  K == 0: all within radius.  With at most
+// kNbRuns runs (every grid grid_build makes) all run bounds are loaded at once and the runs are
+// walked as one flattened candidate range, kOvfUnroll x 64 candidates in flight; the in-radius
+// candidates go to LDS with their points.  The selection (the K smallest (d2, index) keys) and
+// the count depend only on the candidate set, not on the visiting order.
 template <class F>
 __device__ __forceinline__ int32_t for_each_neighbour(const GridRefs &G, float4 q, int32_t K, NbBuf &buf, F &&f)
 {
@@ -59,43 +67,95 @@ __device__ __forceinline__ int32_t for_each_neighbour(const GridRefs &G, float4 
     if (!grid_range(G.g, q.x, q.y, q.z, lo, hi)) return 0;
     const double r2 = (double)G.g.radius * (double)G.g.radius;
     const int lane = lane_id();
+    const int nry = hi[1] - lo[1] + 1;
+    const int nruns = (hi[0] - lo[0] + 1) * nry;
     int32_t count = 0;
     wave_lds_sync();
-    for (int32_t ix = lo[0]; ix <= hi[0]; ++ix)
-        for (int32_t iy = lo[1]; iy <= hi[1]; ++iy) {
-            const int64_t row = ((int64_t)ix * G.g.dims[1] + iy) * G.g.dims[2];
-            const uint32_t s0 = G.start[row + lo[2]], s1 = G.start[row + hi[2] + 1];
-            for (uint32_t b = s0; b < s1; b += kWave) {
-                const uint32_t s = b + lane;
-                bool in = false;
-                double d2 = 0.0;
-                float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (s < s1) {
-                    p = G.sxyzi[s];
-                    d2 = dist2(p.x, p.y, p.z, q.x, q.y, q.z);
-                    in = d2 < r2;
-                }
-                if (K == 0) {
-                    if (in) f(s, G.order[s], p);
-                    count += __popcll(__ballot(in));
-                } else {
-                    const uint64_t m = __ballot(in);
-                    if (in) {
-                        const int32_t pos = count + __popcll(m & ((1ull << lane) - 1ull));
-                        if (pos < kNbCap) {
-                            buf.d2[pos] = d2;
-                            buf.idx[pos] = G.order[s];
-                            buf.slot[pos] = (int32_t)s;
-                        }
-                    }
-                    count += __popcll(m);
-                }
+    // stores one in-radius candidate per flagged lane at the next free LDS positions
+    auto keep = [&](bool in, uint32_t s, double d2, int32_t ix, float4 p) {
+        const uint64_t m = __ballot(in);
+        if (in) {
+            const int32_t pos = count + __popcll(m & ((1ull << lane) - 1ull));
+            if (K == 0) {
+                f(s, ix, p);
+            } else if (pos < kNbCap) {
+                buf.d2[pos] = d2;
+                buf.idx[pos] = ix;
+                buf.slot[pos] = (int32_t)s;
+                buf.p[pos] = p;
             }
         }
+        count += __popcll(m);
+    };
+    if (nruns <= kNbRuns) {
+        uint32_t rs[kNbRuns], cum[kNbRuns + 1];
+#pragma unroll
+        for (int r = 0; r < kNbRuns; ++r) {
+            uint32_t a = 0, b = 0;
+            if (r < nruns) {
+                const int ix = lo[0] + r / nry, iy = lo[1] + r % nry;
+                const int64_t row = ((int64_t)ix * G.g.dims[1] + iy) * G.g.dims[2];
+                a = G.start[row + lo[2]];
+                b = G.start[row + hi[2] + 1];
+            }
+            rs[r] = a;
+            cum[r + 1] = b - a;
+        }
+        cum[0] = 0;
+#pragma unroll
+        for (int r = 0; r < kNbRuns; ++r) cum[r + 1] += cum[r];
+        const uint32_t total = cum[kNbRuns];
+        for (uint32_t f0 = 0; f0 < total; f0 += kWave * kOvfUnroll) {
+            float4 c[kOvfUnroll];
+            int32_t ci[kOvfUnroll];
+            uint32_t cs[kOvfUnroll];
+#pragma unroll
+            for (int u = 0; u < kOvfUnroll; ++u) {
+                const uint32_t fi = f0 + u * kWave + lane;
+                cs[u] = 0;
+                if (fi < total) {
+                    uint32_t sl = rs[0] + fi;
+#pragma unroll
+                    for (int r = 1; r < kNbRuns; ++r)
+                        if (fi >= cum[r]) sl = rs[r] + (fi - cum[r]);
+                    cs[u] = sl;
+                    c[u] = G.sxyzi[sl];
+                    ci[u] = G.order[sl];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kOvfUnroll; ++u) {
+                const bool valid = f0 + u * kWave + lane < total;
+                const double d2 = valid ? dist2(c[u].x, c[u].y, c[u].z, q.x, q.y, q.z) : 0.0;
+                keep(valid && d2 < r2, cs[u], d2, valid ? ci[u] : 0, valid ? c[u] : make_float4(0.f, 0.f, 0.f, 0.f));
+            }
+        }
+    } else {
+        // cell smaller than the radius (not produced by grid_build): run by run
+        for (int32_t ix = lo[0]; ix <= hi[0]; ++ix)
+            for (int32_t iy = lo[1]; iy <= hi[1]; ++iy) {
+                const int64_t row = ((int64_t)ix * G.g.dims[1] + iy) * G.g.dims[2];
+                const uint32_t s0 = G.start[row + lo[2]], s1 = G.start[row + hi[2] + 1];
+                for (uint32_t b = s0; b < s1; b += kWave) {
+                    const uint32_t s = b + lane;
+                    bool in = false;
+                    double d2 = 0.0;
+                    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+                    int32_t id = 0;
+                    if (s < s1) {
+                        p = G.sxyzi[s];
+                        id = G.order[s];
+                        d2 = dist2(p.x, p.y, p.z, q.x, q.y, q.z);
+                        in = d2 < r2;
+                    }
+                    keep(in, s, d2, id, p);
+                }
+            }
+    }
     if (K == 0) return count;
     wave_lds_sync();
     if (count <= K) {
-        for (int32_t e = lane; e < count; e += kWave) f(buf.slot[e], buf.idx[e], G.sxyzi[buf.slot[e]]);
+        for (int32_t e = lane; e < count; e += kWave) f(buf.slot[e], buf.idx[e], buf.p[e]);
         return count;
     }
     if (count <= kNbCap) {
@@ -104,7 +164,7 @@ __device__ __forceinline__ int32_t for_each_neighbour(const GridRefs &G, float4 
             const int32_t ie = buf.idx[e];
             int32_t rank = 0;
             for (int32_t j = 0; j < count; ++j) rank += key_less(buf.d2[j], buf.idx[j], de, ie) ? 1 : 0;
-            if (rank < K) f(buf.slot[e], ie, G.sxyzi[buf.slot[e]]);
+            if (rank < K) f(buf.slot[e], ie, buf.p[e]);
         }
         return K;
     }
@@ -211,7 +271,6 @@ __host__ __device__ inline float4 normal_from_moments(int32_t cnt, const long lo
 // flight and accumulates integer moments; the group then sums them with shuffles.  Integer
 // sums are exact, so the result does not depend on G or on the visiting order.
 constexpr int kNbBlock = 256;
-constexpr int kNbRuns = 9;
 constexpr int kNbUnroll = 4;
 
 struct Moments {
