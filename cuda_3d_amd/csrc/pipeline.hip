@@ -11,11 +11,13 @@
 //              geometry, normals and colour gradient (one neighbourhood walk when the gradient's
 //              radius is the smaller and it has no K cap), grid-ordered attributes;
 //   registration of the new frame onto the previous one, level by level (coarse to fine): a
-//              registration plan built once per frame parity is set up before the fork, and each
-//              level's persistent launch runs on that level's stream once its source is built
-//              and the coarser level's launch has handed it the pose, so the coarse levels
-//              register while the fine ones are still being built; results are copied to pinned
-//              host memory once every level has joined back.
+//              registration plan built once per frame parity is set up before the fork; each
+//              level's stream ends its build by refreshing the level's registration sizes, and
+//              the persistent launches run one after another on the coarsest level's stream,
+//              each once its level is built (so the coarse levels register while the fine ones
+//              are still being built, and a launch follows the one handing it the pose with no
+//              cross-stream wait); results are copied to pinned host memory once every level
+//              has joined back.
 // Nothing in a step depends on host-side sizes: the voxel lattice is computed on the device
 // (its scan covers an upper bound from the camera frustum), and each level's grid uses one
 // geometry over the frustum box (neighbour results do not depend on the grid geometry: points
@@ -44,7 +46,7 @@ struct gca_pipeline_s {
     gca_ctx ctx = nullptr;  // private: front end, registration, every cloud
     gca_ctx lctx[kPipeMaxLevels] = {};  // private, one per level (stream + scratch)
     hipEvent_t ev_fork = nullptr, ev_join[kPipeMaxLevels] = {};
-    hipEvent_t ev_icp[kPipeMaxLevels] = {};  // level l's registration launched (chain order)
+    hipEvent_t ev_built[kPipeMaxLevels] = {};  // level l's source built and its sizes refreshed
     gca_camera cam{};
     gca_bilateral bil{};
     float tmin = 0.f, tmax = 0.f;
@@ -133,16 +135,28 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         c->grid.attrs_valid = false;
         GCA_TRY(grid_gather_attributes(lc, c));
         if (plan) {
-            // this level's registration: once its source is built and the coarser level's
-            // registration (which hands it the pose) has run
-            if (l > 0) GCA_HIP(hipStreamWaitEvent(lc->stream, p->ev_icp[l - 1], 0));
-            GCA_TRY(icp_plan_run_state(lc, plan, l));
-            if (l + 1 < p->nl) GCA_HIP(hipEventRecord(p->ev_icp[l], lc->stream));
+            // the registration chain runs on level 0's stream (after level 0's build): level l
+            // once it is built, right after level l - 1's launch, which hands it the pose.
+            // Enqueued here, not after every level's build, because a graph launch submits its
+            // nodes in this order and a late submission is a late start.
+            gca_ctx ic = p->lctx[0];
+            GCA_TRY(icp_plan_sizes(lc, plan, l));
+            if (l > 0) {
+                GCA_HIP(hipEventRecord(p->ev_built[l], lc->stream));
+                GCA_HIP(hipStreamWaitEvent(ic->stream, p->ev_built[l], 0));
+            }
+            GCA_TRY(icp_plan_persist(ic, plan, l));
         }
-        GCA_HIP(hipEventRecord(p->ev_join[l], lc->stream));
-        GCA_HIP(hipStreamWaitEvent(ctx->stream, p->ev_join[l], 0));
+        if (l > 0 || !plan) {
+            GCA_HIP(hipEventRecord(p->ev_join[l], lc->stream));
+            GCA_HIP(hipStreamWaitEvent(ctx->stream, p->ev_join[l], 0));
+        }
     }
-    if (plan) GCA_TRY(icp_plan_copy_results(ctx, plan));
+    if (plan) {
+        GCA_HIP(hipEventRecord(p->ev_join[0], p->lctx[0]->stream));
+        GCA_HIP(hipStreamWaitEvent(ctx->stream, p->ev_join[0], 0));
+        GCA_TRY(icp_plan_copy_results(ctx, plan));
+    }
     GCA_HIP(hipMemcpyAsync(p->err_host, p->err_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     return GCA_OK;
 }
@@ -224,7 +238,7 @@ int32_t gca_pipeline_destroy(gca_pipeline p)
     if (p->ev_fork) hipEventDestroy(p->ev_fork);
     for (int l = 0; l < kPipeMaxLevels; ++l) {
         if (p->ev_join[l]) hipEventDestroy(p->ev_join[l]);
-        if (p->ev_icp[l]) hipEventDestroy(p->ev_icp[l]);
+        if (p->ev_built[l]) hipEventDestroy(p->ev_built[l]);
         if (p->lctx[l]) gca_ctx_destroy(p->lctx[l]);
     }
     if (p->ctx) gca_ctx_destroy(p->ctx);
@@ -270,7 +284,7 @@ int32_t gca_pipeline_create(gca_ctx ctx, const gca_camera *cam, const gca_bilate
         st = gca_ctx_create(ctx->device, &p->lctx[l]);
         if (st == GCA_OK) p->lctx[l]->prof_owner = p->ctx;
         if (st == GCA_OK && (hipEventCreateWithFlags(&p->ev_join[l], hipEventDisableTiming) != hipSuccess ||
-                             hipEventCreateWithFlags(&p->ev_icp[l], hipEventDisableTiming) != hipSuccess))
+                             hipEventCreateWithFlags(&p->ev_built[l], hipEventDisableTiming) != hipSuccess))
             st = GCA_ERR_DEVICE;
     }
     if (st == GCA_OK && hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess) st = GCA_ERR_DEVICE;
