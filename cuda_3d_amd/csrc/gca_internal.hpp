@@ -209,13 +209,10 @@ struct IcpPlan;
 int32_t icp_plan_create(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud *srcs, const gca_cloud *tgts,
                         const gca_icp_params *params, const float *grad_r, const double *T0, IcpPlan **out);
 int32_t icp_plan_launch(gca_ctx ctx, IcpPlan *plan);
-// The same launch in parts (state set-up; one state's sizes, refreshed from its clouds'
-// headers; one state's persistent launch; results), for overlapping state s's registration with
-// the building of state s + 1's source.  A state's sizes may be refreshed while the previous
-// state's launch runs (that launch writes only its own state and the next state's pose and
-// claim words).
+// The same launch in parts (state set-up; one state's persistent launch, which takes the
+// state's sizes from its clouds' headers, so it may be enqueued before they are final; results),
+// for overlapping state s's registration with the building of state s + 1's source.
 int32_t icp_plan_init(gca_ctx ctx, IcpPlan *plan);
-int32_t icp_plan_sizes(gca_ctx ctx, IcpPlan *plan, int s);
 int32_t icp_plan_persist(gca_ctx ctx, IcpPlan *plan, int s);
 int32_t icp_plan_copy_results(gca_ctx ctx, IcpPlan *plan);
 int32_t icp_plan_results(gca_ctx ctx, const IcpPlan *plan, double *T_out, gca_icp_report *reports);

@@ -734,17 +734,41 @@ __device__ __forceinline__ void unit_commit(const PairDesc &P, IcpState *states,
 #define GCA_ICP_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 8)))
 constexpr int kScanWindow = 64;  // claim words read per scan round (one wavefront's loads)
 
+// refresh (launches over one state only): the state's sizes are taken from its clouds' headers
+// at the start, by every workgroup for itself (workgroup 0 also stores them in the state), for a
+// launch enqueued before its source cloud's size was final.
 template <int MODE>
 __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(const PairDesc *__restrict__ pairs, IcpState *states,
-                                                           int s_first, int n_states, uint32_t *err)
+                                                           int s_first, int n_states, int refresh, uint32_t *err)
 {
     __shared__ double T[12];
     __shared__ double sys[kPartStride];
     __shared__ int s_pick, s_alive, s_vb;
     __shared__ uint32_t s_epoch, s_pick_epoch;
+    __shared__ int s_nsrc, s_ntgt, s_G, s_nvb;
+    if (refresh) {
+        if (threadIdx.x == 0) {
+            const PairDesc &P = pairs[s_first];
+            const int ns = P.src_hdr->n, nt = P.tgt_hdr->n;
+            const int G = icp_lanes_for(P.lanes, ns);
+            const int nvb = icp_units_for(ns, G, P.unit_cap);
+            s_nsrc = ns;
+            s_ntgt = nt;
+            s_G = G;
+            s_nvb = nvb;
+            if (blockIdx.x == 0) {
+                IcpState &S = states[s_first];
+                S.n_src = ns;
+                S.n_tgt = nt;
+                S.G = G;
+                S.nvb = nvb;
+            }
+        }
+        __syncthreads();
+    }
     // A launch over one state needs no more workgroups than the state has units per iteration:
     // the others could only poll (and their polling competes with whatever else runs).
-    if (n_states == 1 && (int)blockIdx.x >= max(states[s_first].nvb, 1)) return;
+    if (n_states == 1 && (int)blockIdx.x >= max(refresh ? s_nvb : states[s_first].nvb, 1)) return;
     // Highest issue priority among the wavefronts sharing a SIMD: a registration iteration is a
     // chain of dependent memory round trips, and in a frame pipeline it runs beside the kernels
     // that build the finer levels, whose wavefronts would otherwise be issued first (measured:
@@ -780,7 +804,7 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
                     s_alive = 1;
                     // first state in round-robin order from the last one served, then the
                     // sub-queue nearest to this workgroup's own
-                    if (e != kEpochWait && c < subq_units(states[p].nvb, x)) {
+                    if (e != kEpochWait && c < subq_units(refresh ? s_nvb : states[p].nvb, x)) {
                         my_rot = r;
                         my_epoch = e;
                         atomicMin(&s_pick, r);
@@ -819,7 +843,7 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
             const unsigned long long old =
                 __hip_atomic_fetch_add(&st->q[x].w, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const uint32_t e = (uint32_t)(old >> 32), c = (uint32_t)old;
-            const bool ok = e < kEpochWait && c < subq_units(st->nvb, x);
+            const bool ok = e < kEpochWait && c < subq_units(refresh ? s_nvb : st->nvb, x);
             s_vb = ok ? x + kSubQ * (int)c : -1;
             s_epoch = e;
         } else if (threadIdx.x >= kWave && threadIdx.x < kWave + 12) {
@@ -836,8 +860,11 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
         idle = 0;
         home = p - s_first;
         const PairDesc &P = pairs[p];
-        const int nvb = st->nvb;
-        icp_unit<MODE>(P, st->n_src, st->G, nvb, st->n_tgt, vb, epoch > 0, T, sys);
+        const int nvb = refresh ? s_nvb : st->nvb;
+        if (refresh)
+            icp_unit<MODE>(P, s_nsrc, s_G, nvb, s_ntgt, vb, epoch > 0, T, sys);
+        else
+            icp_unit<MODE>(P, st->n_src, st->G, nvb, st->n_tgt, vb, epoch > 0, T, sys);
         unit_commit<MODE>(P, states, p, vb, nvb, epoch, sys);
     }
 }
@@ -1081,20 +1108,6 @@ __global__ void k_state_init(IcpState *states, const PairDesc *__restrict__ pair
     s.ticket = 0u;
 }
 
-// Sizes and launch shape of states [s0, s0 + n) from their clouds' headers, for states whose
-// source was still being built when k_state_init ran (a frame pipeline's pyramid levels).
-__global__ void k_state_sizes(IcpState *states, const PairDesc *__restrict__ pairs, int s0, int n)
-{
-    const int p = s0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= s0 + n) return;
-    IcpState &s = states[p];
-    const PairDesc &P = pairs[p];
-    s.n_src = P.src_hdr->n;
-    s.n_tgt = P.tgt_hdr->n;
-    s.G = icp_lanes_for(P.lanes, s.n_src);
-    s.nvb = icp_units_for(s.n_src, s.G, P.unit_cap);
-}
-
 // State of the single-pair kernels (Gauss-Newton build, residuals, objective).
 __global__ void k_state_simple(IcpState *st, const double *__restrict__ T0)
 {
@@ -1291,25 +1304,26 @@ static void states_describe(int n_pairs, int n_levels, const gca_cloud *srcs, co
 
 template <int MODE>
 static void launch_persist_mode(gca_ctx ctx, const PairDesc *pairs, IcpState *states, int s_first, int n_states,
-                                uint32_t *err)
+                                int refresh, uint32_t *err)
 {
     const int blocks = ctx->tune[GCA_TUNE_ICP_PERSIST_BLOCKS] ? ctx->tune[GCA_TUNE_ICP_PERSIST_BLOCKS] : 2 * ctx->n_cu;
     hipLaunchKernelGGL(k_icp_persist<MODE>, dim3((unsigned)blocks), dim3(kIcpBlock), 0, ctx->stream, pairs, states,
-                       s_first, n_states, err);
+                       s_first, n_states, refresh, err);
 }
 
 // The persistent launch over states [s_first, s_first + n_states) (the chain links of their
-// finishers may open states after the range: those run in a later launch).
+// finishers may open states after the range: those run in a later launch).  refresh: one state,
+// whose sizes the launch takes from its clouds' headers.
 static void launch_persist_any(gca_ctx ctx, int mode, const PairDesc *pairs, IcpState *states, int n_states,
-                               uint32_t *err, int s_first = 0)
+                               uint32_t *err, int s_first = 0, int refresh = 0)
 {
     StageTimer timer(ctx, GCA_STAGE_ICP);
     if (mode == GCA_ACCUM_NAIVE_FP32)
-        launch_persist_mode<GCA_ACCUM_NAIVE_FP32>(ctx, pairs, states, s_first, n_states, err);
+        launch_persist_mode<GCA_ACCUM_NAIVE_FP32>(ctx, pairs, states, s_first, n_states, refresh, err);
     else if (mode == GCA_ACCUM_FP64)
-        launch_persist_mode<GCA_ACCUM_FP64>(ctx, pairs, states, s_first, n_states, err);
+        launch_persist_mode<GCA_ACCUM_FP64>(ctx, pairs, states, s_first, n_states, refresh, err);
     else
-        launch_persist_mode<GCA_ACCUM_COMPENSATED>(ctx, pairs, states, s_first, n_states, err);
+        launch_persist_mode<GCA_ACCUM_COMPENSATED>(ctx, pairs, states, s_first, n_states, refresh, err);
 }
 
 // Poses and reports from a host copy of the states.
@@ -1499,18 +1513,10 @@ int32_t icp_plan_init(gca_ctx ctx, IcpPlan *p)
     return GCA_OK;
 }
 
-int32_t icp_plan_sizes(gca_ctx ctx, IcpPlan *p, int s)
-{
-    if (s < 0 || s >= p->n_states) return GCA_ERR_INVALID_VALUE;
-    hipLaunchKernelGGL(k_state_sizes, dim3(1), dim3(64), 0, ctx->stream, p->states, (const PairDesc *)p->pairs, s, 1);
-    GCA_HIP(hipGetLastError());
-    return GCA_OK;
-}
-
 int32_t icp_plan_persist(gca_ctx ctx, IcpPlan *p, int s)
 {
     if (s < 0 || s >= p->n_states) return GCA_ERR_INVALID_VALUE;
-    launch_persist_any(ctx, p->mode, p->pairs, p->states, 1, p->err, s);
+    launch_persist_any(ctx, p->mode, p->pairs, p->states, 1, p->err, s, 1);
     GCA_HIP(hipGetLastError());
     return GCA_OK;
 }

@@ -11,13 +11,12 @@
 //              geometry, normals and colour gradient (one neighbourhood walk when the gradient's
 //              radius is the smaller and it has no K cap), grid-ordered attributes;
 //   registration of the new frame onto the previous one, level by level (coarse to fine): a
-//              registration plan built once per frame parity is set up before the fork; each
-//              level's stream ends its build by refreshing the level's registration sizes, and
-//              the persistent launches run one after another on the coarsest level's stream,
-//              each once its level is built (so the coarse levels register while the fine ones
-//              are still being built, and a launch follows the one handing it the pose with no
-//              cross-stream wait); results are copied to pinned host memory once every level
-//              has joined back.
+//              registration plan built once per frame parity is set up before the fork; the
+//              persistent launches (each takes its level's sizes from the built clouds' headers)
+//              run one after another on the coarsest level's stream, each once its level is
+//              built (so the coarse levels register while the fine ones are still being built,
+//              and a launch follows the one handing it the pose with no cross-stream wait);
+//              results are copied to pinned host memory once every level has joined back.
 // Nothing in a step depends on host-side sizes: the voxel lattice is computed on the device
 // (its scan covers an upper bound from the camera frustum), and each level's grid uses one
 // geometry over the frustum box (neighbour results do not depend on the grid geometry: points
@@ -46,7 +45,7 @@ struct gca_pipeline_s {
     gca_ctx ctx = nullptr;  // private: front end, registration, every cloud
     gca_ctx lctx[kPipeMaxLevels] = {};  // private, one per level (stream + scratch)
     hipEvent_t ev_fork = nullptr, ev_join[kPipeMaxLevels] = {};
-    hipEvent_t ev_built[kPipeMaxLevels] = {};  // level l's source built and its sizes refreshed
+    hipEvent_t ev_built[kPipeMaxLevels] = {};  // level l's source built
     gca_camera cam{};
     gca_bilateral bil{};
     float tmin = 0.f, tmax = 0.f;
@@ -140,7 +139,6 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
             // Enqueued here, not after every level's build, because a graph launch submits its
             // nodes in this order and a late submission is a late start.
             gca_ctx ic = p->lctx[0];
-            GCA_TRY(icp_plan_sizes(lc, plan, l));
             if (l > 0) {
                 GCA_HIP(hipEventRecord(p->ev_built[l], lc->stream));
                 GCA_HIP(hipStreamWaitEvent(ic->stream, p->ev_built[l], 0));

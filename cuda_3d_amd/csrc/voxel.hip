@@ -36,15 +36,13 @@ struct VoxelParams {
     int32_t pad;
 };
 
-// Lattice of a voxel call from the input's device header (one thread), so a launch sequence
-// needs no host round trip: origin = unpadded min bound, dims = floor(extent / v) + 1 per axis
-// in FP32 (point_cloud.cu:205-218, cuda_voxel_grid_down_sample.cu:37-47).  A lattice the
-// reference would refuse, or one with more than max_bits cells, is flagged (bad, *err) and
-// yields an empty output.
-__global__ void k_voxel_params(const CloudHdr *__restrict__ in, float voxel, double max_bits,
-                               VoxelParams *__restrict__ vp, uint32_t *__restrict__ err)
+// Lattice of a voxel call from the input's device header, so a launch sequence needs no host
+// round trip: origin = unpadded min bound, dims = floor(extent / v) + 1 per axis in FP32
+// (point_cloud.cu:205-218, cuda_voxel_grid_down_sample.cu:37-47).  A lattice the reference
+// would refuse, or one with more than max_bits cells, is flagged bad (and yields an empty
+// output).
+__device__ __forceinline__ VoxelParams voxel_params_of(const CloudHdr *__restrict__ in, float voxel, double max_bits)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int n = in->n;
     VoxelParams p{};
     p.voxel = voxel;
@@ -58,11 +56,8 @@ __global__ void k_voxel_params(const CloudHdr *__restrict__ in, float voxel, dou
         nbits *= (double)p.dims[a];
     }
     if (nbits > max_bits) p.bad = 1;
-    if (p.bad) {
-        p.dims[0] = p.dims[1] = p.dims[2] = 1;
-        if (err) atomicOr(err, 1u);
-    }
-    *vp = p;
+    if (p.bad) p.dims[0] = p.dims[1] = p.dims[2] = 1;
+    return p;
 }
 
 __device__ __forceinline__ int64_t voxel_key(const VoxelParams &vp, float4 p)
@@ -110,14 +105,25 @@ __device__ __forceinline__ T run_reduce(T v, int head, Op op)
 }
 
 // Raster-ordered points hit the same voxel in runs, so each wavefront first merges runs and
-// only the first lane of a run issues the atomics.
+// only the first lane of a run issues the atomics.  Every workgroup derives the lattice from the
+// header itself; workgroup 0 also stores it for the later kernels of the call and flags a
+// refused lattice (*err).
 __global__ __launch_bounds__(256) void k_voxel_mark(const float4 *__restrict__ xyzi,
-                                                    const CloudHdr *__restrict__ hdr,
-                                                    const VoxelParams *__restrict__ vpp,
-                                                    uint32_t *__restrict__ bitmap,
+                                                    const CloudHdr *__restrict__ hdr, float voxel,
+                                                    double max_bits, VoxelParams *__restrict__ vp_out,
+                                                    uint32_t *__restrict__ err, uint32_t *__restrict__ bitmap,
                                                     int64_t *__restrict__ lin)
 {
-    const VoxelParams vp = *vpp;
+    __shared__ VoxelParams s_vp;
+    if (threadIdx.x == 0) {
+        s_vp = voxel_params_of(hdr, voxel, max_bits);
+        if (blockIdx.x == 0) {
+            *vp_out = s_vp;
+            if (s_vp.bad && err) atomicOr(err, 1u);
+        }
+    }
+    __syncthreads();
+    const VoxelParams vp = s_vp;
     const int n = vp.bad ? 0 : hdr->n;
     const int stride = gridDim.x * blockDim.x;
     for (int b = blockIdx.x * blockDim.x; b < n; b += stride) {  // block-uniform trip count
@@ -246,7 +252,7 @@ __global__ __launch_bounds__(256) void k_voxel_finalize(unsigned long long *__re
     }
 }
 
-// Kernel sequence of one voxel call: device lattice, mark, popcount scan over `words` bitmap
+// Kernel sequence of one voxel call: mark (with the device lattice), popcount scan over `words` bitmap
 // words (>= the lattice's), accumulate, finalize.  Scratch is sized for `n_cap` input points.
 static int32_t voxel_enqueue(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, int64_t n_cap, int64_t words,
                              double max_bits, uint32_t *err)
@@ -273,8 +279,7 @@ static int32_t voxel_enqueue(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud o
     const unsigned nb = (unsigned)std::min<int64_t>(4096, (nc + 255) / 256);
     ctx->bitmap_dirty = true;  // until the whole sequence is enqueued
     StageTimer timer(ctx, GCA_STAGE_VOXEL);
-    hipLaunchKernelGGL(k_voxel_params, dim3(1), dim3(64), 0, ctx->stream, in->hdr, voxel, max_bits, vp, err);
-    hipLaunchKernelGGL(k_voxel_mark, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->hdr, (const VoxelParams *)vp,
+    hipLaunchKernelGGL(k_voxel_mark, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->hdr, voxel, max_bits, vp, err,
                        ctx->bitmap, lin);
     GCA_HIP(hipGetLastError());
     GCA_TRY(exclusive_scan_u32(ctx, ctx->bitmap, word_pre, words, (uint32_t *)&out->hdr->n, true));
