@@ -194,8 +194,10 @@ int32_t grid_gather_attributes(gca_ctx ctx, gca_cloud c);
 int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn);
 int32_t launch_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn);
 // Normals (radius, max_nn) and colour gradient (gradient_radius <= radius, all neighbours) from
-// one neighbourhood walk; results equal launch_normals followed by launch_gradient.
-int32_t launch_normals_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn, float gradient_radius);
+// one neighbourhood walk; results equal launch_normals followed by launch_gradient.  grid_attrs:
+// the solve also writes the grid-ordered copies (as grid_gather_attributes would).
+int32_t launch_normals_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn, float gradient_radius,
+                                bool grid_attrs = false);
 int32_t launch_transform(gca_ctx ctx, gca_cloud c, const double T[16]);
 int32_t launch_bounds(gca_ctx ctx, gca_cloud c);
 int32_t launch_copy_cloud(gca_ctx ctx, gca_cloud src, gca_cloud dst);

@@ -595,15 +595,20 @@ __global__ __launch_bounds__(kNbBlock) void k_solve(const CloudHdr *__restrict__
 
 // Pass 3 of the joint walk: the normal, then the colour gradient on that normal (the value
 // stored, as a separate gradient call would read it back).
+// order != nullptr: thread s solves point order[s] (the grid's sorted slot s) and also writes
+// its normal and gradient to snrm[s] / sgrad[s], the grid-ordered copies registration reads.
 __global__ __launch_bounds__(kNbBlock) void k_solve_pair(const CloudHdr *__restrict__ hdr,
                                                          const Moments *__restrict__ mom,
                                                          const Moments *__restrict__ mom_g, double rg,
                                                          float4 *__restrict__ nrm, float4 *__restrict__ grad,
+                                                         const int32_t *__restrict__ order,
+                                                         float4 *__restrict__ snrm, float4 *__restrict__ sgrad,
                                                          uint32_t *__restrict__ ovf_n)
 {
     const int n = hdr->n;
     if (blockIdx.x == 0 && threadIdx.x == 0) *ovf_n = 0u;
-    for (int i = blockIdx.x * kNbBlock + threadIdx.x; i < n; i += gridDim.x * kNbBlock) {
+    for (int s = blockIdx.x * kNbBlock + threadIdx.x; s < n; s += gridDim.x * kNbBlock) {
+        const int i = order ? order[s] : s;
         long long S[9];
         const Moments &M = mom[i];
 #pragma unroll
@@ -613,7 +618,12 @@ __global__ __launch_bounds__(kNbBlock) void k_solve_pair(const CloudHdr *__restr
         const Moments &Mg = mom_g[i];
 #pragma unroll
         for (int c = 0; c < 9; ++c) S[c] = Mg.S[c];
-        grad[i] = gradient_from_moments(Mg.cnt, S, nf, rg);
+        const float4 gv = gradient_from_moments(Mg.cnt, S, nf, rg);
+        grad[i] = gv;
+        if (order) {
+            snrm[s] = nf;
+            sgrad[s] = gv;
+        }
     }
 }
 
@@ -658,10 +668,15 @@ int32_t launch_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
     return launch_moments_pipeline<1>(ctx, c, radius, K, c->grad);
 }
 
-int32_t launch_normals_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K, float gradient_radius)
+int32_t launch_normals_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K, float gradient_radius,
+                                bool grid_attrs)
 {
     if (!(gradient_radius <= radius)) return GCA_ERR_INVALID_VALUE;
-    const Grid &G = c->grid;
+    Grid &G = c->grid;
+    if (grid_attrs && !G.snrm) {
+        GCA_HIP(hipMalloc(&G.snrm, sizeof(float4) * G.cap_pts));
+        GCA_HIP(hipMalloc(&G.sgrad, sizeof(float4) * G.cap_pts));
+    }
     const int gs = normals_lanes(ctx);
     const int64_t cap = std::max<int64_t>(c->cap, 1);
     Moments *mom = (Moments *)ctx_scratch(ctx, 17, sizeof(Moments) * cap, false);
@@ -679,8 +694,11 @@ int32_t launch_normals_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t 
                            dim3(kWave * kWavesPerBlock), 0, ctx->stream, c->xyzi, c->nrm, c->hdr, d, G.start,
                            G.order, G.sxyzi, K, (const int32_t *)ovf, (const uint32_t *)ovf_n, mom);
     hipLaunchKernelGGL(k_solve_pair, dim3(grid_for(cap, kNbBlock, 2048)), dim3(kNbBlock), 0, ctx->stream, c->hdr,
-                       (const Moments *)mom, (const Moments *)mom_g, rg, c->nrm, c->grad, ovf_n);
+                       (const Moments *)mom, (const Moments *)mom_g, rg, c->nrm, c->grad,
+                       grid_attrs ? (const int32_t *)G.order : nullptr, grid_attrs ? G.snrm : nullptr,
+                       grid_attrs ? G.sgrad : nullptr, ovf_n);
     GCA_HIP(hipGetLastError());
+    if (grid_attrs) G.attrs_valid = true;
     return GCA_OK;
 }
 

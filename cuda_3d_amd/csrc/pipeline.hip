@@ -9,7 +9,8 @@
 //              run side by side: the small coarse levels fill the device next to the fine one):
 //              voxel lattice from the device header + downsample, neighbour grid of fixed
 //              geometry, normals and colour gradient (one neighbourhood walk when the gradient's
-//              radius is the smaller and it has no K cap), grid-ordered attributes;
+//              radius is the smaller and it has no K cap; its solve then also writes the
+//              grid-ordered attributes, else one gather kernel does);
 //   registration of the new frame onto the previous one, level by level (coarse to fine): a
 //              registration plan built once per frame parity is set up before the fork; the
 //              persistent launches (each takes its level's sizes from the built clouds' headers)
@@ -125,14 +126,15 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         GCA_TRY(launch_voxel_planned(lc, p->full[slot], L.voxel_size, c, p->max_bits[l], p->err_dev));
         GCA_TRY(launch_grid_build(lc, c, p->gdesc[l]));
         if (L.gradient_max_nn == 0 && L.gradient_radius <= L.normal_radius) {
-            GCA_TRY(launch_normals_gradient(lc, c, L.normal_radius, L.normal_max_nn, L.gradient_radius));
+            // the solve writes the grid-ordered copies registration reads
+            GCA_TRY(launch_normals_gradient(lc, c, L.normal_radius, L.normal_max_nn, L.gradient_radius, true));
         } else {
             GCA_TRY(launch_normals(lc, c, L.normal_radius, L.normal_max_nn));
             GCA_TRY(launch_gradient(lc, c, L.gradient_radius, L.gradient_max_nn));
+            c->grid.attrs_valid = false;
         }
         c->has_normals = c->has_gradient = true;
-        c->grid.attrs_valid = false;
-        GCA_TRY(grid_gather_attributes(lc, c));
+        GCA_TRY(grid_gather_attributes(lc, c));  // no launch when the solve wrote them
         if (plan) {
             // the registration chain runs on level 0's stream (after level 0's build): level l
             // once it is built, right after level l - 1's launch, which hands it the pose.
