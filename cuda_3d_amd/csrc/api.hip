@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_transform(float4 *__restrict__ xyzi, fl
 }
 
 // ---- exclusive scan of uint32: one pass with decoupled look-back ---------------------------
-// Chunks of kScanChunk elements are handed out in start order by a counter, so a chunk only
+// Chunks of values are handed out in start order by a counter, so a chunk only
 // ever waits on chunks whose workgroups are already running.  Each chunk publishes one 64-bit
 // word per call: (call sequence << 34) | (state << 32) | value, first with its own total (state
 // 1), then, once the totals before it are known, with its inclusive prefix (state 2).  Wavefront
@@ -143,9 +143,11 @@ __global__ __launch_bounds__(256) void k_transform(float4 *__restrict__ xyzi, fl
 // (graph-capture safe).  A wait that outlasts kScanSpin polls stops and raises the ctx's
 // device error (reported by gca_ctx_synchronize); with chunks taken in start order it cannot
 // happen unless the device stops making progress.
-constexpr int kScanBlock = 256;
+// Chunks of BLOCK x kScanPer values: 256-thread workgroups for short scans (the shortest
+// per-chunk latency), 1024-thread ones beyond kScanWideFrom 256-thread chunks (a look-back covers
+// 64 chunks per memory round trip, so fewer, wider chunks shorten a long scan's prefix chain).
 constexpr int kScanPer = 16;
-constexpr int kScanChunk = kScanBlock * kScanPer;
+constexpr int64_t kScanWideFrom = 128;
 constexpr uint32_t kScanSpin = 1u << 22;
 constexpr unsigned long long kScanAgg = 1ull << 32, kScanInc = 2ull << 32;
 constexpr uint32_t kScanSeqMask = (1u << 30) - 1u;
@@ -157,14 +159,14 @@ struct ScanCtl {
 template <bool POPC>
 __device__ __forceinline__ uint32_t scan_val(uint32_t v) { return POPC ? (uint32_t)__popc(v) : v; }
 
-template <bool POPC>
-__global__ __launch_bounds__(kScanBlock) void k_scan_onepass(const uint32_t *__restrict__ in, int64_t n,
+template <bool POPC, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_scan_onepass(const uint32_t *__restrict__ in, int64_t n,
                                                              int nchunks, unsigned long long *flags, ScanCtl *ctl,
                                                              uint32_t *__restrict__ out,
                                                              uint32_t *__restrict__ total_dev,
                                                              uint32_t *__restrict__ out_end, uint32_t *dev_err)
 {
-    __shared__ uint32_t lds[kScanBlock / kWave + 1];
+    __shared__ uint32_t lds[BLOCK / kWave + 1];
     __shared__ uint32_t s_chunk, s_seq, s_excl;
     if (threadIdx.x == 0) {
         const unsigned long long w = __hip_atomic_fetch_add(&ctl->ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_onepass(const uint32_t *__r
         if (threadIdx.x == 0) __hip_atomic_store(dev_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         return;
     }
-    const int64_t base = (int64_t)c * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+    const int64_t base = (int64_t)c * (BLOCK * kScanPer) + (int64_t)threadIdx.x * kScanPer;
     const bool full = base + kScanPer <= n && ((reinterpret_cast<uintptr_t>(in) & 15u) == 0);
     uint32_t v[kScanPer];
     if (full) {
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_onepass(const uint32_t *__r
         s += v[k];
     }
     uint32_t tot;
-    uint32_t run = block_excl_scan<kScanBlock>(s, lds, &tot);
+    uint32_t run = block_excl_scan<BLOCK>(s, lds, &tot);
     if (threadIdx.x < kWave) {
         const int lane = lane_id();
         const unsigned long long tag = (unsigned long long)(seq & kScanSeqMask) << 34;
@@ -270,18 +272,27 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_onepass(const uint32_t *__r
 int32_t exclusive_scan_u32(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n,
                            uint32_t *total_dev, bool popcount)
 {
-    int64_t nchunks = (n + kScanChunk - 1) / kScanChunk;
+    const bool wide = (n + 256 * kScanPer - 1) / (256 * kScanPer) > kScanWideFrom;
+    const int64_t chunk = (wide ? 1024 : 256) * kScanPer;
+    int64_t nchunks = (n + chunk - 1) / chunk;
     if (nchunks < 1) nchunks = 1;
     if (nchunks > 1024 * 64) return GCA_ERR_INVALID_VALUE;
     unsigned long long *flags = (unsigned long long *)ctx_scratch(ctx, 15, sizeof(unsigned long long) * nchunks, true);
     ScanCtl *ctl = (ScanCtl *)ctx_scratch(ctx, 23, sizeof(ScanCtl), true);
     if (!flags || !ctl) return GCA_ERR_OUT_OF_MEMORY;
-    if (popcount)
-        hipLaunchKernelGGL(k_scan_onepass<true>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream, in, n,
-                           (int)nchunks, flags, ctl, out, total_dev, out + n, ctx->dev_err);
+    const dim3 grid((unsigned)nchunks);
+    if (popcount && wide)
+        hipLaunchKernelGGL((k_scan_onepass<true, 1024>), grid, dim3(1024), 0, ctx->stream, in, n, (int)nchunks, flags,
+                           ctl, out, total_dev, out + n, ctx->dev_err);
+    else if (popcount)
+        hipLaunchKernelGGL((k_scan_onepass<true, 256>), grid, dim3(256), 0, ctx->stream, in, n, (int)nchunks, flags,
+                           ctl, out, total_dev, out + n, ctx->dev_err);
+    else if (wide)
+        hipLaunchKernelGGL((k_scan_onepass<false, 1024>), grid, dim3(1024), 0, ctx->stream, in, n, (int)nchunks, flags,
+                           ctl, out, total_dev, out + n, ctx->dev_err);
     else
-        hipLaunchKernelGGL(k_scan_onepass<false>, dim3((unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream, in, n,
-                           (int)nchunks, flags, ctl, out, total_dev, out + n, ctx->dev_err);
+        hipLaunchKernelGGL((k_scan_onepass<false, 256>), grid, dim3(256), 0, ctx->stream, in, n, (int)nchunks, flags,
+                           ctl, out, total_dev, out + n, ctx->dev_err);
     GCA_HIP(hipGetLastError());
     return GCA_OK;
 }
