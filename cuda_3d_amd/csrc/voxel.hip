@@ -149,11 +149,12 @@ __global__ __launch_bounds__(256) void k_voxel_mark(const float4 *__restrict__ x
 // in which they arrive does not change the result.  (Device-scope 64-bit atomics are what bound
 // the per-run version: ~8-14 G/s.)
 constexpr int kVoxChunk = 2048;
+constexpr int kVoxBlock = 1024;  // 2 points per thread: short per-thread chains, many waves per chunk
 constexpr int kVoxTableBits = 10;
 constexpr int kVoxTable = 1 << kVoxTableBits;
 constexpr int kVoxProbe = 16;
 
-__global__ __launch_bounds__(256) void k_voxel_accum(const float4 *__restrict__ xyzi,
+__global__ __launch_bounds__(kVoxBlock) void k_voxel_accum(const float4 *__restrict__ xyzi,
                                                      const float4 *__restrict__ rgb,
                                                      const CloudHdr *__restrict__ hdr,
                                                      const VoxelParams *__restrict__ vpp,
@@ -284,7 +285,7 @@ static int32_t voxel_enqueue(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud o
     GCA_HIP(hipGetLastError());
     GCA_TRY(exclusive_scan_u32(ctx, ctx->bitmap, word_pre, words, (uint32_t *)&out->hdr->n, true));
     const unsigned nb_acc = (unsigned)std::min<int64_t>(4096, (nc + kVoxChunk - 1) / kVoxChunk);
-    hipLaunchKernelGGL(k_voxel_accum, dim3(nb_acc), dim3(256), 0, ctx->stream, in->xyzi, in->rgb, in->hdr,
+    hipLaunchKernelGGL(k_voxel_accum, dim3(nb_acc), dim3(kVoxBlock), 0, ctx->stream, in->xyzi, in->rgb, in->hdr,
                        (const VoxelParams *)vp, ctx->bitmap, word_pre, lin, acc);
     hipLaunchKernelGGL(k_voxel_finalize, dim3(nb), dim3(256), 0, ctx->stream, acc, in->hdr, out->hdr,
                        ctx->bitmap, out->xyzi, out->rgb);
