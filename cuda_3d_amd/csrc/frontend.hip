@@ -9,10 +9,11 @@
 // arguments), then back-projection, depth->colour re-projection and BGR sampling in the
 // reference's FP32 operation order.  Results go to a per-pixel staging buffer and each block
 // records its count of valid pixels.
-// Kernel B: exclusive scan of the block counts (raster order == block order).
-// Kernel C: stable compaction (wave ballot + in-block prefix) into the SoA cloud, plus each
-// block's min/max into its own slot.
-// Kernel D: one workgroup folds the block bounds into the cloud header (no contended atomics).
+// Kernel B: stable compaction (wave ballot + in-block prefix) into the SoA cloud at the block's
+// offset, which each block sums itself from the counts of the blocks before it in raster order
+// (at most a few thousand L2-resident words: no separate scan launch); the last block writes
+// the count.  Each block's min/max goes into its own slot.
+// Kernel C: one workgroup folds the block bounds into the cloud header (no contended atomics).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restr
                                                          float4 *__restrict__ st_rgb,
                                                          uint8_t *__restrict__ st_valid,
                                                          uint32_t *__restrict__ block_cnt,
-                                                         CloudHdr *__restrict__ hdr)
+                                                         CloudHdr *__restrict__ hdr, uint32_t *clear_word)
 {
     __shared__ uint16_t tile[2 * kMaxR + 1][kRowSeg + 2 * kMaxR];
     __shared__ uint32_t wcount[kRowSeg / kWave];
@@ -62,6 +63,7 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restr
     else if (fp.mode == GCA_BILATERAL_REFERENCE_ADAPTIVE) R = 5;
 
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (clear_word) *clear_word = 0u;
         for (int a = 0; a < 3; ++a) {
             hdr->bmin_key[a] = float_to_key(FLT_MAX);
             hdr->bmax_key[a] = float_to_key(FLT_MIN);  // geometry_util.cuh:92
@@ -166,35 +168,46 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restr
 __global__ __launch_bounds__(kRowSeg) void k_rgbd_compact(const float4 *__restrict__ st_xyzi,
                                                           const float4 *__restrict__ st_rgb,
                                                           const uint8_t *__restrict__ st_valid,
-                                                          const uint32_t *__restrict__ block_off,
+                                                          const uint32_t *__restrict__ block_cnt,
                                                           int32_t W, int32_t nbx,
                                                           float4 *__restrict__ out_xyzi,
                                                           float4 *__restrict__ out_rgb,
-                                                          float *__restrict__ bpart)
+                                                          float *__restrict__ bpart, CloudHdr *__restrict__ hdr)
 {
     __shared__ uint32_t woff[kRowSeg / kWave];
+    __shared__ uint32_t wpre[kRowSeg / kWave];
     __shared__ float smin[3][kRowSeg / kWave], smax[3][kRowSeg / kWave];
     const int y = blockIdx.y;
     const int x = blockIdx.x * kRowSeg + threadIdx.x;
     const int64_t p = (int64_t)y * W + x;
+    const int64_t b = (int64_t)y * nbx + blockIdx.x;  // raster order of the block
+    // this block's offset: the counts of the blocks before it (strided over the threads)
+    uint32_t pre = 0;
+    for (int64_t k = threadIdx.x; k < b; k += kRowSeg) pre += block_cnt[k];
+    pre = wave_sum(pre);
     bool valid = (x < W) && st_valid[p];
     uint64_t m = __ballot(valid);
     const int w = threadIdx.x / kWave;
-    if (lane_id() == 0) woff[w] = (uint32_t)__popcll(m);
+    if (lane_id() == 0) {
+        woff[w] = (uint32_t)__popcll(m);
+        wpre[w] = pre;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t s = 0;
+        for (int i = 0; i < kRowSeg / kWave; ++i) s += wpre[i];
         for (int i = 0; i < kRowSeg / kWave; ++i) {
             uint32_t t = woff[i];
             woff[i] = s;
             s += t;
         }
+        if (b == (int64_t)gridDim.x * gridDim.y - 1) hdr->n = (int32_t)s;  // the last block: the count
     }
     __syncthreads();
     float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     if (valid) {
         uint64_t below = m & ((1ull << lane_id()) - 1ull);
-        uint32_t dst = block_off[(int64_t)y * nbx + blockIdx.x] + woff[w] + (uint32_t)__popcll(below);
+        uint32_t dst = woff[w] + (uint32_t)__popcll(below);
         float4 a = st_xyzi[p];
         out_xyzi[dst] = a;
         out_rgb[dst] = st_rgb[p];
@@ -305,7 +318,7 @@ static int32_t build_lut(gca_ctx ctx, const gca_bilateral *f, FrontParams *fp)
 }
 
 int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr, const gca_camera *cam,
-                         const gca_bilateral *bil, float tmin, float tmax, gca_cloud out)
+                         const gca_bilateral *bil, float tmin, float tmax, gca_cloud out, uint32_t *clear_word)
 {
     const int W = cam->width, H = cam->height;
     const int64_t P = (int64_t)W * H;
@@ -329,17 +342,15 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
     float4 *st_rgb = (float4 *)ctx_scratch(ctx, 1, sizeof(float4) * P, false);
     uint8_t *st_valid = (uint8_t *)ctx_scratch(ctx, 2, (size_t)P, false);
     uint32_t *bcnt = (uint32_t *)ctx_scratch(ctx, 3, sizeof(uint32_t) * (nblocks + 1), false);
-    uint32_t *boff = (uint32_t *)ctx_scratch(ctx, 4, sizeof(uint32_t) * (nblocks + 1), false);
     float *bpart = (float *)ctx_scratch(ctx, 20, sizeof(float) * 6 * nblocks, false);
-    if (!st_xyzi || !st_rgb || !st_valid || !bcnt || !boff || !bpart) return GCA_ERR_OUT_OF_MEMORY;
+    if (!st_xyzi || !st_rgb || !st_valid || !bcnt || !bpart) return GCA_ERR_OUT_OF_MEMORY;
     dim3 grid(fp.nbx, H);
     StageTimer timer(ctx, GCA_STAGE_FRONTEND);
     hipLaunchKernelGGL(k_rgbd_pixels, grid, dim3(kRowSeg), 0, ctx->stream, depth, bgr,
-                       (const double *)ctx->lut, fp, st_xyzi, st_rgb, st_valid, bcnt, out->hdr);
+                       (const double *)ctx->lut, fp, st_xyzi, st_rgb, st_valid, bcnt, out->hdr, clear_word);
     GCA_HIP(hipGetLastError());
-    GCA_TRY(exclusive_scan_u32(ctx, bcnt, boff, nblocks, (uint32_t *)&out->hdr->n));
     hipLaunchKernelGGL(k_rgbd_compact, grid, dim3(kRowSeg), 0, ctx->stream, st_xyzi, st_rgb, st_valid,
-                       boff, W, fp.nbx, out->xyzi, out->rgb, bpart);
+                       (const uint32_t *)bcnt, W, fp.nbx, out->xyzi, out->rgb, bpart, out->hdr);
     hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, ctx->stream, (const float *)bpart, nblocks, out->hdr);
     GCA_HIP(hipGetLastError());
     return GCA_OK;

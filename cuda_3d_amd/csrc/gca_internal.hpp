@@ -175,9 +175,10 @@ inline GridDesc query_desc(const Grid &G, float radius)
 }
 
 // Kernel launchers (one translation unit each) -------------------------------------------
+// clear_word (optional): a device word the first kernel sets to zero (a caller's per-step flag).
 int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
                          const gca_camera *cam, const gca_bilateral *bil, float tmin, float tmax,
-                         gca_cloud out);
+                         gca_cloud out, uint32_t *clear_word = nullptr);
 int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out);
 // No host round trip: the lattice is computed on the device; the bitmap scan covers max_bits
 // cells (a lattice beyond that flags *err and gives an empty output); scratch by capacity.

@@ -70,11 +70,6 @@ struct gca_pipeline_s {
 
 namespace {
 
-__global__ void k_zero_word(uint32_t *w)
-{
-    if (threadIdx.x == 0) *w = 0u;
-}
-
 void drop_graphs(gca_pipeline p)
 {
     for (int s = 0; s < 2; ++s) {
@@ -113,8 +108,8 @@ int32_t make_plan(gca_pipeline p, int slot)
 int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *bgr)
 {
     gca_ctx ctx = p->ctx;
-    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(64), 0, ctx->stream, p->err_dev);
-    GCA_TRY(launch_from_rgbd(ctx, depth, bgr, &p->cam, &p->bil, p->tmin, p->tmax, p->full[slot]));
+    // the front end's first kernel also clears the step's voxel-lattice error word
+    GCA_TRY(launch_from_rgbd(ctx, depth, bgr, &p->cam, &p->bil, p->tmin, p->tmax, p->full[slot], p->err_dev));
     IcpPlan *plan = p->frames >= 1 ? p->plan[slot] : nullptr;
     if (plan) GCA_TRY(icp_plan_init(ctx, plan));
     GCA_HIP(hipEventRecord(p->ev_fork, ctx->stream));
