@@ -6,11 +6,13 @@
 // The reference sorts int3 keys with a comparison sort and reduces by key.  Here the output
 // order (lexicographic (kx, ky, kz) == ascending linear key) comes from an occupancy bitmap
 // over the voxel lattice instead of a sort:
-//   1. mark: key in the reference's FP32 formula, set bit L = (kx*ny + ky)*nz + kz;
-//   2. popcount prefix over the bitmap words (reduce-then-scan) -> rank of every occupied
-//      voxel in lexicographic order, and the voxel count M;
-//   3. accumulate: each point adds its coordinates and colour in 2^-30 fixed point (int64
-//      atomics: exact, so the sums do not depend on the order in which points arrive);
+//   1. mark: the lattice from the input's device header (every workgroup for itself), then
+//      the key in the reference's FP32 formula, set bit L = (kx*ny + ky)*nz + kz;
+//   2. popcount prefix over the bitmap words (one-pass scan with decoupled look-back) -> rank
+//      of every occupied voxel in lexicographic order, and the voxel count M;
+//   3. accumulate: each point adds its coordinates and colour in 2^-30 fixed point, first into
+//      a workgroup's LDS table, then once per voxel and workgroup into int64 sums in global
+//      memory (integer atomics: exact, so the sums do not depend on the order of arrival);
 //   4. finalize: mean = sum / count, intensity from the mean colour; the bitmap words and
 //      accumulators are cleared on the way out, so no memset is needed before the next call.
 // The output keeps the input's bounding box, as the reference does (point_cloud.cu:224-226).
