@@ -96,6 +96,12 @@ def run_batch(g, ctx, n_pairs, levels, iters, seed=2026):
     lat = np.array(lat) * 1e3
     sizes = bp.sizes
     iters_run = [[r.iterations for r in row] for row in breps]
+    # algorithmic bytes of the batched launch: per pair, level and iteration 24 B per source
+    # point (x, y, z, I read; previous match read and written) + 48 B per target point
+    # (x, y, z, I, normal, gradient read once), as for the frame's launches
+    algo = sum(iters_run[k][l] * (24.0 * sizes[l][k] + 48.0 * bp.tgt_sizes[l][k])
+               for k in range(n_pairs) for l in range(len(levels)))
+    achieved = algo / tb_s / 1e9
     # registration quality against the generating transforms (not a parity bar)
     truth_err = []
     for k, G in enumerate(bp.truth):
@@ -113,6 +119,8 @@ def run_batch(g, ctx, n_pairs, levels, iters, seed=2026):
         "batched_latency_ms_p50": float(np.percentile(blat, 50)), "batched_latency_ms_p99": float(np.percentile(blat, 99)),
         "plain_latency_ms_p50": float(np.percentile(lat, 50)), "plain_latency_ms_p99": float(np.percentile(lat, 99)),
         "max_abs_pose_diff_batched_vs_plain": diff,
+        "batched_roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": algo},
         "rotation_err_vs_truth_rad_median": float(np.median(truth_err)),
     }
 
@@ -247,7 +255,7 @@ def main():
     if not args.no_batch and args.batch_pairs > 0:
         batch = run_batch(g, ctx, args.batch_pairs, levels, iters)
         log(f"batch: {batch['batched_pairs_per_s']:.1f} pairs/s batched vs {batch['plain_pairs_per_s']:.1f} plain "
-            f"({batch['speedup']:.2f}x)")
+            f"({batch['speedup']:.2f}x); batched launch {batch['batched_roofline']['achieved']:.0f} GB/s algorithmic")
 
     out = {
         "metric": "ms/frame 640x480 depth->ICP; batched ICP pairs/s vs plain at equal pose error",

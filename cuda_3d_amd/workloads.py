@@ -87,6 +87,7 @@ class BatchPairs:
         self.grad_r = [2.0 * v for v in levels]
         self.T0 = np.tile(np.eye(4), (n_pairs, 1, 1))
         self.sizes = [[self.src[k][l].points_number() for k in range(n_pairs)] for l in range(len(levels))]
+        self.tgt_sizes = [[self.tgt[k][l].points_number() for k in range(n_pairs)] for l in range(len(levels))]
         if log:
             log(f"batch: {n_pairs} pairs, sampled {int(self.counts.sum())} points per side, finest level "
                 f"{min(self.sizes[-1])}..{max(self.sizes[-1])} ({time.perf_counter() - t0:.1f}s)")
