@@ -217,7 +217,10 @@ int32_t icp_plan_launch(gca_ctx ctx, IcpPlan *plan);
 // for overlapping state s's registration with the building of state s + 1's source.
 int32_t icp_plan_init(gca_ctx ctx, IcpPlan *plan);
 int32_t icp_plan_persist(gca_ctx ctx, IcpPlan *plan, int s);
-int32_t icp_plan_copy_results(gca_ctx ctx, IcpPlan *plan);
+// Results into the plan's pinned host memory (one kernel); extra: an optional caller word copied
+// to host_extra (pinned) by the same kernel.
+int32_t icp_plan_copy_results(gca_ctx ctx, IcpPlan *plan, const uint32_t *extra = nullptr,
+                              uint32_t *host_extra = nullptr);
 int32_t icp_plan_results(gca_ctx ctx, const IcpPlan *plan, double *T_out, gca_icp_report *reports);
 void icp_plan_destroy(IcpPlan *plan);
 

@@ -1521,11 +1521,27 @@ int32_t icp_plan_persist(gca_ctx ctx, IcpPlan *p, int s)
     return GCA_OK;
 }
 
-int32_t icp_plan_copy_results(gca_ctx ctx, IcpPlan *p)
+// The states, the launch's error word and an optional caller word, stored by one workgroup
+// straight into pinned host memory (device-accessible): one launch instead of a copy each.
+__global__ __launch_bounds__(256) void k_results_to_host(const uint4 *__restrict__ states, int n16,
+                                                         const uint32_t *__restrict__ err,
+                                                         const uint32_t *__restrict__ extra, uint4 *host_states,
+                                                         uint32_t *host_err, uint32_t *host_extra)
 {
-    GCA_HIP(hipMemcpyAsync(p->host_states, p->states, sizeof(IcpState) * p->n_states, hipMemcpyDeviceToHost,
-                           ctx->stream));
-    GCA_HIP(hipMemcpyAsync(p->host_err, p->err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    for (int k = threadIdx.x; k < n16; k += blockDim.x) host_states[k] = states[k];
+    if (threadIdx.x == 0) {
+        *host_err = *err;
+        if (extra) *host_extra = *extra;
+    }
+}
+
+int32_t icp_plan_copy_results(gca_ctx ctx, IcpPlan *p, const uint32_t *extra, uint32_t *host_extra)
+{
+    static_assert(sizeof(IcpState) % 16 == 0, "states are copied as 16-byte words");
+    const int n16 = (int)(sizeof(IcpState) * p->n_states / 16);
+    hipLaunchKernelGGL(k_results_to_host, dim3(1), dim3(256), 0, ctx->stream, (const uint4 *)p->states, n16,
+                       (const uint32_t *)p->err, extra, (uint4 *)p->host_states, p->host_err, host_extra);
+    GCA_HIP(hipGetLastError());
     return GCA_OK;
 }
 
