@@ -148,10 +148,12 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         }
     }
     if (plan) {
+        // the registration results and the step's lattice error word, by one kernel right after
+        // the last registration launch on its stream (every level's build is ordered before that
+        // launch, so the error word is final)
+        GCA_TRY(icp_plan_copy_results(p->lctx[0], plan, p->err_dev, p->err_host));
         GCA_HIP(hipEventRecord(p->ev_join[0], p->lctx[0]->stream));
         GCA_HIP(hipStreamWaitEvent(ctx->stream, p->ev_join[0], 0));
-        // the registration results and the step's lattice error word, by one kernel
-        GCA_TRY(icp_plan_copy_results(ctx, plan, p->err_dev, p->err_host));
     } else {
         GCA_HIP(hipMemcpyAsync(p->err_host, p->err_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     }
