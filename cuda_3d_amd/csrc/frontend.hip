@@ -28,6 +28,11 @@ namespace gca {
 
 constexpr int kRowSeg = 256;
 constexpr int kMaxR = 15;
+constexpr int kStageBatch = 8;  // staging loads in flight per thread
+constexpr int kWrLds = 4096;    // longest range-weight table staged in LDS (32 KB)
+static_assert(sizeof(double) * ((2 * kMaxR + 1) * (2 * kMaxR + 1) + kWrLds) +
+                      sizeof(uint16_t) * (2 * kMaxR + 1) * (kRowSeg + 2 * kMaxR) <= 60 * 1024,
+              "the pixel kernel's dynamic LDS must stay within a workgroup's 64 KB");
 
 struct FrontParams {
     int32_t W, H, nbx;
@@ -49,10 +54,16 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restr
                                                          float4 *__restrict__ st_rgb,
                                                          uint8_t *__restrict__ st_valid,
                                                          uint32_t *__restrict__ block_cnt,
-                                                         CloudHdr *__restrict__ hdr, uint32_t *clear_word)
+                                                         CloudHdr *__restrict__ hdr, uint32_t *clear_word,
+                                                         int n_ws, int n_wr)
 {
-    __shared__ uint16_t tile[2 * kMaxR + 1][kRowSeg + 2 * kMaxR];
+    // dynamic LDS, sized by the host for this filter: the spatial and range weight tables (fixed
+    // mode: every tap reads one of each) followed by the depth tile of the filter's own radius
+    extern __shared__ __align__(16) unsigned char front_lds[];
     __shared__ uint32_t wcount[kRowSeg / kWave];
+    double *s_ws = reinterpret_cast<double *>(front_lds);
+    double *s_wr = s_ws + n_ws;
+    uint16_t *tile = reinterpret_cast<uint16_t *>(s_wr + n_wr);
 
     const int W = fp.W, H = fp.H;
     const int y = blockIdx.y;
@@ -72,20 +83,46 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restr
 
     // stage rows y-R .. y+R, columns x0-R .. x0+255+R (out of frame -> 0, which the filter
     // skips exactly like the reference skips out-of-frame and zero taps)
+    // (each thread's loads go out kStageBatch at a time before any is written to LDS: one
+    // memory round trip per batch, not per element)
     const int rows = 2 * R + 1, cols = kRowSeg + 2 * R;
-    for (int k = threadIdx.x; k < rows * cols; k += kRowSeg) {
-        int ry = k / cols, rx = k - ry * cols;
-        int gy = y - R + ry, gx = x0 - R + rx;
-        uint16_t v = 0;
-        if (gy >= 0 && gy < H && gx >= 0 && gx < W) v = depth[(int64_t)gy * W + gx];
-        tile[ry][rx] = v;
+    for (int b0 = 0; b0 < rows * cols; b0 += kRowSeg * kStageBatch) {
+        uint16_t v[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {  // unconditional loads from clamped (in-frame) addresses
+            const int k = b0 + u * kRowSeg + (int)threadIdx.x;
+            const int ry = k / cols, rx = k - ry * cols;
+            const int gy = y - R + ry, gx = x0 - R + rx;
+            const bool in = k < rows * cols && gy >= 0 && gy < H && gx >= 0 && gx < W;
+            const int cy = min(max(gy, 0), H - 1), cx = min(max(gx, 0), W - 1);
+            const uint16_t d = depth[(int64_t)cy * W + cx];
+            v[u] = in ? d : (uint16_t)0;
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+            const int k = b0 + u * kRowSeg + (int)threadIdx.x;
+            if (k < rows * cols) tile[k] = v[u];
+        }
+    }
+    for (int b0 = 0; b0 < n_ws + n_wr; b0 += kRowSeg * kStageBatch) {
+        double w[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {  // unconditional loads from clamped table indices
+            const int k = min(b0 + u * kRowSeg + (int)threadIdx.x, n_ws + n_wr - 1);
+            w[u] = lut[k < n_ws ? fp.ws_off[0] + k : fp.wr_off[0] + (k - n_ws)];
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+            const int k = b0 + u * kRowSeg + (int)threadIdx.x;
+            if (k < n_ws + n_wr) s_ws[k] = w[u];  // s_wr follows s_ws
+        }
     }
     __syncthreads();
 
     bool valid = false;
     float4 oxyz = make_float4(0.f, 0.f, 0.f, 0.f), orgb = make_float4(0.f, 0.f, 0.f, 0.f);
     if (x < W) {
-        const uint16_t d0 = tile[R][threadIdx.x + R];
+        const uint16_t d0 = tile[R * cols + threadIdx.x + R];
         const float z0 = (float)d0 * fp.scale;
         if (d0 != 0 && !(z0 < fp.tmin || z0 > fp.tmax)) {
             double f = (double)d0;
@@ -100,23 +137,28 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restr
                 r = (int)(1.0f * (float)step);
             }
             if (r >= 0) {
-                const double *ws = lut + fp.ws_off[s];
-                const double *wr = lut + fp.wr_off[s];
                 const int wl = fp.wr_len[s];
                 const int side = 2 * r + 1;
                 double sw = 0.0, swd = 0.0;
-                for (int dy = -r; dy <= r; ++dy) {
-                    const uint16_t *row = &tile[R + dy][threadIdx.x + R];
-                    for (int dx = -r; dx <= r; ++dx) {
-                        uint16_t dn = row[dx];
-                        if (!dn) continue;
-                        int diff = abs((int)d0 - (int)dn);
-                        double wrv = diff < wl ? wr[diff] : 0.0;
-                        double w = __dmul_rn(ws[(dy + r) * side + (dx + r)], wrv);
-                        sw = __dadd_rn(sw, w);
-                        swd = __dadd_rn(swd, __dmul_rn(w, (double)dn));
+                auto taps = [&](const double *ws, const double *wr) {
+                    for (int dy = -r; dy <= r; ++dy) {
+                        const uint16_t *row = &tile[(R + dy) * cols + threadIdx.x + R];
+                        for (int dx = -r; dx <= r; ++dx) {
+                            uint16_t dn = row[dx];
+                            if (!dn) continue;
+                            int diff = abs((int)d0 - (int)dn);
+                            double wrv = diff < wl ? wr[diff] : 0.0;
+                            double w = __dmul_rn(ws[(dy + r) * side + (dx + r)], wrv);
+                            sw = __dadd_rn(sw, w);
+                            swd = __dadd_rn(swd, __dmul_rn(w, (double)dn));
+                        }
                     }
-                }
+                };
+                // two copies of the loop, so each reads its tables with the right instructions
+                if (n_ws > 0)
+                    taps(s_ws, s_wr);
+                else
+                    taps(lut + fp.ws_off[s], lut + fp.wr_off[s]);
                 f = swd / sw;
             }
             const float z = (float)f * fp.scale;
@@ -345,9 +387,17 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
     float *bpart = (float *)ctx_scratch(ctx, 20, sizeof(float) * 6 * nblocks, false);
     if (!st_xyzi || !st_rgb || !st_valid || !bcnt || !bpart) return GCA_ERR_OUT_OF_MEMORY;
     dim3 grid(fp.nbx, H);
+    // LDS of the pixel kernel: weight tables (fixed mode; the range table while it is short)
+    // and the depth tile of the filter's radius
+    const int R = bil->mode == GCA_BILATERAL_FIXED ? bil->radius : (bil->mode == GCA_BILATERAL_REFERENCE_ADAPTIVE ? 5 : 0);
+    const bool stage = bil->mode == GCA_BILATERAL_FIXED && fp.wr_len[0] <= kWrLds;  // both tables or neither
+    const int n_ws = stage ? (2 * R + 1) * (2 * R + 1) : 0;
+    const int n_wr = stage ? fp.wr_len[0] : 0;
+    const size_t lds = sizeof(double) * (size_t)(n_ws + n_wr) + sizeof(uint16_t) * (size_t)(2 * R + 1) * (kRowSeg + 2 * R);
     StageTimer timer(ctx, GCA_STAGE_FRONTEND);
-    hipLaunchKernelGGL(k_rgbd_pixels, grid, dim3(kRowSeg), 0, ctx->stream, depth, bgr,
-                       (const double *)ctx->lut, fp, st_xyzi, st_rgb, st_valid, bcnt, out->hdr, clear_word);
+    hipLaunchKernelGGL(k_rgbd_pixels, grid, dim3(kRowSeg), lds, ctx->stream, depth, bgr,
+                       (const double *)ctx->lut, fp, st_xyzi, st_rgb, st_valid, bcnt, out->hdr, clear_word, n_ws,
+                       n_wr);
     GCA_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_rgbd_compact, grid, dim3(kRowSeg), 0, ctx->stream, st_xyzi, st_rgb, st_valid,
                        (const uint32_t *)bcnt, W, fp.nbx, out->xyzi, out->rgb, bpart, out->hdr);
