@@ -67,12 +67,16 @@ def level(frame):
 
 # ---- front end ---------------------------------------------------------------------------------
 
-@pytest.mark.parametrize("mode", ["fixed", "adaptive", "none"])
+@pytest.mark.parametrize("mode", ["fixed", "fixed_wide", "adaptive", "none"])
 def test_frontend_bitexact(frame, ctx, mode):
+    """fixed: 5x5 with its weight tables in LDS; fixed_wide: 15x15 with a range table too long for
+    LDS (sigma_r = 200 mm: ~5500 entries), read from global memory."""
     import cuda_3d_amd as g
 
     if mode == "fixed":
         bg, bo = frame.bil, frame.bilo
+    elif mode == "fixed_wide":
+        bg, bo = g.bilateral(g.BILATERAL_FIXED, 7, 4.0, 200.0), O.Bilateral(1, 7, 4.0, 200.0)
     elif mode == "adaptive":
         bg, bo = g.bilateral(g.BILATERAL_REFERENCE_ADAPTIVE, 0, 0, 0), O.Bilateral(2, 0, 0, 0)
     else:
