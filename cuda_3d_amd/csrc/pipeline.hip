@@ -17,7 +17,8 @@
 //              run one after another on the coarsest level's stream, each once its level is
 //              built (so the coarse levels register while the fine ones are still being built,
 //              and a launch follows the one handing it the pose with no cross-stream wait);
-//              results are copied to pinned host memory once every level has joined back.
+//              one small kernel right after the last level's launch, on the same stream, stores
+//              the results and the step's error words into pinned host memory.
 // Nothing in a step depends on host-side sizes: the voxel lattice is computed on the device
 // (its scan covers an upper bound from the camera frustum), and each level's grid uses one
 // geometry over the frustum box (neighbour results do not depend on the grid geometry: points
