@@ -93,6 +93,15 @@ struct Comp {
 // ---- fixed point ------------------------------------------------------------------------
 __device__ __forceinline__ long long fx(double v) { return __double2ll_rn(v); }
 
+// fx for |v| < 2^51: adding 1.5 * 2^52 rounds v to an integer (to nearest, ties to even: the
+// same rounding as fx), which then sits in the low mantissa bits; two instructions instead of
+// the general conversion's six.  Only for values bounded by construction.
+__device__ __forceinline__ long long fx_small(double v)
+{
+    const double t = __dadd_rn(v, 6755399441055744.0);  // 1.5 * 2^52
+    return __double_as_longlong(t) - 0x4338000000000000ll;
+}
+
 // ---- grid --------------------------------------------------------------------------------
 __device__ __forceinline__ int32_t grid_cell(double p, float o, float s, int32_t dim)
 {

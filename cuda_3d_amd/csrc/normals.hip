@@ -354,7 +354,10 @@ __device__ __forceinline__ int32_t group_walk(const GridRefs &G, float4 q, int g
     return cnt;
 }
 
-// Neighbourhood moments (2^-40 fixed point) of one neighbour p of query q, u = (p - q) / r:
+// Neighbourhood moments (2^-40 fixed point) of one neighbour p of query q, u = (p - q) / r.
+// Every neighbour is within the radius, so |u_a| <= 1 (to rounding) and the u and u u^T terms
+// are below 2^41 in fixed point: fx_small converts them; the u dI terms use the general fx.
+//
 //   normals:  S = sum u, sum u u^T (xx xy xz yy yz zz)
 //   gradient: S = sum u u^T, sum u dI with dI = I_p - I_q (raw: the projection onto the query's
 //             tangent plane is applied to the sums, gradient_from_moments)
@@ -366,26 +369,26 @@ __device__ __forceinline__ void add_moments(long long S[9], float4 q, float4 p, 
         const double u0 = ((double)p.x - (double)q.x) * inv_r;
         const double u1 = ((double)p.y - (double)q.y) * inv_r;
         const double u2 = ((double)p.z - (double)q.z) * inv_r;
-        S[0] += fx(u0 * kFx40);
-        S[1] += fx(u1 * kFx40);
-        S[2] += fx(u2 * kFx40);
-        S[3] += fx((u0 * u0) * kFx40);
-        S[4] += fx((u0 * u1) * kFx40);
-        S[5] += fx((u0 * u2) * kFx40);
-        S[6] += fx((u1 * u1) * kFx40);
-        S[7] += fx((u1 * u2) * kFx40);
-        S[8] += fx((u2 * u2) * kFx40);
+        S[0] += fx_small(u0 * kFx40);
+        S[1] += fx_small(u1 * kFx40);
+        S[2] += fx_small(u2 * kFx40);
+        S[3] += fx_small((u0 * u0) * kFx40);
+        S[4] += fx_small((u0 * u1) * kFx40);
+        S[5] += fx_small((u0 * u2) * kFx40);
+        S[6] += fx_small((u1 * u1) * kFx40);
+        S[7] += fx_small((u1 * u2) * kFx40);
+        S[8] += fx_small((u2 * u2) * kFx40);
     } else {
         const double u0 = ((double)p.x - (double)q.x) * inv_r;
         const double u1 = ((double)p.y - (double)q.y) * inv_r;
         const double u2 = ((double)p.z - (double)q.z) * inv_r;
         const double dI = (double)p.w - (double)q.w;
-        S[0] += fx((u0 * u0) * kFx40);
-        S[1] += fx((u0 * u1) * kFx40);
-        S[2] += fx((u0 * u2) * kFx40);
-        S[3] += fx((u1 * u1) * kFx40);
-        S[4] += fx((u1 * u2) * kFx40);
-        S[5] += fx((u2 * u2) * kFx40);
+        S[0] += fx_small((u0 * u0) * kFx40);
+        S[1] += fx_small((u0 * u1) * kFx40);
+        S[2] += fx_small((u0 * u2) * kFx40);
+        S[3] += fx_small((u1 * u1) * kFx40);
+        S[4] += fx_small((u1 * u2) * kFx40);
+        S[5] += fx_small((u2 * u2) * kFx40);
         S[6] += fx((u0 * dI) * kFx40);
         S[7] += fx((u1 * dI) * kFx40);
         S[8] += fx((u2 * dI) * kFx40);
