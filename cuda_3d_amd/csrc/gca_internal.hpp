@@ -76,6 +76,13 @@ struct Grid {
     bool attrs_valid = false;
 };
 
+// Grid counting done by a voxel call's finalize for its output points (grid_prepare'd buffers).
+struct GridCount {
+    GridDesc d;
+    uint32_t *count;
+    uint32_t *cell_of;
+};
+
 }  // namespace gca
 
 struct gca_ctx_s {
@@ -183,8 +190,11 @@ int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out);
 // No host round trip: the lattice is computed on the device; the bitmap scan covers max_bits
 // cells (a lattice beyond that flags *err and gives an empty output); scratch by capacity.
 int32_t launch_voxel_planned(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, double max_bits,
-                             uint32_t *err);
-int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d);
+                             uint32_t *err, const GridCount *gc = nullptr);
+// Grid buffers of c for geometry d (allocation only; no launch).
+int32_t grid_prepare(gca_ctx ctx, gca_cloud c, const GridDesc &d);
+// counted: the points' cells and counts were already produced (GridCount of a voxel call).
+int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d, bool counted = false);
 int32_t launch_nn(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t *out);
 int32_t launch_radius(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t max_nn,
                       int32_t *offsets, int32_t *idx, int64_t idx_cap, int64_t *total);

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_grid_scatter(const float4 *__restrict__
     }
 }
 
-int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d)
+int32_t grid_prepare(gca_ctx ctx, gca_cloud c, const GridDesc &d)
 {
     Grid &G = c->grid;
     G.valid = false;
@@ -91,11 +91,20 @@ int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d)
         GCA_HIP(hipMalloc(&G.cell_of, sizeof(uint32_t) * c->cap));
         G.cap_pts = c->cap;
     }
+    return GCA_OK;
+}
+
+int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d, bool counted)
+{
+    Grid &G = c->grid;
+    GCA_TRY(grid_prepare(ctx, c, d));
     const unsigned nb = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (c->cap + 255) / 256));
     StageTimer timer(ctx, GCA_STAGE_GRID);
-    hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, ctx->stream, c->xyzi, c->hdr, d, G.count,
-                       G.cell_of);
-    GCA_HIP(hipGetLastError());
+    if (!counted) {
+        hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, ctx->stream, c->xyzi, c->hdr, d, G.count,
+                           G.cell_of);
+        GCA_HIP(hipGetLastError());
+    }
     GCA_TRY(exclusive_scan_u32(ctx, G.count, G.start, d.ncells, nullptr));
     hipLaunchKernelGGL(k_grid_scatter, dim3(nb), dim3(256), 0, ctx->stream, c->xyzi, c->hdr, G.count,
                        G.start, G.cell_of, G.order, G.sxyzi);

@@ -119,8 +119,12 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         gca_cloud c = p->lvc[slot][l];
         const gca_level_params &L = p->lv[l];
         GCA_HIP(hipStreamWaitEvent(lc->stream, p->ev_fork, 0));
-        GCA_TRY(launch_voxel_planned(lc, p->full[slot], L.voxel_size, c, p->max_bits[l], p->err_dev));
-        GCA_TRY(launch_grid_build(lc, c, p->gdesc[l]));
+        // the voxel call's finalize also does the grid's count pass for its output points
+        GCA_TRY(cloud_reserve(c, std::max<int64_t>(p->full[slot]->cap, 1)));
+        GCA_TRY(grid_prepare(lc, c, p->gdesc[l]));
+        const GridCount gc{p->gdesc[l], c->grid.count, c->grid.cell_of};
+        GCA_TRY(launch_voxel_planned(lc, p->full[slot], L.voxel_size, c, p->max_bits[l], p->err_dev, &gc));
+        GCA_TRY(launch_grid_build(lc, c, p->gdesc[l], true));
         if (L.gradient_max_nn == 0 && L.gradient_radius <= L.normal_radius) {
             // the solve writes the grid-ordered copies registration reads
             GCA_TRY(launch_normals_gradient(lc, c, L.normal_radius, L.normal_max_nn, L.gradient_radius, true));

@@ -228,12 +228,16 @@ __global__ __launch_bounds__(kVoxBlock) void k_voxel_accum(const float4 *__restr
     }
 }
 
+// gc_count != nullptr: each output point is also counted into its cell of the grid gc (and its
+// cell recorded), which is the first kernel of that grid's build.
 __global__ __launch_bounds__(256) void k_voxel_finalize(unsigned long long *__restrict__ acc,
                                                         const CloudHdr *__restrict__ in_hdr,
                                                         CloudHdr *__restrict__ out_hdr,
                                                         uint32_t *__restrict__ bitmap,
                                                         float4 *__restrict__ out_xyzi,
-                                                        float4 *__restrict__ out_rgb)
+                                                        float4 *__restrict__ out_rgb, GridDesc gd,
+                                                        uint32_t *__restrict__ gc_count,
+                                                        uint32_t *__restrict__ gc_cell_of)
 {
     const int m = out_hdr->n;
     if (blockIdx.x == 0 && threadIdx.x < 3) {
@@ -248,6 +252,14 @@ __global__ __launch_bounds__(256) void k_voxel_finalize(unsigned long long *__re
         for (int c = 0; c < 6; ++c) o[c] = (float)(((double)(long long)a[c] / cnt) * (1.0 / kFx30));
         out_xyzi[r] = make_float4(o[0], o[1], o[2], intensity_of(o[3], o[4], o[5]));
         out_rgb[r] = make_float4(o[3], o[4], o[5], 0.0f);
+        if (gc_count) {  // the grid's count pass for this point (same cell rule as k_grid_count)
+            const int32_t cx = grid_cell(o[0], gd.origin[0], gd.cell, gd.dims[0]);
+            const int32_t cy = grid_cell(o[1], gd.origin[1], gd.cell, gd.dims[1]);
+            const int32_t cz = grid_cell(o[2], gd.origin[2], gd.cell, gd.dims[2]);
+            const uint32_t lin = (uint32_t)(((int64_t)cx * gd.dims[1] + cy) * gd.dims[2] + cz);
+            gc_cell_of[r] = lin;
+            atomicAdd(&gc_count[lin], 1u);
+        }
         const int64_t L = (int64_t)a[7];
         bitmap[L >> 5] = 0u;
 #pragma unroll
@@ -258,7 +270,7 @@ __global__ __launch_bounds__(256) void k_voxel_finalize(unsigned long long *__re
 // Kernel sequence of one voxel call: mark (with the device lattice), popcount scan over `words` bitmap
 // words (>= the lattice's), accumulate, finalize.  Scratch is sized for `n_cap` input points.
 static int32_t voxel_enqueue(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, int64_t n_cap, int64_t words,
-                             double max_bits, uint32_t *err)
+                             double max_bits, uint32_t *err, const GridCount *gc = nullptr)
 {
     if (ctx->bitmap_words < words || ctx->bitmap_dirty) {
         GCA_HIP(hipStreamSynchronize(ctx->stream));
@@ -290,7 +302,8 @@ static int32_t voxel_enqueue(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud o
     hipLaunchKernelGGL(k_voxel_accum, dim3(nb_acc), dim3(kVoxBlock), 0, ctx->stream, in->xyzi, in->rgb, in->hdr,
                        (const VoxelParams *)vp, ctx->bitmap, word_pre, lin, acc);
     hipLaunchKernelGGL(k_voxel_finalize, dim3(nb), dim3(256), 0, ctx->stream, acc, in->hdr, out->hdr,
-                       ctx->bitmap, out->xyzi, out->rgb);
+                       ctx->bitmap, out->xyzi, out->rgb, gc ? gc->d : GridDesc{}, gc ? gc->count : nullptr,
+                       gc ? gc->cell_of : nullptr);
     GCA_HIP(hipGetLastError());
     ctx->bitmap_dirty = false;
     cloud_points_changed(out);
@@ -315,11 +328,12 @@ int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out)
     return voxel_enqueue(ctx, in, voxel, out, n, ((int64_t)nbits + 31) / 32, kVoxelMaxBits, nullptr);
 }
 
-int32_t launch_voxel_planned(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, double max_bits, uint32_t *err)
+int32_t launch_voxel_planned(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, double max_bits, uint32_t *err,
+                             const GridCount *gc)
 {
     const int64_t words = ((int64_t)max_bits + 31) / 32;
     GCA_TRY(cloud_reserve(out, std::max<int64_t>(in->cap, 1)));
-    return voxel_enqueue(ctx, in, voxel, out, in->cap, words, max_bits, err);
+    return voxel_enqueue(ctx, in, voxel, out, in->cap, words, max_bits, err, gc);
 }
 
 }  // namespace gca
