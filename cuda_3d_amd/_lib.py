@@ -107,6 +107,7 @@ _SIGS = {
     "gca_cloud_device_arrays": (i32, [vp, P(vp), P(vp), P(vp), P(vp), P(vp)]),
     "gca_cloud_bounds": (i32, [vp, vp, vp]),
     "gca_cloud_copy": (i32, [vp, vp, vp]),
+    "gca_cloud_append": (i32, [vp, vp, vp]),
     "gca_cloud_from_rgbd": (i32, [vp, vp, vp, P(Camera), P(Bilateral), f32, f32, vp]),
     "gca_voxel_downsample": (i32, [vp, vp, f32, vp]),
     "gca_radius_outlier_removal": (i32, [vp, vp, f32, i32, vp]),

@@ -267,6 +267,12 @@ class PointCloud:
         check(lib.gca_cloud_copy(self.ctx.handle, self.handle, out.handle), "gca_cloud_copy")
         return out
 
+    def __iadd__(self, other: "PointCloud") -> "PointCloud":
+        """point_cloud::operator+= (point_cloud.cu:55-73): other's points appended; normals kept
+        only when both clouds have them; bounds recomputed."""
+        check(lib.gca_cloud_append(self.ctx.handle, self.handle, other.handle), "gca_cloud_append")
+        return self
+
     def voxel_grid_down_sample(self, voxel_size: float, out: Optional["PointCloud"] = None):
         """point_cloud::voxel_grid_down_sample (point_cloud.cu:198-229); None on bad size."""
         pc = out or PointCloud(self.ctx, 1)

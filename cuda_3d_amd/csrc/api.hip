@@ -792,6 +792,59 @@ int32_t gca_cloud_copy(gca_ctx ctx, gca_cloud src, gca_cloud dst)
     return GCA_OK;
 }
 
+int32_t gca_cloud_append(gca_ctx ctx, gca_cloud dst, gca_cloud src)
+{
+    // point_cloud::operator+= (point_cloud.cu:55-73)
+    if (!ctx || !dst || !src) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(cloud_sync_header(dst));
+    GCA_TRY(cloud_sync_header(src));
+    GCA_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t nd = dst->host_hdr.n, ns = src->host_hdr.n, total = nd + ns;
+    if (total > (int64_t)INT_MAX / 2) return GCA_ERR_INVALID_VALUE;
+    const bool normals = dst->has_normals && src->has_normals;
+    if (total > dst->cap) {
+        // grow, keeping dst's points (and normals when they survive)
+        const int64_t cap = std::max<int64_t>(total, dst->cap + dst->cap / 2);
+        const size_t b = sizeof(float4) * (size_t)cap;
+        float4 *a[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int k = 0; k < 4; ++k)
+            if (hipMalloc(&a[k], b) != hipSuccess) {
+                for (int j = 0; j < k; ++j) hipFree(a[j]);
+                return GCA_ERR_OUT_OF_MEMORY;
+            }
+        const size_t keep = sizeof(float4) * (size_t)nd;
+        if (keep) {
+            GCA_HIP(hipMemcpyAsync(a[0], dst->xyzi, keep, hipMemcpyDeviceToDevice, ctx->stream));
+            GCA_HIP(hipMemcpyAsync(a[1], dst->rgb, keep, hipMemcpyDeviceToDevice, ctx->stream));
+            if (normals) GCA_HIP(hipMemcpyAsync(a[2], dst->nrm, keep, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        GCA_HIP(hipStreamSynchronize(ctx->stream));
+        hipFree(dst->xyzi);
+        hipFree(dst->rgb);
+        hipFree(dst->nrm);
+        hipFree(dst->grad);
+        dst->xyzi = a[0];
+        dst->rgb = a[1];
+        dst->nrm = a[2];
+        dst->grad = a[3];
+        dst->cap = cap;  // (when src is dst, its arrays are these new ones: the first nd entries)
+    }
+    if (ns > 0) {
+        const size_t b = sizeof(float4) * (size_t)ns;
+        GCA_HIP(hipMemcpyAsync(dst->xyzi + nd, src->xyzi, b, hipMemcpyDeviceToDevice, ctx->stream));
+        GCA_HIP(hipMemcpyAsync(dst->rgb + nd, src->rgb, b, hipMemcpyDeviceToDevice, ctx->stream));
+        if (normals) GCA_HIP(hipMemcpyAsync(dst->nrm + nd, src->nrm, b, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    dst->host_hdr.n = (int32_t)total;
+    GCA_HIP(hipMemcpyAsync(&dst->hdr->n, &dst->host_hdr.n, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    GCA_TRY(launch_bounds(ctx, dst));  // get_min_bound / get_max_bound recompute them (point_cloud.cu:107-132)
+    GCA_HIP(hipStreamSynchronize(ctx->stream));
+    dst->has_normals = normals;
+    dst->has_gradient = false;
+    cloud_points_changed(dst);
+    return GCA_OK;
+}
+
 int32_t gca_cloud_from_rgbd(gca_ctx ctx, const uint16_t *depth_dev, const uint8_t *bgr_dev,
                             const gca_camera *cam, const gca_bilateral *filter, float tmin,
                             float tmax, gca_cloud out)

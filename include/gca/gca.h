@@ -210,6 +210,11 @@ int32_t gca_cloud_device_arrays(gca_cloud cloud, void **xyzi, void **rgb, void *
 int32_t gca_cloud_bounds(gca_cloud cloud, float min_bound[3], float max_bound[3]);
 /* Reference: point_cloud copy ctor / operator= (point_cloud.cu:29-49). */
 int32_t gca_cloud_copy(gca_ctx ctx, gca_cloud src, gca_cloud dst);
+/* dst += src: src's points (xyz, intensity, colour) appended after dst's; normals kept only
+ * when both clouds have them (else dst has none); the colour gradient is dropped; the bounds are
+ * recomputed over all points.  dst == src appends a copy of the cloud to itself.  Synchronises
+ * (sizes are read on the host).  Reference: point_cloud::operator+= (point_cloud.cu:55-73). */
+int32_t gca_cloud_append(gca_ctx ctx, gca_cloud dst, gca_cloud src);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Front end                                                                                 */

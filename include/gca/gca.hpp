@@ -97,6 +97,11 @@ public:
         if (this != &other) ok(gca_cloud_copy(m_ctx->handle(), other.m_cloud, m_cloud));
         return *this;
     }
+    point_cloud &operator+=(const point_cloud &other)  // point_cloud.cu:55-73
+    {
+        ok(gca_cloud_append(m_ctx->handle(), m_cloud, other.m_cloud));
+        return *this;
+    }
 
     gca_cloud handle() const { return m_cloud; }
     const std::shared_ptr<context> &ctx() const { return m_ctx; }

@@ -93,6 +93,49 @@ def test_frontend_bitexact(frame, ctx, mode):
     np.testing.assert_array_equal(pc.get_max_bound(), mx)
 
 
+def test_cloud_append(ctx):
+    """point_cloud::operator+= (point_cloud.cu:55-73): the concatenation bit for bit, growth past
+    the capacity, normals only when both clouds have them, bounds over every point (the max with
+    the reference's FLT_MIN start), and a cloud appended to itself."""
+    import cuda_3d_amd as g
+
+    rng = np.random.default_rng(11)
+
+    def cloud(n, lo, hi, normals):
+        pc = g.PointCloud.from_numpy(rng.uniform(lo, hi, (n, 3)), rng.uniform(0, 1, (n, 3)), ctx=ctx)
+        if normals:
+            v = rng.normal(size=(n, 3))
+            pc.set_normals(v / np.linalg.norm(v, axis=1, keepdims=True))
+        return pc
+
+    a, b = cloud(3000, -1.0, 2.0, True), cloud(5000, -3.0, 1.0, True)
+    (ax, ar, ai), (bx, br, bi) = a.download(), b.download()
+    an, bn = a.download_normals(), b.download_normals()
+    a += b
+    x, r, i = a.download()
+    assert len(a) == 8000 and a.has_normals()
+    np.testing.assert_array_equal(x, np.concatenate([ax, bx]))
+    np.testing.assert_array_equal(r, np.concatenate([ar, br]))
+    np.testing.assert_array_equal(i, np.concatenate([ai, bi]))
+    np.testing.assert_array_equal(a.download_normals(), np.concatenate([an, bn]))
+    mn, mx = O.bounds(np.concatenate([x, i[:, None]], 1))
+    np.testing.assert_array_equal(a.get_min_bound(), mn)
+    np.testing.assert_array_equal(a.get_max_bound(), mx)
+
+    c = cloud(700, 0.5, 0.6, False)  # no normals: the sum has none (point_cloud.cu:64-70)
+    cx, _, _ = c.download()
+    a += c
+    assert len(a) == 8700 and not a.has_normals()
+    np.testing.assert_array_equal(a.download()[0], np.concatenate([ax, bx, cx]))
+
+    d = cloud(1000, 2.0, 3.0, True)  # all coordinates positive: max bound from the points
+    dx, _, _ = d.download()
+    d += d
+    np.testing.assert_array_equal(d.download()[0], np.concatenate([dx, dx]))
+    assert d.has_normals()
+    np.testing.assert_array_equal(d.get_max_bound(), dx.max(0))
+
+
 def test_frontend_edge_frames(ctx):
     import cuda_3d_amd as g
 
