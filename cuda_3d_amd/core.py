@@ -174,6 +174,19 @@ class PointCloud:
         return pc
 
     @staticmethod
+    def create_from_ply(path: str, ctx: Optional[Context] = None, with_normals: bool = False):
+        """loadPointCloud + removeNaNPoints + create_from_pcl (test_color_icp.cpp:298-343,
+        point_cloud.cu:481-504).  The reference goes through pcl::PointXYZRGB and so drops the
+        file's normals; ``with_normals`` uploads them instead."""
+        from .ply import read_ply
+
+        v = read_ply(path)
+        pc = PointCloud.from_numpy(v.xyz, v.rgb(), ctx)
+        if with_normals and v.normals is not None and len(v.xyz):
+            pc.set_normals(v.normals)
+        return pc
+
+    @staticmethod
     def create_from_rgbd(depth, bgr, camera: L.Camera, filter: Optional[L.Bilateral] = None,
                          threshold_min_in_meter: float = 0.0, threshold_max_in_meter: float = 10.0,
                          ctx: Optional[Context] = None, out: Optional["PointCloud"] = None):

@@ -300,6 +300,40 @@ def _T(T):
     return np.ascontiguousarray(np.asarray(T, np.float64).reshape(16))
 
 
+def transform(xyzi, T, vectors=None):
+    """point_cloud::transform (point_cloud.cu:93-105): transform_point_functor / transform_normal_functor
+    (geometry_util.cuh:133-165) apply R p + t to the points and R v to normals.  The reference
+    multiplies in FP32; this restatement (like the product) forms ((r0 x + r1 y) + r2 z) + t in FP64
+    and rounds once to FP32, so it sits within an FP32 ulp or two of the reference's arithmetic.
+    Returns (xyzi', vectors')."""
+    T = np.asarray(T, np.float64).reshape(4, 4)
+    x = _f4(xyzi).astype(np.float64)
+    out = _f4(xyzi).copy()
+    for r in range(3):
+        out[:, r] = (((T[r, 0] * x[:, 0] + T[r, 1] * x[:, 1]) + T[r, 2] * x[:, 2]) + T[r, 3]).astype(np.float32)
+    vout = None
+    if vectors is not None:
+        v = _f4(vectors).astype(np.float64)
+        vout = _f4(vectors).copy()
+        for r in range(3):
+            vout[:, r] = ((T[r, 0] * v[:, 0] + T[r, 1] * v[:, 1]) + T[r, 2] * v[:, 2]).astype(np.float32)
+    return out, vout
+
+
+def accumulate(model_xyzi, model_rgb, frame_xyzi, frame_rgb, T, voxel=None):
+    """One reconstruction step of apps/app.cpp:336-348: the model is moved by T
+    (point_cloud::transform), the new frame is appended (point_cloud::operator+=,
+    point_cloud.cu:55-73: plain concatenation, model first) and, when ``voxel`` is given, the sum
+    is voxel-downsampled (app.cpp:345-349).  Returns (xyzi, rgb)."""
+    moved, _ = transform(model_xyzi, T)
+    xyzi = np.concatenate([moved, _f4(frame_xyzi)])
+    rgb = np.concatenate([_f4(model_rgb), _f4(frame_rgb)])
+    if voxel is None:
+        return xyzi, rgb
+    ox, orgb, _ = voxel_downsample(xyzi, rgb, voxel)
+    return ox, orgb
+
+
 def build_gn(src, tgt, tn, tg, nn, lam, T=None):
     """cuda_build_gauss_newton_color_icp (.cu:431-474) in FP64. Returns 29 doubles."""
     T = np.eye(4) if T is None else T

@@ -136,6 +136,89 @@ def test_cloud_append(ctx):
     np.testing.assert_array_equal(d.get_max_bound(), dx.max(0))
 
 
+def test_accumulation_step(frame):
+    """One reconstruction step of apps/app.cpp:336-348 through the library: the model (the target
+    frame voxelised at 1 cm) is moved by the frame pose, the source frame is appended and the sum is
+    voxelised at 8 mm.  Bit-exact against the oracle at every stage."""
+    T = frame.G
+    model = frame.tgt.voxel_grid_down_sample(0.01)
+    mx, mrgb, _ = O.voxel_downsample(frame.otx, frame.otr, 0.01)
+    model.transform(T)
+    tx, _ = O.transform(mx, T)
+    x, _, i = model.download()
+    np.testing.assert_array_equal(x, tx[:, :3])
+    np.testing.assert_array_equal(i, tx[:, 3])
+    model += frame.src
+    ax, argb = O.accumulate(mx, mrgb, frame.osx, frame.osr, T)
+    x, rgb, i = model.download()
+    np.testing.assert_array_equal(x, ax[:, :3])
+    np.testing.assert_array_equal(rgb, argb[:, :3])
+    mn, mxb = O.bounds(ax)
+    np.testing.assert_array_equal(model.get_min_bound(), mn)
+    np.testing.assert_array_equal(model.get_max_bound(), mxb)
+    out = model.voxel_grid_down_sample(0.008)
+    vx, vrgb = O.accumulate(mx, mrgb, frame.osx, frame.osr, T, 0.008)
+    x, rgb, i = out.download()
+    assert len(x) == len(vx) > 1000
+    np.testing.assert_array_equal(x, vx[:, :3])
+    np.testing.assert_array_equal(i, vx[:, 3])
+    np.testing.assert_array_equal(rgb, vrgb[:, :3])
+
+
+def test_ply_ingest_and_self_registration(ctx):
+    """data/frag_115.ply (every fifth vertex) through create_from_ply, then the reference's own
+    example of apps/test_color_icp.cpp:399-410 on it: voxel 0.02, target normals at 0.05,
+    color_icp(100, 0.05, 0.04), with the source the same fragment moved by a known pose.  Upload and
+    transform bit-exact; the pose within 1e-5 of the oracle's."""
+    import os
+
+    import cuda_3d_amd as g
+    from cuda_3d_amd.ply import read_ply
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "frag_115_every5.ply")
+    v = read_ply(path)
+    tgt = g.PointCloud.create_from_ply(path, ctx=ctx)
+    x, rgb, inten = tgt.download()
+    assert not tgt.has_normals() and len(x) == len(v.xyz)
+    np.testing.assert_array_equal(x, v.xyz)
+    np.testing.assert_array_equal(rgb, v.rgb())
+    with_n = g.PointCloud.create_from_ply(path, ctx=ctx, with_normals=True)
+    np.testing.assert_array_equal(with_n.download_normals(), v.normals)
+
+    txyzi = np.concatenate([x, inten[:, None]], 1)
+    trgb = np.concatenate([rgb, np.zeros((len(x), 1), np.float32)], 1)
+    K = np.eye(4)
+    c, s_ = np.cos(0.03), np.sin(0.03)
+    K[:3, :3] = [[c, -s_, 0], [s_, c, 0], [0, 0, 1]]
+    K[:3, 3] = [0.02, -0.01, 0.015]
+    src = tgt.copy()
+    src.transform(K)
+    sxyzi, _ = O.transform(txyzi, K)
+    np.testing.assert_array_equal(src.download()[0], sxyzi[:, :3])
+
+    ds, dt = src.voxel_grid_down_sample(0.02), tgt.voxel_grid_down_sample(0.02)
+    dt.estimate_normals(0.05)
+    icp = g.ColorICP(100, 0.05, 0.04, conv_rotation=0.0, conv_translation_sq=0.0)
+    icp.set_source_point_cloud(ds)
+    icp.set_target_point_cloud(dt)
+    assert icp.align()
+    T = icp.get_final_transformation_matrix()
+
+    bmin, bmax = O.bounds(txyzi)
+    osx, _, _ = O.voxel_downsample(sxyzi, trgb, 0.02)
+    otx, _, _ = O.voxel_downsample(txyzi, trgb, 0.02)
+    np.testing.assert_array_equal(dt.download()[0], otx[:, :3])
+    on = O.estimate_normals(otx, 0.05, 0, bmin, bmax)
+    og = O.color_gradient(otx, on, 0.04, 0, bmin, bmax)
+    To, rep = O.icp_align(osx, otx, on, og, O.icp_params(100, 0.05, LAM, 0.0, 0.0), None, bmin, bmax)
+    er, et = pose_err(T, To)
+    kr, kt = pose_err(T, np.linalg.inv(K))
+    print(f"[gpu-parity] frag_115 pose err vs oracle {er:.3e} rad {et:.3e} m; vs truth {kr:.3e} rad {kt:.3e} m",
+          flush=True)
+    assert er <= 1e-5 and et <= 1e-5
+    assert kr < 1e-2 and kt < 1e-2
+
+
 def test_frontend_edge_frames(ctx):
     import cuda_3d_amd as g
 

@@ -163,6 +163,53 @@ def test_voxel_edge_cases():
 
 # ---- neighbours -----------------------------------------------------------------------------
 
+def test_transform_closed_forms_and_reference_fp32():
+    """transform: identity and a quarter turn about z are exact, T then T^-1 returns the points,
+    normals stay unit, and the result is within a few FP32 ulps of the reference's own FP32
+    product R p + t (geometry_util.cuh:133-149)."""
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-3, 3, (2000, 3)), rng.uniform(0, 1, (2000, 1))], 1).astype(np.float32)
+    same, _ = O.transform(x, np.eye(4))
+    np.testing.assert_array_equal(same, x)
+    Tz = np.eye(4)
+    Tz[:3, :3] = Rotation.from_euler("z", 90, degrees=True).as_matrix()
+    q, _ = O.transform(x, Tz)
+    np.testing.assert_array_equal(q[:, :3], np.stack([-x[:, 1], x[:, 0], x[:, 2]], 1))
+    np.testing.assert_array_equal(q[:, 3], x[:, 3])
+    T = np.eye(4)
+    T[:3, :3] = Rotation.from_rotvec([0.3, -0.2, 0.5]).as_matrix()
+    T[:3, 3] = [0.4, -1.2, 2.0]
+    v = rng.normal(size=(2000, 3))
+    v = np.concatenate([v / np.linalg.norm(v, axis=1, keepdims=True), np.zeros((2000, 1))], 1)
+    y, vy = O.transform(x, T, v)
+    back, vb = O.transform(y, np.linalg.inv(T), vy)
+    assert np.abs(back[:, :3] - x[:, :3]).max() < 4e-6
+    assert np.abs(np.linalg.norm(vy[:, :3], axis=1) - 1).max() < 3e-7
+    assert np.abs(vb[:, :3] - v[:, :3]).max() < 3e-7
+    R32, t32, p32 = T[:3, :3].astype(np.float32), T[:3, 3].astype(np.float32), x[:, :3]
+    ref = np.stack([((R32[r, 0] * p32[:, 0] + R32[r, 1] * p32[:, 1]) + R32[r, 2] * p32[:, 2]) + t32[r]
+                    for r in range(3)], 1)
+    scale = np.abs(p32).sum(1, keepdims=True) + np.abs(t32)
+    assert (np.abs(y[:, :3] - ref) <= 4 * np.finfo(np.float32).eps * scale).all()
+
+
+def test_accumulate_is_transform_then_concatenation():
+    rng = np.random.default_rng(6)
+    m = rng.uniform(-1, 1, (500, 4)).astype(np.float32)
+    f = rng.uniform(-1, 1, (300, 4)).astype(np.float32)
+    mr, fr = rng.uniform(0, 1, (500, 4)).astype(np.float32), rng.uniform(0, 1, (300, 4)).astype(np.float32)
+    T = np.eye(4)
+    T[:3, 3] = [0.5, 0.0, -0.25]  # dyadic shifts of values in [-1, 1]: exact
+    x, r = O.accumulate(m, mr, f, fr, T)
+    np.testing.assert_array_equal(x[:500, :3], m[:, :3] + np.float32([0.5, 0.0, -0.25]))
+    np.testing.assert_array_equal(x[500:], f)
+    np.testing.assert_array_equal(r, np.concatenate([mr, fr]))
+    vx, vr = O.accumulate(m, mr, f, fr, T, 0.1)
+    ox, orgb, _ = O.voxel_downsample(x, r, 0.1)
+    np.testing.assert_array_equal(vx, ox)
+    np.testing.assert_array_equal(vr, orgb)
+
+
 def test_nn_and_radius_match_brute_force():
     ref, qry, r = f4(GOLD["nb_ref"]), f4(GOLD["nb_qry"]), float(GOLD["nb_radius"])
     np.testing.assert_array_equal(O.nn_search(qry, ref, r), GOLD["nb_nn"])
