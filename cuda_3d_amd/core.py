@@ -305,6 +305,37 @@ class PointCloud:
         check(st, "gca_radius_outlier_removal")
         return pc
 
+    def euclidean_cluster_labels(self, cluster_tolerance: float, min_cluster_size: int,
+                                 max_cluster_size: int):
+        """Per-point cluster numbers (device int32 tensor, -1 = dropped) from
+        gca_euclidean_clustering, with the cluster count and the number of components above
+        max_cluster_size.  None when the reference returns nothing (no normals, bad sizes)."""
+        t = _torch()
+        n = self.points_number()
+        labels = t.empty(max(n, 1), dtype=t.int32, device="cuda")
+        nc, no = C.c_int32(), C.c_int32()
+        t.cuda.synchronize()
+        st = lib.gca_euclidean_clustering(self.ctx.handle, self.handle, cluster_tolerance, min_cluster_size,
+                                          max_cluster_size, C.c_void_p(labels.data_ptr()), C.byref(nc),
+                                          C.byref(no))
+        if st in (L.GCA_ERR_INVALID_VALUE, L.GCA_ERR_NOT_READY):
+            return None
+        check(st, "gca_euclidean_clustering")
+        return labels[:n], nc.value, no.value
+
+    def euclidean_clustering(self, cluster_tolerance: float, min_cluster_size: int, max_cluster_size: int):
+        """point_cloud::euclidean_clustering (point_cloud.cu:284-335): a list of clusters, each an
+        ascending int32 array of point indices, in the order of their lowest index; [] where the
+        reference returns an empty result."""
+        r = self.euclidean_cluster_labels(cluster_tolerance, min_cluster_size, max_cluster_size)
+        if r is None or r[1] == 0:
+            return []
+        lab = r[0].cpu().numpy()
+        idx = np.argsort(lab, kind="stable").astype(np.int32)
+        lab_sorted = lab[idx]
+        cuts = np.searchsorted(lab_sorted, np.arange(r[1] + 1))
+        return [idx[cuts[k]:cuts[k + 1]] for k in range(r[1])]
+
     def estimate_normals(self, search_radius: float, max_nn: int = 0) -> bool:
         """point_cloud::estimate_normals (point_cloud.cu:145-181)."""
         st = lib.gca_estimate_normals(self.ctx.handle, self.handle, search_radius, max_nn)

@@ -886,6 +886,33 @@ int32_t gca_radius_outlier_removal(gca_ctx ctx, gca_cloud in, float radius, int3
     return GCA_OK;
 }
 
+int32_t gca_euclidean_clustering(gca_ctx ctx, gca_cloud c, float tolerance, int32_t min_cluster_size,
+                                 int32_t max_cluster_size, int32_t *labels_dev, int32_t *n_clusters,
+                                 int32_t *n_oversized)
+{
+    if (!ctx || !c || !labels_dev || !n_clusters) return GCA_ERR_INVALID_VALUE;
+    // cuda_clustering.cu:38-42 and point_cloud.cu:297-302
+    if (min_cluster_size < 0 || max_cluster_size <= 0 || max_cluster_size < min_cluster_size)
+        return GCA_ERR_INVALID_VALUE;
+    if (!(tolerance > 0.0f)) return GCA_ERR_INVALID_VALUE;
+    if (!c->has_normals) return GCA_ERR_NOT_READY;  // point_cloud.cu:290-295
+    GCA_TRY(cloud_sync_header(c));
+    *n_clusters = 0;
+    if (n_oversized) *n_oversized = 0;
+    if (c->host_hdr.n == 0) return GCA_OK;
+    GCA_TRY(grid_build(ctx, c, tolerance));
+    uint32_t *counts = (uint32_t *)ctx_scratch(ctx, 26, 2 * sizeof(uint32_t), false);
+    if (!counts) return GCA_ERR_OUT_OF_MEMORY;
+    GCA_TRY(launch_euclidean_clustering(ctx, c, tolerance, (uint32_t)min_cluster_size,
+                                        (uint32_t)max_cluster_size, labels_dev, counts));
+    uint32_t h[2] = {0, 0};
+    GCA_HIP(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    GCA_HIP(hipStreamSynchronize(ctx->stream));
+    *n_clusters = (int32_t)h[0];
+    if (n_oversized) *n_oversized = (int32_t)h[1];
+    return GCA_OK;
+}
+
 int32_t gca_nn_search(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t *out_dev)
 {
     if (!ctx || !q || !ref || !out_dev) return GCA_ERR_INVALID_VALUE;

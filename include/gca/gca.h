@@ -232,6 +232,18 @@ int32_t gca_cloud_from_rgbd(gca_ctx ctx, const uint16_t *depth_dev, const uint8_
  * via point_cloud::voxel_grid_down_sample (point_cloud.cu:198-229). */
 int32_t gca_voxel_downsample(gca_ctx ctx, gca_cloud in, float voxel_size, gca_cloud out);
 
+/* Euclidean clustering: the connected components of the graph joining points closer than
+ * tolerance.  labels_dev (device, one int32 per point) receives each point's cluster number, or -1
+ * when its component has fewer than min_cluster_size points; clusters are numbered in the order of
+ * their lowest point index, as the reference's seeds run.  Components above max_cluster_size
+ * points are kept whole and counted in *n_oversized (may be NULL); the reference cuts them where
+ * its queue reaches that size.  GCA_ERR_NOT_READY without normals, like the reference's empty
+ * result.  Reference: cuda_euclidean_clustering (src/geometry/cuda_clustering.cuh, .cu:31-110) via
+ * point_cloud::euclidean_clustering (point_cloud.hpp:71-73, point_cloud.cu:284-335). */
+int32_t gca_euclidean_clustering(gca_ctx ctx, gca_cloud cloud, float tolerance,
+                                 int32_t min_cluster_size, int32_t max_cluster_size,
+                                 int32_t *labels_dev, int32_t *n_clusters, int32_t *n_oversized);
+
 /* Radius outlier removal: keep a point when more than min_neighbors points (itself
  * included) lie within radius.  Reference: cuda_grid_radius_outliers_removal
  * (src/geometry/cuda_nn_search.cuh:37-41, .cu:725-800) via

@@ -267,6 +267,39 @@ def radius_search(q_xyzi, r_xyzi, radius, max_nn=0, rbmin=None, rbmax=None):
     return offs, idx[:total].copy()
 
 
+def euclidean_clustering(xyzi, tolerance, min_cluster_size, max_cluster_size, bmin=None, bmax=None):
+    """cuda_euclidean_clustering (cuda_clustering.cu:31-110): for each unvisited point in index
+    order, a breadth-first search over the radius-neighbour lists (cuda_search_radius_neighbors,
+    here radius_search) that stops expanding once its queue holds max_cluster_size points
+    (:75-96); the queue is kept as a cluster when it has at least min_cluster_size points
+    (:98-103).  Returns the clusters as int32 arrays in queue order.  Pure Python: small inputs."""
+    if min_cluster_size < 0 or max_cluster_size <= 0 or max_cluster_size < min_cluster_size:
+        raise ValueError("invalid cluster sizes")  # :38-42
+    xyzi = _f4(xyzi)
+    n = len(xyzi)
+    if n == 0:
+        return []
+    offs, idx = radius_search(xyzi, xyzi, tolerance, 0, bmin, bmax)
+    visited = np.zeros(n, bool)
+    out = []
+    for i in range(n):
+        if visited[i]:
+            continue
+        queue = [i]
+        visited[i] = True
+        k = 0
+        while k < len(queue) and len(queue) < max_cluster_size:
+            p = queue[k]
+            for j in idx[offs[p]:offs[p + 1]].tolist():
+                if not visited[j]:
+                    visited[j] = True
+                    queue.append(j)
+            k += 1
+        if len(queue) >= min_cluster_size:
+            out.append(np.array(queue, np.int32))
+    return out
+
+
 def estimate_normals(xyzi, radius, max_nn=0, bmin=None, bmax=None):
     """cuda_estimate_normals (cuda_estimate_normals.cu:153-199)."""
     xyzi = _f4(xyzi)

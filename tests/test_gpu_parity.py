@@ -219,6 +219,48 @@ def test_ply_ingest_and_self_registration(ctx):
     assert kr < 1e-2 and kt < 1e-2
 
 
+def _clusters_equal(got, want):
+    assert len(got) == len(want)
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, np.sort(b))
+
+
+def test_euclidean_clustering_synthetic(ctx):
+    """Connected components of the radius graph against the oracle's seeded search, on blobs,
+    a 300-point chain (deep union-find trees), two bridged blobs and isolated points."""
+    import cuda_3d_amd as g
+    from test_oracle import cluster_scene
+
+    tol = 0.02
+    x = cluster_scene(3, tol)
+    pc = g.PointCloud.from_numpy(x[:, :3], np.full((len(x), 3), 0.5), ctx=ctx)
+    assert pc.euclidean_clustering(tol, 10, len(x)) == []  # no normals: empty (point_cloud.cu:290-295)
+    pc.set_normals(np.tile([0.0, 0.0, 1.0], (len(x), 1)))
+    xi = pc.download_xyzi()
+    for mn in (0, 10, 100):
+        print(f"[gpu-parity] clustering synthetic min {mn}", flush=True)
+        _clusters_equal(pc.euclidean_clustering(tol, mn, len(x)), O.euclidean_clustering(xi, tol, mn, len(x)))
+    labels, nc, no = pc.euclidean_cluster_labels(tol, 10, 500)
+    whole = O.euclidean_clustering(xi, tol, 10, len(x))
+    assert nc == len(whole) and no == sum(len(c) > 500 for c in whole) > 0
+    lab = labels.cpu().numpy()
+    for k, c in enumerate(whole):
+        assert (lab[c] == k).all()
+    assert (lab == -1).sum() == len(x) - sum(len(c) for c in whole)
+    assert pc.euclidean_clustering(tol, 5, 4) == [] and pc.euclidean_clustering(0.0, 1, 10) == []
+
+
+def test_euclidean_clustering_frame(level):
+    """The 2 cm target of each frame with its normals (a copy: the shared cloud keeps its grid),
+    tolerance 3 cm."""
+    vc, ox = level["gpu"].copy(), level["ox"]
+    assert vc.has_normals()
+    print(f"[gpu-parity] clustering frame n={len(ox)}", flush=True)
+    got = vc.euclidean_clustering(0.03, 20, 10 ** 6)
+    _clusters_equal(got, O.euclidean_clustering(ox, 0.03, 20, 10 ** 6))
+    assert len(got) >= 1
+
+
 def test_frontend_edge_frames(ctx):
     import cuda_3d_amd as g
 

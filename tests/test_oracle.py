@@ -210,6 +210,56 @@ def test_accumulate_is_transform_then_concatenation():
     np.testing.assert_array_equal(vr, orgb)
 
 
+def cluster_scene(seed, tol):
+    """Blobs of uneven sizes, a long chain (spacing 0.6 tol: one neighbour each way), two blobs
+    bridged by a chain, and isolated points; shuffled so indices are unrelated to geometry."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.normal(c, 0.3 * tol * s, (m, 3)) for c, m, s in
+             (((0, 0, 0), 400, 3), ((1, 0, 0), 60, 1), ((0, 1, 0), 1500, 5), ((1, 1, 1), 12, 1),
+              ((2, 0, 0), 250, 3), ((2, 0.5, 0), 250, 3))]
+    parts.append(np.stack([np.linspace(0, 0.6 * tol * 299, 300), np.full(300, 3.0), np.zeros(300)], 1))
+    parts.append(np.stack([np.full(60, 2.0), np.linspace(0.03, 0.47, 60), np.zeros(60)], 1))
+    parts.append(rng.uniform(4, 6, (40, 3)))
+    x = np.concatenate(parts)
+    x = x[rng.permutation(len(x))]
+    return np.concatenate([x, rng.uniform(0, 1, (len(x), 1))], 1).astype(np.float32)
+
+
+def test_clustering_matches_connected_components():
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    tol = 0.02
+    x = cluster_scene(3, tol)
+    pairs = cKDTree(x[:, :3].astype(np.float64)).query_pairs(tol, output_type="ndarray")
+    d = np.linalg.norm(x[pairs[:, 0], :3].astype(np.float64) - x[pairs[:, 1], :3], axis=1)
+    pairs = pairs[d < tol]
+    A = coo_matrix((np.ones(len(pairs)), (pairs[:, 0], pairs[:, 1])), shape=(len(x), len(x)))
+    nc, lab = connected_components(A, directed=False)
+    sizes = np.bincount(lab)
+    for mn in (0, 10, 100):
+        cl = O.euclidean_clustering(x, tol, mn, len(x))
+        want = [np.flatnonzero(lab == c) for c in np.unique(lab) if sizes[c] >= max(mn, 1)]
+        want.sort(key=lambda a: a[0])
+        assert len(cl) == len(want)
+        for a, b in zip(cl, want):
+            np.testing.assert_array_equal(np.sort(a), b)
+            assert a[0] == b[0]  # the seed is the lowest index
+
+
+def test_clustering_truncates_at_max_size_like_the_reference():
+    """A chain with one neighbour each way, searched from index 0 with max 3: the queue stops at
+    3 points, so the chain of 10 comes out as 0-2, 3-5, 6-8, 9 (cuda_clustering.cu:75-96)."""
+    tol = 0.1
+    x = np.zeros((10, 4), np.float32)
+    x[:, 0] = np.arange(10) * 0.06
+    cl = O.euclidean_clustering(x, tol, 1, 3)
+    assert [c.tolist() for c in cl] == [[0, 1, 2], [3, 4, 5], [6, 7, 8], [9]]
+    assert [c.tolist() for c in O.euclidean_clustering(x, tol, 2, 3)] == [[0, 1, 2], [3, 4, 5], [6, 7, 8]]
+    with pytest.raises(ValueError):
+        O.euclidean_clustering(x, tol, 4, 3)
+
+
 def test_nn_and_radius_match_brute_force():
     ref, qry, r = f4(GOLD["nb_ref"]), f4(GOLD["nb_qry"]), float(GOLD["nb_radius"])
     np.testing.assert_array_equal(O.nn_search(qry, ref, r), GOLD["nb_nn"])
