@@ -67,9 +67,9 @@ __device__ __forceinline__ int32_t cc_root(int32_t *parent, int32_t x)
     return x;
 }
 
-__device__ __forceinline__ void cc_union(int32_t *parent, int32_t a, int32_t b)
+// Links the trees of u and v (any nodes of them; the point's own root is passed in).
+__device__ __forceinline__ void cc_union(int32_t *parent, int32_t u, int32_t v)
 {
-    int32_t u = cc_find(parent, a), v = cc_find(parent, b);
     while (u != v) {
         if (u > v) {
             const int32_t t = u;
@@ -109,6 +109,7 @@ __global__ __launch_bounds__(kCcBlock) void k_cc_hook(const CloudHdr *__restrict
     const int32_t i = order[s];
     int32_t lo[3], hi[3];
     if (!grid_range(g, p.x, p.y, p.z, lo, hi)) return;
+    int32_t ri = cc_find(parent, i);  // this point's root, refreshed after each link
     for (int32_t ix = lo[0]; ix <= hi[0]; ++ix)
         for (int32_t iy = lo[1]; iy <= hi[1]; ++iy) {
             const int64_t row = ((int64_t)ix * g.dims[1] + iy) * g.dims[2];
@@ -116,7 +117,16 @@ __global__ __launch_bounds__(kCcBlock) void k_cc_hook(const CloudHdr *__restrict
             for (uint32_t t = start[row + lo[2]]; t < e; ++t) {
                 const float4 q = sxyzi[t];
                 const int32_t j = order[t];
-                if (j < i && dist2(q.x, q.y, q.z, p.x, p.y, p.z) < r2) cc_union(parent, i, j);
+                if (j < i && dist2(q.x, q.y, q.z, p.x, p.y, p.z) < r2) {
+                    const int32_t rj = cc_find(parent, j);
+                    if (rj != ri) {
+                        ri = cc_find(parent, ri);  // short walk: ri was a root not long ago
+                        if (rj != ri) {
+                            cc_union(parent, ri, rj);  // a compare-and-swap only for distinct roots
+                            ri = cc_find(parent, ri);
+                        }
+                    }
+                }
             }
         }
 }
