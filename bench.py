@@ -142,6 +142,25 @@ def copy_calibration(nbytes: int, reps: int = 5):
                       "GB_per_s": 2 * nbytes / dt / 1e9}), flush=True)
 
 
+def run_clustering(g, ctx, depth, bgr, cam, bil, tolerance=0.01, calls=20):
+    """Euclidean clustering (point_cloud::euclidean_clustering) of the full-resolution target
+    frame cloud: median host wall time of gca_euclidean_clustering over ``calls`` calls, the
+    cloud's grid at the tolerance already built (the reference's call also rebuilds its radius
+    lists each time; here the grid persists on the cloud)."""
+    import numpy as np
+
+    pc = g.PointCloud.create_from_rgbd(depth, bgr, cam, bil, 0.2, 3.5, ctx=ctx)
+    pc.set_normals(np.tile([0.0, 0.0, 1.0], (pc.points_number(), 1)))  # the call requires normals
+    r = pc.euclidean_cluster_labels(tolerance, 20, 10 ** 7)
+    ts = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        r = pc.euclidean_cluster_labels(tolerance, 20, 10 ** 7)
+        ts.append(time.perf_counter() - t0)
+    return {"points": pc.points_number(), "tolerance": tolerance, "min_cluster_size": 20,
+            "clusters": r[1], "ms_per_call": float(np.median(ts) * 1e3), "calls": calls}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +169,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-pairs", type=int, default=64)
+    ap.add_argument("--no-clustering", action="store_true")
     ap.add_argument("--no-roofline-pass", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly (no hipGraph replay)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -257,6 +277,12 @@ def main():
         log(f"batch: {batch['batched_pairs_per_s']:.1f} pairs/s batched vs {batch['plain_pairs_per_s']:.1f} plain "
             f"({batch['speedup']:.2f}x); batched launch {batch['batched_roofline']['achieved']:.0f} GB/s algorithmic")
 
+    clustering = None
+    if not args.no_clustering:
+        clustering = run_clustering(g, ctx, td, tc, cam, bil)
+        log(f"clustering: {clustering['points']} points, tolerance {clustering['tolerance']} m: "
+            f"{clustering['clusters']} clusters, {clustering['ms_per_call']:.3f} ms per call (host wall)")
+
     out = {
         "metric": "ms/frame 640x480 depth->ICP; batched ICP pairs/s vs plain at equal pose error",
         "value": ms, "unit": "ms/frame", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -269,7 +295,7 @@ def main():
                    "iterations_per_level": iters_seen[-2:] if iters_seen else None,
                    "hip_graph": not args.no_graph, "latency_ms_per_frame_with_readback": lat_ms,
                    "points_full": full_n, "points_per_level": sizes},
-        "roofline": roof, "cpu_baseline": cpu, "batch": batch, "stages": stages,
+        "roofline": roof, "cpu_baseline": cpu, "batch": batch, "clustering": clustering, "stages": stages,
     }
     print(json.dumps(out), flush=True)
 
