@@ -36,9 +36,12 @@ namespace {
 
 constexpr int kCcBlock = 256;
 
+// Workgroup scope lets the load hit L1.  A stale parent is still an ancestor and pointers only
+// decrease, so walks on stale values end, and every link is validated by an agent-scope
+// compare-and-swap; each failed one strictly lowers the larger of the two roots.
 __device__ __forceinline__ int32_t cc_load(int32_t *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Root of x with path halving.
