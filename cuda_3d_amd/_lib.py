@@ -112,6 +112,7 @@ _SIGS = {
     "gca_voxel_downsample": (i32, [vp, vp, f32, vp]),
     "gca_radius_outlier_removal": (i32, [vp, vp, f32, i32, vp]),
     "gca_euclidean_clustering": (i32, [vp, vp, f32, i32, i32, vp, vp, vp]),
+    "gca_euclidean_clustering_host": (i32, [vp, vp, f32, i32, i32, vp, i64, vp, vp]),
     "gca_nn_search": (i32, [vp, vp, vp, f32, vp]),
     "gca_nn_search_host": (i32, [vp, vp, vp, f32, vp, i64, P(i64)]),
     "gca_radius_search": (i32, [vp, vp, vp, f32, i32, vp, vp, i64, P(i64)]),

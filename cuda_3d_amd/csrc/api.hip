@@ -913,6 +913,25 @@ int32_t gca_euclidean_clustering(gca_ctx ctx, gca_cloud c, float tolerance, int3
     return GCA_OK;
 }
 
+int32_t gca_euclidean_clustering_host(gca_ctx ctx, gca_cloud c, float tolerance, int32_t min_cluster_size,
+                                      int32_t max_cluster_size, int32_t *labels_host, int64_t labels_cap,
+                                      int32_t *n_clusters, int32_t *n_oversized)
+{
+    if (!ctx || !c || !n_clusters) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(cloud_sync_header(c));
+    const int64_t n = c->host_hdr.n;
+    if (n > 0 && (!labels_host || labels_cap < n)) return GCA_ERR_INVALID_VALUE;
+    int32_t *lab = (int32_t *)ctx_scratch(ctx, 27, sizeof(int32_t) * (size_t)std::max<int64_t>(n, 1), false);
+    if (!lab) return GCA_ERR_OUT_OF_MEMORY;
+    GCA_TRY(gca_euclidean_clustering(ctx, c, tolerance, min_cluster_size, max_cluster_size, lab, n_clusters,
+                                     n_oversized));
+    if (n > 0) {
+        GCA_HIP(hipMemcpyAsync(labels_host, lab, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+        GCA_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return GCA_OK;
+}
+
 int32_t gca_nn_search(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int32_t *out_dev)
 {
     if (!ctx || !q || !ref || !out_dev) return GCA_ERR_INVALID_VALUE;

@@ -90,7 +90,7 @@ struct gca_ctx_s {
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     // scratch buffers owned by the ctx, grown on demand
-    static constexpr int kScratchSlots = 27;
+    static constexpr int kScratchSlots = 28;
     void *scratch[kScratchSlots] = {nullptr};
     size_t scratch_bytes[kScratchSlots] = {0};
     bool scratch_zeroed[kScratchSlots] = {false};

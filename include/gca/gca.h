@@ -244,6 +244,13 @@ int32_t gca_euclidean_clustering(gca_ctx ctx, gca_cloud cloud, float tolerance,
                                  int32_t min_cluster_size, int32_t max_cluster_size,
                                  int32_t *labels_dev, int32_t *n_clusters, int32_t *n_oversized);
 
+/* gca_euclidean_clustering with the labels copied to host memory (labels_cap >= the cloud's
+ * point count); for callers without device buffers (the C++ facade). */
+int32_t gca_euclidean_clustering_host(gca_ctx ctx, gca_cloud cloud, float tolerance,
+                                      int32_t min_cluster_size, int32_t max_cluster_size,
+                                      int32_t *labels_host, int64_t labels_cap, int32_t *n_clusters,
+                                      int32_t *n_oversized);
+
 /* Radius outlier removal: keep a point when more than min_neighbors points (itself
  * included) lie within radius.  Reference: cuda_grid_radius_outliers_removal
  * (src/geometry/cuda_nn_search.cuh:37-41, .cu:725-800) via

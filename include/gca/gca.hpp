@@ -197,6 +197,23 @@ public:
     // point (host copy; empty on invalid input, point_cloud.cu:512-516).
     static std::vector<int32_t> nn_search(const point_cloud &query_pc, const point_cloud &reference_pc, float radius);
 
+    // point_cloud::euclidean_clustering (point_cloud.cu:284-335): clusters as ascending point
+    // indices, in the order of their lowest index; empty where the reference returns nothing.
+    std::vector<std::vector<int32_t>> euclidean_clustering(float cluster_tolerance, int32_t min_cluster_size,
+                                                           int32_t max_cluster_size) const
+    {
+        std::vector<int32_t> lab(points_number());
+        int32_t nc = 0;
+        if (gca_euclidean_clustering_host(m_ctx->handle(), m_cloud, cluster_tolerance, min_cluster_size,
+                                          max_cluster_size, lab.empty() ? nullptr : lab.data(),
+                                          (int64_t)lab.size(), &nc, nullptr) != GCA_OK)
+            return {};
+        std::vector<std::vector<int32_t>> out((size_t)nc);
+        for (size_t i = 0; i < lab.size(); ++i)
+            if (lab[i] >= 0) out[(size_t)lab[i]].push_back((int32_t)i);
+        return out;
+    }
+
 private:
     std::pair<std::array<float, 3>, std::array<float, 3>> bounds() const
     {

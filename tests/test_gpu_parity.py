@@ -247,6 +247,16 @@ def test_euclidean_clustering_synthetic(ctx):
     for k, c in enumerate(whole):
         assert (lab[c] == k).all()
     assert (lab == -1).sum() == len(x) - sum(len(c) for c in whole)
+    import ctypes as C
+
+    host = np.full(len(x), -7, np.int32)
+    hc, ho = C.c_int32(), C.c_int32()
+    assert g._lib.lib.gca_euclidean_clustering_host(ctx.handle, pc.handle, tol, 10, 500, host.ctypes.data_as(C.c_void_p),
+                                                    len(x), C.byref(hc), C.byref(ho)) == 0
+    np.testing.assert_array_equal(host, lab)
+    assert (hc.value, ho.value) == (nc, no)
+    assert g._lib.lib.gca_euclidean_clustering_host(ctx.handle, pc.handle, tol, 10, 500, host.ctypes.data_as(C.c_void_p),
+                                                    len(x) - 1, C.byref(hc), C.byref(ho)) != 0  # too small
     assert pc.euclidean_clustering(tol, 5, 4) == [] and pc.euclidean_clustering(0.0, 1, 10) == []
 
 
