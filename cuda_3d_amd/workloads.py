@@ -57,7 +57,7 @@ class BatchPairs:
     """n_pairs x len(levels) clouds ready for gca_color_icp_batch_multiscale."""
 
     def __init__(self, ctx, n_pairs: int = 64, levels=LEVELS, iters: int = 30, seed: int = 2026,
-                 scale: float = 2.0, conv=(1e-6, 1e-6), log=None):
+                 scale: float = 2.0, conv=(1e-6, 1e-6), log=None, keep_host: bool = False):
         from . import core as g
         from . import scene
 
@@ -66,8 +66,11 @@ class BatchPairs:
         t0 = time.perf_counter()
         self.src, self.tgt = [], []   # [pair][level]
         self.truth = []
+        self.host = [] if keep_host else None   # [pair] (src_xyz, src_rgb, tgt_xyz, tgt_rgb)
         for k, n in enumerate(self.counts):
             sx, sr, tx, tr, G = scene.sample_surface_pair(seed + 17 * k, int(n), scale=scale)
+            if keep_host:
+                self.host.append((sx, sr, tx, tr))
             s_full = g.PointCloud.from_numpy(sx, sr, ctx=ctx)
             t_full = g.PointCloud.from_numpy(tx, tr, ctx=ctx)
             srow, trow = [], []
@@ -83,6 +86,7 @@ class BatchPairs:
             self.truth.append(G)
         ctx.synchronize()
         self.levels = tuple(levels)
+        self.iters, self.conv = iters, tuple(conv)
         self.params = [g.icp_params(iters, 2.5 * v, LAMBDA, 0, conv[0], conv[1]) for v in levels]
         self.grad_r = [2.0 * v for v in levels]
         self.T0 = np.tile(np.eye(4), (n_pairs, 1, 1))

@@ -110,7 +110,7 @@ def lib():
         L.orc_intensity.argtypes = [C.c_float] * 3
         L.orc_voxel_downsample.restype = C.c_int64
         L.orc_voxel_downsample.argtypes = [f32p, f32p, C.c_int64, f32p, f32p, C.c_float, f32p,
-                                           f32p, i64p]
+                                           f32p, i32p]
         L.orc_nn_search.argtypes = [f32p, C.c_int64, f32p, C.c_int64, f32p, f32p, C.c_float, i32p]
         L.orc_radius_search.restype = C.c_int64
         L.orc_radius_search.argtypes = [f32p, C.c_int64, f32p, C.c_int64, f32p, f32p, C.c_float,
@@ -194,14 +194,15 @@ def intensity(r, g, b):
 
 
 def voxel_downsample(xyzi, rgb, voxel, bmin=None, bmax=None):
-    """cuda_voxel_grid_downsample (cuda_voxel_grid_down_sample.cu:123-161). Returns (xyzi, rgb, keys)."""
+    """cuda_voxel_grid_downsample (cuda_voxel_grid_down_sample.cu:123-161). Returns (xyzi, rgb,
+    keys) with keys int32 [M, 3] (kx, ky, kz) in the reference's lexicographic order."""
     xyzi, rgb = _f4(xyzi), _f4(rgb)
     if bmin is None:
         bmin, bmax = bounds(xyzi)
     n = len(xyzi)
     ox = np.zeros((max(n, 1), 4), np.float32)
     orgb = np.zeros((max(n, 1), 4), np.float32)
-    keys = np.zeros(max(n, 1), np.int64)
+    keys = np.zeros((max(n, 1), 3), np.int32)
     m = lib().orc_voxel_downsample(xyzi, rgb, n, _b3(bmin), _b3(bmax), voxel, ox, orgb, keys)
     if m < 0:
         raise ValueError("invalid voxel size")

@@ -153,35 +153,39 @@ void orc_bounds(const float *xyzi, int64_t n, float mn[3], float mx[3])
 /* Voxel grid downsample                                                                     */
 
 typedef struct {
-    int64_t key;
+    int32_t k[3];
     int64_t idx;
 } key_idx;
 
+/* Lexicographic (kx, ky, kz), then source index: the reference's int3 comparison sort
+ * (cuda_voxel_grid_down_sample.cu:50-62,136) made stable. */
 static int cmp_key_idx(const void *a, const void *b)
 {
     const key_idx *x = (const key_idx *)a, *y = (const key_idx *)b;
-    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    for (int c = 0; c < 3; ++c)
+        if (x->k[c] != y->k[c]) return x->k[c] < y->k[c] ? -1 : 1;
     return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+static int same_key(const key_idx *x, const key_idx *y)
+{
+    return x->k[0] == y->k[0] && x->k[1] == y->k[1] && x->k[2] == y->k[2];
 }
 
 int64_t orc_voxel_downsample(const float *xyzi, const float *rgb, int64_t n, const float bmin[3],
                              const float bmax[3], float voxel, float *out_xyzi, float *out_rgb,
-                             int64_t *keys_out)
+                             int32_t *keys_out)
 {
     if (!(voxel > 0.0f)) return -1; /* point_cloud.cu:200-204 */
-    int64_t dims[3];
     for (int a = 0; a < 3; ++a) {
         float ext = bmax[a] - bmin[a];
         if ((double)voxel * 2147483647.0 < (double)ext) return -1; /* point_cloud.cu:213-218 */
-        dims[a] = (int64_t)floorf(ext / voxel) + 1;
     }
     if (n == 0) return 0;
-    if ((double)dims[0] * (double)dims[1] * (double)dims[2] > 17179869184.0) return -1;
     key_idx *ki = (key_idx *)malloc(sizeof(key_idx) * (size_t)n);
     for (int64_t i = 0; i < n; ++i) {
-        int64_t k[3];
-        for (int a = 0; a < 3; ++a) k[a] = (int64_t)floorf((xyzi[4 * i + a] - bmin[a]) / voxel);
-        ki[i].key = (k[0] * dims[1] + k[1]) * dims[2] + k[2];
+        /* __float2int_rd((p - min) / v) per axis, FP32 (cuda_voxel_grid_down_sample.cu:26-48) */
+        for (int a = 0; a < 3; ++a) ki[i].k[a] = (int32_t)floorf((xyzi[4 * i + a] - bmin[a]) / voxel);
         ki[i].idx = i;
     }
     qsort(ki, (size_t)n, sizeof(key_idx), cmp_key_idx);
@@ -189,7 +193,7 @@ int64_t orc_voxel_downsample(const float *xyzi, const float *rgb, int64_t n, con
     for (int64_t s = 0; s < n;) {
         int64_t e = s;
         int64_t S[6] = {0, 0, 0, 0, 0, 0};
-        while (e < n && ki[e].key == ki[s].key) {
+        while (e < n && same_key(&ki[e], &ki[s])) {
             int64_t i = ki[e].idx;
             for (int a = 0; a < 3; ++a) {
                 S[a] += llrint((double)xyzi[4 * i + a] * FX30);
@@ -208,7 +212,8 @@ int64_t orc_voxel_downsample(const float *xyzi, const float *rgb, int64_t n, con
         out_rgb[4 * m + 1] = o[4];
         out_rgb[4 * m + 2] = o[5];
         out_rgb[4 * m + 3] = 0.0f;
-        if (keys_out) keys_out[m] = ki[s].key;
+        if (keys_out)
+            for (int a = 0; a < 3; ++a) keys_out[3 * m + a] = ki[s].k[a];
         ++m;
         s = e;
     }
@@ -722,7 +727,12 @@ double orc_residuals(const float *src_xyzi, int64_t ns, const float *tgt_xyzi, c
             if (out) out[i] = v;
             total += v;
         } else {
+#if ORC_PLANT == 7 /* squares the already-weighted residuals */
+            double v = (double)lambda * rg * rg + (1.0 - (double)lambda) * rc * rc;
+#else
+            /* compute_RMSE_functor (cuda_color_icp_build_least_square.cu:82-132): unweighted */
             double v = (double)lambda * rgr * rgr + (1.0 - (double)lambda) * rcr * rcr;
+#endif
             if (out) out[i] = v;
             total += v;
         }

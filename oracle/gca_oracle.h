@@ -52,11 +52,13 @@ void orc_bounds(const float *xyzi, int64_t n, float mn[3], float mx[3]);
 float orc_intensity(float r, float g, float b);
 
 /* cuda_voxel_grid_down_sample.cu:26-161 with point_cloud.cu:198-229.  Keys in FP32 as the
- * reference; per-voxel sums in 2^-30 fixed point.  keys_out (int64 linear key, may be NULL).
- * Returns M, or -1 for voxel <= 0 / voxel too small. */
+ * reference, ordered lexicographically on (kx, ky, kz) like its int3 comparison sort (:136);
+ * per-voxel sums in 2^-30 fixed point.  keys_out: int32[3*M] (kx, ky, kz) per output voxel, may
+ * be NULL.  Returns M, or -1 for voxel <= 0 / an axis beyond INT_MAX voxels (point_cloud.cu:213-218,
+ * the reference's only size guard). */
 int64_t orc_voxel_downsample(const float *xyzi, const float *rgb, int64_t n, const float bmin[3],
                              const float bmax[3], float voxel, float *out_xyzi, float *out_rgb,
-                             int64_t *keys_out);
+                             int32_t *keys_out);
 
 /* Grid over a cloud with bounds (bmin,bmax) for search radius r (cuda_nn_search.cu:557-608;
  * padding 1.5 r from point_cloud.cu:155-160,528-537). */

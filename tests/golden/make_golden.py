@@ -11,6 +11,9 @@ so these vectors are the external pins of the FP64 oracle:
   normals     np.linalg.eigh of the centred covariance of brute-force neighbour sets
   gradient    np.linalg.solve of the regularised 3x3 normal equations
   jacobian    central finite differences of the colored-ICP residuals over the Euler pose
+  residuals   the per-point residual kinds of cuda_color_icp_build_least_square.cu:13-132:
+              weighted pair (sqrt(l) r_g, sqrt(1-l) r_c), |.|+|.| of the pair, and the
+              unweighted objective l r_g^2 + (1-l) r_c^2; zero where there is no correspondence
   solve6      np.linalg.solve + scipy Rotation.from_euler('xyz') for Rz Ry Rx
 Run: python tests/golden/make_golden.py
 """
@@ -214,8 +217,14 @@ def gn_case(rng):
         J[:, k] = (residuals(e, T, src, tgt, tn, tg, nn, lam) - residuals(-e, T, src, tgt, tn, tg, nn, lam)) / (2 * h)
     JTJ = J.T @ J
     JTr = J.T @ r0
+    # per-point residual kinds, scattered back to every source point (0 without a match)
+    pair = np.zeros((ns, 2))
+    pair[nn >= 0] = r0.reshape(-1, 2)
+    raw = pair / np.array([np.sqrt(lam), np.sqrt(1 - lam)])
     return dict(gn_src=src, gn_tgt=tgt, gn_tn=tn, gn_tg=tg, gn_nn=nn, gn_T=T, gn_lam=lam,
-                gn_JTJ=JTJ, gn_JTr=JTr, gn_r2=float(r0 @ r0), gn_ncorr=int((nn >= 0).sum()))
+                gn_JTJ=JTJ, gn_JTr=JTr, gn_r2=float(r0 @ r0), gn_ncorr=int((nn >= 0).sum()),
+                res_pair=pair, res_abs=np.abs(pair).sum(1),
+                res_rmse=lam * raw[:, 0] ** 2 + (1 - lam) * raw[:, 1] ** 2)
 
 
 def solve_case(rng):

@@ -139,10 +139,7 @@ def test_voxel_matches_numpy_unique():
         tol = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64) + 2.0 ** -30
         assert np.all(np.abs(got.astype(np.float64) - want) <= tol)
     assert np.mean(ox[:, :3] == mean[:, :3].astype(np.float32)) > 0.95
-    mn = xyz.min(0)
-    dims = (np.floor((xyz.max(0) - mn) / np.float32(v)) + 1).astype(np.int64)
-    k = GOLD["vox_keys"]
-    np.testing.assert_array_equal(keys, (k[:, 0] * dims[1] + k[:, 1]) * dims[2] + k[:, 2])
+    np.testing.assert_array_equal(keys, GOLD["vox_keys"])
     c = orgb.astype(np.float64)
     inten = (0.2126 * c[:, 0] + 0.7152 * c[:, 1] + 0.0722 * c[:, 2]).astype(np.float32)
     np.testing.assert_array_equal(ox[:, 3], inten)
@@ -159,6 +156,20 @@ def test_voxel_edge_cases():
         O.voxel_downsample(same, same, 0.0)
     with pytest.raises(ValueError):
         O.voxel_downsample(f4([[0, 0, 0], [1e4, 1e4, 1e4]]), f4(np.zeros((2, 3))), 1e-9)
+
+
+def test_voxel_huge_lattice_is_not_refused():
+    """The reference guards only each axis against INT_MAX voxels (point_cloud.cu:213-218): a
+    lattice of 2^50 cells (2 km at 1 mm per axis, far beyond any dense table) is valid, and its
+    voxels come out in (kx, ky, kz) order."""
+    rng = np.random.default_rng(11)
+    xyz = np.concatenate([rng.uniform(0, 0.01, (40, 3)), [[0, 0, 0], [2000.0, 1500.0, 1800.0]]])
+    v = np.float32(0.001)
+    ox, _, keys = O.voxel_downsample(f4(xyz), f4(np.zeros_like(xyz)), v)
+    mn = xyz.astype(np.float32).min(0)
+    want = np.unique(np.floor((xyz.astype(np.float32) - mn) / v).astype(np.int64), axis=0)
+    np.testing.assert_array_equal(keys, want)
+    assert len(ox) == len(want) and keys[-1, 0] > 10 ** 6
 
 
 # ---- neighbours -----------------------------------------------------------------------------
@@ -284,14 +295,47 @@ def test_radius_sets_agree_with_kdtree():
     assert mism == 0
 
 
-def test_grid_cells_formula_and_cell_growth():
+def _cell_box_violations(p, cells, origin, cell, dims):
+    """Exact-arithmetic check (fractions) that every point lies in the half-open box of its cell,
+    origin + c * cell <= p < origin + (c + 1) * cell, except that a point below the first cell
+    sits in cell 0 and one beyond the last in cell dims - 1."""
+    from fractions import Fraction as F
+
+    bad = 0
+    for i in range(len(p)):
+        for a in range(3):
+            x, o, s, c = F(float(p[i, a])), F(float(origin[a])), F(float(cell)), int(cells[i, a])
+            lo_ok = c == 0 or o + c * s <= x
+            hi_ok = c == dims[a] - 1 or x < o + (c + 1) * s
+            below = x < o + c * s
+            above = x >= o + (c + 1) * s
+            bad += not (lo_ok and hi_ok) or (below and c != 0) or (above and c != dims[a] - 1)
+            bad += not (0 <= c < dims[a])
+    return bad
+
+
+def test_grid_cells_box_property_and_reference_extent():
+    """Cell binning pinned by properties, not by its own formula: each point lies inside its
+    cell's box (exact arithmetic), points outside the grid clamp to the border cells, the origin is
+    the bound padded by 1.5 r (point_cloud.cu:528-537) and the cell counts follow the reference's
+    FP32 extent formula int((max - min) / cell) + 1 (cuda_nn_search.cu:557-562)."""
     rng = np.random.default_rng(4)
     p = f4(rng.uniform(-1, 1, size=(500, 3)))
-    cells, origin, cell, dims = O.grid_cells(p, 0.1)
+    r = np.float32(0.1)
+    cells, origin, cell, dims = O.grid_cells(p, float(r))
     mn, mx = O.bounds(p)
-    assert np.all(origin == mn - np.float32(1.5) * np.float32(0.1)) and cell == np.float32(0.1)
-    exp = np.clip(np.floor((p[:, :3].astype(np.float64) - origin.astype(np.float64)) / np.float64(cell)), 0, dims - 1)
-    np.testing.assert_array_equal(cells, exp.astype(np.int32))
+    pad = np.float32(1.5) * r
+    assert np.all(origin == mn - pad) and cell == r
+    want_dims = ((mx + pad) - (mn - pad)) / r
+    np.testing.assert_array_equal(dims, want_dims.astype(np.int32) + 1)
+    assert _cell_box_violations(p[:, :3], cells, origin, cell, dims) == 0
+    # a grid over a sub-box: points beyond it clamp to the border cells
+    bmin, bmax = np.float32([-0.3, -0.2, -0.1]), np.float32([0.2, 0.3, 0.4])
+    cells2, origin2, cell2, dims2 = O.grid_cells(p, float(r), bmin, bmax)
+    np.testing.assert_array_equal(dims2, (((bmax + pad) - (bmin - pad)) / r).astype(np.int32) + 1)
+    assert _cell_box_violations(p[:, :3], cells2, origin2, cell2, dims2) == 0
+    outside = (p[:, :3] < origin2) | (p[:, :3] >= origin2 + dims2 * cell2)
+    assert outside.any() and np.all((cells2 == 0) | (cells2 == dims2 - 1) | ~outside)
     # tiny radius: the cell edge doubles until the dense grid fits the cap
     _, _, cell2, dims2 = O.grid_cells(p, 1e-4)
     assert cell2 > np.float32(1e-4) and np.prod(dims2.astype(np.float64)) <= 2 ** 25
@@ -384,6 +428,23 @@ def test_gauss_newton_matches_finite_difference_jacobian():
     np.testing.assert_allclose(out[21:27], G["gn_JTr"], rtol=1e-6, atol=1e-7 * np.abs(G["gn_JTr"]).max())
     assert abs(out[27] - G["gn_r2"]) < 1e-9 * max(1, G["gn_r2"])
     assert out[28] == G["gn_ncorr"]
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_residual_kinds_match_golden(kind):
+    """Per-point residuals (cuda_color_icp_build_least_square.cu:13-132) against the numpy vectors
+    of tests/golden/make_golden.py, and the identities that tie the kinds together: the objective
+    kind sums to the weighted squares of the pair kind, i.e. to the Gauss-Newton r^2 of the same
+    correspondences, and the absolute kind is |pair0| + |pair1|."""
+    G = GOLD
+    args = (G["gn_src"], G["gn_tgt"], f4(G["gn_tn"]), f4(G["gn_tg"]), G["gn_nn"], float(G["gn_lam"]))
+    per, total = O.residuals(*args, kind, G["gn_T"])
+    want = {0: G["res_pair"], 1: G["res_abs"], 2: G["res_rmse"]}[kind]
+    np.testing.assert_allclose(per, want, rtol=1e-9, atol=1e-12)
+    assert abs(total - want.sum()) <= 1e-9 * max(1.0, np.abs(want).sum())
+    if kind == 2:
+        assert abs(total - G["gn_r2"]) <= 1e-9 * G["gn_r2"]
+        assert np.all(per[G["gn_nn"] < 0] == 0)
 
 
 def test_solve6_and_euler():
@@ -482,6 +543,7 @@ PLANTS = {
     4: ("largest instead of smallest eigenvector", lambda: test_normals_match_numpy_eigh(15)),
     5: ("nearest-neighbour tie toward the higher index", lambda: test_nn_and_radius_match_brute_force()),
     6: ("Gaussian weight with the 1/2 factor", lambda: test_bilateral_matches_direct_fp64_loop()),
+    7: ("objective residual squares the weighted residuals", lambda: test_residual_kinds_match_golden(2)),
 }
 
 
