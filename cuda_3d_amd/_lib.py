@@ -113,6 +113,9 @@ _SIGS = {
     "gca_radius_outlier_removal": (i32, [vp, vp, f32, i32, vp]),
     "gca_euclidean_clustering": (i32, [vp, vp, f32, i32, i32, vp, vp, vp]),
     "gca_euclidean_clustering_host": (i32, [vp, vp, f32, i32, i32, vp, i64, vp, vp]),
+    "gca_residual_segmentation": (i32, [vp, vp, f32, vp, i32, i32, vp, vp, vp]),
+    "gca_residual_segmentation_host": (i32, [vp, vp, f32, vp, i32, i32, vp, i64, vp, vp]),
+    "gca_abs_residual_host": (i32, [vp, vp, vp, f32, f32, f32, vp, i64, P(i64), P(C.c_double)]),
     "gca_nn_search": (i32, [vp, vp, vp, f32, vp]),
     "gca_nn_search_host": (i32, [vp, vp, vp, f32, vp, i64, P(i64)]),
     "gca_radius_search": (i32, [vp, vp, vp, f32, i32, vp, vp, i64, P(i64)]),

@@ -308,26 +308,46 @@ class PointCloud:
     def euclidean_cluster_labels(self, cluster_tolerance: float, min_cluster_size: int,
                                  max_cluster_size: int):
         """Per-point cluster numbers (device int32 tensor, -1 = dropped) from
-        gca_euclidean_clustering, with the cluster count and the number of components above
+        gca_euclidean_clustering, with the cluster count and the number of components cut at
         max_cluster_size.  None when the reference returns nothing (no normals, bad sizes)."""
         t = _torch()
         n = self.points_number()
         labels = t.empty(max(n, 1), dtype=t.int32, device="cuda")
-        nc, no = C.c_int32(), C.c_int32()
+        nc, ncut = C.c_int32(), C.c_int32()
         t.cuda.synchronize()
         st = lib.gca_euclidean_clustering(self.ctx.handle, self.handle, cluster_tolerance, min_cluster_size,
                                           max_cluster_size, C.c_void_p(labels.data_ptr()), C.byref(nc),
-                                          C.byref(no))
+                                          C.byref(ncut))
         if st in (L.GCA_ERR_INVALID_VALUE, L.GCA_ERR_NOT_READY):
             return None
         check(st, "gca_euclidean_clustering")
-        return labels[:n], nc.value, no.value
+        return labels[:n], nc.value, ncut.value
 
-    def euclidean_clustering(self, cluster_tolerance: float, min_cluster_size: int, max_cluster_size: int):
-        """point_cloud::euclidean_clustering (point_cloud.cu:284-335): a list of clusters, each an
-        ascending int32 array of point indices, in the order of their lowest index; [] where the
-        reference returns an empty result."""
-        r = self.euclidean_cluster_labels(cluster_tolerance, min_cluster_size, max_cluster_size)
+    def residual_segment_labels(self, cluster_tolerance: float, abs_residual, min_cluster_size: int,
+                                max_cluster_size: int):
+        """gca_residual_segmentation: per-point segment numbers (device int32 tensor, -1 =
+        dropped), the segment count and the number of components cut at max_cluster_size.
+        abs_residual: one float per point (numpy or a cuda tensor).  None where the reference
+        returns nothing (bad tolerance or sizes)."""
+        t = _torch()
+        n = self.points_number()
+        res = abs_residual if isinstance(abs_residual, t.Tensor) else device_tensor(np.asarray(abs_residual, np.float32))
+        if res.numel() < n:
+            raise ValueError("abs_residual needs one value per point")
+        res = res.to(device="cuda", dtype=t.float32).contiguous()
+        labels = t.empty(max(n, 1), dtype=t.int32, device="cuda")
+        nc, ncut = C.c_int32(), C.c_int32()
+        t.cuda.synchronize()
+        st = lib.gca_residual_segmentation(self.ctx.handle, self.handle, cluster_tolerance, C.c_void_p(res.data_ptr()),
+                                           min_cluster_size, max_cluster_size, C.c_void_p(labels.data_ptr()),
+                                           C.byref(nc), C.byref(ncut))
+        if st == L.GCA_ERR_INVALID_VALUE:
+            return None
+        check(st, "gca_residual_segmentation")
+        return labels[:n], nc.value, ncut.value
+
+    @staticmethod
+    def _clusters_from_labels(r):
         if r is None or r[1] == 0:
             return []
         lab = r[0].cpu().numpy()
@@ -335,6 +355,21 @@ class PointCloud:
         lab_sorted = lab[idx]
         cuts = np.searchsorted(lab_sorted, np.arange(r[1] + 1))
         return [idx[cuts[k]:cuts[k + 1]] for k in range(r[1])]
+
+    def euclidean_clustering(self, cluster_tolerance: float, min_cluster_size: int, max_cluster_size: int):
+        """point_cloud::euclidean_clustering (point_cloud.cu:284-335): a list of clusters in seed
+        order, each an ascending int32 array of point indices (the reference lists a cluster in
+        its search-queue order); [] where the reference returns an empty result."""
+        return self._clusters_from_labels(
+            self.euclidean_cluster_labels(cluster_tolerance, min_cluster_size, max_cluster_size))
+
+    def icp_residual_segmentation(self, cluster_tolerance: float, abs_residual, min_cluster_size: int,
+                                  max_cluster_size: int):
+        """point_cloud::icp_residual_segmentation (point_cloud.cu:391-435): segments in seed order,
+        each an ascending int32 array of point indices; abs_residual as from
+        ColorICP.get_abs_residual; [] where the reference returns an empty result."""
+        return self._clusters_from_labels(
+            self.residual_segment_labels(cluster_tolerance, abs_residual, min_cluster_size, max_cluster_size))
 
     def estimate_normals(self, search_radius: float, max_nn: int = 0) -> bool:
         """point_cloud::estimate_normals (point_cloud.cu:145-181)."""

@@ -888,7 +888,7 @@ int32_t gca_radius_outlier_removal(gca_ctx ctx, gca_cloud in, float radius, int3
 
 int32_t gca_euclidean_clustering(gca_ctx ctx, gca_cloud c, float tolerance, int32_t min_cluster_size,
                                  int32_t max_cluster_size, int32_t *labels_dev, int32_t *n_clusters,
-                                 int32_t *n_oversized)
+                                 int32_t *n_cut)
 {
     if (!ctx || !c || !labels_dev || !n_clusters) return GCA_ERR_INVALID_VALUE;
     // cuda_clustering.cu:38-42 and point_cloud.cu:297-302
@@ -898,7 +898,7 @@ int32_t gca_euclidean_clustering(gca_ctx ctx, gca_cloud c, float tolerance, int3
     if (!c->has_normals) return GCA_ERR_NOT_READY;  // point_cloud.cu:290-295
     GCA_TRY(cloud_sync_header(c));
     *n_clusters = 0;
-    if (n_oversized) *n_oversized = 0;
+    if (n_cut) *n_cut = 0;
     if (c->host_hdr.n == 0) return GCA_OK;
     GCA_TRY(grid_build(ctx, c, tolerance));
     uint32_t *counts = (uint32_t *)ctx_scratch(ctx, 26, 2 * sizeof(uint32_t), false);
@@ -909,13 +909,60 @@ int32_t gca_euclidean_clustering(gca_ctx ctx, gca_cloud c, float tolerance, int3
     GCA_HIP(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     GCA_HIP(hipStreamSynchronize(ctx->stream));
     *n_clusters = (int32_t)h[0];
-    if (n_oversized) *n_oversized = (int32_t)h[1];
+    if (n_cut) *n_cut = (int32_t)h[1];
+    return GCA_OK;
+}
+
+int32_t gca_residual_segmentation(gca_ctx ctx, gca_cloud c, float tolerance, const float *abs_residual_dev,
+                                  int32_t min_cluster_size, int32_t max_cluster_size, int32_t *labels_dev,
+                                  int32_t *n_clusters, int32_t *n_cut)
+{
+    if (!ctx || !c || !abs_residual_dev || !labels_dev || !n_clusters) return GCA_ERR_INVALID_VALUE;
+    // cuda_clustering.cu:260-264 and point_cloud.cu:397-402; no normals needed (point_cloud.cu:391-435)
+    if (min_cluster_size < 0 || max_cluster_size <= 0 || max_cluster_size < min_cluster_size)
+        return GCA_ERR_INVALID_VALUE;
+    if (!(tolerance > 0.0f)) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(cloud_sync_header(c));
+    *n_clusters = 0;
+    if (n_cut) *n_cut = 0;
+    if (c->host_hdr.n == 0) return GCA_OK;
+    GCA_TRY(grid_build(ctx, c, tolerance));
+    uint32_t *counts = (uint32_t *)ctx_scratch(ctx, 26, 2 * sizeof(uint32_t), false);
+    if (!counts) return GCA_ERR_OUT_OF_MEMORY;
+    GCA_TRY(launch_euclidean_clustering(ctx, c, tolerance, (uint32_t)min_cluster_size, (uint32_t)max_cluster_size,
+                                        labels_dev, counts, abs_residual_dev));
+    uint32_t h[2] = {0, 0};
+    GCA_HIP(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    GCA_HIP(hipStreamSynchronize(ctx->stream));
+    *n_clusters = (int32_t)h[0];
+    if (n_cut) *n_cut = (int32_t)h[1];
+    return GCA_OK;
+}
+
+int32_t gca_residual_segmentation_host(gca_ctx ctx, gca_cloud c, float tolerance, const float *abs_residual_host,
+                                       int32_t min_cluster_size, int32_t max_cluster_size, int32_t *labels_host,
+                                       int64_t labels_cap, int32_t *n_clusters, int32_t *n_cut)
+{
+    if (!ctx || !c || !n_clusters) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(cloud_sync_header(c));
+    const int64_t n = c->host_hdr.n;
+    if (n > 0 && (!labels_host || labels_cap < n || !abs_residual_host)) return GCA_ERR_INVALID_VALUE;
+    int32_t *lab = (int32_t *)ctx_scratch(ctx, 27, sizeof(int32_t) * (size_t)std::max<int64_t>(n, 1), false);
+    float *res = (float *)ctx_scratch(ctx, 33, sizeof(float) * (size_t)std::max<int64_t>(n, 1), false);
+    if (!lab || !res) return GCA_ERR_OUT_OF_MEMORY;
+    if (n > 0) GCA_HIP(hipMemcpyAsync(res, abs_residual_host, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    GCA_TRY(gca_residual_segmentation(ctx, c, tolerance, res, min_cluster_size, max_cluster_size, lab, n_clusters,
+                                      n_cut));
+    if (n > 0) {
+        GCA_HIP(hipMemcpyAsync(labels_host, lab, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+        GCA_HIP(hipStreamSynchronize(ctx->stream));
+    }
     return GCA_OK;
 }
 
 int32_t gca_euclidean_clustering_host(gca_ctx ctx, gca_cloud c, float tolerance, int32_t min_cluster_size,
                                       int32_t max_cluster_size, int32_t *labels_host, int64_t labels_cap,
-                                      int32_t *n_clusters, int32_t *n_oversized)
+                                      int32_t *n_clusters, int32_t *n_cut)
 {
     if (!ctx || !c || !n_clusters) return GCA_ERR_INVALID_VALUE;
     GCA_TRY(cloud_sync_header(c));
@@ -924,7 +971,7 @@ int32_t gca_euclidean_clustering_host(gca_ctx ctx, gca_cloud c, float tolerance,
     int32_t *lab = (int32_t *)ctx_scratch(ctx, 27, sizeof(int32_t) * (size_t)std::max<int64_t>(n, 1), false);
     if (!lab) return GCA_ERR_OUT_OF_MEMORY;
     GCA_TRY(gca_euclidean_clustering(ctx, c, tolerance, min_cluster_size, max_cluster_size, lab, n_clusters,
-                                     n_oversized));
+                                     n_cut));
     if (n > 0) {
         GCA_HIP(hipMemcpyAsync(labels_host, lab, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
         GCA_HIP(hipStreamSynchronize(ctx->stream));

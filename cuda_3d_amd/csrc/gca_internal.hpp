@@ -90,7 +90,7 @@ struct gca_ctx_s {
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     // scratch buffers owned by the ctx, grown on demand
-    static constexpr int kScratchSlots = 28;
+    static constexpr int kScratchSlots = 40;
     void *scratch[kScratchSlots] = {nullptr};
     size_t scratch_bytes[kScratchSlots] = {0};
     bool scratch_zeroed[kScratchSlots] = {false};
@@ -200,9 +200,13 @@ int32_t launch_radius(gca_ctx ctx, gca_cloud q, gca_cloud ref, float radius, int
                       int32_t *offsets, int32_t *idx, int64_t idx_cap, int64_t *total);
 int32_t launch_grid_cells(gca_ctx ctx, gca_cloud c, int32_t *cells);
 int32_t launch_outlier(gca_ctx ctx, gca_cloud in, float radius, int32_t min_nn, gca_cloud out);
-// labels of the cloud's own grid (built at `tolerance`); counts_dev[0] = clusters, [1] = oversized
+// labels of the cloud's own grid (built at `tolerance`); counts_dev[0] = clusters, [1] =
+// components cut at max_size.  res (device, one float per point, may be null): residual gate of
+// the residual-based segmentation.  Synchronises (the cut is planned on the host).  The header's
+// host copy must be current.
 int32_t launch_euclidean_clustering(gca_ctx ctx, gca_cloud c, float tolerance, uint32_t min_size,
-                                    uint32_t max_size, int32_t *label, uint32_t *counts_dev);
+                                    uint32_t max_size, int32_t *label, uint32_t *counts_dev,
+                                    const float *res = nullptr);
 // Copies normals and colour gradients into the grid's cell order (c->grid.snrm / sgrad).
 int32_t grid_gather_attributes(gca_ctx ctx, gca_cloud c);
 int32_t launch_normals(gca_ctx ctx, gca_cloud c, float radius, int32_t max_nn);

@@ -280,6 +280,15 @@ struct Sum1<GCA_ACCUM_FP64> {
     __device__ double value() const { return s; }
 };
 
+// a + b in the precision of the accumulator kind: FP32 for the naive ablation (every sum of the
+// reduction rounds to FP32, as a plain FP32 reduction would), FP64 otherwise.
+template <int MODE>
+__device__ __forceinline__ double acc_add(double a, double b)
+{
+    if constexpr (MODE == GCA_ACCUM_NAIVE_FP32) return (double)((float)a + (float)b);
+    else return a + b;
+}
+
 __device__ __forceinline__ void xform_f32(const double *T, float4 p, float &x, float &y, float &z)
 {
     const double a = p.x, b = p.y, c = p.z;
@@ -337,7 +346,7 @@ __device__ __forceinline__ void block_partial(Acc<MODE> &A, int ncorr, int lead,
     if (threadIdx.x < 28) {
         const int k = threadIdx.x;
         double v = 0.0;
-        for (int i = 0; i < nw; ++i) v += wacc[i].value(k);
+        for (int i = 0; i < nw; ++i) v = acc_add<MODE>(v, wacc[i].value(k));
         out[k] = v;
     } else if (threadIdx.x == 28) {
         int c = 0;
@@ -386,12 +395,16 @@ __device__ __forceinline__ void pass_partial(const float (*rows)[kRowStride], in
 // Ends with a barrier.
 constexpr int kSumBatch = 32;
 
-__device__ __forceinline__ void sum_partials(const double *partials, int n, double *sys)
+// naive: the GCA_ACCUM_NAIVE_FP32 ablation, whose sums across units are FP32 too (the
+// correspondence count, entry 28, stays exact).
+__device__ __forceinline__ void sum_partials(const double *partials, int n, double *sys, bool naive = false)
 {
     __shared__ double part[8][kPartStride];
     const int v = threadIdx.x % kPartStride, grp = threadIdx.x / kPartStride;  // 8 groups of 32
+    const bool f32 = naive && v < kNSys - 1;
     if (v < kNSys) {
         double a = 0.0;
+        float af = 0.f;
         for (int b0 = grp; b0 < n; b0 += 8 * kSumBatch) {
             double x[kSumBatch];
 #pragma unroll
@@ -399,16 +412,25 @@ __device__ __forceinline__ void sum_partials(const double *partials, int n, doub
                 const int b = b0 + 8 * u;
                 x[u] = b < n ? partials[(int64_t)b * kPartStride + v] : 0.0;
             }
+            if (f32) {
 #pragma unroll
-            for (int u = 0; u < kSumBatch; ++u) a += x[u];
+                for (int u = 0; u < kSumBatch; ++u) af += (float)x[u];
+            } else {
+#pragma unroll
+                for (int u = 0; u < kSumBatch; ++u) a += x[u];
+            }
         }
-        part[grp][v] = a;
+        part[grp][v] = f32 ? (double)af : a;
     }
     __syncthreads();
     if (threadIdx.x < kNSys) {
         double a = 0.0;
-        for (int g = 0; g < 8; ++g) a += part[g][threadIdx.x];
-        sys[threadIdx.x] = a;
+        float af = 0.f;
+        for (int g = 0; g < 8; ++g) {
+            if (f32) af += (float)part[g][threadIdx.x];
+            else a += part[g][threadIdx.x];
+        }
+        sys[threadIdx.x] = f32 ? (double)af : a;
     }
     __syncthreads();
 }
@@ -589,7 +611,7 @@ __device__ __forceinline__ void icp_unit(const PairDesc &P, int n, int G, int nv
             rows[28][threadIdx.x] = c;
         }
         pass_partial<MODE>(rows, Q, first ? sys : bsys);
-        if (!first && threadIdx.x < kNSys) sys[threadIdx.x] += bsys[threadIdx.x];
+        if (!first && threadIdx.x < kNSys) sys[threadIdx.x] = acc_add<MODE>(sys[threadIdx.x], bsys[threadIdx.x]);
         first = false;
         __syncthreads();  // slot[] and the block-partial scratch are reused by the next pass
     }
@@ -624,7 +646,8 @@ __device__ __forceinline__ double euler_elem(int t, const double *trig, const do
 // stopping rule of color_icp.cu:216-228, then the hand-over: the next epoch of this state, or
 // the pose to the next level of the chain.  A failed solve takes the identity step (which also
 // satisfies the stopping rule), as the reference does.
-__device__ __noinline__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double *partials, int nvb, double *sys)
+__device__ __noinline__ void icp_finish(IcpState *states, int p, uint32_t epoch, const double *partials, int nvb, double *sys,
+                                        bool naive)
 {
     IcpState *st = states + p;
     __shared__ double Tc[16], Tn[16], dT[16], xi[6], trig[6];
@@ -643,7 +666,7 @@ __device__ __noinline__ void icp_finish(IcpState *states, int p, uint32_t epoch,
         f_crot = load_coherent(&st->conv_rot);
         f_ct2 = load_coherent(&st->conv_t2);
     }
-    sum_partials(partials, nvb, sys);  // after the acquire in unit_commit
+    sum_partials(partials, nvb, sys, naive);  // after the acquire in unit_commit
     if (t == 0) f_ok = solve6(sys, sys + 21, xi) ? 1 : 0;
     __syncthreads();
     if (t < 3) {
@@ -717,7 +740,7 @@ __device__ __forceinline__ void unit_commit(const PairDesc &P, IcpState *states,
     __syncthreads();
     if (!last) return;
     fence_acquire_agent();
-    icp_finish(states, p, epoch, P.partials, nvb, sys);
+    icp_finish(states, p, epoch, P.partials, nvb, sys, MODE == GCA_ACCUM_NAIVE_FP32);
 }
 
 // ---- persistent launch: every iteration of every state --------------------------------------
@@ -926,7 +949,7 @@ __global__ __launch_bounds__(kIcpBlock) void k_build_gn(PairDesc P, const float4
     __syncthreads();
     if (!last) return;
     fence_acquire_agent();
-    sum_partials(P.partials, P.nblk, sys);
+    sum_partials(P.partials, P.nblk, sys, MODE == GCA_ACCUM_NAIVE_FP32);
     if (threadIdx.x < kNSys) st->sys[threadIdx.x] = sys[threadIdx.x];
     if (threadIdx.x == 0) st->ticket = 0u;
 }
@@ -1646,6 +1669,35 @@ int32_t gca_compute_residuals(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const i
     GCA_HIP(hipMemcpyAsync(&s, res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     GCA_HIP(hipStreamSynchronize(ctx->stream));
     if (sum) *sum = s;
+    return GCA_OK;
+}
+
+int32_t gca_abs_residual_host(gca_ctx ctx, gca_cloud src, gca_cloud tgt, float max_corr, float lambda, float grad_r,
+                              float *out_host, int64_t capacity, int64_t *n_out, double *mean)
+{
+    if (!ctx || !src || !tgt || !n_out || capacity < 0) return GCA_ERR_INVALID_VALUE;
+    if (!(max_corr > 0.0f) || lambda > 1.0f || !(lambda >= 0.0f)) return GCA_ERR_INVALID_VALUE;
+    if (!tgt->has_normals) return GCA_ERR_NOT_READY;  // color_icp.cu:104-110
+    if (!tgt->has_gradient) {                        // color_icp.cu:112-117
+        if (!(grad_r > 0.0f)) return GCA_ERR_NOT_READY;
+        GCA_TRY(gca_compute_color_gradient(ctx, tgt, grad_r, 0));
+    }
+    GCA_TRY(cloud_sync_header(src));
+    const int64_t n = src->host_hdr.n;
+    *n_out = n;
+    if (mean) *mean = 0.0;
+    if (n > capacity || (n > 0 && !out_host)) return GCA_ERR_INVALID_VALUE;
+    if (n == 0) return GCA_OK;
+    int32_t *nn = (int32_t *)ctx_scratch(ctx, 34, sizeof(int32_t) * (size_t)n, false);
+    float *res = (float *)ctx_scratch(ctx, 35, sizeof(float) * (size_t)n, false);
+    if (!nn || !res) return GCA_ERR_OUT_OF_MEMORY;
+    GCA_TRY(gca_nn_search(ctx, src, tgt, max_corr, nn));  // color_icp.cu:120
+    const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    double sum = 0.0;
+    GCA_TRY(gca_compute_residuals(ctx, src, tgt, nn, lambda, I, GCA_RESIDUAL_ABS, res, &sum));
+    GCA_HIP(hipMemcpyAsync(out_host, res, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    GCA_HIP(hipStreamSynchronize(ctx->stream));
+    if (mean) *mean = sum / (double)n;  // color_icp.cu:134-135
     return GCA_OK;
 }
 

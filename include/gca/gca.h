@@ -92,7 +92,8 @@ typedef struct gca_bilateral {
 
 enum {
     GCA_ACCUM_COMPENSATED = 0, /* FP32 per-point terms, TwoSum-compensated FP32 partials, FP64 merge */
-    GCA_ACCUM_NAIVE_FP32 = 1,  /* ablation: plain FP32 partial sums, FP64 merge */
+    GCA_ACCUM_NAIVE_FP32 = 1,  /* ablation: every sum FP32 (within and across work units), like a
+                                  plain FP32 transform_reduce */
     GCA_ACCUM_FP64 = 2         /* FP64 partials throughout */
 };
 
@@ -232,24 +233,45 @@ int32_t gca_cloud_from_rgbd(gca_ctx ctx, const uint16_t *depth_dev, const uint8_
  * via point_cloud::voxel_grid_down_sample (point_cloud.cu:198-229). */
 int32_t gca_voxel_downsample(gca_ctx ctx, gca_cloud in, float voxel_size, gca_cloud out);
 
-/* Euclidean clustering: the connected components of the graph joining points closer than
- * tolerance.  labels_dev (device, one int32 per point) receives each point's cluster number, or -1
- * when its component has fewer than min_cluster_size points; clusters are numbered in the order of
- * their lowest point index, as the reference's seeds run.  Components above max_cluster_size
- * points are kept whole and counted in *n_oversized (may be NULL); the reference cuts them where
- * its queue reaches that size.  GCA_ERR_NOT_READY without normals, like the reference's empty
- * result.  Reference: cuda_euclidean_clustering (src/geometry/cuda_clustering.cuh, .cu:31-110) via
- * point_cloud::euclidean_clustering (point_cloud.hpp:71-73, point_cloud.cu:284-335). */
+/* Euclidean clustering: the reference's seeded breadth-first searches over the graph joining
+ * points closer than tolerance.  Seeds run in index order; a search stops expanding once its queue
+ * holds max_cluster_size points, and the rest of that component is left to later seeds; a queue
+ * of at least min_cluster_size points is a cluster.  labels_dev (device, one int32 per point)
+ * receives each point's cluster number (seed order) or -1.  Neighbours are visited in the order
+ * (squared distance, index); only the pieces of a component above max_cluster_size depend on it.
+ * *n_cut (may be NULL) counts the components that were cut.  GCA_ERR_NOT_READY without normals,
+ * like the reference's empty result.  Synchronises.  Reference: cuda_euclidean_clustering
+ * (src/geometry/cuda_clustering.cuh, .cu:31-110) via point_cloud::euclidean_clustering
+ * (point_cloud.hpp:71-73, point_cloud.cu:284-335). */
 int32_t gca_euclidean_clustering(gca_ctx ctx, gca_cloud cloud, float tolerance,
                                  int32_t min_cluster_size, int32_t max_cluster_size,
-                                 int32_t *labels_dev, int32_t *n_clusters, int32_t *n_oversized);
+                                 int32_t *labels_dev, int32_t *n_clusters, int32_t *n_cut);
 
 /* gca_euclidean_clustering with the labels copied to host memory (labels_cap >= the cloud's
  * point count); for callers without device buffers (the C++ facade). */
 int32_t gca_euclidean_clustering_host(gca_ctx ctx, gca_cloud cloud, float tolerance,
                                       int32_t min_cluster_size, int32_t max_cluster_size,
                                       int32_t *labels_host, int64_t labels_cap, int32_t *n_clusters,
-                                      int32_t *n_oversized);
+                                      int32_t *n_cut);
+
+/* Residual-based segmentation: gca_euclidean_clustering's searches with an edge followed only when
+ * the two points' absolute ICP residuals differ by less than 0.003 (|r_a - r_b| in FP32).
+ * abs_residual_dev: one float per point (device), e.g. gca_compute_residuals(GCA_RESIDUAL_ABS)
+ * of this cloud as the ICP source (color_icp::get_abs_residual, color_icp.cu:90-137).  No normals
+ * needed.  Reference: cuda_residual_based_segmentation (src/geometry/cuda_clustering.cuh,
+ * .cu:251-341) via point_cloud::icp_residual_segmentation (point_cloud.hpp:79-81,
+ * point_cloud.cu:391-435). */
+int32_t gca_residual_segmentation(gca_ctx ctx, gca_cloud cloud, float tolerance,
+                                  const float *abs_residual_dev, int32_t min_cluster_size,
+                                  int32_t max_cluster_size, int32_t *labels_dev, int32_t *n_clusters,
+                                  int32_t *n_cut);
+
+/* gca_residual_segmentation with host residuals in and host labels out (labels_cap >= the cloud's
+ * point count), for callers without device buffers (the C++ facade). */
+int32_t gca_residual_segmentation_host(gca_ctx ctx, gca_cloud cloud, float tolerance,
+                                       const float *abs_residual_host, int32_t min_cluster_size,
+                                       int32_t max_cluster_size, int32_t *labels_host,
+                                       int64_t labels_cap, int32_t *n_clusters, int32_t *n_cut);
 
 /* Radius outlier removal: keep a point when more than min_neighbors points (itself
  * included) lie within radius.  Reference: cuda_grid_radius_outliers_removal
@@ -333,6 +355,16 @@ enum {
 int32_t gca_compute_residuals(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const int32_t *nn_dev,
                               float lambda_geometric, const double T[16], int32_t kind,
                               float *out_dev, double *sum);
+
+/* Per-point absolute residuals of src (as it stands) against tgt: nearest target point within
+ * max_corr, |sqrt(l) r_g| + |sqrt(1-l) r_c| per source point (0 without a correspondence) into
+ * out_host (capacity floats; *n_points = the source count), and their mean over all source points
+ * (*mean may be NULL).  The target's colour gradient is computed with gradient_radius if absent.
+ * Synchronises.  Reference: color_icp::get_abs_residual / get_abs_residual_host
+ * (src/registration/color_icp.cu:90-150), whose source has been moved to the final pose. */
+int32_t gca_abs_residual_host(gca_ctx ctx, gca_cloud src, gca_cloud tgt, float max_corr,
+                              float lambda_geometric, float gradient_radius, float *out_host,
+                              int64_t capacity, int64_t *n_points, double *mean);
 
 /* Host FP64 LDL^T solve of (J^T J) xi = -J^T r and the Euler (Rz Ry Rx) increment.  Identity
  * and GCA_ERR_NOT_SPD if the factorisation fails.  Reference: solve_JTJ_JTr<true>,

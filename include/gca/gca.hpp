@@ -197,8 +197,9 @@ public:
     // point (host copy; empty on invalid input, point_cloud.cu:512-516).
     static std::vector<int32_t> nn_search(const point_cloud &query_pc, const point_cloud &reference_pc, float radius);
 
-    // point_cloud::euclidean_clustering (point_cloud.cu:284-335): clusters as ascending point
-    // indices, in the order of their lowest index; empty where the reference returns nothing.
+    // point_cloud::euclidean_clustering (point_cloud.cu:284-335): clusters in seed order, each as
+    // ascending point indices (the reference lists a cluster in its search-queue order); empty
+    // where the reference returns nothing.
     std::vector<std::vector<int32_t>> euclidean_clustering(float cluster_tolerance, int32_t min_cluster_size,
                                                            int32_t max_cluster_size) const
     {
@@ -208,13 +209,38 @@ public:
                                           max_cluster_size, lab.empty() ? nullptr : lab.data(),
                                           (int64_t)lab.size(), &nc, nullptr) != GCA_OK)
             return {};
+        return clusters_from_labels(lab, nc);
+    }
+
+    // point_cloud::icp_residual_segmentation (point_cloud.cu:391-435): segments of the radius
+    // graph whose edges join points with absolute residuals less than 0.003 apart, as
+    // euclidean_clustering returns clusters; abs_residual has one value per point (as from
+    // color_icp::get_abs_residual_host).  Empty where the reference returns nothing.
+    std::vector<std::vector<int32_t>> icp_residual_segmentation(float cluster_tolerance,
+                                                                const std::vector<float> &abs_residual,
+                                                                int32_t min_cluster_size,
+                                                                int32_t max_cluster_size) const
+    {
+        std::vector<int32_t> lab(points_number());
+        int32_t nc = 0;
+        if (abs_residual.size() < lab.size()) return {};
+        if (gca_residual_segmentation_host(m_ctx->handle(), m_cloud, cluster_tolerance,
+                                           abs_residual.empty() ? nullptr : abs_residual.data(), min_cluster_size,
+                                           max_cluster_size, lab.empty() ? nullptr : lab.data(), (int64_t)lab.size(),
+                                           &nc, nullptr) != GCA_OK)
+            return {};
+        return clusters_from_labels(lab, nc);
+    }
+
+private:
+    static std::vector<std::vector<int32_t>> clusters_from_labels(const std::vector<int32_t> &lab, int32_t nc)
+    {
         std::vector<std::vector<int32_t>> out((size_t)nc);
         for (size_t i = 0; i < lab.size(); ++i)
             if (lab[i] >= 0) out[(size_t)lab[i]].push_back((int32_t)i);
         return out;
     }
 
-private:
     std::pair<std::array<float, 3>, std::array<float, 3>> bounds() const
     {
         std::pair<std::array<float, 3>, std::array<float, 3>> b{};
@@ -268,6 +294,23 @@ public:
         m_source->transform(m_T);
         m_done = true;
         return true;
+    }
+
+    // color_icp::get_abs_residual_host (color_icp.cu:90-150): per source point |r_g| + |r_c|
+    // (weighted) against its nearest target point within the correspondence distance, the source
+    // standing where align left it, and the mean over the source points; empty on invalid state.
+    std::pair<std::vector<float>, float> get_abs_residual_host() const
+    {
+        if (!m_source || !m_target) return {};
+        std::vector<float> out(m_source->points_number());
+        int64_t n = 0;
+        double mean = 0.0;
+        if (!ok(gca_abs_residual_host(m_source->ctx()->handle(), m_source->handle(), m_target->handle(),
+                                      m_params.max_correspondence_distance, m_params.lambda_geometric, m_grad_r,
+                                      out.empty() ? nullptr : out.data(), (int64_t)out.size(), &n, &mean)))
+            return {};
+        out.resize((size_t)n);
+        return {std::move(out), (float)mean};
     }
 
     const mat4x4d &get_final_transformation_matrix() const { return m_T; }

@@ -268,18 +268,26 @@ def radius_search(q_xyzi, r_xyzi, radius, max_nn=0, rbmin=None, rbmax=None):
     return offs, idx[:total].copy()
 
 
-def euclidean_clustering(xyzi, tolerance, min_cluster_size, max_cluster_size, bmin=None, bmax=None):
+def euclidean_clustering(xyzi, tolerance, min_cluster_size, max_cluster_size, bmin=None, bmax=None,
+                         residual=None):
     """cuda_euclidean_clustering (cuda_clustering.cu:31-110): for each unvisited point in index
     order, a breadth-first search over the radius-neighbour lists (cuda_search_radius_neighbors,
-    here radius_search) that stops expanding once its queue holds max_cluster_size points
-    (:75-96); the queue is kept as a cluster when it has at least min_cluster_size points
-    (:98-103).  Returns the clusters as int32 arrays in queue order.  Pure Python: small inputs."""
+    here radius_search, so each list in (squared distance, index) order) that stops expanding once
+    its queue holds max_cluster_size points (:75-96); the queue is kept as a cluster when it has at
+    least min_cluster_size points (:98-103).  With ``residual`` (one float per point) this is
+    cuda_residual_based_segmentation (:251-341): a neighbour is taken only when
+    |residual[p] - residual[j]| (FP32) < 0.003 for the point p being expanded (:291-293, :316-319),
+    and is otherwise left unvisited.  Returns the clusters as int32 arrays in queue order.  Pure
+    Python: small inputs."""
     if min_cluster_size < 0 or max_cluster_size <= 0 or max_cluster_size < min_cluster_size:
         raise ValueError("invalid cluster sizes")  # :38-42
     xyzi = _f4(xyzi)
     n = len(xyzi)
     if n == 0:
         return []
+    res = None if residual is None else np.ascontiguousarray(residual, np.float32).reshape(-1)
+    if res is not None and len(res) < n:
+        raise ValueError("one residual per point")
     offs, idx = radius_search(xyzi, xyzi, tolerance, 0, bmin, bmax)
     visited = np.zeros(n, bool)
     out = []
@@ -292,9 +300,12 @@ def euclidean_clustering(xyzi, tolerance, min_cluster_size, max_cluster_size, bm
         while k < len(queue) and len(queue) < max_cluster_size:
             p = queue[k]
             for j in idx[offs[p]:offs[p + 1]].tolist():
-                if not visited[j]:
-                    visited[j] = True
-                    queue.append(j)
+                if visited[j]:
+                    continue
+                if res is not None and not float(abs(res[p] - res[j])) < 0.003:
+                    continue
+                visited[j] = True
+                queue.append(j)
             k += 1
         if len(queue) >= min_cluster_size:
             out.append(np.array(queue, np.int32))

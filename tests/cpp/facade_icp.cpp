@@ -3,7 +3,9 @@
 // normals, color_icp::align.  Input: a binary file
 //   int64 n_src, int64 n_tgt, float32 src_xyz[n_src*3], src_rgb[n_src*3],
 //   float32 tgt_xyz[n_tgt*3], tgt_rgb[n_tgt*3]
-// Output: one JSON line with the pose (row-major), iterations and the reference "RMSE".
+// Output: one JSON line with the pose (row-major), iterations, the reference "RMSE", the
+// per-point absolute residuals at the final pose (color_icp::get_abs_residual_host), the target's
+// Euclidean clusters and the moved source's residual segments ([size, first index] each).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -55,9 +57,22 @@ int main(int argc, char **argv)
         return 1;
     }
     const auto &T = icp.get_final_transformation_matrix();
+    const auto res = icp.get_abs_residual_host();
+    const auto clusters = tgt->euclidean_clustering(0.02f, 20, 2000);
+    const auto moved = icp.get_transformed_source_point_cloud();
+    const auto segments = moved->icp_residual_segmentation(0.02f, res.first, 10, 1500);
     printf("{\"T\": [");
     for (int k = 0; k < 16; ++k) printf("%s%.17g", k ? ", " : "", T[k]);
-    printf("], \"iterations\": %d, \"rmse\": %.17g, \"n_src\": %zu, \"moved_src\": %zu}\n", icp.report().iterations,
-           (double)icp.get_RSME(), src->points_number(), icp.get_transformed_source_point_cloud()->points_number());
+    printf("], \"iterations\": %d, \"rmse\": %.17g, \"n_src\": %zu, \"moved_src\": %zu", icp.report().iterations,
+           (double)icp.get_RSME(), src->points_number(), moved->points_number());
+    printf(", \"abs_mean\": %.9g, \"abs\": [", (double)res.second);
+    for (size_t k = 0; k < res.first.size(); ++k) printf("%s%.9g", k ? ", " : "", (double)res.first[k]);
+    printf("], \"clusters\": [");
+    for (size_t k = 0; k < clusters.size(); ++k)
+        printf("%s[%zu, %d]", k ? ", " : "", clusters[k].size(), clusters[k].empty() ? -1 : clusters[k][0]);
+    printf("], \"segments\": [");
+    for (size_t k = 0; k < segments.size(); ++k)
+        printf("%s[%zu, %d]", k ? ", " : "", segments[k].size(), segments[k].empty() ? -1 : segments[k][0]);
+    printf("]}\n");
     return 0;
 }

@@ -242,11 +242,12 @@ def test_euclidean_clustering_synthetic(ctx):
         _clusters_equal(pc.euclidean_clustering(tol, mn, len(x)), O.euclidean_clustering(xi, tol, mn, len(x)))
     labels, nc, no = pc.euclidean_cluster_labels(tol, 10, 500)
     whole = O.euclidean_clustering(xi, tol, 10, len(x))
-    assert nc == len(whole) and no == sum(len(c) > 500 for c in whole) > 0
+    cut = O.euclidean_clustering(xi, tol, 10, 500)
+    assert nc == len(cut) and no == sum(len(c) > 500 for c in whole) > 0
     lab = labels.cpu().numpy()
-    for k, c in enumerate(whole):
+    for k, c in enumerate(cut):
         assert (lab[c] == k).all()
-    assert (lab == -1).sum() == len(x) - sum(len(c) for c in whole)
+    assert (lab == -1).sum() == len(x) - sum(len(c) for c in cut)
     import ctypes as C
 
     host = np.full(len(x), -7, np.int32)
@@ -260,6 +261,63 @@ def test_euclidean_clustering_synthetic(ctx):
     assert pc.euclidean_clustering(tol, 5, 4) == [] and pc.euclidean_clustering(0.0, 1, 10) == []
 
 
+@pytest.mark.parametrize("mx", [500, 100, 37, 3, 1])
+def test_euclidean_clustering_truncation(ctx, mx):
+    """max_cluster_size below the largest components: the reference's searches stop at that
+    size and leave the rest to later seeds (cuda_clustering.cu:75-96); the device replays them."""
+    import cuda_3d_amd as g
+    from test_oracle import cluster_scene
+
+    tol = 0.02
+    x = cluster_scene(3, tol)
+    pc = g.PointCloud.from_numpy(x[:, :3], np.full((len(x), 3), 0.5), ctx=ctx)
+    pc.set_normals(np.tile([0.0, 0.0, 1.0], (len(x), 1)))
+    xi = pc.download_xyzi()
+    for mn in (0, min(10, mx)):
+        print(f"[gpu-parity] clustering truncation max {mx} min {mn}", flush=True)
+        want = O.euclidean_clustering(xi, tol, mn, mx)
+        got = pc.euclidean_clustering(tol, mn, mx)
+        _clusters_equal(got, want)
+        assert max(len(c) for c in want) >= mx
+
+
+def test_residual_segmentation_synthetic(ctx):
+    """Residual-based segmentation (cuda_clustering.cu:251-341) against the oracle, with residuals
+    that cut many edges, whole and with max_cluster_size cutting components."""
+    import cuda_3d_amd as g
+    from test_oracle import cluster_scene
+
+    tol = 0.02
+    x = cluster_scene(5, tol)
+    rng = np.random.default_rng(9)
+    res = rng.choice(np.float32([0.0, 0.002, 0.004, 0.0055, 0.01]), len(x)).astype(np.float32)
+    res += rng.uniform(0, 1e-4, len(x)).astype(np.float32)
+    pc = g.PointCloud.from_numpy(x[:, :3], np.full((len(x), 3), 0.5), ctx=ctx)  # no normals needed
+    xi = pc.download_xyzi()
+    for mn, mx in ((0, len(x)), (5, len(x)), (5, 60), (1, 7)):
+        print(f"[gpu-parity] residual segmentation synthetic min {mn} max {mx}", flush=True)
+        _clusters_equal(pc.icp_residual_segmentation(tol, res, mn, mx),
+                        O.euclidean_clustering(xi, tol, mn, mx, residual=res))
+    assert pc.icp_residual_segmentation(0.0, res, 1, 10) == [] and pc.icp_residual_segmentation(tol, res, 5, 4) == []
+
+
+def test_residual_segmentation_frame(level, gn_case):
+    """Both frames: the 2 cm source moved by the test pose, its absolute residuals against the
+    2 cm target (gca_compute_residuals, ABS kind), segmented at 3 cm, whole and cut at 300."""
+    import cuda_3d_amd as g
+
+    moved = gn_case["src"].copy()
+    moved.transform(gn_case["T"])
+    res, _ = g.compute_residuals(gn_case["src"], level["gpu"], gn_case["nn"], 1, LAM, gn_case["T"])
+    res = np.asarray(res, np.float32)
+    xi = moved.download_xyzi()
+    for mn, mx in ((5, 10 ** 6), (5, 300)):
+        print(f"[gpu-parity] residual segmentation frame n={len(xi)} min {mn} max {mx}", flush=True)
+        got = moved.icp_residual_segmentation(0.03, res, mn, mx)
+        _clusters_equal(got, O.euclidean_clustering(xi, 0.03, mn, mx, residual=res))
+        assert len(got) >= 2
+
+
 def test_euclidean_clustering_frame(level):
     """The 2 cm target of each frame with its normals (a copy: the shared cloud keeps its grid),
     tolerance 3 cm."""
@@ -269,6 +327,11 @@ def test_euclidean_clustering_frame(level):
     got = vc.euclidean_clustering(0.03, 20, 10 ** 6)
     _clusters_equal(got, O.euclidean_clustering(ox, 0.03, 20, 10 ** 6))
     assert len(got) >= 1
+    # cut at a size far below the frame's surfaces
+    got = vc.euclidean_clustering(0.03, 20, 1000)
+    want = O.euclidean_clustering(ox, 0.03, 20, 1000)
+    _clusters_equal(got, want)
+    assert len(want) > len(O.euclidean_clustering(ox, 0.03, 20, 10 ** 6))
 
 
 def test_frontend_edge_frames(ctx):
@@ -805,6 +868,14 @@ def test_cpp_facade_matches_python(ctx, tmp_path):
     assert icp.align()
     np.testing.assert_array_equal(np.array(out["T"]).reshape(4, 4), icp.get_final_transformation_matrix())
     assert out["iterations"] == icp.report.iterations and out["n_src"] == len(sx) == out["moved_src"]
+    # get_abs_residual_host, euclidean_clustering and icp_residual_segmentation through the facade
+    res, mean = icp.get_abs_residual()
+    np.testing.assert_array_equal(np.array(out["abs"], np.float32), res)
+    assert abs(out["abs_mean"] - mean) <= 1e-6 * max(abs(mean), 1e-30)
+    clusters = t.euclidean_clustering(0.02, 20, 2000)
+    assert out["clusters"] == [[len(c), int(c[0])] for c in clusters] and len(clusters) > 0
+    segments = icp.get_transformed_source_point_cloud().icp_residual_segmentation(0.02, res, 10, 1500)
+    assert out["segments"] == [[len(c), int(c[0])] for c in segments] and len(segments) > 0
 
 
 # ---- precision variant and differentiation at their stated sizes -----------------------------------
@@ -838,6 +909,40 @@ def test_precision_variant_1280x720(ctx):
     print(f"[gpu-parity] 1280x720 finest level {fine} points; pose error vs FP64 oracle (rad, m): {errs}", flush=True)
     assert max(errs["compensated"]) <= 1e-5 and max(errs["fp64"]) <= 1e-5
     assert all(np.isfinite(errs["naive_fp32"]))
+    # The accumulation error itself: the Gauss-Newton system of the finest level at the oracle's
+    # final pose, where J^T r nearly cancels, built in each mode and compared with the oracle's
+    # FP64 build over the same correspondences.
+    v = 0.01
+    sf, tf = src.voxel_grid_down_sample(v), tgt.voxel_grid_down_sample(v)
+    tf.estimate_normals(2.5 * v, 30)
+    tf.compute_color_gradient(2.0 * v, 0)
+    osf, _, _ = O.voxel_downsample(osx, osr, v)
+    otf, _, _ = O.voxel_downsample(otx, otr, v)
+    on = O.estimate_normals(otf, 2.5 * v, 30)
+    og = O.color_gradient(otf, on, 2.0 * v, 0)
+    moved = sf.copy()
+    moved.transform(To)
+    nn = g.PointCloud.nn_search(moved, tf, 2.5 * v)
+    o = O.build_gn(osf, otf, on, og, nn, LAM, To)
+    oJ = np.zeros((6, 6))
+    k = 0
+    for a in range(6):
+        for b in range(a, 6):
+            oJ[a, b] = oJ[b, a] = o[k]
+            k += 1
+    sys_err = {}
+    for mode, name in ((0, "compensated"), (1, "naive_fp32"), (2, "fp64")):
+        JTJ, JTr, r2, nc = g.build_gauss_newton(sf, tf, nn, LAM, To, mode)
+        assert nc == int(o[28])
+        sys_err[name] = (float(np.abs(JTJ - oJ).max() / np.abs(oJ).max()),
+                         float(np.abs(JTr - o[21:27]).max() / np.abs(o[21:27]).max()),
+                         float(abs(r2 - o[27]) / o[27]))
+    print(f"[gpu-parity] 1280x720 finest-level system at the final pose, {int(o[28])} correspondences; "
+          f"relative error vs FP64 (JTJ, JTr, r^2): {sys_err}", flush=True)
+    # J^T r at the solution is a sum that cancels: its error is dominated by the FP32 rounding of
+    # the per-point terms in every mode, so it is reported, and only J^T J and r^2 are bounded
+    for name in ("compensated", "fp64"):
+        assert sys_err[name][0] <= 1e-5 and sys_err[name][2] <= 1e-5, (name, sys_err[name])
 
 
 def test_differentiation_100k_32_poses(ctx):

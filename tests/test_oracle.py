@@ -258,6 +258,54 @@ def test_clustering_matches_connected_components():
             assert a[0] == b[0]  # the seed is the lowest index
 
 
+def test_residual_segmentation_matches_gated_components():
+    """cuda_residual_based_segmentation (cuda_clustering.cu:251-341) without truncation is the
+    connected components of the radius graph restricted to edges whose residuals differ by less
+    than 0.003 (FP32 difference): pinned by scipy on that graph, with residuals laid out so that
+    many edges are cut (steps of 0.002 and 0.004 along the chain, random residuals in blobs)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    tol = 0.02
+    x = cluster_scene(5, tol)
+    rng = np.random.default_rng(9)
+    res = rng.choice(np.float32([0.0, 0.002, 0.004, 0.0055, 0.01]), len(x)).astype(np.float32)
+    res += rng.uniform(0, 1e-4, len(x)).astype(np.float32)
+    pairs = cKDTree(x[:, :3].astype(np.float64)).query_pairs(tol, output_type="ndarray")
+    d = np.linalg.norm(x[pairs[:, 0], :3].astype(np.float64) - x[pairs[:, 1], :3], axis=1)
+    gate = np.abs(res[pairs[:, 0]] - res[pairs[:, 1]]).astype(np.float64) < 0.003
+    keep = (d < tol) & gate
+    assert 0.2 < gate[d < tol].mean() < 0.9  # the gate is exercised both ways
+    pairs = pairs[keep]
+    A = coo_matrix((np.ones(len(pairs)), (pairs[:, 0], pairs[:, 1])), shape=(len(x), len(x)))
+    nc, lab = connected_components(A, directed=False)
+    sizes = np.bincount(lab)
+    for mn in (0, 5, 50):
+        cl = O.euclidean_clustering(x, tol, mn, len(x), residual=res)
+        want = [np.flatnonzero(lab == c) for c in np.unique(lab) if sizes[c] >= max(mn, 1)]
+        want.sort(key=lambda a: a[0])
+        assert len(cl) == len(want)
+        for a, b in zip(cl, want):
+            np.testing.assert_array_equal(np.sort(a), b)
+    # residuals all equal: the gate never fires, so the segments are the Euclidean clusters
+    flat = O.euclidean_clustering(x, tol, 10, len(x), residual=np.zeros(len(x), np.float32))
+    plain = O.euclidean_clustering(x, tol, 10, len(x))
+    assert len(flat) == len(plain) and all(np.array_equal(a, b) for a, b in zip(flat, plain))
+
+
+def test_residual_gate_is_checked_against_the_point_being_expanded():
+    """Three collinear points 0 - 1 - 2 (spacing 0.6 tol, so 0 and 2 are not neighbours) with
+    residuals 0, 0.002, 0.004: 0-1 and 1-2 pass the gate (0.002 < 0.003), so all three are one
+    segment, although |r0 - r2| = 0.004; with r1 = 0.0035 neither edge passes."""
+    tol = 0.1
+    x = np.zeros((3, 4), np.float32)
+    x[:, 0] = np.arange(3) * 0.06
+    seg = O.euclidean_clustering(x, tol, 1, 10, residual=np.float32([0, 0.002, 0.004]))
+    assert [s.tolist() for s in seg] == [[0, 1, 2]]
+    seg = O.euclidean_clustering(x, tol, 1, 10, residual=np.float32([0, 0.0035, 0.0]))
+    assert [s.tolist() for s in seg] == [[0], [1], [2]]
+
+
 def test_clustering_truncates_at_max_size_like_the_reference():
     """A chain with one neighbour each way, searched from index 0 with max 3: the queue stops at
     3 points, so the chain of 10 comes out as 0-2, 3-5, 6-8, 9 (cuda_clustering.cu:75-96)."""
