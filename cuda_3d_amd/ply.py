@@ -64,8 +64,12 @@ def _header(buf: bytes):
 
 
 def read_ply(path: str, remove_nan: bool = True) -> PlyVertices:
-    """Vertices of a PLY file.  ``remove_nan`` drops points with a non-finite coordinate
-    (removeNaNFromPointCloudBruteForce, test_color_icp.cpp:223-259)."""
+    """Vertices of a PLY file.  ``remove_nan`` is the reference's removeNaNPoints
+    (test_color_icp.cpp:335-342): it drops points with a non-finite coordinate
+    (removeNaNFromPointCloudBruteForce, :223-259) and points whose packed colour word is zero
+    (removeNaNRGBFromPointCloud, :262-290).  PCL's PLY reader packs r << 16 | g << 8 | b with a zero
+    alpha byte unless the file has an alpha property, so a pure-black vertex of a coloured file is
+    dropped; a file without colour keeps PCL's default alpha of 255 and loses nothing here."""
     with open(path, "rb") as f:
         buf = f.read()
     fmt, elements, off = _header(buf)
@@ -106,6 +110,12 @@ def read_ply(path: str, remove_nan: bool = True) -> PlyVertices:
         nrm = np.stack([cols["nx"], cols["ny"], cols["nz"]], 1).astype(np.float32)
     if remove_nan:
         keep = np.isfinite(xyz).all(1)
+        if rgb8 is not None:
+            word_zero = (rgb8 == 0).all(1)
+            alpha = cols.get("alpha")
+            if alpha is not None:
+                word_zero &= np.asarray(alpha).astype(np.uint8) == 0
+            keep &= ~word_zero
         if not keep.all():
             xyz = xyz[keep]
             rgb8 = None if rgb8 is None else rgb8[keep]

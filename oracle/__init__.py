@@ -68,7 +68,7 @@ class LevelParams(C.Structure):
 
 
 def build():
-    """Compile the oracle (make -C oracle)."""
+    """Compile the oracle and the CPU baseline (make -C oracle)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
@@ -471,3 +471,50 @@ def multiscale_icp(src_xyzi, src_rgb, tgt_xyzi, tgt_rgb, levels, T0=None):
     if rc != 0:
         raise ValueError("invalid level parameters")
     return T.reshape(4, 4), list(reps)
+
+
+# ---- CPU baseline (oracle/cpu_pipeline.cpp): the benchmark's frame step, FP32, OpenMP ----------
+
+class CpuPipelineParams(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float),
+                ("cy", C.c_float), ("scale", C.c_float), ("tmin", C.c_float), ("tmax", C.c_float),
+                ("radius", C.c_int), ("sigma_space", C.c_float), ("sigma_depth", C.c_float),
+                ("n_levels", C.c_int), ("voxel", C.c_float * 8), ("max_iter", C.c_int), ("lam", C.c_float),
+                ("conv_rot", C.c_double), ("conv_t2", C.c_double)]
+
+
+_cpb = None
+
+
+def _cpu_lib():
+    global _cpb
+    if _cpb is None:
+        path = os.path.join(_HERE, "_build", "libgca_cpu_pipeline.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.cpb_run.restype = C.c_int
+        L.cpb_run.argtypes = [u16p, u8p, u16p, u8p, C.POINTER(CpuPipelineParams), C.c_int, C.c_int, C.c_int,
+                              f64p, f64p, i32p, i64p]
+        _cpb = L
+    return _cpb
+
+
+def cpu_pipeline_run(frame_a, frame_b, W, H, fx, fy, cx, cy, levels, max_iter, conv=(1e-6, 1e-6),
+                     threads=0, warmup=3, reps=20, lam=0.968, bilateral=(2, 2.0, 30.0), tmin=0.2, tmax=3.5):
+    """Times the frame step of frame_b ((depth, bgr)) registered onto frame_a with the
+    multithreaded FP32 CPU pipeline: front end, per level voxel / grid / normals (r 2.5 v, K 30) /
+    colour gradient (r 2 v), colored ICP (corr 2.5 v).  threads = 0: OpenMP's default.  Returns
+    dict(ms=[reps], threads, T, iterations, points)."""
+    (da, ca), (db, cb) = frame_a, frame_b
+    prm = CpuPipelineParams(W, H, fx, fy, cx, cy, 0.001, tmin, tmax, bilateral[0], bilateral[1], bilateral[2],
+                            len(levels), (C.c_float * 8)(*levels), max_iter, lam, conv[0], conv[1])
+    ms = np.zeros(max(reps, 1))
+    T = np.zeros(16)
+    its = np.zeros(8, np.int32)
+    pts = np.zeros(9, np.int64)
+    args = [np.ascontiguousarray(a, dt).reshape(-1) for a, dt in ((da, np.uint16), (ca, np.uint8), (db, np.uint16),
+                                                                   (cb, np.uint8))]
+    used = _cpu_lib().cpb_run(*args, C.byref(prm), threads, warmup, reps, ms, T, its, pts)
+    return dict(ms=ms[:reps].tolist(), threads=int(used), T=T.reshape(4, 4), iterations=its[:len(levels)].tolist(),
+                points=pts[:1 + len(levels)].tolist())

@@ -45,6 +45,29 @@ def test_ascii_with_leading_element_and_nan(tmp_path):
     assert len(read_ply(str(p), remove_nan=False).xyz) == 3
 
 
+def test_black_vertices_are_dropped_like_the_reference(tmp_path):
+    """removeNaNRGBFromPointCloud (test_color_icp.cpp:262-290) drops points whose packed colour
+    word is zero: a pure-black vertex without alpha, or with alpha 0; black with alpha 255 and any
+    vertex of a file without colour stay."""
+    p = tmp_path / "black.ply"
+    p.write_text("ply\nformat ascii 1.0\nelement vertex 4\nproperty float x\nproperty float y\n"
+                 "property float z\nproperty uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n"
+                 "0 0 1 0 0 0\n1 0 1 0 0 1\n2 0 1 1 0 0\n3 0 1 0 0 0\n")
+    v = read_ply(str(p))
+    np.testing.assert_array_equal(v.xyz[:, 0], np.float32([1, 2]))
+    assert v.rgb8.tolist() == [[0, 0, 1], [1, 0, 0]]
+    assert len(read_ply(str(p), remove_nan=False).xyz) == 4
+    q = tmp_path / "alpha.ply"
+    q.write_text("ply\nformat ascii 1.0\nelement vertex 2\nproperty float x\nproperty float y\n"
+                 "property float z\nproperty uchar red\nproperty uchar green\nproperty uchar blue\n"
+                 "property uchar alpha\nend_header\n0 0 1 0 0 0 255\n1 0 1 0 0 0 0\n")
+    np.testing.assert_array_equal(read_ply(str(q)).xyz[:, 0], np.float32([0]))
+    r = tmp_path / "plain.ply"
+    r.write_text("ply\nformat ascii 1.0\nelement vertex 2\nproperty float x\nproperty float y\n"
+                 "property float z\nend_header\n0 0 0\n1 1 1\n")
+    assert len(read_ply(str(r)).xyz) == 2
+
+
 def test_binary_with_leading_element_and_no_colour(tmp_path):
     p = tmp_path / "b.ply"
     xyz = np.float32([[1, 2, 3], [-4, 5.5, 6]])
