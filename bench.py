@@ -9,14 +9,23 @@ reference's fixed stopping rule (color_icp.hpp:45-47: cos(theta_step) >= 1 - 1e-
 registers a real frame pair.  Depth and colour are resident in HBM before the timed region.
 
 The JSON line also carries:
-  roofline      the dominant kernel (the persistent ICP launch, one per frame), algorithmic
-                bytes per launch = sum over levels of iterations x (24 M_src + 48 M_tgt)
-                (source x,y,z,I read + previous match read and written; target x,y,z,I, normal
-                and gradient read once) over its average launch time, from hipEvents on the
-                library's stream in a second, identical pass;
-  cpu_baseline  the FP64 oracle (C, one core) on one frame of the same workload;
+  roofline      the dominant kernel (the resident ICP launch, one per pyramid level),
+                algorithmic bytes per launch = iterations x (24 M_src + 48 M_tgt) (source
+                x,y,z,I read + previous match read and written; target x,y,z,I, normal and
+                gradient read once) over its average launch time, from hipEvents on the
+                launching stream in a second, identical pass;
+  cpu_baseline  the same frame step as a multithreaded FP32 C++ program (oracle/cpu_pipeline.cpp)
+                on the host cores: median of 20 steps after 3 warm-ups, at all threads (value)
+                and at one thread, with the core count and CPU model;
+  fixed_iterations  the same workload with the stop thresholds at 0: exactly 30 Gauss-Newton
+                iterations per level;
+  sequence      8 distinct frames of one scene (poses within the motion bounds of each other)
+                stepped in order, each registered onto the one before;
+  h2d           host -> device upload of one frame (depth + BGR), pinned and pageable, and the
+                PCIe-inclusive frame time (value + pageable upload: the reference uploads
+                pageable frames synchronously before each step);
   batch         64 uneven pairs x 3 levels in one persistent launch vs per-pair, per-level
-                plain calls, same stopping rule (poses must agree bit for bit).
+                plain calls, same stopping rule.
 """
 from __future__ import annotations
 
@@ -38,7 +47,7 @@ def log(msg):
 
 def read_traffic():
     """HBM bytes per ICP launch measured with rocprofv3 --pmc (committed summary)."""
-    path = os.path.join(ROOT, "profiles", "icp_persist_traffic_640x480.md")
+    path = os.path.join(ROOT, "profiles", "icp_resident_traffic_640x480.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -50,22 +59,124 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(frames, cam, levels, iters):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(frames, cm, levels, iters, reps=20, warmup=3):
+    """The frame step (front end, levels, 3-level ICP onto the previous frame) of the multithreaded
+    FP32 CPU pipeline: median ms at all threads and at one thread."""
     import numpy as np
 
     import oracle as O
 
-    (td, tc), (sd, sc) = frames
-    camo = O.make_camera(cam.width, cam.height, cam.depth_intrin.fx, cam.depth_intrin.fy,
-                         cam.depth_intrin.cx, cam.depth_intrin.cy)
-    bo = O.Bilateral(1, 2, 2.0, 30.0)
-    lo = [O.LevelParams(v, 2.5 * v, 30, 2 * v, 0, O.icp_params(iters, 2.5 * v, 0.968, 1e-6, 1e-6)) for v in levels]
-    tx, tr, _, _ = O.cloud_from_rgbd(td, tc, camo, bo, 0.2, 3.5)
+    W, H = cm.width, cm.height
+    out = {}
+    for key, th in (("all", 0), ("one", 1)):
+        r = O.cpu_pipeline_run(frames[0], frames[1], W, H, cm.fx, cm.fy, cm.cx, cm.cy, levels, iters,
+                               threads=th, warmup=warmup, reps=reps)
+        out[key] = (float(np.median(r["ms"])), r["threads"], r["iterations"])
+    return out
+
+
+def h2d_upload(td, tc, reps=20):
+    """One frame's upload (uint16 depth + BGR8) with synchronous copies: from pinned host memory
+    and from pageable numpy arrays.  Median ms of each."""
+    import numpy as np
+    import torch
+
+    d_dev = torch.empty(td.shape, dtype=torch.int16, device="cuda")
+    c_dev = torch.empty(tc.shape, dtype=torch.uint8, device="cuda")
+    d_pin = torch.from_numpy(td.view(np.int16)).pin_memory()
+    c_pin = torch.from_numpy(tc).pin_memory()
+    d_pag = torch.from_numpy(td.view(np.int16).copy())
+    c_pag = torch.from_numpy(tc.copy())
+    res = {}
+    for name, (d, c) in (("pinned", (d_pin, c_pin)), ("pageable", (d_pag, c_pag))):
+        ts = []
+        for k in range(reps + 2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            d_dev.copy_(d)
+            c_dev.copy_(c)
+            torch.cuda.synchronize()
+            if k >= 2:
+                ts.append((time.perf_counter() - t0) * 1e3)
+        res[name] = float(np.median(ts))
+    res["bytes"] = int(td.nbytes + tc.nbytes)
+    return res
+
+
+def level_iteration_us(reports):
+    """Per level: device time of one Gauss-Newton iteration (active time / iterations)."""
+    return [r.active_us / r.iterations if r.iterations else 0.0 for r in reports]
+
+
+def run_fixed(g, ctx, cam, bil, ptrs, levels, iters, steps, graph):
+    """The headline workload with the stop thresholds at 0 (every level runs `iters`)."""
+    import numpy as np
+
+    from cuda_3d_amd.pipeline import FramePipeline
+
+    pipe = FramePipeline(ctx, cam, bil, levels, iters, conv_rotation=0.0, conv_translation_sq=0.0, graph=graph)
+    for k in range(6):
+        pipe.step(*ptrs[k % 2])
+    pipe.ctx.synchronize()
     t0 = time.perf_counter()
-    sx, sr, _, _ = O.cloud_from_rgbd(sd, sc, camo, bo, 0.2, 3.5)  # the new frame's front end
-    T, _ = O.multiscale_icp(sx, sr, tx, tr, lo)                   # prep + 3-level registration
-    dt = time.perf_counter() - t0
-    return dt * 1e3, T
+    for k in range(steps):
+        pipe.step(*ptrs[k % 2])
+    pipe.ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    per_it = []
+    for k in range(6):
+        pipe.step(*ptrs[k % 2])
+        per_it.append(level_iteration_us(pipe.result()[1]))
+    its = pipe.iterations()
+    pipe.close()
+    return {"ms_per_frame": ms, "iterations_per_level": its, "steps": steps,
+            "icp_us_per_iteration_per_level": np.median(np.array(per_it), axis=0).tolist()}
+
+
+def run_sequence(g, ctx, cam, bil, levels, iters, W, H, n_frames, passes, graph):
+    """n_frames distinct frames stepped in order (each onto the one before), `passes` times."""
+    import numpy as np
+
+    from cuda_3d_amd import scene
+    from cuda_3d_amd.pipeline import FramePipeline
+
+    frames, poses, _ = scene.frame_sequence(31, n_frames, W, H)
+    dev = [(g.device_tensor(d.view(np.int16)), g.device_tensor(c)) for d, c in frames]
+    ptrs = [(d.data_ptr(), c.data_ptr()) for d, c in dev]
+    pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=graph)
+    errs, its = [], []
+    for k in range(n_frames + 1):  # warm-up pass with a readback per frame: pose error vs truth
+        pipe.step(*ptrs[k % n_frames])
+        T, reps = pipe.result()
+        if T is None:
+            continue
+        a, b = poses[(k - 1) % n_frames], poses[k % n_frames]
+        truth = np.linalg.inv(a) @ b
+        dR = T[:3, :3] @ truth[:3, :3].T
+        errs.append(float(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1))))
+        its.append([r.iterations for r in reps])
+    pipe.ctx.synchronize()
+    t0 = time.perf_counter()
+    steps = n_frames * passes
+    for k in range(steps):
+        pipe.step(*ptrs[k % n_frames])
+    pipe.ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    captures, updates = pipe.graph_stats()
+    pipe.close()
+    return {"frames": n_frames, "steps": steps, "ms_per_frame": ms, "graph_captures": captures,
+            "graph_node_updates": updates, "iterations_per_level": its,
+            "rotation_err_vs_truth_rad_median": float(np.median(errs)), "rotation_err_vs_truth_rad_max": float(max(errs))}
 
 
 def run_batch(g, ctx, n_pairs, levels, iters, seed=2026):
@@ -170,6 +281,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-pairs", type=int, default=64)
     ap.add_argument("--no-clustering", action="store_true")
+    ap.add_argument("--no-fixed", action="store_true", help="skip the fixed-iterations run")
+    ap.add_argument("--no-sequence", action="store_true", help="skip the 8-frame sequence run")
     ap.add_argument("--no-roofline-pass", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly (no hipGraph replay)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -221,14 +334,17 @@ def main():
     ms = (t1 - t0) * 1e3 / args.steps
     log(f"{ms:.3f} ms/frame")
     # latency of one frame when the host waits for each pose
-    lat = []
+    lat, per_it = [], []
     for _ in range(min(args.steps, 20)):
         t2 = time.perf_counter()
         step()
-        pipe.result()
+        _, reps = pipe.result()
         lat.append((time.perf_counter() - t2) * 1e3)
+        per_it.append(level_iteration_us(reps))
     lat_ms = float(np.median(lat))
-    log(f"{lat_ms:.3f} ms per frame with a pose readback after each frame")
+    it_us = np.median(np.array(per_it), axis=0).tolist()
+    log(f"{lat_ms:.3f} ms per frame with a pose readback after each frame; "
+        f"ICP us per iteration by level {[round(x, 1) for x in it_us]}")
 
     sizes = [pipe.level_sizes(s) for s in (0, 1)]
     full_n = [pipe.full_size(s) for s in (0, 1)]
@@ -254,7 +370,7 @@ def main():
         per_launch_bytes = algo / max(n_icp, 1)
         avg_launch_ms = icp_ms / max(n_icp, 1)
         achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": "k_icp_persist", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        roof = {"bound": "hbm", "kernel": "k_icp_resident", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
                 "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_us": avg_launch_ms * 1e3,
                 "launches": n_icp}
@@ -264,12 +380,32 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle, 1 core) on one frame ...")
-        cpu_ms, _ = cpu_baseline([(td, tc), (sd, sc)], cam, levels, iters)
-        cpu = {"value": cpu_ms, "unit": "ms/frame", "cores": 1, "kind": "port",
-               "sample": "one 640x480 frame: front end + voxel/normals/gradient at 3 levels + 3-level colored ICP "
-                         "(max 30 it/level, reference stopping rule), FP64 oracle in C on one host core"}
-        log(f"cpu baseline {cpu_ms:.0f} ms/frame")
+        log("cpu baseline (FP32 C++ pipeline, OpenMP) on the same frame step ...")
+        cb = cpu_baseline([(td, tc), (sd, sc)], cm, levels, iters)
+        cpu = {"value": cb["all"][0], "unit": "ms/frame", "cores": cb["all"][1], "kind": "port",
+               "value_1_thread": cb["one"][0], "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+               "iterations_per_level": cb["all"][2],
+               "sample": "the benchmark's frame step (640x480 front end, 3 levels of voxel / normals / gradient, "
+                         "3-level colored ICP onto the previous frame, reference stopping rule) as a multithreaded "
+                         "FP32 C++ program (oracle/cpu_pipeline.cpp): median of 20 steps after 3 warm-ups"}
+        log(f"cpu baseline {cb['all'][0]:.1f} ms/frame at {cb['all'][1]} threads, {cb['one'][0]:.1f} ms at 1 thread")
+
+    fixed = None
+    if not args.no_fixed:
+        fixed = run_fixed(g, ctx, cam, bil, ptrs, levels, iters, args.steps, not args.no_graph)
+        log(f"fixed {iters} iterations per level: {fixed['ms_per_frame']:.3f} ms/frame; ICP us per iteration "
+            f"by level {[round(x, 1) for x in fixed['icp_us_per_iteration_per_level']]}")
+
+    seq = None
+    if not args.no_sequence:
+        seq = run_sequence(g, ctx, cam, bil, levels, iters, W, H, 8, max(2, args.steps // 8), not args.no_graph)
+        log(f"sequence of {seq['frames']} distinct frames: {seq['ms_per_frame']:.3f} ms/frame "
+            f"({seq['graph_captures']} captures, {seq['graph_node_updates']} node updates); rotation error vs "
+            f"truth median {seq['rotation_err_vs_truth_rad_median']:.2e} rad")
+
+    h2d = h2d_upload(td, tc)
+    h2d["pcie_inclusive_ms_per_frame"] = ms + h2d["pageable"]
+    log(f"upload of one frame ({h2d['bytes']} B): {h2d['pinned']:.3f} ms pinned, {h2d['pageable']:.3f} ms pageable")
 
     batch = None
     if not args.no_batch and args.batch_pairs > 0:
@@ -294,8 +430,10 @@ def main():
                    "width": W, "height": H, "levels": list(levels), "max_iters_per_level": iters,
                    "iterations_per_level": iters_seen[-2:] if iters_seen else None,
                    "hip_graph": not args.no_graph, "latency_ms_per_frame_with_readback": lat_ms,
+                   "icp_us_per_iteration_per_level": it_us,
                    "points_full": full_n, "points_per_level": sizes},
-        "roofline": roof, "cpu_baseline": cpu, "batch": batch, "clustering": clustering, "stages": stages,
+        "roofline": roof, "cpu_baseline": cpu, "fixed_iterations": fixed, "sequence": seq, "h2d": h2d,
+        "batch": batch, "clustering": clustering, "stages": stages,
     }
     print(json.dumps(out), flush=True)
 

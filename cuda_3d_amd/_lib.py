@@ -69,7 +69,8 @@ class IcpParams(C.Structure):
 class IcpReport(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32),
                 ("solver_failures", C.c_int32), ("n_correspondences", C.c_int32),
-                ("rmse", C.c_double), ("fitness", C.c_double), ("latency_us", C.c_double)]
+                ("rmse", C.c_double), ("fitness", C.c_double), ("latency_us", C.c_double),
+                ("active_us", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -138,6 +139,7 @@ _SIGS = {
     "gca_pipeline_set_graph": (i32, [vp, i32]),
     "gca_pipeline_ctx": (i32, [vp, P(vp)]),
     "gca_pipeline_sizes": (i32, [vp, i32, P(i64), vp]),
+    "gca_pipeline_graph_stats": (i32, [vp, P(i64), P(i64)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)

@@ -522,7 +522,7 @@ int32_t gca_ctx_set_tuning(gca_ctx c, int32_t key, int32_t value)
         return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_ICP_MAX_UNITS && value > 1024) return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_ICP_PERSIST_BLOCKS && value > 65536) return GCA_ERR_INVALID_VALUE;
-    if (key == GCA_TUNE_ICP_LAUNCH && value > 2) return GCA_ERR_INVALID_VALUE;
+    if (key == GCA_TUNE_ICP_LAUNCH && value > 3) return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_NORMALS_LANES && !pow2(value, 64)) return GCA_ERR_INVALID_VALUE;
     if (c->tune[key] != value) ++c->tune_version;
     c->tune[key] = value;

@@ -359,6 +359,66 @@ static int32_t build_lut(gca_ctx ctx, const gca_bilateral *f, FrontParams *fp)
     return GCA_OK;
 }
 
+struct FrontLaunch {
+    dim3 grid, block;
+    unsigned lds = 0;
+    // k_rgbd_pixels' arguments, in order
+    const uint16_t *depth = nullptr;
+    const uint8_t *bgr = nullptr;
+    const double *lut = nullptr;
+    FrontParams fp{};
+    float4 *st_xyzi = nullptr, *st_rgb = nullptr;
+    uint8_t *st_valid = nullptr;
+    uint32_t *bcnt = nullptr;
+    CloudHdr *hdr = nullptr;
+    uint32_t *clear_word = nullptr;
+    int n_ws = 0, n_wr = 0;
+    bool valid = false;
+};
+
+FrontLaunch *front_launch_new() { return new (std::nothrow) FrontLaunch(); }
+
+void front_launch_free(FrontLaunch *f) { delete f; }
+
+hipGraphNode_t front_launch_node(const FrontLaunch *f, hipGraph_t g)
+{
+    if (!f || !f->valid || !g) return nullptr;
+    size_t n = 0;
+    if (hipGraphGetNodes(g, nullptr, &n) != hipSuccess || n == 0) return nullptr;
+    std::vector<hipGraphNode_t> nodes(n);
+    if (hipGraphGetNodes(g, nodes.data(), &n) != hipSuccess) return nullptr;
+    hipGraphNode_t found = nullptr;
+    for (size_t k = 0; k < n; ++k) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(nodes[k], &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams kp{};
+        if (hipGraphKernelNodeGetParams(nodes[k], &kp) != hipSuccess) continue;
+        if (kp.func == reinterpret_cast<void *>(&k_rgbd_pixels)) {
+            if (found) return nullptr;  // more than one: not the graph of one pipeline step
+            found = nodes[k];
+        }
+    }
+    return found;
+}
+
+hipError_t front_launch_update(FrontLaunch *f, hipGraphExec_t exec, hipGraphNode_t node, const uint16_t *depth,
+                               const uint8_t *bgr)
+{
+    if (!f || !f->valid || !exec || !node) return hipErrorInvalidValue;
+    f->depth = depth;
+    f->bgr = bgr;
+    void *args[] = {&f->depth, &f->bgr, &f->lut, &f->fp, &f->st_xyzi, &f->st_rgb, &f->st_valid,
+                    &f->bcnt, &f->hdr, &f->clear_word, &f->n_ws, &f->n_wr};
+    hipKernelNodeParams kp{};
+    kp.func = reinterpret_cast<void *>(&k_rgbd_pixels);
+    kp.gridDim = f->grid;
+    kp.blockDim = f->block;
+    kp.sharedMemBytes = f->lds;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    return hipGraphExecKernelNodeSetParams(exec, node, &kp);
+}
+
 int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr, const gca_camera *cam,
                          const gca_bilateral *bil, float tmin, float tmax, gca_cloud out, uint32_t *clear_word)
 {
@@ -395,6 +455,10 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
     const int n_wr = stage ? fp.wr_len[0] : 0;
     const size_t lds = sizeof(double) * (size_t)(n_ws + n_wr) + sizeof(uint16_t) * (size_t)(2 * R + 1) * (kRowSeg + 2 * R);
     StageTimer timer(ctx, GCA_STAGE_FRONTEND);
+    if (FrontLaunch *r = ctx->front_rec) {
+        *r = FrontLaunch{grid, dim3(kRowSeg), (unsigned)lds, depth, bgr, (const double *)ctx->lut, fp, st_xyzi,
+                         st_rgb, st_valid, bcnt, out->hdr, clear_word, n_ws, n_wr, true};
+    }
     hipLaunchKernelGGL(k_rgbd_pixels, grid, dim3(kRowSeg), lds, ctx->stream, depth, bgr,
                        (const double *)ctx->lut, fp, st_xyzi, st_rgb, st_valid, bcnt, out->hdr, clear_word, n_ws,
                        n_wr);

@@ -83,6 +83,8 @@ struct GridCount {
     uint32_t *cell_of;
 };
 
+struct FrontLaunch;  // a recorded front-end launch (pipeline graph updates)
+
 }  // namespace gca
 
 struct gca_ctx_s {
@@ -124,6 +126,9 @@ struct gca_ctx_s {
     // being captured into a graph)
     bool prof_on = false;
     bool capturing = false;
+    // set by a pipeline while it captures a step: launch_from_rgbd records its pixel kernel's
+    // launch there (see FrontLaunch)
+    gca::FrontLaunch *front_rec = nullptr;
     gca_ctx_s *prof_owner = nullptr;  // stage records go to this ctx's profiler when set
     struct ProfRec {
         int stage;
@@ -180,6 +185,18 @@ inline GridDesc query_desc(const Grid &G, float radius)
     d.radius = radius;
     return d;
 }
+
+// The launch of the front end's pixel kernel (the one kernel that reads the frame buffers),
+// recorded while a pipeline step is captured, so that the instantiated graph can be pointed at
+// other frame buffers by updating that kernel node alone.
+struct FrontLaunch;
+FrontLaunch *front_launch_new();
+void front_launch_free(FrontLaunch *f);
+// The recorded launch's kernel node in g, or nullptr.
+hipGraphNode_t front_launch_node(const FrontLaunch *f, hipGraph_t g);
+// exec's instance of that node reads depth / bgr from now on (launches already made keep theirs).
+hipError_t front_launch_update(FrontLaunch *f, hipGraphExec_t exec, hipGraphNode_t node, const uint16_t *depth,
+                               const uint8_t *bgr);
 
 // Kernel launchers (one translation unit each) -------------------------------------------
 // clear_word (optional): a device word the first kernel sets to zero (a caller's per-step flag).

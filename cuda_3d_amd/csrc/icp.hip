@@ -73,8 +73,11 @@ struct IcpState {
     int32_t iter, max_iter, done, converged, solver_failures, n_src;
     int32_t G, nvb, n_tgt, next;  // lanes per query, work units, target size, next level (-1)
     long long t_init, t_done;     // device wall clock at set-up and when the state finished
+    long long t_start;            // ... and when its first iteration's work started (0 = not yet)
     QWord q[kSubQ];                        // sub-queue x holds units x, x + kSubQ, ...
     alignas(256) uint32_t ticket;          // units completed in the current epoch
+    QWord bar[kSubQ];                      // resident launch: arrivals of workgroup group x (b % kSubQ)
+    alignas(256) uint32_t bar_top;         // ... and of completed groups
     alignas(256) uint32_t pad_end;
 };
 
@@ -869,6 +872,7 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
             const bool ok = e < kEpochWait && c < subq_units(refresh ? s_nvb : st->nvb, x);
             s_vb = ok ? x + kSubQ * (int)c : -1;
             s_epoch = e;
+            if (ok && e == 0 && c == 0 && x == 0) st->t_start = wall_clock64();  // unit 0 of the first epoch
         } else if (threadIdx.x >= kWave && threadIdx.x < kWave + 12) {
             T[threadIdx.x - kWave] = load_coherent(&st->T[threadIdx.x - kWave]);
         }
@@ -907,11 +911,196 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_iter(const 
     const int vb = (int)blockIdx.x - unit_first[p];
     const uint32_t epoch = (uint32_t)st->iter;
     if (threadIdx.x < 12) T[threadIdx.x] = st->T[threadIdx.x];
+    if (threadIdx.x == 0 && vb == 0 && epoch == 0) st->t_start = wall_clock64();
     __syncthreads();
     const PairDesc &P = pairs[p];
     const int nvb = st->nvb;
     icp_unit<MODE>(P, st->n_src, st->G, nvb, st->n_tgt, vb, epoch > 0, T, sys);
     unit_commit<MODE>(P, states, p, vb, nvb, epoch, sys);
+}
+
+// ---- resident launch: one state, every iteration, no scheduler ---------------------------------
+// Workgroup b < nvb runs unit b of every iteration.  After its unit it publishes the partial
+// (write-through) and arrives at the iteration's barrier: a release increment of its group's
+// counter (group b % kSubQ, each counter on its own line), and the arrival that completes a group
+// increments the top counter.  Every workgroup then waits for the top counter, reads all nvb
+// partials with coherent loads, sums them in the finisher's fixed order (sum_partials), solves and
+// updates the pose itself: every workgroup computes the same bits, so no pose is handed over
+// inside the launch, and the result is bit-identical to the persistent and per-iteration launches
+// of the same shape.  Partials alternate between two banks by iteration parity: a bank is
+// overwritten only after every workgroup has passed the barrier of the iteration after the one
+// that read it.  The workgroups keep their source points and previous matches from one
+// iteration to the next only through memory they alone touch (the matched slots).
+// Progress needs all nvb workgroups resident at once, so the launch is used only when nvb is at
+// most two per CU (the occupancy allows four); kernels running beside it only delay it, as their
+// workgroups end and free the slots the rest need.  A wait beyond the spin limit flags the error
+// word instead of hanging.
+__device__ __forceinline__ uint32_t load_coherent_u32(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// sum_partials over coherent loads (partials written in this launch by other workgroups, read
+// without an acquire so that the target data cached in L2 survives).  Same order and arithmetic.
+__device__ __forceinline__ void sum_partials_coherent(const double *partials, int n, double *sys, bool naive)
+{
+    __shared__ double part[8][kPartStride];
+    const int v = threadIdx.x % kPartStride, grp = threadIdx.x / kPartStride;
+    const bool f32 = naive && v < kNSys - 1;
+    if (v < kNSys) {
+        double a = 0.0;
+        float af = 0.f;
+        for (int b0 = grp; b0 < n; b0 += 8 * kSumBatch) {
+            double x[kSumBatch];
+#pragma unroll
+            for (int u = 0; u < kSumBatch; ++u) {
+                const int b = b0 + 8 * u;
+                x[u] = b < n ? load_coherent(partials + (int64_t)b * kPartStride + v) : 0.0;
+            }
+            if (f32) {
+#pragma unroll
+                for (int u = 0; u < kSumBatch; ++u) af += (float)x[u];
+            } else {
+#pragma unroll
+                for (int u = 0; u < kSumBatch; ++u) a += x[u];
+            }
+        }
+        part[grp][v] = f32 ? (double)af : a;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNSys) {
+        double a = 0.0;
+        float af = 0.f;
+        for (int g = 0; g < 8; ++g) {
+            if (f32) af += (float)part[g][threadIdx.x];
+            else a += part[g][threadIdx.x];
+        }
+        sys[threadIdx.x] = f32 ? (double)af : a;
+    }
+    __syncthreads();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_resident(const PairDesc *__restrict__ pairs,
+                                                                               IcpState *states, int s, int refresh,
+                                                                               uint32_t *err)
+{
+    __shared__ double Tc[16], Tn[16], dT[16], xi[6], trig[6];
+    __shared__ double sys[kPartStride];
+    __shared__ int f_ok, f_done, f_stop;
+    __shared__ double f_crot, f_ct2;
+    __shared__ int f_max, f_next, f_iter0, f_fail0;
+    __shared__ int s_nsrc, s_ntgt, s_G, s_nvb;
+    IcpState *st = states + s;
+    const PairDesc &P = pairs[s];
+    const int t = threadIdx.x;
+    if (t == 0) {
+        int ns, nt, G, nvb;
+        if (refresh) {
+            ns = P.src_hdr->n;
+            nt = P.tgt_hdr->n;
+            G = icp_lanes_for(P.lanes, ns);
+            nvb = icp_units_for(ns, G, P.unit_cap);
+        } else {
+            ns = st->n_src;
+            nt = st->n_tgt;
+            G = st->G;
+            nvb = st->nvb;
+        }
+        s_nsrc = ns;
+        s_ntgt = nt;
+        s_G = G;
+        s_nvb = nvb;
+        f_max = st->max_iter;
+        f_next = st->next;
+        f_iter0 = st->iter;
+        f_fail0 = st->solver_failures;
+        f_crot = st->conv_rot;
+        f_ct2 = st->conv_t2;
+        f_stop = st->done;
+        if (refresh && blockIdx.x == 0) {
+            st->n_src = ns;
+            st->n_tgt = nt;
+            st->G = G;
+            st->nvb = nvb;
+        }
+    } else if (t >= 64 && t < 80) {
+        Tc[t - 64] = st->T[t - 64];
+    }
+    __syncthreads();
+    const int nvb = s_nvb, vb = (int)blockIdx.x;
+    if (vb >= nvb || f_stop) return;
+    if (vb == 0 && t == 0) st->t_start = wall_clock64();
+    __builtin_amdgcn_s_setprio(3);  // a chain of dependent round trips; see k_icp_persist
+    const int grp = vb % kSubQ;
+    const uint64_t grp_size = subq_units(nvb, grp);
+    const uint32_t n_groups = (uint32_t)min(nvb, kSubQ);
+    int iter = f_iter0, fails = f_fail0;
+    for (uint32_t e = 0;; ++e) {
+        double *bank = P.partials + (size_t)(e & 1u) * (size_t)P.unit_cap * kPartStride;
+        icp_unit<MODE>(P, s_nsrc, s_G, nvb, s_ntgt, vb, iter > 0, Tc, sys);
+        if (t < kNSys) store_coherent(&bank[(int64_t)vb * kPartStride + t], sys[t]);
+        if (t == 0) {
+            const unsigned long long old =
+                __hip_atomic_fetch_add(&st->bar[grp].w, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (old + 1 == grp_size * (e + 1))
+                __hip_atomic_fetch_add(&st->bar_top, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t want = n_groups * (e + 1);
+            uint32_t spins = 0;
+            while (load_coherent_u32(&st->bar_top) < want) {
+                if (++spins > kSpinLimit) {
+                    atomicOr(err, 2u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            f_stop = spins > kSpinLimit ? 1 : 0;
+        }
+        __syncthreads();
+        if (f_stop) return;
+        sum_partials_coherent(bank, nvb, sys, MODE == GCA_ACCUM_NAIVE_FP32);
+        if (t == 0) f_ok = solve6(sys, sys + 21, xi) ? 1 : 0;
+        __syncthreads();
+        if (t < 3) {
+            double sn, cs;
+            sincos(xi[t], &sn, &cs);
+            trig[2 * t] = sn;
+            trig[2 * t + 1] = cs;
+        }
+        __syncthreads();
+        if (t < 16) dT[t] = euler_elem(t, trig, xi);
+        __syncthreads();
+        if (t < 16) {
+            const int i = t >> 2, j = t & 3;
+            Tn[t] = ((dT[4 * i] * Tc[j] + dT[4 * i + 1] * Tc[4 + j]) + dT[4 * i + 2] * Tc[8 + j]) + dT[4 * i + 3] * Tc[12 + j];
+        } else if (t == 64) {
+            const double cos_t = 0.5 * (dT[0] + dT[5] + dT[10] - 1.0);
+            const double tt = (dT[3] * dT[3] + dT[7] * dT[7]) + dT[11] * dT[11];
+            const bool conv = cos_t >= 1.0 - f_crot && tt <= f_ct2;
+            f_done = (conv || iter + 1 >= f_max) ? (conv ? 2 : 1) : 0;
+        }
+        __syncthreads();
+        ++iter;
+        if (!f_ok) ++fails;
+        if (t < 16) Tc[t] = Tn[t];
+        const int done = f_done;
+        if (done && vb == 0) {
+            if (t < 16) {
+                st->T[t] = Tn[t];
+                if (f_next >= 0) states[f_next].T[t] = Tn[t];
+            } else if (t >= 32 && t < 32 + kNSys) {
+                st->sys[t - 32] = sys[t - 32];
+            } else if (t == 64) {
+                st->iter = iter;
+                st->solver_failures = fails;
+                st->converged = done == 2 ? 1 : 0;
+                st->done = 1;
+                st->t_done = wall_clock64();
+            }
+        }
+        __syncthreads();  // Tc is read by the next unit; f_done is rewritten by the next finish
+        if (done) return;
+    }
 }
 
 // Gauss-Newton system for given correspondences (no solve): writes sys into st->sys.
@@ -1125,10 +1314,15 @@ __global__ void k_state_init(IcpState *states, const PairDesc *__restrict__ pair
     s.done = I.max_iter <= 0 ? 1 : 0;
     s.t_init = wall_clock64();
     s.t_done = s.t_init;
+    s.t_start = 0;
     const unsigned long long w0 = I.max_iter <= 0 ? ((unsigned long long)kEpochDone << 32)
                                                   : (I.open ? 0ull : ((unsigned long long)kEpochWait << 32));
-    for (int x = 0; x < kSubQ; ++x) s.q[x].w = w0;
+    for (int x = 0; x < kSubQ; ++x) {
+        s.q[x].w = w0;
+        s.bar[x].w = 0ull;
+    }
     s.ticket = 0u;
+    s.bar_top = 0u;
 }
 
 // State of the single-pair kernels (Gauss-Newton build, residuals, objective).
@@ -1310,7 +1504,7 @@ static void states_describe(int n_pairs, int n_levels, const gca_cloud *srcs, co
         for (int l = 0; l < n_levels; ++l) {
             const int s = k * n_levels + l;
             const gca_icp_params &pl = params[l];
-            P[s] = make_pair(srcs[s], tgts[s], partials + (int64_t)s * unit_cap * kPartStride, nnprev + nn_off[s],
+            P[s] = make_pair(srcs[s], tgts[s], partials + (int64_t)s * 2 * unit_cap * kPartStride, nnprev + nn_off[s],
                              pl.lambda_geometric, pl.max_correspondence_distance);
             memcpy(P[s].lanes, lanes, sizeof(int32_t) * 3);
             P[s].unit_cap = unit_cap;
@@ -1349,6 +1543,31 @@ static void launch_persist_any(gca_ctx ctx, int mode, const PairDesc *pairs, Icp
         launch_persist_mode<GCA_ACCUM_COMPENSATED>(ctx, pairs, states, s_first, n_states, refresh, err);
 }
 
+// The resident launch (k_icp_resident) over state s: one workgroup per unit, so it is used only
+// while a state's units fit two per CU (unit_cap <= 2 CUs).  Launch strategy 0 (default) takes it
+// for one-pair calls, 3 for every call.
+static bool resident_fits(gca_ctx ctx, int32_t unit_cap) { return unit_cap <= 2 * std::max(ctx->n_cu, 1); }
+
+static bool use_resident(gca_ctx ctx, int n_pairs, int32_t unit_cap)
+{
+    const int32_t L = ctx->tune[GCA_TUNE_ICP_LAUNCH];
+    if (!resident_fits(ctx, unit_cap)) return false;
+    return L == 3 || (L == 0 && n_pairs == 1);
+}
+
+static void launch_resident_any(gca_ctx ctx, int mode, const PairDesc *pairs, IcpState *states, int s, int refresh,
+                                int32_t unit_cap, uint32_t *err)
+{
+    StageTimer timer(ctx, GCA_STAGE_ICP);
+    const dim3 g((unsigned)unit_cap), b(kIcpBlock);
+    if (mode == GCA_ACCUM_NAIVE_FP32)
+        hipLaunchKernelGGL(k_icp_resident<GCA_ACCUM_NAIVE_FP32>, g, b, 0, ctx->stream, pairs, states, s, refresh, err);
+    else if (mode == GCA_ACCUM_FP64)
+        hipLaunchKernelGGL(k_icp_resident<GCA_ACCUM_FP64>, g, b, 0, ctx->stream, pairs, states, s, refresh, err);
+    else
+        hipLaunchKernelGGL(k_icp_resident<GCA_ACCUM_COMPENSATED>, g, b, 0, ctx->stream, pairs, states, s, refresh, err);
+}
+
 // Poses and reports from a host copy of the states.
 static void states_report(gca_ctx ctx, const IcpState *S, int n_pairs, int n_states, const std::vector<int32_t> &tail,
                           const double *T0, double *T_out, gca_icp_report *reps)
@@ -1369,6 +1588,7 @@ static void states_report(gca_ctx ctx, const IcpState *S, int n_pairs, int n_sta
         r.rmse = n > 0 ? S[s].sys[27] / n : 0.0;  // color_icp.cu:231
         r.fitness = n > 0 ? S[s].sys[28] / n : 0.0;
         r.latency_us = ticks_per_us > 0 ? (double)(S[s].t_done - t0) / ticks_per_us : 0.0;
+        r.active_us = ticks_per_us > 0 && S[s].t_start > 0 ? (double)(S[s].t_done - S[s].t_start) / ticks_per_us : 0.0;
     }
 }
 
@@ -1394,7 +1614,7 @@ static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_clou
         }
     }
     IcpWork w;
-    GCA_TRY(icp_work(ctx, n_states, n_pairs, (int64_t)n_states * unit_cap, nn_off[n_states], table_units, w));
+    GCA_TRY(icp_work(ctx, n_states, n_pairs, (int64_t)n_states * 2 * unit_cap, nn_off[n_states], table_units, w));
     std::vector<PairDesc> P;
     std::vector<StateInit> I;
     std::vector<int32_t> tail;
@@ -1408,7 +1628,12 @@ static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_clou
 
     const int mode = params[0].accumulate_mode;
     std::vector<int32_t> us, uf(n_states, 0);
-    if (!per_iter) {
+    if (!per_iter && use_resident(ctx, n_pairs, unit_cap)) {
+        // one launch per state, pair by pair and coarse to fine: a state's launch hands its pose to
+        // the next level of its chain before that level's launch starts
+        for (int s = 0; s < n_states; ++s) launch_resident_any(ctx, mode, w.pairs, w.states, s, 0, unit_cap, w.err);
+        GCA_HIP(hipGetLastError());
+    } else if (!per_iter) {
         launch_persist_any(ctx, mode, w.pairs, w.states, n_states, w.err);
         GCA_HIP(hipGetLastError());
     } else {
@@ -1452,6 +1677,7 @@ static int32_t run_states(gca_ctx ctx, int n_pairs, int n_levels, const gca_clou
 // ---- registration plans: set up once, launched many times without host round trips --------
 struct IcpPlan {
     int n_pairs = 0, n_levels = 0, n_states = 0, mode = 0;
+    int32_t unit_cap = 0;
     IcpState *states = nullptr;
     double *partials = nullptr;
     int32_t *nnprev = nullptr;
@@ -1495,6 +1721,7 @@ int32_t icp_plan_create(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud 
     p->n_levels = n_levels;
     p->n_states = n_states;
     p->mode = params[0].accumulate_mode;
+    p->unit_cap = unit_cap;
     p->T0h.assign(T0, T0 + 16 * n_pairs);
     auto fail = [&](hipError_t e) {
         icp_plan_destroy(p);
@@ -1502,7 +1729,7 @@ int32_t icp_plan_create(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud 
     };
     hipError_t e;
     if ((e = hipMalloc(&p->states, sizeof(IcpState) * n_states)) != hipSuccess) return fail(e);
-    if ((e = hipMalloc(&p->partials, sizeof(double) * kPartStride * n_states * unit_cap)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->partials, sizeof(double) * kPartStride * n_states * 2 * unit_cap)) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&p->nnprev, sizeof(int32_t) * std::max<int64_t>(nn_off[n_states], 1))) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&p->pairs, sizeof(PairDesc) * n_states)) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&p->init, sizeof(StateInit) * n_states)) != hipSuccess) return fail(e);
@@ -1539,7 +1766,10 @@ int32_t icp_plan_init(gca_ctx ctx, IcpPlan *p)
 int32_t icp_plan_persist(gca_ctx ctx, IcpPlan *p, int s)
 {
     if (s < 0 || s >= p->n_states) return GCA_ERR_INVALID_VALUE;
-    launch_persist_any(ctx, p->mode, p->pairs, p->states, 1, p->err, s, 1);
+    if (use_resident(ctx, p->n_pairs, p->unit_cap))
+        launch_resident_any(ctx, p->mode, p->pairs, p->states, s, 1, p->unit_cap, p->err);
+    else
+        launch_persist_any(ctx, p->mode, p->pairs, p->states, 1, p->err, s, 1);
     GCA_HIP(hipGetLastError());
     return GCA_OK;
 }
@@ -1805,7 +2035,8 @@ int32_t gca_multiscale_icp(gca_ctx ctx, gca_cloud src_full, gca_cloud tgt_full, 
         prm[l] = L.icp;
         gr[l] = L.gradient_radius;
     }
-    // all levels in one launch, each starting from the pose the previous one ends with
+    // the levels in order, each starting from the pose the previous one ends with (one resident
+    // launch per level, or one persistent launch over the chain)
     std::vector<gca_icp_report> rep(n_levels);
     std::vector<gca_cloud> S(cs, cs + n_levels), Tg(ct, ct + n_levels);
     GCA_TRY(run_states(ctx, 1, n_levels, S.data(), Tg.data(), prm, gr, T0 ? T0 : I, T_out, rep.data()));

@@ -66,7 +66,11 @@ struct gca_pipeline_s {
     int eager_since_reset = 0;  // eager steps since plans were (re)built
     uint64_t tune_version = 0;
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};       // kept for its front-end node
+    gca::FrontLaunch *front[2] = {nullptr, nullptr};  // the front end's recorded launch per parity
+    hipGraphNode_t front_node[2] = {nullptr, nullptr};
     const void *cap_depth[2] = {nullptr, nullptr}, *cap_bgr[2] = {nullptr, nullptr};
+    int64_t captures = 0, updates = 0;
 };
 
 namespace {
@@ -75,7 +79,10 @@ void drop_graphs(gca_pipeline p)
 {
     for (int s = 0; s < 2; ++s) {
         if (p->exec[s]) hipGraphExecDestroy(p->exec[s]);
+        if (p->graph[s]) hipGraphDestroy(p->graph[s]);
         p->exec[s] = nullptr;
+        p->graph[s] = nullptr;
+        p->front_node[s] = nullptr;
         p->cap_depth[s] = p->cap_bgr[s] = nullptr;
     }
 }
@@ -169,7 +176,9 @@ int32_t capture(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
 {
     gca_ctx ctx = p->ctx;
     hipGraph_t g = nullptr;
+    if (!p->front[slot] && !(p->front[slot] = front_launch_new())) return GCA_ERR_OUT_OF_MEMORY;
     ctx->capturing = true;
+    ctx->front_rec = p->front[slot];
     for (int l = 0; l < p->nl; ++l) p->lctx[l]->capturing = true;
     hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
     int32_t st = e == hipSuccess ? enqueue(p, slot, depth, bgr) : hip_status(e);
@@ -178,6 +187,7 @@ int32_t capture(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         if (st == GCA_OK && e2 != hipSuccess) st = hip_status(e2);
     }
     ctx->capturing = false;
+    ctx->front_rec = nullptr;
     for (int l = 0; l < p->nl; ++l) p->lctx[l]->capturing = false;
     if (st == GCA_OK && !g) st = GCA_ERR_DEVICE;
     hipGraphExec_t ex = nullptr;
@@ -185,12 +195,20 @@ int32_t capture(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         if (e != hipSuccess) st = hip_status(e);
     }
-    if (g) hipGraphDestroy(g);
-    if (st != GCA_OK) return st;
+    if (st != GCA_OK) {
+        if (g) hipGraphDestroy(g);
+        return st;
+    }
     if (p->exec[slot]) hipGraphExecDestroy(p->exec[slot]);
+    if (p->graph[slot]) hipGraphDestroy(p->graph[slot]);
     p->exec[slot] = ex;
+    p->graph[slot] = g;
+    // the pixel kernel's node: a later step with other frame buffers updates this node instead
+    // of capturing again (nullptr: such a step captures)
+    p->front_node[slot] = front_launch_node(p->front[slot], g);
     p->cap_depth[slot] = depth;
     p->cap_bgr[slot] = bgr;
+    ++p->captures;
     return GCA_OK;
 }
 
@@ -233,6 +251,7 @@ int32_t gca_pipeline_destroy(gca_pipeline p)
         if (p->lctx[l]) hipStreamSynchronize(p->lctx[l]->stream);
     drop_plans(p);
     for (int s = 0; s < 2; ++s) {
+        front_launch_free(p->front[s]);
         if (p->full[s]) gca_cloud_destroy(p->full[s]);
         for (int l = 0; l < p->nl; ++l)
             if (p->lvc[s][l]) gca_cloud_destroy(p->lvc[s][l]);
@@ -352,8 +371,22 @@ int32_t gca_pipeline_step(gca_pipeline p, const uint16_t *depth_dev, const uint8
     // step of each frame parity with its plan in place
     const bool graph = p->use_graph && p->frames >= 1 && p->eager_since_reset >= 2 && p->plan[0] && p->plan[1];
     if (graph) {
-        if (!p->exec[slot] || p->cap_depth[slot] != depth_dev || p->cap_bgr[slot] != bgr_dev)
+        if (!p->exec[slot]) {
             GCA_TRY(capture(p, slot, depth_dev, bgr_dev));
+        } else if (p->cap_depth[slot] != depth_dev || p->cap_bgr[slot] != bgr_dev) {
+            // new frame buffers: point the pixel kernel's node at them (capture again only if
+            // the update is refused)
+            if (p->front_node[slot] &&
+                front_launch_update(p->front[slot], p->exec[slot], p->front_node[slot], depth_dev, bgr_dev) ==
+                    hipSuccess) {
+                p->cap_depth[slot] = depth_dev;
+                p->cap_bgr[slot] = bgr_dev;
+                ++p->updates;
+            } else {
+                (void)hipGetLastError();
+                GCA_TRY(capture(p, slot, depth_dev, bgr_dev));
+            }
+        }
         GCA_HIP(hipGraphLaunch(p->exec[slot], p->ctx->stream));
     } else {
         GCA_TRY(enqueue(p, slot, depth_dev, bgr_dev));
@@ -382,6 +415,14 @@ int32_t gca_pipeline_result(gca_pipeline p, double T[16], gca_icp_report *report
     }
     GCA_TRY(icp_plan_results(p->ctx, p->plan[p->last], T, reports));
     *valid = 1;
+    return GCA_OK;
+}
+
+int32_t gca_pipeline_graph_stats(gca_pipeline p, int64_t *captures, int64_t *node_updates)
+{
+    if (!p || !captures || !node_updates) return GCA_ERR_INVALID_VALUE;
+    *captures = p->captures;
+    *node_updates = p->updates;
     return GCA_OK;
 }
 

@@ -74,6 +74,12 @@ class FramePipeline:
         """Gauss-Newton iterations of each level in the last registration."""
         return [r.iterations for r in self.result()[1]]
 
+    def graph_stats(self):
+        """(graph captures, front-end node updates) so far."""
+        cap, upd = C.c_int64(), C.c_int64()
+        L.check(L.lib.gca_pipeline_graph_stats(self.handle, C.byref(cap), C.byref(upd)), "gca_pipeline_graph_stats")
+        return int(cap.value), int(upd.value)
+
     def sizes(self, slot: int):
         full = C.c_int64()
         lv = np.zeros(len(self.levels), np.int64)

@@ -187,3 +187,19 @@ def sample_surface_pair(seed: int, n_points: int, max_angle_deg=15.0, max_trans=
         out.append((xyz, np.ascontiguousarray(rgb)))
     (sx, sr), (tx, tr) = out
     return sx, sr, tx, tr, G
+
+
+def frame_sequence(seed: int, n_frames: int, width: int, height: int, max_angle_deg=10.0, max_trans=0.1):
+    """n_frames distinct frames of one seeded scene, each from its own camera pose.  The poses
+    scatter around the identity by at most half the motion bounds each, so any two frames (in
+    particular consecutive ones, and the last and the first) differ by at most max_angle_deg /
+    max_trans.  Returns ([(depth, bgr)], [pose], camera model); pose k maps frame k's camera
+    coordinates into the first frame's world."""
+    cam = camera_for(width, height)
+    scene = make_scene(seed)
+    frames, poses = [], []
+    for k in range(n_frames):
+        G = np.eye(4) if k == 0 else random_rigid(seed + 104729 * (k + 1), 0.5 * max_angle_deg, 0.5 * max_trans)
+        frames.append(render(scene, cam, G, seed + 1 + k))
+        poses.append(G)
+    return frames, poses, cam

@@ -118,6 +118,9 @@ typedef struct gca_icp_report {
     double fitness;          /* n_correspondences / N_src */
     double latency_us;       /* device clock from the call's set-up until this registration
                                 finished (per-pair latency inside a batched launch) */
+    double active_us;        /* device clock from the start of this registration's first
+                                iteration until it finished (active_us / iterations: the time
+                                of one Gauss-Newton iteration) */
 } gca_icp_report;
 
 /* Per-level schedule of the multi-scale driver (coarse to fine). */
@@ -168,7 +171,10 @@ enum {
     GCA_TUNE_ICP_MAX_UNITS = 3,    /* cap on work units (FP64 partials) per pair and iteration, 1..1024 */
     GCA_TUNE_NORMALS_LANES = 4,    /* lanes per normal/gradient query: 1,2,4,8,16,32,64 */
     GCA_TUNE_ICP_PERSIST_BLOCKS = 5, /* workgroups of the persistent ICP launch (0 = 2 per CU), 1..65536 */
-    GCA_TUNE_ICP_LAUNCH = 6,       /* 0/1 = one persistent launch per call, 2 = one launch per iteration */
+    GCA_TUNE_ICP_LAUNCH = 6,       /* 0 = resident launches for one-pair calls, a persistent launch for
+                                      several pairs; 1 = one persistent launch per call; 2 = one launch
+                                      per iteration; 3 = one resident launch per pair and level.  Every
+                                      strategy gives the same bits for the same unit partition */
     GCA_TUNE_ICP_LANES_BATCH = 7,  /* lanes per correspondence query in calls with several pairs (0 = 1) */
     GCA_TUNE_COUNT = 8
 };
@@ -442,8 +448,12 @@ int32_t gca_pipeline_step(gca_pipeline pipeline, const uint16_t *depth_dev, cons
 int32_t gca_pipeline_result(gca_pipeline pipeline, double T[16], gca_icp_report *reports,
                             int32_t *valid);
 /* Replay each frame parity's step as a hipGraph (captured after one eager step per parity, and
- * again when the input pointers or the ctx tuning change). */
+ * again when the ctx tuning changes).  A step with other frame buffers than the graph's updates
+ * the one kernel node that reads them (hipGraphExecKernelNodeSetParams) instead of capturing. */
 int32_t gca_pipeline_set_graph(gca_pipeline pipeline, int32_t enable);
+/* Graph captures and front-end node updates since the pipeline was created (no reference
+ * counterpart: replay bookkeeping). */
+int32_t gca_pipeline_graph_stats(gca_pipeline pipeline, int64_t *captures, int64_t *node_updates);
 /* The pipeline's own ctx (its stream, stage profiler and tuning). */
 int32_t gca_pipeline_ctx(gca_pipeline pipeline, gca_ctx *ctx);
 /* Point counts of a frame slot (0 / 1; the last step used slot (frames - 1) % 2): the full

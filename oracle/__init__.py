@@ -58,7 +58,8 @@ class IcpReport(C.Structure):
     """gca_icp_report (include/gca/gca.h), which the oracle fills; latency_us stays 0 here."""
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32),
                 ("solver_failures", C.c_int32), ("n_correspondences", C.c_int32),
-                ("rmse", C.c_double), ("fitness", C.c_double), ("latency_us", C.c_double)]
+                ("rmse", C.c_double), ("fitness", C.c_double), ("latency_us", C.c_double),
+                ("active_us", C.c_double)]
 
 
 class LevelParams(C.Structure):

@@ -623,9 +623,10 @@ def uneven_pairs(ctx):
 
 
 def rep_key(r):
-    """Report fields that are results (latency_us is timing)."""
+    """Report fields that are results (latency_us and active_us are timing)."""
     d = r.as_dict()
     d.pop("latency_us")
+    d.pop("active_us")
     return d
 
 
@@ -685,17 +686,29 @@ def test_batch_matches_single_pairs(ctx, uneven_pairs):
 
 
 def test_launch_strategies_identical(ctx, uneven_pairs):
-    """One persistent launch and one launch per iteration run the same units and finish."""
+    """One persistent launch, one launch per iteration and one resident launch per pair (every
+    workgroup finishing each iteration itself) run the same units and the same finish."""
     import cuda_3d_amd as g
 
     srcs, tgts = uneven_pairs
     p = icp_p(g, 12, 0.05)
     out = {}
-    for launch in (1, 2):
+    for launch in (1, 2, 3):
+        print(f"[gpu-parity] launch strategy {launch}", flush=True)
         with tuned(ctx, icp_launch=launch):
             out[launch] = g.batch_icp(srcs, tgts, p, [np.eye(4)] * len(srcs))
-    np.testing.assert_array_equal(out[1][0], out[2][0])
-    assert [rep_key(r) for r in out[1][1]] == [rep_key(r) for r in out[2][1]]
+    for launch in (2, 3):
+        np.testing.assert_array_equal(out[1][0], out[launch][0])
+        assert [rep_key(r) for r in out[1][1]] == [rep_key(r) for r in out[launch][1]]
+    # one pair, three levels: resident launches level by level (the default) against one
+    # persistent launch over the chain
+    lv = [g.level_params(v, icp=icp_p(g, 10, 2.5 * v, conv=(1e-6, 1e-6))) for v in (0.04, 0.02)]
+    res = {}
+    for launch in (0, 1):
+        with tuned(ctx, icp_launch=launch):
+            res[launch] = g.multiscale_icp(srcs[1], tgts[1], lv)
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    assert [rep_key(r) for r in res[0][1]] == [rep_key(r) for r in res[1][1]]
 
 
 def test_batch_multiscale_matches_sequential(ctx, frame):
@@ -1084,6 +1097,37 @@ def test_pipeline_graph_replays_match_eager(ctx):
         # unrelated work on another ctx and stream between replays
         other.voxel_grid_down_sample(0.05)
         ctx.synchronize()
+    eager.close()
+    graph.close()
+
+
+def test_pipeline_graph_follows_new_frame_buffers(ctx):
+    """Frames arriving in four different device buffers (each frame of the pair in two copies,
+    used in turn): the replayed graphs follow the new buffers through a node update, so exactly
+    two captures happen (one per frame parity), and every pose equals the eager pipeline's."""
+    import cuda_3d_amd as g
+    from cuda_3d_amd.pipeline import FramePipeline
+
+    cam, dev, _, _ = _pipeline_frames(g, 320, 240, 23)
+    copies = [(d.clone(), c.clone()) for d, c in dev]
+    bufs = [dev[0], dev[1], copies[0], copies[1]]  # frame A, B, A, B at four addresses
+    bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
+    eager = FramePipeline(ctx, cam, bil, graph=False)
+    graph = FramePipeline(ctx, cam, bil, graph=True)
+    for k in range(14):
+        d, c = bufs[k % 4] if k < 8 else bufs[(k * 3) % 4]
+        eager.step(d.data_ptr(), c.data_ptr())
+        Te, re = eager.result()
+        graph.step(d.data_ptr(), c.data_ptr())
+        Tg, rg = graph.result()
+        if k == 0:
+            assert Te is None and Tg is None
+            continue
+        np.testing.assert_array_equal(Tg, Te)
+        assert [rep_key(r) for r in rg] == [rep_key(r) for r in re]
+    captures, updates = graph.graph_stats()
+    print(f"[gpu-parity] rotating frame buffers: {captures} captures, {updates} node updates", flush=True)
+    assert captures == 2 and updates >= 6
     eager.close()
     graph.close()
 
