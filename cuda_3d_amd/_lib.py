@@ -125,6 +125,7 @@ _SIGS = {
     "gca_compute_color_gradient": (i32, [vp, vp, f32, i32]),
     "gca_transform": (i32, [vp, vp, vp]),
     "gca_build_gauss_newton": (i32, [vp, vp, vp, vp, f32, vp, i32, vp]),
+    "gca_build_gauss_newton_noalloc": (i32, [vp, vp, vp, vp, f32, vp, i32, vp]),
     "gca_compute_residuals": (i32, [vp, vp, vp, vp, f32, vp, i32, vp, vp]),
     "gca_solve_gauss_newton": (i32, [vp, vp, vp, vp]),
     "gca_color_icp_align": (i32, [vp, vp, vp, P(IcpParams), f32, vp, vp, P(IcpReport)]),

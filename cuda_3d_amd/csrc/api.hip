@@ -475,6 +475,7 @@ int32_t gca_ctx_create(int32_t device, gca_ctx *out)
         return GCA_ERR_DEVICE;
     }
     *c->dev_err = 0u;
+    c->noalloc_scratch = noalloc_setup(device);  // this device's static scratch (no allocation)
     *out = c;
     return GCA_OK;
 }

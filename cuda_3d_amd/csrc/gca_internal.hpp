@@ -129,6 +129,9 @@ struct gca_ctx_s {
     // set by a pipeline while it captures a step: launch_from_rgbd records its pixel kernel's
     // launch there (see FrontLaunch)
     gca::FrontLaunch *front_rec = nullptr;
+    // the device address of this device's static scratch of the allocation-free entry points
+    // (gca::NoAllocScratch), set by gca_ctx_create
+    void *noalloc_scratch = nullptr;
     gca_ctx_s *prof_owner = nullptr;  // stage records go to this ctx's profiler when set
     struct ProfRec {
         int stage;
@@ -197,6 +200,17 @@ hipGraphNode_t front_launch_node(const FrontLaunch *f, hipGraph_t g);
 // exec's instance of that node reads depth / bgr from now on (launches already made keep theirs).
 hipError_t front_launch_update(FrontLaunch *f, hipGraphExec_t exec, hipGraphNode_t node, const uint16_t *depth,
                                const uint8_t *bgr);
+
+// Static device scratch of the allocation-free entry points: one per device, at the address
+// noalloc_setup resolves once per device (noalloc.hip); the library never allocates it.
+constexpr int kNoAllocMaxBlocks = 1024;  // partials of up to this many workgroups
+struct NoAllocScratch {
+    alignas(256) uint32_t ticket;                // last-block ticket, zero between calls
+    alignas(256) double partials[kNoAllocMaxBlocks * 32];
+};
+// Device address of `device`'s scratch (resolved on the first call for that device, with the
+// device current); nullptr if the runtime refuses.
+void *noalloc_setup(int device);
 
 // Kernel launchers (one translation unit each) -------------------------------------------
 // clear_word (optional): a device word the first kernel sets to zero (a caller's per-step flag).

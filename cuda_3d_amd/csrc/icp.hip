@@ -1103,15 +1103,16 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_resident(co
     }
 }
 
-// Gauss-Newton system for given correspondences (no solve): writes sys into st->sys.
+// Gauss-Newton system for given correspondences (no solve).  ticket: zero at the launch, left at
+// zero; out: the 29 sums (FP64).
 template <int MODE>
 __global__ __launch_bounds__(kIcpBlock) void k_build_gn(PairDesc P, const float4 *__restrict__ tgt_xyzi,
                                                         const int32_t *__restrict__ nn, double t00, double t01,
                                                         double t02, double t03, double t10, double t11,
                                                         double t12, double t13, double t20, double t21,
-                                                        double t22, double t23, float sl, float sc)
+                                                        double t22, double t23, float sl, float sc,
+                                                        uint32_t *ticket, double *out)
 {
-    IcpState *st = P.st;
     const double T[12] = {t00, t01, t02, t03, t10, t11, t12, t13, t20, t21, t22, t23};
     const int n = P.src_hdr->n;
     Acc<MODE> A;
@@ -1134,13 +1135,32 @@ __global__ __launch_bounds__(kIcpBlock) void k_build_gn(PairDesc P, const float4
     if (threadIdx.x < kNSys) P.partials[(int64_t)blockIdx.x * kPartStride + threadIdx.x] = sys[threadIdx.x];
     fence_release_agent();
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&st->ticket, 1u) == (uint32_t)(P.nblk - 1);
+    if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == (uint32_t)(P.nblk - 1);
     __syncthreads();
     if (!last) return;
     fence_acquire_agent();
     sum_partials(P.partials, P.nblk, sys, MODE == GCA_ACCUM_NAIVE_FP32);
-    if (threadIdx.x < kNSys) st->sys[threadIdx.x] = sys[threadIdx.x];
-    if (threadIdx.x == 0) st->ticket = 0u;
+    if (threadIdx.x < kNSys) out[threadIdx.x] = sys[threadIdx.x];
+    if (threadIdx.x == 0) *ticket = 0u;
+}
+
+__global__ void k_ticket_clear(uint32_t *ticket) { *ticket = 0u; }
+
+// k_build_gn over given partials (nblk workgroups' worth), ticket and output, on ctx's stream.
+static void launch_build_gn(gca_ctx ctx, const PairDesc &P, const float4 *tgt_xyzi, const int32_t *nn, const double *T,
+                            float lambda, int32_t mode, uint32_t *ticket, double *out)
+{
+    const float sl = sqrtf(lambda), sc = sqrtf(1.0f - lambda);
+    const dim3 g((unsigned)P.nblk), b(kIcpBlock);
+    if (mode == GCA_ACCUM_NAIVE_FP32)
+        hipLaunchKernelGGL(k_build_gn<GCA_ACCUM_NAIVE_FP32>, g, b, 0, ctx->stream, P, tgt_xyzi, nn, T[0], T[1], T[2],
+                           T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11], sl, sc, ticket, out);
+    else if (mode == GCA_ACCUM_FP64)
+        hipLaunchKernelGGL(k_build_gn<GCA_ACCUM_FP64>, g, b, 0, ctx->stream, P, tgt_xyzi, nn, T[0], T[1], T[2], T[3],
+                           T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11], sl, sc, ticket, out);
+    else
+        hipLaunchKernelGGL(k_build_gn<GCA_ACCUM_COMPENSATED>, g, b, 0, ctx->stream, P, tgt_xyzi, nn, T[0], T[1], T[2],
+                           T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11], sl, sc, ticket, out);
 }
 
 // ---- generic FP64 block sums with last-block finish (residuals, objective) -----------------
@@ -1860,22 +1880,39 @@ int32_t gca_build_gauss_newton(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const 
     PairDesc P;
     int64_t n;
     GCA_TRY(single_pair(ctx, src, tgt, w, P, n));
-    const float sl = sqrtf(lambda), sc = sqrtf(1.0f - lambda);
-    dim3 g((unsigned)P.nblk), b(kIcpBlock);
-    if (mode == GCA_ACCUM_NAIVE_FP32)
-        hipLaunchKernelGGL(k_build_gn<GCA_ACCUM_NAIVE_FP32>, g, b, 0, ctx->stream, P, tgt->xyzi, nn_dev, T[0], T[1],
-                           T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11], sl, sc);
-    else if (mode == GCA_ACCUM_FP64)
-        hipLaunchKernelGGL(k_build_gn<GCA_ACCUM_FP64>, g, b, 0, ctx->stream, P, tgt->xyzi, nn_dev, T[0], T[1], T[2],
-                           T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11], sl, sc);
-    else
-        hipLaunchKernelGGL(k_build_gn<GCA_ACCUM_COMPENSATED>, g, b, 0, ctx->stream, P, tgt->xyzi, nn_dev, T[0], T[1],
-                           T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11], sl, sc);
+    launch_build_gn(ctx, P, tgt->xyzi, nn_dev, T, lambda, mode, &w.states->ticket, w.states->sys);
     GCA_HIP(hipGetLastError());
     IcpState S;
     GCA_HIP(hipMemcpyAsync(&S, w.states, offsetof(IcpState, q), hipMemcpyDeviceToHost, ctx->stream));
     GCA_HIP(hipStreamSynchronize(ctx->stream));
     memcpy(out, S.sys, sizeof(double) * kNSys);
+    return GCA_OK;
+}
+
+int32_t gca_build_gauss_newton_noalloc(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const int32_t *nn_dev,
+                                       float lambda, const double T[16], int32_t mode, double *out_dev)
+{
+    if (!ctx || !src || !tgt || !nn_dev || !T || !out_dev) return GCA_ERR_INVALID_VALUE;
+    if (lambda > 1.0f || !(lambda >= 0.0f) || mode < 0 || mode > 2) return GCA_ERR_INVALID_VALUE;  // .cu:438-441
+    if (!tgt->has_normals || !tgt->has_gradient) return GCA_ERR_NOT_READY;                       // .cu:449-452
+    if (src->ctx != ctx || tgt->ctx != ctx) return GCA_ERR_INVALID_VALUE;
+    // the device's static scratch (set up by gca_ctx_create); workgroups stride over the source
+    // points, so any cloud fits the fixed partial slots
+    NoAllocScratch *sc = (NoAllocScratch *)ctx->noalloc_scratch;
+    if (!sc) return GCA_ERR_DEVICE;
+    const int32_t nb = icp_blocks(src->cap);  // capacity, known on the host: no size read-back
+    if (nb > kNoAllocMaxBlocks) return GCA_ERR_INVALID_VALUE;
+    PairDesc P{};
+    P.src = src->xyzi;
+    P.src_hdr = src->hdr;
+    P.tgt_hdr = tgt->hdr;
+    P.nrm = tgt->nrm;
+    P.grad = tgt->grad;
+    P.partials = sc->partials;
+    P.nblk = nb;
+    hipLaunchKernelGGL(k_ticket_clear, dim3(1), dim3(1), 0, ctx->stream, &sc->ticket);
+    launch_build_gn(ctx, P, tgt->xyzi, nn_dev, T, lambda, mode, &sc->ticket, out_dev);
+    GCA_HIP(hipGetLastError());
     return GCA_OK;
 }
 

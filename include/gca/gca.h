@@ -350,6 +350,20 @@ int32_t gca_build_gauss_newton(gca_ctx ctx, gca_cloud src, gca_cloud tgt, const 
                                float lambda_geometric, const double T[16],
                                int32_t accumulate_mode, double out[29]);
 
+/* gca_build_gauss_newton for callers that cannot allocate (no counterpart in the reference; for
+ * code under HIP graph capture, tight serving loops and harnesses that forbid hipMalloc): the same
+ * kernels and sums, with the result left in device memory (out_dev: 29 doubles, same layout) and
+ * no allocation, host read-back or synchronisation (the call only enqueues on the ctx stream).
+ * The work is split by the source cloud's capacity (no size read-back), so the sums equal
+ * gca_build_gauss_newton's bit for bit when the cloud is full, and to the FP64 summation order
+ * otherwise.  Scratch: a static device array of 257 KiB per GPU (partials of up to 1024
+ * workgroups and a ticket), which every GPU the library runs on holds until the process ends; the
+ * workgroups stride over the source points, so any cloud fits.  Calls that overlap on one GPU
+ * share that array and race. */
+int32_t gca_build_gauss_newton_noalloc(gca_ctx ctx, gca_cloud src, gca_cloud tgt,
+                                       const int32_t *nn_dev, float lambda_geometric,
+                                       const double T[16], int32_t accumulate_mode, double *out_dev);
+
 enum {
     GCA_RESIDUAL_PAIR = 0, /* out: float2 (sqrt(l) r_g, sqrt(1-l) r_c) per point, .cu:268-308 */
     GCA_RESIDUAL_ABS = 1,  /* out: |r_g|+|r_c| weighted, .cu:310-350 */
