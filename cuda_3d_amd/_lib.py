@@ -31,7 +31,7 @@ ACCUM_COMPENSATED, ACCUM_NAIVE_FP32, ACCUM_FP64 = range(3)
 RESIDUAL_PAIR, RESIDUAL_ABS, RESIDUAL_RMSE = range(3)
 STAGES = ("frontend", "voxel", "grid", "normals", "gradient", "icp")
 TUNE_KEYS = ("icp_lanes_small", "icp_lanes_medium", "icp_lanes_large", "icp_max_units", "normals_lanes",
-             "icp_persist_blocks", "icp_launch", "icp_lanes_batch")
+             "icp_persist_blocks", "icp_launch", "icp_lanes_batch", "frontend", "grid_cell", "voxel")
 
 
 class GcaError(RuntimeError):

@@ -2,11 +2,12 @@
 shapes Thrust picks).
 
 The knobs are the ctx tuning keys (GCA_TUNE_*): lanes per neighbourhood query for normals and
-colour gradients, lanes per correspondence query of the ICP by cloud size, the cap on ICP work
-units per iteration (one FP64 partial each) and the workgroup count of the persistent ICP
-launch.  None of them changes a result beyond the last bits of the ICP's FP64 partial sums
-(neighbour sets, moments, voxel and grid output are exact whatever the shape), so the tuner
-only measures time.
+colour gradients, lanes per correspondence query of the ICP by cloud size (the size of a query's
+lane group within the 256-thread workgroup), the cap on ICP work units per iteration (one FP64
+partial each, one workgroup each in a resident launch), the workgroup count of the persistent ICP
+launch, and the neighbour-grid cell edge (in quarters of the search radius).  None of them
+changes a result beyond the last bits of the ICP's FP64 partial sums (neighbour sets, moments,
+voxel and grid output are exact whatever the shape or cell), so the tuner only measures time.
 
 Method: coordinate descent.  Starting from the current values, each knob in turn is swept over
 its candidates with the others held, and the value with the lowest median time of the stages
@@ -40,6 +41,7 @@ def space(n_cu: int) -> Dict[str, List[int]]:
         "icp_max_units": [64, 128, 256, 512, 1024],
         "icp_persist_blocks": [n_cu // 2, n_cu, 2 * n_cu, 4 * n_cu, 8 * n_cu],
         "icp_lanes_batch": [1, 2, 4, 8],
+        "grid_cell": [4, 5, 6, 8],
     }
 
 
@@ -51,12 +53,13 @@ STAGES_OF = {
     "icp_max_units": ("icp",),
     "icp_persist_blocks": ("icp",),
     "icp_lanes_batch": ("icp",),
+    "grid_cell": ("grid", "normals", "gradient", "icp"),
 }
 # knobs a workload does not exercise (one-pair calls never read the batch lanes, the batch
 # never reads the one-pair lanes or the normals shape)
 UNUSED_BY = {
     "frame": ("icp_lanes_batch",),
-    "batch": ("normals_lanes", "icp_lanes_small", "icp_lanes_medium", "icp_lanes_large"),
+    "batch": ("normals_lanes", "icp_lanes_small", "icp_lanes_medium", "icp_lanes_large", "grid_cell"),
 }
 
 

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgca.so")
-SOURCES = ["api.hip", "frontend.hip", "voxel.hip", "grid.hip", "normals.hip", "icp.hip", "pipeline.hip", "cluster.hip",
+SOURCES = ["api.hip", "frontend.hip", "voxel.hip", "sort.hip", "grid.hip", "normals.hip", "icp.hip", "pipeline.hip", "cluster.hip",
            "noalloc.hip"]
 ARCH = os.environ.get("GCA_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

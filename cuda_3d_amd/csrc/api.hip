@@ -132,29 +132,14 @@ __global__ __launch_bounds__(256) void k_transform(float4 *__restrict__ xyzi, fl
 }
 
 // ---- exclusive scan of uint32: one pass with decoupled look-back ---------------------------
-// Chunks of values are handed out in start order by a counter, so a chunk only
-// ever waits on chunks whose workgroups are already running.  Each chunk publishes one 64-bit
-// word per call: (call sequence << 34) | (state << 32) | value, first with its own total (state
-// 1), then, once the totals before it are known, with its inclusive prefix (state 2).  Wavefront
-// 0 looks back over 64 predecessor words at a time and stops at the nearest inclusive prefix.
-// Flag, value and sequence travel in one word, so no ordering between separate stores is
-// needed, and words left by earlier calls carry an older sequence.  The workgroup that takes
-// the last chunk re-arms the counter with the next sequence: nothing is cleared between calls
-// (graph-capture safe).  A wait that outlasts kScanSpin polls stops and raises the ctx's
-// device error (reported by gca_ctx_synchronize); with chunks taken in start order it cannot
-// happen unless the device stops making progress.
+// (lookback_exclusive, gca_device.hpp).  The workgroup that takes the last chunk re-arms the
+// counter with the next sequence, so nothing is cleared between calls; a look-back that gives up
+// raises the ctx's device error (reported by gca_ctx_synchronize).
 // Chunks of BLOCK x kScanPer values: 256-thread workgroups for short scans (the shortest
 // per-chunk latency), 1024-thread ones beyond kScanWideFrom 256-thread chunks (a look-back covers
 // 64 chunks per memory round trip, so fewer, wider chunks shorten a long scan's prefix chain).
 constexpr int kScanPer = 16;
 constexpr int64_t kScanWideFrom = 128;
-constexpr uint32_t kScanSpin = 1u << 22;
-constexpr unsigned long long kScanAgg = 1ull << 32, kScanInc = 2ull << 32;
-constexpr uint32_t kScanSeqMask = (1u << 30) - 1u;
-
-struct ScanCtl {
-    unsigned long long ctr;  // (sequence << 32) | chunks taken
-};
 
 template <bool POPC>
 __device__ __forceinline__ uint32_t scan_val(uint32_t v) { return POPC ? (uint32_t)__popc(v) : v; }
@@ -207,39 +192,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_onepass(const uint32_t *__restri
     uint32_t run = block_excl_scan<BLOCK>(s, lds, &tot);
     if (threadIdx.x < kWave) {
         const int lane = lane_id();
-        const unsigned long long tag = (unsigned long long)(seq & kScanSeqMask) << 34;
-        uint32_t excl = 0;
-        if (c == 0) {
-            if (lane == 0) __hip_atomic_store(&flags[0], tag | kScanInc | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            if (lane == 0) __hip_atomic_store(&flags[c], tag | kScanAgg | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int end = c;  // window: chunks end - 64 .. end - 1, lane j reads chunk end - 1 - j
-            uint32_t spins = 0;
-            for (;;) {
-                const int k = end - 1 - lane;
-                const unsigned long long w =
-                    k >= 0 ? __hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag | kScanInc);
-                const uint32_t st = (uint32_t)(w >> 32) & 3u;
-                const bool valid = (w >> 34) == (seq & kScanSeqMask) && st != 0u;
-                const unsigned long long inc = __ballot(valid && st == 2u);
-                const int first = inc ? __builtin_ctzll(inc) : kWave;  // nearest inclusive prefix
-                const unsigned long long need = first >= kWave - 1 ? ~0ull : ((2ull << first) - 1ull);
-                if (__ballot(!valid) & need) {
-                    if (++spins > kScanSpin) {
-                        if (lane == 0) __hip_atomic_store(dev_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += wave_sum(lane <= first ? (uint32_t)w : 0u);
-                if (first < kWave) break;
-                end -= kWave;
-            }
-            if (lane == 0)
-                __hip_atomic_store(&flags[c], tag | kScanInc | (uint32_t)(excl + tot), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
+        const uint32_t excl = lookback_exclusive(flags, c, seq, tot, dev_err);
         if (lane == 0) {
             s_excl = excl;
             if (c == nchunks - 1) {
@@ -366,7 +319,7 @@ int32_t launch_transform(gca_ctx ctx, gca_cloud c, const double *T)
 }
 
 // Host-side canonical grid geometry; identical FP32 formulas to the oracle.
-void grid_params(const float bmin[3], const float bmax[3], float r, GridDesc *d)
+void grid_params(const float bmin[3], const float bmax[3], float r, GridDesc *d, int32_t cell_quarters)
 {
     float pad = 1.5f * r;
     float top[3];
@@ -374,7 +327,7 @@ void grid_params(const float bmin[3], const float bmax[3], float r, GridDesc *d)
         d->origin[a] = bmin[a] - pad;
         top[a] = bmax[a] + pad;
     }
-    float s = r;
+    float s = cell_quarters > 4 ? r * (0.25f * (float)cell_quarters) : r;
     for (;;) {
         double total = 1.0;
         for (int a = 0; a < 3; ++a) {
@@ -388,19 +341,25 @@ void grid_params(const float bmin[3], const float bmax[3], float r, GridDesc *d)
     d->cell = s;
     d->radius = r;
     d->ncells = (int64_t)d->dims[0] * d->dims[1] * d->dims[2];
+    d->cell_q = cell_quarters > 4 ? cell_quarters : 0;
+    d->pad = 0;
 }
 
 int32_t grid_build(gca_ctx ctx, gca_cloud c, float radius, bool exact)
 {
-    if (c->grid.valid) {
+    // the tuned cell edge (GCA_TUNE_GRID_CELL) applies to grids that only serve searches; an exact
+    // build keeps the canonical cell = radius (cell binning as the reference's)
+    const int32_t q = exact || ctx->tune[GCA_TUNE_GRID_CELL] <= 4 ? 0 : ctx->tune[GCA_TUNE_GRID_CELL];
+    if (c->grid.valid && c->grid.desc.cell_q == q) {
         if (c->grid.desc.radius == radius) return GCA_OK;
-        if (!exact && c->grid.desc.cell >= radius && c->grid.desc.cell <= 1.5f * radius) return GCA_OK;
+        // a canonical grid built for a nearby radius serves searches too (its cell is >= radius)
+        if (!exact && q == 0 && c->grid.desc.cell >= radius && c->grid.desc.cell <= 1.5f * radius) return GCA_OK;
     }
     GCA_TRY(cloud_sync_header(c));
     float mn[3], mx[3];
     cloud_host_bounds(c, mn, mx);
     GridDesc d;
-    grid_params(mn, mx, radius, &d);
+    grid_params(mn, mx, radius, &d, q);
     GCA_TRY(launch_grid_build(ctx, c, d));
     return GCA_OK;
 }
@@ -524,6 +483,9 @@ int32_t gca_ctx_set_tuning(gca_ctx c, int32_t key, int32_t value)
     if (key == GCA_TUNE_ICP_MAX_UNITS && value > 1024) return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_ICP_PERSIST_BLOCKS && value > 65536) return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_ICP_LAUNCH && value > 3) return GCA_ERR_INVALID_VALUE;
+    if (key == GCA_TUNE_FRONTEND && value > 2) return GCA_ERR_INVALID_VALUE;
+    if (key == GCA_TUNE_VOXEL && value > 2) return GCA_ERR_INVALID_VALUE;
+    if (key == GCA_TUNE_GRID_CELL && value != 0 && (value < 4 || value > 12)) return GCA_ERR_INVALID_VALUE;
     if (key == GCA_TUNE_NORMALS_LANES && !pow2(value, 64)) return GCA_ERR_INVALID_VALUE;
     if (c->tune[key] != value) ++c->tune_version;
     c->tune[key] = value;

@@ -47,39 +47,27 @@ struct FrontParams {
     int32_t wr_len[6];
 };
 
-__global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restrict__ depth,
-                                                         const uint8_t *__restrict__ bgr,
-                                                         const double *__restrict__ lut, FrontParams fp,
-                                                         float4 *__restrict__ st_xyzi,
-                                                         float4 *__restrict__ st_rgb,
-                                                         uint8_t *__restrict__ st_valid,
-                                                         uint32_t *__restrict__ block_cnt,
-                                                         CloudHdr *__restrict__ hdr, uint32_t *clear_word,
-                                                         int n_ws, int n_wr)
+// The pixel work of a workgroup (the row segment x0 .. x0 + 255 of row y): stage the depth rows
+// y-R .. y+R and the weight tables in LDS, then for this thread's pixel the bilateral filter,
+// back-projection, depth -> colour re-projection and BGR sampling in the reference's FP32
+// operation order.  Returns whether the pixel is a valid point; contains a barrier (every
+// thread of the workgroup calls it).
+__device__ __forceinline__ bool front_pixel(const uint16_t *__restrict__ depth, const uint8_t *__restrict__ bgr,
+                                            const double *__restrict__ lut, const FrontParams &fp, int y, int x0,
+                                            int n_ws, int n_wr, float4 &oxyz, float4 &orgb)
 {
     // dynamic LDS, sized by the host for this filter: the spatial and range weight tables (fixed
     // mode: every tap reads one of each) followed by the depth tile of the filter's own radius
     extern __shared__ __align__(16) unsigned char front_lds[];
-    __shared__ uint32_t wcount[kRowSeg / kWave];
     double *s_ws = reinterpret_cast<double *>(front_lds);
     double *s_wr = s_ws + n_ws;
     uint16_t *tile = reinterpret_cast<uint16_t *>(s_wr + n_wr);
 
     const int W = fp.W, H = fp.H;
-    const int y = blockIdx.y;
-    const int x0 = blockIdx.x * kRowSeg;
     const int x = x0 + threadIdx.x;
     int R = 0;
     if (fp.mode == GCA_BILATERAL_FIXED) R = fp.radius;
     else if (fp.mode == GCA_BILATERAL_REFERENCE_ADAPTIVE) R = 5;
-
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        if (clear_word) *clear_word = 0u;
-        for (int a = 0; a < 3; ++a) {
-            hdr->bmin_key[a] = float_to_key(FLT_MAX);
-            hdr->bmax_key[a] = float_to_key(FLT_MIN);  // geometry_util.cuh:92
-        }
-    }
 
     // stage rows y-R .. y+R, columns x0-R .. x0+255+R (out of frame -> 0, which the filter
     // skips exactly like the reference skips out-of-frame and zero taps)
@@ -120,7 +108,8 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restr
     __syncthreads();
 
     bool valid = false;
-    float4 oxyz = make_float4(0.f, 0.f, 0.f, 0.f), orgb = make_float4(0.f, 0.f, 0.f, 0.f);
+    oxyz = make_float4(0.f, 0.f, 0.f, 0.f);
+    orgb = make_float4(0.f, 0.f, 0.f, 0.f);
     if (x < W) {
         const uint16_t d0 = tile[R * cols + threadIdx.x + R];
         const float z0 = (float)d0 * fp.scale;
@@ -192,6 +181,35 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restr
                 }
             }
         }
+    }
+    return valid;
+}
+
+__global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restrict__ depth,
+                                                         const uint8_t *__restrict__ bgr,
+                                                         const double *__restrict__ lut, FrontParams fp,
+                                                         float4 *__restrict__ st_xyzi,
+                                                         float4 *__restrict__ st_rgb,
+                                                         uint8_t *__restrict__ st_valid,
+                                                         uint32_t *__restrict__ block_cnt,
+                                                         CloudHdr *__restrict__ hdr, uint32_t *clear_word,
+                                                         int n_ws, int n_wr)
+{
+    __shared__ uint32_t wcount[kRowSeg / kWave];
+    const int W = fp.W;
+    const int y = blockIdx.y;
+    const int x0 = blockIdx.x * kRowSeg;
+    const int x = x0 + threadIdx.x;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (clear_word) *clear_word = 0u;
+        for (int a = 0; a < 3; ++a) {
+            hdr->bmin_key[a] = float_to_key(FLT_MAX);
+            hdr->bmax_key[a] = float_to_key(FLT_MIN);  // geometry_util.cuh:92
+        }
+    }
+    float4 oxyz, orgb;
+    const bool valid = front_pixel(depth, bgr, lut, fp, y, x0, n_ws, n_wr, oxyz, orgb);
+    if (x < W) {
         const int64_t p = (int64_t)y * W + x;
         st_xyzi[p] = oxyz;
         st_rgb[p] = orgb;
@@ -205,6 +223,81 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restr
         for (int i = 0; i < kRowSeg / kWave; ++i) s += wcount[i];
         block_cnt[(int64_t)y * fp.nbx + blockIdx.x] = s;
     }
+}
+
+// Block bounds of the valid points of a workgroup into its slot (FLT_MAX / -FLT_MAX if it has
+// none): one small kernel folds the slots, instead of every block hitting the same six words
+// with atomics.
+__device__ __forceinline__ void block_bounds(bool valid, float4 a, float *bp)
+{
+    __shared__ float smin[3][kRowSeg / kWave], smax[3][kRowSeg / kWave];
+    const int w = threadIdx.x / kWave;
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    if (valid) {
+        mn[0] = mx[0] = a.x;
+        mn[1] = mx[1] = a.y;
+        mn[2] = mx[2] = a.z;
+    }
+    for (int k = 0; k < 3; ++k) {
+        float v0 = wave_min(mn[k]), v1 = wave_max(mx[k]);
+        if (lane_id() == 0) smin[k][w] = v0, smax[k][w] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        float v0 = smin[k][0], v1 = smax[k][0];
+        for (int q = 1; q < kRowSeg / kWave; ++q) v0 = fminf(v0, smin[k][q]), v1 = fmaxf(v1, smax[k][q]);
+        bp[k] = v0;
+        bp[3 + k] = v1;
+    }
+}
+
+// One-pass front end: the pixel work, the workgroup's offset in the compacted cloud by decoupled
+// look-back (workgroup-wide, lookback_exclusive_block: every chunk is resident at once), the
+// stable compaction and the block bounds, in one kernel.
+// Workgroups take row segments in raster order from a counter (so a look-back only waits on
+// segments already being processed); the last one writes the point count and re-arms the counter.
+// No per-pixel staging: each valid pixel is written once, to its place in the cloud.
+__global__ __launch_bounds__(kRowSeg) void k_rgbd_onepass(const uint16_t *__restrict__ depth,
+                                                          const uint8_t *__restrict__ bgr,
+                                                          const double *__restrict__ lut, FrontParams fp,
+                                                          unsigned long long *flags, ScanCtl *ctl, int nchunks,
+                                                          float4 *__restrict__ out_xyzi, float4 *__restrict__ out_rgb,
+                                                          float *__restrict__ bpart, CloudHdr *__restrict__ hdr,
+                                                          uint32_t *clear_word, int n_ws, int n_wr, uint32_t *dev_err)
+{
+    __shared__ uint32_t lds[kRowSeg / kWave + 1];
+    __shared__ unsigned long long lb[4 * (kRowSeg / kWave)];
+    __shared__ uint32_t s_chunk, s_seq;
+    if (threadIdx.x == 0) {
+        const unsigned long long w = __hip_atomic_fetch_add(&ctl->ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_chunk = (uint32_t)w;
+        s_seq = (uint32_t)(w >> 32);
+    }
+    __syncthreads();
+    const int c = (int)s_chunk;
+    const uint32_t seq = s_seq;
+    if (c >= nchunks) {  // only after an interrupted call left the counter behind
+        if (threadIdx.x == 0) __hip_atomic_store(dev_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return;
+    }
+    const int y = c / fp.nbx, x0 = (c - y * fp.nbx) * kRowSeg;
+    if (c == 0 && threadIdx.x == 0 && clear_word) *clear_word = 0u;
+    float4 oxyz, orgb;
+    const bool valid = front_pixel(depth, bgr, lut, fp, y, x0, n_ws, n_wr, oxyz, orgb);
+    uint32_t tot;
+    const uint32_t rank = block_excl_scan<kRowSeg>(valid ? 1u : 0u, lds, &tot);
+    const uint32_t excl = lookback_exclusive_block<kRowSeg>(flags, c, seq, tot, dev_err, lb);
+    if (threadIdx.x == 0 && c == nchunks - 1) {
+        hdr->n = (int32_t)(excl + tot);
+        __hip_atomic_store(&ctl->ctr, (unsigned long long)(seq + 1u) << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (valid) {
+        const uint32_t dst = excl + rank;
+        out_xyzi[dst] = oxyz;
+        out_rgb[dst] = orgb;
+    }
+    block_bounds(valid, oxyz, bpart + (int64_t)c * 6);
 }
 
 __global__ __launch_bounds__(kRowSeg) void k_rgbd_compact(const float4 *__restrict__ st_xyzi,
@@ -360,9 +453,10 @@ static int32_t build_lut(gca_ctx ctx, const gca_bilateral *f, FrontParams *fp)
 }
 
 struct FrontLaunch {
+    bool onepass = false;  // k_rgbd_onepass, else k_rgbd_pixels
     dim3 grid, block;
     unsigned lds = 0;
-    // k_rgbd_pixels' arguments, in order
+    // the kernel's arguments (those of k_rgbd_pixels, then those only k_rgbd_onepass takes)
     const uint16_t *depth = nullptr;
     const uint8_t *bgr = nullptr;
     const double *lut = nullptr;
@@ -373,7 +467,17 @@ struct FrontLaunch {
     CloudHdr *hdr = nullptr;
     uint32_t *clear_word = nullptr;
     int n_ws = 0, n_wr = 0;
+    unsigned long long *flags = nullptr;
+    ScanCtl *ctl = nullptr;
+    int nchunks = 0;
+    float4 *out_xyzi = nullptr, *out_rgb = nullptr;
+    float *bpart = nullptr;
+    uint32_t *dev_err = nullptr;
     bool valid = false;
+    void *func() const
+    {
+        return onepass ? reinterpret_cast<void *>(&k_rgbd_onepass) : reinterpret_cast<void *>(&k_rgbd_pixels);
+    }
 };
 
 FrontLaunch *front_launch_new() { return new (std::nothrow) FrontLaunch(); }
@@ -393,7 +497,7 @@ hipGraphNode_t front_launch_node(const FrontLaunch *f, hipGraph_t g)
         if (hipGraphNodeGetType(nodes[k], &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
         hipKernelNodeParams kp{};
         if (hipGraphKernelNodeGetParams(nodes[k], &kp) != hipSuccess) continue;
-        if (kp.func == reinterpret_cast<void *>(&k_rgbd_pixels)) {
+        if (kp.func == f->func()) {
             if (found) return nullptr;  // more than one: not the graph of one pipeline step
             found = nodes[k];
         }
@@ -407,14 +511,16 @@ hipError_t front_launch_update(FrontLaunch *f, hipGraphExec_t exec, hipGraphNode
     if (!f || !f->valid || !exec || !node) return hipErrorInvalidValue;
     f->depth = depth;
     f->bgr = bgr;
-    void *args[] = {&f->depth, &f->bgr, &f->lut, &f->fp, &f->st_xyzi, &f->st_rgb, &f->st_valid,
-                    &f->bcnt, &f->hdr, &f->clear_word, &f->n_ws, &f->n_wr};
+    void *pix[] = {&f->depth, &f->bgr, &f->lut, &f->fp, &f->st_xyzi, &f->st_rgb, &f->st_valid,
+                   &f->bcnt, &f->hdr, &f->clear_word, &f->n_ws, &f->n_wr};
+    void *one[] = {&f->depth, &f->bgr, &f->lut, &f->fp, &f->flags, &f->ctl, &f->nchunks, &f->out_xyzi,
+                   &f->out_rgb, &f->bpart, &f->hdr, &f->clear_word, &f->n_ws, &f->n_wr, &f->dev_err};
     hipKernelNodeParams kp{};
-    kp.func = reinterpret_cast<void *>(&k_rgbd_pixels);
+    kp.func = f->func();
     kp.gridDim = f->grid;
     kp.blockDim = f->block;
     kp.sharedMemBytes = f->lds;
-    kp.kernelParams = args;
+    kp.kernelParams = f->onepass ? one : pix;
     kp.extra = nullptr;
     return hipGraphExecKernelNodeSetParams(exec, node, &kp);
 }
@@ -440,14 +546,10 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
     if (bil->mode != GCA_BILATERAL_NONE) GCA_TRY(build_lut(ctx, bil, &fp));
     GCA_TRY(cloud_reserve(out, P));
     const int64_t nblocks = (int64_t)fp.nbx * H;
-    float4 *st_xyzi = (float4 *)ctx_scratch(ctx, 0, sizeof(float4) * P, false);
-    float4 *st_rgb = (float4 *)ctx_scratch(ctx, 1, sizeof(float4) * P, false);
-    uint8_t *st_valid = (uint8_t *)ctx_scratch(ctx, 2, (size_t)P, false);
-    uint32_t *bcnt = (uint32_t *)ctx_scratch(ctx, 3, sizeof(uint32_t) * (nblocks + 1), false);
+    const bool onepass = ctx->tune[GCA_TUNE_FRONTEND] != 2;
     float *bpart = (float *)ctx_scratch(ctx, 20, sizeof(float) * 6 * nblocks, false);
-    if (!st_xyzi || !st_rgb || !st_valid || !bcnt || !bpart) return GCA_ERR_OUT_OF_MEMORY;
-    dim3 grid(fp.nbx, H);
-    // LDS of the pixel kernel: weight tables (fixed mode; the range table while it is short)
+    if (!bpart) return GCA_ERR_OUT_OF_MEMORY;
+    // LDS of the pixel work: weight tables (fixed mode; the range table while it is short)
     // and the depth tile of the filter's radius
     const int R = bil->mode == GCA_BILATERAL_FIXED ? bil->radius : (bil->mode == GCA_BILATERAL_REFERENCE_ADAPTIVE ? 5 : 0);
     const bool stage = bil->mode == GCA_BILATERAL_FIXED && fp.wr_len[0] <= kWrLds;  // both tables or neither
@@ -455,17 +557,77 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
     const int n_wr = stage ? fp.wr_len[0] : 0;
     const size_t lds = sizeof(double) * (size_t)(n_ws + n_wr) + sizeof(uint16_t) * (size_t)(2 * R + 1) * (kRowSeg + 2 * R);
     StageTimer timer(ctx, GCA_STAGE_FRONTEND);
-    if (FrontLaunch *r = ctx->front_rec) {
-        *r = FrontLaunch{grid, dim3(kRowSeg), (unsigned)lds, depth, bgr, (const double *)ctx->lut, fp, st_xyzi,
-                         st_rgb, st_valid, bcnt, out->hdr, clear_word, n_ws, n_wr, true};
+    if (onepass) {
+        // look-back words and the segment counter: zeroed once when allocated, then carried
+        // from call to call by their sequence numbers
+        unsigned long long *flags =
+            (unsigned long long *)ctx_scratch(ctx, 36, sizeof(unsigned long long) * (size_t)nblocks, true);
+        ScanCtl *ctl = (ScanCtl *)ctx_scratch(ctx, 37, sizeof(ScanCtl), true);
+        if (!flags || !ctl) return GCA_ERR_OUT_OF_MEMORY;
+        if (nblocks > INT_MAX) return GCA_ERR_INVALID_VALUE;
+        const dim3 grid((unsigned)nblocks);
+        if (FrontLaunch *r = ctx->front_rec) {
+            *r = FrontLaunch{};
+            r->onepass = true;
+            r->grid = grid;
+            r->block = dim3(kRowSeg);
+            r->lds = (unsigned)lds;
+            r->depth = depth;
+            r->bgr = bgr;
+            r->lut = (const double *)ctx->lut;
+            r->fp = fp;
+            r->hdr = out->hdr;
+            r->clear_word = clear_word;
+            r->n_ws = n_ws;
+            r->n_wr = n_wr;
+            r->flags = flags;
+            r->ctl = ctl;
+            r->nchunks = (int)nblocks;
+            r->out_xyzi = out->xyzi;
+            r->out_rgb = out->rgb;
+            r->bpart = bpart;
+            r->dev_err = ctx->dev_err;
+            r->valid = true;
+        }
+        hipLaunchKernelGGL(k_rgbd_onepass, grid, dim3(kRowSeg), lds, ctx->stream, depth, bgr, (const double *)ctx->lut,
+                           fp, flags, ctl, (int)nblocks, out->xyzi, out->rgb, bpart, out->hdr, clear_word, n_ws, n_wr,
+                           ctx->dev_err);
+        GCA_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, ctx->stream, (const float *)bpart, nblocks, out->hdr);
+    } else {
+        float4 *st_xyzi = (float4 *)ctx_scratch(ctx, 0, sizeof(float4) * P, false);
+        float4 *st_rgb = (float4 *)ctx_scratch(ctx, 1, sizeof(float4) * P, false);
+        uint8_t *st_valid = (uint8_t *)ctx_scratch(ctx, 2, (size_t)P, false);
+        uint32_t *bcnt = (uint32_t *)ctx_scratch(ctx, 3, sizeof(uint32_t) * (nblocks + 1), false);
+        if (!st_xyzi || !st_rgb || !st_valid || !bcnt) return GCA_ERR_OUT_OF_MEMORY;
+        dim3 grid(fp.nbx, H);
+        if (FrontLaunch *r = ctx->front_rec) {
+            *r = FrontLaunch{};
+            r->grid = grid;
+            r->block = dim3(kRowSeg);
+            r->lds = (unsigned)lds;
+            r->depth = depth;
+            r->bgr = bgr;
+            r->lut = (const double *)ctx->lut;
+            r->fp = fp;
+            r->st_xyzi = st_xyzi;
+            r->st_rgb = st_rgb;
+            r->st_valid = st_valid;
+            r->bcnt = bcnt;
+            r->hdr = out->hdr;
+            r->clear_word = clear_word;
+            r->n_ws = n_ws;
+            r->n_wr = n_wr;
+            r->valid = true;
+        }
+        hipLaunchKernelGGL(k_rgbd_pixels, grid, dim3(kRowSeg), lds, ctx->stream, depth, bgr,
+                           (const double *)ctx->lut, fp, st_xyzi, st_rgb, st_valid, bcnt, out->hdr, clear_word, n_ws,
+                           n_wr);
+        GCA_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_rgbd_compact, grid, dim3(kRowSeg), 0, ctx->stream, st_xyzi, st_rgb, st_valid,
+                           (const uint32_t *)bcnt, W, fp.nbx, out->xyzi, out->rgb, bpart, out->hdr);
+        hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, ctx->stream, (const float *)bpart, nblocks, out->hdr);
     }
-    hipLaunchKernelGGL(k_rgbd_pixels, grid, dim3(kRowSeg), lds, ctx->stream, depth, bgr,
-                       (const double *)ctx->lut, fp, st_xyzi, st_rgb, st_valid, bcnt, out->hdr, clear_word, n_ws,
-                       n_wr);
-    GCA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_rgbd_compact, grid, dim3(kRowSeg), 0, ctx->stream, st_xyzi, st_rgb, st_valid,
-                       (const uint32_t *)bcnt, W, fp.nbx, out->xyzi, out->rgb, bpart, out->hdr);
-    hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, ctx->stream, (const float *)bpart, nblocks, out->hdr);
     GCA_HIP(hipGetLastError());
     return GCA_OK;
 }

@@ -76,6 +76,139 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds, u
     return res;
 }
 
+// ---- decoupled look-back of one-pass scans ----------------------------------------------
+// Chunks (workgroups) are handed out in start order by a counter, so a chunk only ever waits on
+// chunks whose workgroups are already running.  Each chunk publishes one 64-bit word per call:
+// (call sequence << 34) | (state << 32) | value, first with its own total (state 1), then, once the
+// totals before it are known, with its inclusive prefix (state 2).  Wavefront 0 looks back over 64
+// predecessor words at a time and stops at the nearest inclusive prefix.  Flag, value and sequence
+// travel in one word, so no ordering between separate stores is needed, and words left by earlier
+// calls carry an older sequence: nothing is cleared between calls (graph-capture safe).  A wait
+// that outlasts kScanSpin polls stops and raises the device error word; with chunks taken in start
+// order it cannot happen unless the device stops making progress.
+constexpr uint32_t kScanSpin = 1u << 22;
+constexpr unsigned long long kScanAgg = 1ull << 32, kScanInc = 2ull << 32;
+constexpr uint32_t kScanSeqMask = (1u << 30) - 1u;
+
+struct ScanCtl {
+    unsigned long long ctr;  // (sequence << 32) | chunks taken
+};
+
+// Run by the 64 lanes of wavefront 0 of chunk c (sequence seq, chunk total tot): publishes the
+// chunk's words and returns its exclusive prefix (on every lane).
+__device__ __forceinline__ uint32_t lookback_exclusive(unsigned long long *flags, int c, uint32_t seq, uint32_t tot,
+                                                       uint32_t *dev_err)
+{
+    const int lane = lane_id();
+    const unsigned long long tag = (unsigned long long)(seq & kScanSeqMask) << 34;
+    uint32_t excl = 0;
+    if (c == 0) {
+        if (lane == 0) __hip_atomic_store(&flags[0], tag | kScanInc | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+    }
+    if (lane == 0) __hip_atomic_store(&flags[c], tag | kScanAgg | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int end = c;  // window: chunks end - 64 .. end - 1, lane j reads chunk end - 1 - j
+    uint32_t spins = 0;
+    for (;;) {
+        const int k = end - 1 - lane;
+        const unsigned long long w =
+            k >= 0 ? __hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag | kScanInc);
+        const uint32_t st = (uint32_t)(w >> 32) & 3u;
+        const bool valid = (w >> 34) == (seq & kScanSeqMask) && st != 0u;
+        const unsigned long long inc = __ballot(valid && st == 2u);
+        const int first = inc ? __builtin_ctzll(inc) : kWave;  // nearest inclusive prefix
+        const unsigned long long need = first >= kWave - 1 ? ~0ull : ((2ull << first) - 1ull);
+        if (__ballot(!valid) & need) {
+            if (++spins > kScanSpin) {
+                if (lane == 0) __hip_atomic_store(dev_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += wave_sum(lane <= first ? (uint32_t)w : 0u);
+        if (first < kWave) break;
+        end -= kWave;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&flags[c], tag | kScanInc | (uint32_t)(excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return excl;
+}
+
+// Workgroup-wide variant of the look-back, for kernels with many chunks that all start together
+// (every chunk resident at once publishes its total at about the same time, so the look-back
+// walks over totals rather than stopping at an early inclusive prefix): all BLOCK threads read
+// BLOCK predecessor words per round, so a walk over C chunks takes C / BLOCK rounds instead of
+// C / 64.  Same words and protocol as lookback_exclusive; called by every thread of the
+// workgroup (the result is on every thread).  lds: 4 * (BLOCK / 64) words of 64 bits.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t lookback_exclusive_block(unsigned long long *flags, int c, uint32_t seq,
+                                                             uint32_t tot, uint32_t *dev_err,
+                                                             unsigned long long *lds)
+{
+    constexpr int NW = BLOCK / kWave;
+    const int lane = lane_id(), wv = threadIdx.x / kWave;
+    const unsigned long long tag = (unsigned long long)(seq & kScanSeqMask) << 34;
+    if (c == 0) {
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&flags[0], tag | kScanInc | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(&flags[c], tag | kScanAgg | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t excl = 0, spins = 0;
+    int end = c;  // window: chunks end - BLOCK .. end - 1, thread t reads chunk end - 1 - t
+    for (int round = 0;; ++round) {
+        unsigned long long *buf = lds + (round & 1) * 2 * NW;  // [0, NW): masks, [NW, 2NW): sums
+        const int k = end - 1 - (int)threadIdx.x;
+        const unsigned long long w =
+            k >= 0 ? __hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag | kScanInc);
+        const uint32_t st = (uint32_t)(w >> 32) & 3u;
+        const bool valid = (w >> 34) == (seq & kScanSeqMask) && st != 0u;
+        const unsigned long long inc = __ballot(valid && st == 2u);
+        const unsigned long long bad = __ballot(!valid);
+        const int f = inc ? __builtin_ctzll(inc) : kWave;  // this wavefront's nearest inclusive prefix
+        const uint32_t s_all = wave_sum(valid ? (uint32_t)w : 0u);
+        const uint32_t s_first = wave_sum(valid && lane <= f ? (uint32_t)w : 0u);
+        if (lane == 0) {
+            const unsigned long long need = f >= kWave - 1 ? ~0ull : ((2ull << f) - 1ull);
+            // bit 0: an inclusive prefix in this wavefront; bit 1: a word not yet valid in the part
+            // of this wavefront that would be summed
+            buf[wv] = (inc ? 1ull : 0ull) | ((bad & need) ? 2ull : 0ull);
+            buf[NW + wv] = ((unsigned long long)s_first << 32) | s_all;
+        }
+        __syncthreads();
+        bool wait = false, done = false;
+        uint32_t sum = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const unsigned long long m = buf[i], s = buf[NW + i];
+            if (!done) {
+                wait |= (m & 2ull) != 0ull;
+                if (m & 1ull) {
+                    sum += (uint32_t)(s >> 32);
+                    done = true;
+                } else {
+                    sum += (uint32_t)s;
+                }
+            }
+        }
+        if (wait) {
+            if (++spins > kScanSpin) {
+                if (threadIdx.x == 0) __hip_atomic_store(dev_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += sum;
+        if (done) break;
+        end -= BLOCK;
+    }
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&flags[c], tag | kScanInc | (uint32_t)(excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return excl;
+}
+
 // ---- FP32 TwoSum-compensated accumulator -------------------------------------------------
 struct Comp {
     float s, c;

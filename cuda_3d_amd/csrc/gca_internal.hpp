@@ -60,6 +60,9 @@ struct GridDesc {
     int32_t dims[3];
     float radius;
     int64_t ncells;
+    int32_t cell_q;  // 0: canonical cell (the radius, doubled while too many cells); else the tuned
+                     // edge in quarters of the radius
+    int32_t pad;
 };
 
 struct Grid {
@@ -92,7 +95,7 @@ struct gca_ctx_s {
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     // scratch buffers owned by the ctx, grown on demand
-    static constexpr int kScratchSlots = 40;
+    static constexpr int kScratchSlots = 48;
     void *scratch[kScratchSlots] = {nullptr};
     size_t scratch_bytes[kScratchSlots] = {0};
     bool scratch_zeroed[kScratchSlots] = {false};
@@ -120,7 +123,7 @@ struct gca_ctx_s {
     gca_cloud level_clouds[16] = {nullptr};
     // launch-shape tuning (GCA_TUNE_*), 0 = heuristic; the version counts changes (objects that
     // bake shapes in, e.g. a frame pipeline's plans and graphs, rebuild when it moves)
-    int32_t tune[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int32_t tune[GCA_TUNE_COUNT] = {};
     uint64_t tune_version = 0;
     // stage profiler: event pairs recorded on `stream` while enabled (not while the stream is
     // being captured into a graph)
@@ -273,12 +276,17 @@ int32_t icp_plan_results(gca_ctx ctx, const IcpPlan *plan, double *T_out, gca_ic
 void icp_plan_destroy(IcpPlan *plan);
 
 // Host-side canonical grid geometry (same FP32 formulas as the oracle)
-void grid_params(const float bmin[3], const float bmax[3], float r, GridDesc *d);
+// cell_quarters > 4: cell edge = r * cell_quarters / 4 (a tuned cell; results do not change).
+void grid_params(const float bmin[3], const float bmax[3], float r, GridDesc *d, int32_t cell_quarters = 0);
 
 // Block-wide exclusive scan of uint32 arrays (3 kernels, reduce-then-scan)
 // out has n + 1 entries (out[n] = total); popcount = scan the popcounts of the words.
 int32_t exclusive_scan_u32(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n,
                            uint32_t *total_dev, bool popcount = false);
+// Stable LSD radix sort of n (64-bit key, 32-bit value) pairs by the low `bits` key bits, 8 per
+// pass; the result ends in (k0, v0) if *sorted_in_first, else in (k1, v1).
+int32_t radix_sort_u64(gca_ctx ctx, unsigned long long *k0, uint32_t *v0, unsigned long long *k1, uint32_t *v1,
+                       int64_t n, int bits, bool *sorted_in_first);
 
 }  // namespace gca
 

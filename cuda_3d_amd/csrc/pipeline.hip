@@ -239,6 +239,22 @@ void frustum_box(const gca_camera &c, float tmin, float tmax, float mn[3], float
     }
 }
 
+// Per level: one grid geometry over the frustum box (cell = the normals / correspondence radius,
+// the largest of the level, or the tuned multiple of it), and the voxel lattice bound of the box.
+void set_geometry(gca_pipeline p)
+{
+    float bmn[3], bmx[3];
+    frustum_box(p->cam, p->tmin, p->tmax, bmn, bmx);
+    for (int l = 0; l < p->nl; ++l) {
+        const gca_level_params &L = p->lv[l];
+        const float r = std::max(std::max(L.normal_radius, L.icp.max_correspondence_distance), L.gradient_radius);
+        grid_params(bmn, bmx, r, &p->gdesc[l], p->ctx->tune[GCA_TUNE_GRID_CELL]);
+        double bits = 1.0;
+        for (int a = 0; a < 3; ++a) bits *= std::floor((double)(bmx[a] - bmn[a]) / (double)L.voxel_size) + 2.0;
+        p->max_bits[l] = bits;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -323,18 +339,7 @@ int32_t gca_pipeline_create(gca_ctx ctx, const gca_camera *cam, const gca_bilate
         gca_pipeline_destroy(p);
         return st;
     }
-    // per level: one grid geometry over the frustum box (cell = the normals / correspondence
-    // radius, the largest of the level), and the voxel lattice bound of the box
-    float bmn[3], bmx[3];
-    frustum_box(*cam, tmin, tmax, bmn, bmx);
-    for (int l = 0; l < n_levels; ++l) {
-        const gca_level_params &L = levels[l];
-        const float r = std::max(std::max(L.normal_radius, L.icp.max_correspondence_distance), L.gradient_radius);
-        grid_params(bmn, bmx, r, &p->gdesc[l]);
-        double bits = 1.0;
-        for (int a = 0; a < 3; ++a) bits *= std::floor((double)(bmx[a] - bmn[a]) / (double)L.voxel_size) + 2.0;
-        p->max_bits[l] = bits;
-    }
+    set_geometry(p);
     p->tune_version = p->ctx->tune_version;
     *out = p;
     return GCA_OK;
@@ -361,7 +366,9 @@ int32_t gca_pipeline_step(gca_pipeline p, const uint16_t *depth_dev, const uint8
     const int slot = (int)(p->frames & 1);
     if (p->ctx->tune_version != p->tune_version) {  // plans and graphs bake the launch shapes in
         GCA_HIP(hipStreamSynchronize(p->ctx->stream));
+        for (int l = 0; l < p->nl; ++l) GCA_HIP(hipStreamSynchronize(p->lctx[l]->stream));
         drop_plans(p);
+        set_geometry(p);  // the grid cell may be a tuned one
         p->tune_version = p->ctx->tune_version;
     }
     for (int l = 0; l < p->nl; ++l)  // the level ctxs launch with the pipeline ctx's shapes

@@ -4,8 +4,9 @@
 // and point_cloud::voxel_grid_down_sample (src/geometry/point_cloud.cu:198-229).
 //
 // The reference sorts int3 keys with a comparison sort and reduces by key.  Here the output
-// order (lexicographic (kx, ky, kz) == ascending linear key) comes from an occupancy bitmap
-// over the voxel lattice instead of a sort:
+// order (lexicographic (kx, ky, kz) == ascending linear key) comes, for lattices up to 2^34
+// cells, from an occupancy bitmap over the voxel lattice instead of a sort (beyond that, and by
+// tuning, from a radix sort of packed keys and a segmented reduce: the sorted path below):
 //   1. mark: the lattice from the input's device header (every workgroup for itself), then
 //      the key in the reference's FP32 formula, set bit L = (kx*ny + ky)*nz + kz;
 //   2. popcount prefix over the bitmap words (one-pass scan with decoupled look-back) -> rank
@@ -62,18 +63,24 @@ __device__ __forceinline__ VoxelParams voxel_params_of(const CloudHdr *__restric
     return p;
 }
 
-__device__ __forceinline__ int64_t voxel_key(const VoxelParams &vp, float4 p)
+// The voxel's integer coordinates: __float2int_rd((p - min) / v) in FP32
+// (cuda_voxel_grid_down_sample.cu:37-47), clamped to the lattice.
+__device__ __forceinline__ void voxel_key3(const VoxelParams &vp, float4 p, int64_t k[3])
 {
     const float c[3] = {p.x, p.y, p.z};
-    int64_t k[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        // __float2int_rd((p - min) / v) in FP32 (cuda_voxel_grid_down_sample.cu:37-47)
         float q = floorf(__fdiv_rn(__fsub_rn(c[a], vp.bmin[a]), vp.voxel));
         int64_t v = (int64_t)q;
         v = v < 0 ? 0 : (v >= vp.dims[a] ? vp.dims[a] - 1 : v);
         k[a] = v;
     }
+}
+
+__device__ __forceinline__ int64_t voxel_key(const VoxelParams &vp, float4 p)
+{
+    int64_t k[3];
+    voxel_key3(vp, p, k);
     return (k[0] * vp.dims[1] + k[1]) * vp.dims[2] + k[2];
 }
 
@@ -260,11 +267,162 @@ __global__ __launch_bounds__(256) void k_voxel_finalize(unsigned long long *__re
             gc_cell_of[r] = lin;
             atomicAdd(&gc_count[lin], 1u);
         }
-        const int64_t L = (int64_t)a[7];
-        bitmap[L >> 5] = 0u;
+        if (bitmap) {
+            const int64_t L = (int64_t)a[7];
+            bitmap[L >> 5] = 0u;
+        }
 #pragma unroll
         for (int c = 0; c < 8; ++c) a[c] = 0ull;
     }
+}
+
+// ---- sorted path: radix sort of packed keys, then a segmented reduce -------------------------
+// For lattices beyond the bitmap's reach (or by tuning).  Key of a point: (kx << (by + bz)) |
+// (ky << bz) | kz, with bx, by, bz the bit widths of dims - 1, so ascending keys are the
+// reference's lexicographic (kx, ky, kz) order.  Widths beyond 64 bits in all are sorted in two
+// stable passes: by (ky << bz) | kz, then by kx.  Equal keys form segments; a segment's rank
+// (its output position) is the count of segment starts before it.  The sums are the same
+// fixed-point integers as the bitmap path's, so both paths give the same bits.
+__global__ __launch_bounds__(256) void k_vsort_key(const float4 *__restrict__ xyzi, int n, VoxelParams vp,
+                                                   int shift_x, int shift_y, int split,
+                                                   unsigned long long *__restrict__ keys, uint32_t *__restrict__ idx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t k[3];
+    voxel_key3(vp, xyzi[i], k);
+    const unsigned long long lo = ((unsigned long long)k[1] << shift_y) | (unsigned long long)k[2];
+    keys[i] = split ? lo : (((unsigned long long)k[0] << shift_x) | lo);
+    idx[i] = (uint32_t)i;
+}
+
+// Second key of a split sort: kx of the point at each sorted position.
+__global__ __launch_bounds__(256) void k_vsort_rekey(const float4 *__restrict__ xyzi, const uint32_t *__restrict__ idx,
+                                                     int n, VoxelParams vp, unsigned long long *__restrict__ keys)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t k[3];
+    voxel_key3(vp, xyzi[idx[i]], k);
+    keys[i] = (unsigned long long)k[0];
+}
+
+// flag[i] = 1 where sorted position i starts a voxel.  A split sort leaves only kx in the keys, so
+// it compares the full coordinates of the two points instead.
+__global__ __launch_bounds__(256) void k_vsort_flag(const unsigned long long *__restrict__ keys,
+                                                    const uint32_t *__restrict__ idx, const float4 *__restrict__ xyzi,
+                                                    int n, VoxelParams vp, int split, uint32_t *__restrict__ flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool head = i == 0 || keys[i] != keys[i - 1];
+    if (!head && split) {
+        int64_t a[3], b[3];
+        voxel_key3(vp, xyzi[idx[i]], a);
+        voxel_key3(vp, xyzi[idx[i - 1]], b);
+        head = a[1] != b[1] || a[2] != b[2];
+    }
+    flag[i] = head ? 1u : 0u;
+}
+
+// Segmented reduce: each wavefront merges the runs of equal rank among its 64 sorted positions;
+// the first lane of a run stores its sums directly when the whole voxel lies in this wavefront
+// (one writer), and adds them atomically otherwise (voxels that cross wavefronts; the table is
+// zero there, since the finalize pass clears what it reads).
+__global__ __launch_bounds__(256) void k_vsort_reduce(const float4 *__restrict__ xyzi, const float4 *__restrict__ rgb,
+                                                      const uint32_t *__restrict__ idx,
+                                                      const uint32_t *__restrict__ flag,
+                                                      const uint32_t *__restrict__ seg, int n,
+                                                      unsigned long long *__restrict__ acc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // wavefronts cover 64 consecutive positions
+    const int lane = lane_id();
+    const bool on = i < n;
+    unsigned long long v[7] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+    int64_t r = -1 - (int64_t)lane;  // unique keys for inactive lanes
+    uint32_t f = 0u;
+    if (on) {
+        f = flag[i];
+        r = (int64_t)seg[i] + (int64_t)f - 1;
+        const uint32_t j = idx[i];
+        const float4 p = xyzi[j], c = rgb[j];
+        v[0] = (unsigned long long)fx((double)p.x * kFx30);
+        v[1] = (unsigned long long)fx((double)p.y * kFx30);
+        v[2] = (unsigned long long)fx((double)p.z * kFx30);
+        v[3] = (unsigned long long)fx((double)c.x * kFx30);
+        v[4] = (unsigned long long)fx((double)c.y * kFx30);
+        v[5] = (unsigned long long)fx((double)c.z * kFx30);
+        v[6] = 1ull;
+    }
+    const int head = run_head(r);
+    const int64_t r_next = __shfl_down(r, 1, kWave);
+    // does the voxel end inside this wavefront at this lane?
+    uint32_t ends = 0u;
+    if (on) {
+        if (lane < kWave - 1 && i + 1 < n) ends = r_next != r ? 1u : 0u;
+        else ends = (i + 1 >= n || flag[i + 1]) ? 1u : 0u;
+    }
+    ends = run_reduce(ends, head, [](uint32_t a, uint32_t c) { return a | c; });
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        v[k] = run_reduce(v[k], head, [](unsigned long long a, unsigned long long c) { return a + c; });
+    if (on && head == lane) {
+        unsigned long long *a = acc + 8 * r;
+        if (f && ends) {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) a[k] = v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) atomicAdd(&a[k], v[k]);
+        }
+    }
+}
+
+// Sorted-path kernel sequence of one voxel call, on a lattice planned on the host (vp).
+static int32_t voxel_sort_enqueue(gca_ctx ctx, gca_cloud in, const VoxelParams &vp, gca_cloud out, int n)
+{
+    int w[3];
+    for (int a = 0; a < 3; ++a)
+        w[a] = vp.dims[a] > 1 ? 64 - __builtin_clzll((unsigned long long)(vp.dims[a] - 1)) : 0;
+    const bool split = w[0] + w[1] + w[2] > 64;
+    const int64_t nc = std::max(n, 1);
+    auto *k0 = (unsigned long long *)ctx_scratch(ctx, 40, sizeof(unsigned long long) * nc, false);
+    auto *k1 = (unsigned long long *)ctx_scratch(ctx, 41, sizeof(unsigned long long) * nc, false);
+    auto *i0 = (uint32_t *)ctx_scratch(ctx, 42, sizeof(uint32_t) * nc, false);
+    auto *i1 = (uint32_t *)ctx_scratch(ctx, 43, sizeof(uint32_t) * nc, false);
+    auto *flag = (uint32_t *)ctx_scratch(ctx, 44, sizeof(uint32_t) * nc, false);
+    auto *seg = (uint32_t *)ctx_scratch(ctx, 45, sizeof(uint32_t) * (nc + 1), false);
+    auto *acc = (unsigned long long *)ctx_scratch(ctx, 7, sizeof(unsigned long long) * 8 * nc, true);
+    if (!k0 || !k1 || !i0 || !i1 || !flag || !seg || !acc) return GCA_ERR_OUT_OF_MEMORY;
+    StageTimer timer(ctx, GCA_STAGE_VOXEL);
+    const unsigned nb = (unsigned)((nc + 255) / 256);
+    hipLaunchKernelGGL(k_vsort_key, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, n, vp, w[1] + w[2], w[2],
+                       split ? 1 : 0, k0, i0);
+    GCA_HIP(hipGetLastError());
+    bool first = true;
+    GCA_TRY(radix_sort_u64(ctx, k0, i0, k1, i1, n, split ? w[1] + w[2] : w[0] + w[1] + w[2], &first));
+    unsigned long long *ks = first ? k0 : k1, *kt = first ? k1 : k0;
+    uint32_t *is = first ? i0 : i1, *it = first ? i1 : i0;
+    if (split) {
+        hipLaunchKernelGGL(k_vsort_rekey, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, is, n, vp, ks);
+        GCA_HIP(hipGetLastError());
+        GCA_TRY(radix_sort_u64(ctx, ks, is, kt, it, n, w[0], &first));
+        if (!first) {
+            std::swap(ks, kt);
+            std::swap(is, it);
+        }
+    }
+    hipLaunchKernelGGL(k_vsort_flag, dim3(nb), dim3(256), 0, ctx->stream, ks, is, in->xyzi, n, vp, split ? 1 : 0,
+                       flag);
+    GCA_HIP(hipGetLastError());
+    GCA_TRY(exclusive_scan_u32(ctx, flag, seg, n, (uint32_t *)&out->hdr->n));  // n = 0: M = 0
+    hipLaunchKernelGGL(k_vsort_reduce, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->rgb, is, flag, seg, n, acc);
+    GCA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_voxel_finalize, dim3((unsigned)std::min<int64_t>(4096, nb)), dim3(256), 0, ctx->stream, acc,
+                       in->hdr, out->hdr, nullptr, out->xyzi, out->rgb, GridDesc{}, nullptr, nullptr);
+    GCA_HIP(hipGetLastError());
+    cloud_points_changed(out);
+    return GCA_OK;
 }
 
 // Kernel sequence of one voxel call: mark (with the device lattice), popcount scan over `words` bitmap
@@ -316,15 +474,19 @@ int32_t launch_voxel(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out)
     const int64_t n = in->host_hdr.n;
     float mn[3], mx[3];
     cloud_host_bounds(in, mn, mx);
+    VoxelParams vp{};
+    vp.voxel = voxel;
     double nbits = 1.0;
-    for (int a = 0; a < 3; ++a) {  // the device lattice below uses the same FP32 formulas
+    for (int a = 0; a < 3; ++a) {  // the same FP32 formulas as voxel_params_of
         const float ext = mx[a] - mn[a];
         if (n > 0 && (double)voxel * (double)INT_MAX < (double)ext)  // point_cloud.cu:213-218
             return GCA_ERR_INVALID_VALUE;
-        nbits *= n > 0 ? (double)((int64_t)floorf(ext / voxel) + 1) : 1.0;
+        vp.bmin[a] = mn[a];
+        vp.dims[a] = n > 0 ? (int64_t)floorf(ext / voxel) + 1 : 1;
+        nbits *= (double)vp.dims[a];
     }
-    if (nbits > kVoxelMaxBits) return GCA_ERR_INVALID_VALUE;  // voxel too small for the lattice
     GCA_TRY(cloud_reserve(out, std::max<int64_t>(n, 1)));
+    if (ctx->tune[GCA_TUNE_VOXEL] == 2 || nbits > kVoxelMaxBits) return voxel_sort_enqueue(ctx, in, vp, out, (int)n);
     return voxel_enqueue(ctx, in, voxel, out, n, ((int64_t)nbits + 31) / 32, kVoxelMaxBits, nullptr);
 }
 

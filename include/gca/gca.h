@@ -176,7 +176,16 @@ enum {
                                       per iteration; 3 = one resident launch per pair and level.  Every
                                       strategy gives the same bits for the same unit partition */
     GCA_TUNE_ICP_LANES_BATCH = 7,  /* lanes per correspondence query in calls with several pairs (0 = 1) */
-    GCA_TUNE_COUNT = 8
+    GCA_TUNE_FRONTEND = 8,         /* 0/1 = one-pass front end (pixels, look-back offset, compaction in
+                                      one kernel), 2 = pixels, then compaction (two kernels); same bits */
+    GCA_TUNE_GRID_CELL = 9,        /* neighbour-grid cell edge in quarters of the search radius, 4..12
+                                      (0 = 4: cell = radius); any edge >= the radius gives the same
+                                      neighbours.  Exact grids (gca_grid_cells) keep cell = radius */
+    GCA_TUNE_VOXEL = 10,           /* gca_voxel_downsample binning: 0 = occupancy bitmap over the voxel
+                                      lattice up to 2^34 cells, radix sort of the packed keys beyond;
+                                      1 = always the bitmap (same fallback); 2 = always the sort.  Same
+                                      bits either way */
+    GCA_TUNE_COUNT = 11
 };
 int32_t gca_ctx_set_tuning(gca_ctx ctx, int32_t key, int32_t value);
 int32_t gca_ctx_get_tuning(gca_ctx ctx, int32_t key, int32_t *value);

@@ -11,7 +11,8 @@ class FakeCtx:
 
     def __init__(self):
         self.t = {"normals_lanes": 0, "icp_lanes_small": 0, "icp_lanes_medium": 0, "icp_lanes_large": 0,
-                  "icp_max_units": 0, "icp_persist_blocks": 0, "icp_launch": 0, "icp_lanes_batch": 0}
+                  "icp_max_units": 0, "icp_persist_blocks": 0, "icp_launch": 0, "icp_lanes_batch": 0,
+                  "frontend": 0, "grid_cell": 0}
         self.on = False
         self.calls = 0
 
@@ -35,7 +36,8 @@ class FakeCtx:
         icp = (3.0 + (lg(t["icp_lanes_small"]) - 2) ** 2 + (lg(t["icp_lanes_medium"]) - 3) ** 2
                + (lg(t["icp_lanes_large"]) - 1) ** 2 + abs(lg(t["icp_max_units"]) - 8) + 0.01 * t["icp_persist_blocks"]
                + (lg(t["icp_lanes_batch"]) - 1) ** 2)
-        return {"normals": (3, normals), "gradient": (3, 0.5), "icp": (1, icp)}
+        grid = 0.5 + 0.1 * (t["grid_cell"] - 5) ** 2  # cell edge minimal at 5 quarters of the radius
+        return {"normals": (3, normals), "gradient": (3, 0.5), "icp": (1, icp), "grid": (3, grid)}
 
 
 def test_coordinate_descent_finds_separable_minimum():
@@ -47,6 +49,7 @@ def test_coordinate_descent_finds_separable_minimum():
     assert best["icp_max_units"] == 256
     assert best["icp_persist_blocks"] == 128  # smallest candidate: time grows with it
     assert best["icp_lanes_batch"] == 2
+    assert best["grid_cell"] == 5
     # the ctx is left at the best settings, and every measurement is recorded
     assert all(ctx.t[k] == v for k, v in best.items())
     assert len(rows) >= sum(len(v) for v in autotune.space(256).values())

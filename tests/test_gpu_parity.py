@@ -67,12 +67,23 @@ def level(frame):
 
 # ---- front end ---------------------------------------------------------------------------------
 
+@pytest.mark.parametrize("kernels", ["onepass", "two_pass"])
 @pytest.mark.parametrize("mode", ["fixed", "fixed_wide", "adaptive", "none"])
-def test_frontend_bitexact(frame, ctx, mode):
+def test_frontend_bitexact(frame, ctx, mode, kernels):
     """fixed: 5x5 with its weight tables in LDS; fixed_wide: 15x15 with a range table too long for
-    LDS (sigma_r = 200 mm: ~5500 entries), read from global memory."""
+    LDS (sigma_r = 200 mm: ~5500 entries), read from global memory.  onepass: pixels, look-back
+    offset and compaction in one kernel (default); two_pass: pixels to staging, then compaction."""
     import cuda_3d_amd as g
 
+    print(f"[gpu-parity] front end {kernels} {mode}", flush=True)
+    if kernels == "two_pass":
+        with tuned(ctx, frontend=2):
+            _frontend_bitexact(g, frame, ctx, mode)
+    else:
+        _frontend_bitexact(g, frame, ctx, mode)
+
+
+def _frontend_bitexact(g, frame, ctx, mode):
     if mode == "fixed":
         bg, bo = frame.bil, frame.bilo
     elif mode == "fixed_wide":
@@ -382,11 +393,64 @@ def test_voxel_edge_cases(ctx):
     same = g.PointCloud.from_numpy(np.tile([[0.5, 0.5, 1.0]], (100, 1)), np.full((100, 3), 0.5), ctx=ctx)
     assert same.voxel_grid_down_sample(0.02).points_number() == 1
     huge = g.PointCloud.from_numpy([[0, 0, 0], [1e4, 1e4, 1e4]], np.zeros((2, 3)), ctx=ctx)
-    assert huge.voxel_grid_down_sample(1e-6) is None  # voxel too small for the lattice
+    assert huge.voxel_grid_down_sample(1e-6) is None  # over INT_MAX voxels per axis (point_cloud.cu:213-218)
     # repeated calls leave no state behind (bitmap and accumulators self-clear)
     a = same.voxel_grid_down_sample(0.02).download()[0]
     b = same.voxel_grid_down_sample(0.02).download()[0]
     np.testing.assert_array_equal(a, b)
+
+
+def _voxel_vs_oracle(cloud, xyzi, rgb, v):
+    vc = cloud.voxel_grid_down_sample(v)
+    xyz, col, inten = vc.download()
+    ox, orgb, _ = O.voxel_downsample(xyzi, rgb, v)
+    assert len(xyz) == len(ox)
+    np.testing.assert_array_equal(xyz, ox[:, :3])
+    np.testing.assert_array_equal(col, orgb[:, :3])
+    np.testing.assert_array_equal(inten, ox[:, 3])
+    return len(ox)
+
+
+@pytest.mark.parametrize("v", [0.04, 0.02, 0.01])
+def test_voxel_sorted_path_bitexact(frame, ctx, v):
+    """The radix-sort binning (tuning voxel=2) against the oracle on the frame; the default
+    (bitmap) path on the same cloud is covered by test_voxel_bitexact."""
+    with tuned(ctx, voxel=2):
+        _voxel_vs_oracle(frame.tgt, frame.otx, frame.otr, v)
+
+
+def _clustered_points(rng, n_vox, per, extent, v):
+    """n_vox voxel centres in [0, extent)^3, `per` jittered points inside each voxel."""
+    c = (np.floor(rng.uniform(0, extent / v, (n_vox, 3))) + 0.5) * v
+    p = np.repeat(c, per, 0) + rng.uniform(-0.3 * v, 0.3 * v, (n_vox * per, 3))
+    p = np.concatenate([p, [[0.0, 0.0, 0.0], [extent, extent, extent]]])  # pin the bounds
+    rgb = rng.uniform(0, 1, (len(p), 3))
+    order = rng.permutation(len(p))
+    return p[order], rgb[order]
+
+
+@pytest.mark.parametrize("case", ["48bit", "63bit", "90bit_split"])
+def test_voxel_lattice_beyond_bitmap(ctx, case):
+    """Lattices the occupancy bitmap cannot hold (over 2^34 cells) go through the sorted path with
+    no refusal, as the reference's int3 sort takes any lattice under INT_MAX voxels per axis
+    (point_cloud.cu:213-218): 2^48 cells (6 radix passes, 200k points over several sort tiles),
+    2^63 (8 passes), and 2^90 (keys wider than 64 bits: sorted by (ky, kz), then kx).
+    Bit-exact against the oracle, in either tuning."""
+    import cuda_3d_amd as g
+
+    rng = np.random.default_rng({"48bit": 1, "63bit": 2, "90bit_split": 3}[case])
+    n_vox, per, extent, v = {"48bit": (50000, 4, 50.0, 0.001), "63bit": (3000, 3, 2000.0, 0.001),
+                             "90bit_split": (3000, 3, 1000.0, 1e-6)}[case]
+    p, rgb = _clustered_points(rng, n_vox, per, extent, v)
+    pc = g.PointCloud.from_numpy(p, rgb, ctx=ctx)
+    xyzi, col, inten = pc.download()
+    ox = np.concatenate([xyzi, inten[:, None]], 1).astype(np.float32)
+    orgb = np.concatenate([col, np.zeros((len(col), 1))], 1).astype(np.float32)
+    print(f"[voxel-sort] {case}: {len(p)} points", flush=True)
+    m = _voxel_vs_oracle(pc, ox, orgb, v)
+    assert m >= n_vox * 0.99
+    with tuned(ctx, voxel=1):  # bitmap asked for: the lattice still falls back to the sort
+        _voxel_vs_oracle(pc, ox, orgb, v)
 
 
 # ---- neighbour search -------------------------------------------------------------------------------
