@@ -122,9 +122,11 @@ def run_fixed(g, ctx, cam, bil, ptrs, levels, iters, steps, graph):
     """The headline workload with the stop thresholds at 0 (every level runs `iters`)."""
     import numpy as np
 
+    from cuda_3d_amd import autotune
     from cuda_3d_amd.pipeline import FramePipeline
 
     pipe = FramePipeline(ctx, cam, bil, levels, iters, conv_rotation=0.0, conv_translation_sq=0.0, graph=graph)
+    autotune.apply(pipe.ctx, workload="frame")
     for k in range(6):
         pipe.step(*ptrs[k % 2])
     pipe.ctx.synchronize()
@@ -147,13 +149,14 @@ def run_sequence(g, ctx, cam, bil, levels, iters, W, H, n_frames, passes, graph)
     """n_frames distinct frames stepped in order (each onto the one before), `passes` times."""
     import numpy as np
 
-    from cuda_3d_amd import scene
+    from cuda_3d_amd import autotune, scene
     from cuda_3d_amd.pipeline import FramePipeline
 
     frames, poses, _ = scene.frame_sequence(31, n_frames, W, H)
     dev = [(g.device_tensor(d.view(np.int16)), g.device_tensor(c)) for d, c in frames]
     ptrs = [(d.data_ptr(), c.data_ptr()) for d, c in dev]
     pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=graph)
+    autotune.apply(pipe.ctx, workload="frame")
     errs, its = [], []
     for k in range(n_frames + 1):  # warm-up pass with a readback per frame: pose error vs truth
         pipe.step(*ptrs[k % n_frames])
@@ -300,7 +303,7 @@ def main():
     import torch
 
     import cuda_3d_amd as g
-    from cuda_3d_amd import scene
+    from cuda_3d_amd import autotune, scene
     from cuda_3d_amd.pipeline import FramePipeline
 
     W, H, levels, iters = 640, 480, (0.04, 0.02, 0.01), 30
@@ -312,6 +315,9 @@ def main():
     ptrs = [(d.data_ptr(), c.data_ptr()) for d, c in dev]
     pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=not args.no_graph)
     pctx = pipe.ctx  # the pipeline's own stream, profiler and tuning
+    # the launch shapes autotuned on MI355X (cuda_3d_amd/tuning_mi355x.json), per workload
+    tuning = {"table": os.path.relpath(autotune.DEFAULT_TABLE, ROOT),
+              "frame": autotune.apply(pctx, workload="frame"), "batch": autotune.apply(ctx, workload="batch")}
 
     k = 0
 
@@ -431,7 +437,7 @@ def main():
                    "iterations_per_level": iters_seen[-2:] if iters_seen else None,
                    "hip_graph": not args.no_graph, "latency_ms_per_frame_with_readback": lat_ms,
                    "icp_us_per_iteration_per_level": it_us,
-                   "points_full": full_n, "points_per_level": sizes},
+                   "points_full": full_n, "points_per_level": sizes, "tuning": tuning},
         "roofline": roof, "cpu_baseline": cpu, "fixed_iterations": fixed, "sequence": seq, "h2d": h2d,
         "batch": batch, "clustering": clustering, "stages": stages,
     }

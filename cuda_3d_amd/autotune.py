@@ -13,9 +13,12 @@ Method: coordinate descent.  Starting from the current values, each knob in turn
 its candidates with the others held, and the value with the lowest median time of the stages
 it drives (ctx stage profiler: hipEvents on the library's stream) is kept; `rounds` passes.
 
-    python -m cuda_3d_amd.autotune --out cuda_3d_amd/tuning_mi355x.json     # on the GPU
+    python -m cuda_3d_amd.autotune --workload frame --out cuda_3d_amd/tuning_mi355x.json  # on the GPU
+    python -m cuda_3d_amd.autotune --workload batch --out cuda_3d_amd/tuning_mi355x.json
 
-writes {"tuning": {...}, "workload": ..., "measurements": [...]}; `apply(ctx, path)` loads it.
+each writes its workload's entry {"tuning": {...}, "workload": ..., "measurements": [[round, knob,
+value, ms], ...]} into the table (other workloads' entries are kept); `apply(ctx, path, workload)`
+loads one.
 """
 from __future__ import annotations
 
@@ -103,14 +106,25 @@ def tune(ctx, workload: Callable[[], None], knobs: Dict[str, List[int]], rounds:
     return best, rows
 
 
-def apply(ctx, path: str = DEFAULT_TABLE) -> dict:
-    """Apply a saved table to a ctx (keys unknown to this build are ignored)."""
+def load(path: str = DEFAULT_TABLE, workload: str = "frame") -> dict:
+    """The saved settings of one workload ({} when the table or the entry is missing)."""
     from . import _lib as L
 
+    if not os.path.exists(path):
+        return {}
     with open(path) as fh:
-        table = json.load(fh)["tuning"]
-    settings = {k: int(v) for k, v in table.items() if k in L.TUNE_KEYS}
-    ctx.set_tuning(**settings)
+        entry = json.load(fh).get(workload)
+    if not entry:
+        return {}
+    return {k: int(v) for k, v in entry["tuning"].items() if k in L.TUNE_KEYS}
+
+
+def apply(ctx, path: str = DEFAULT_TABLE, workload: str = "frame") -> dict:
+    """Apply a saved table's settings for `workload` to a ctx (keys unknown to this build are
+    ignored)."""
+    settings = load(path, workload)
+    if settings:
+        ctx.set_tuning(**settings)
     return settings
 
 
@@ -152,10 +166,16 @@ def main(argv=None):
         what = f"64 uneven pairs x 3 levels in one persistent launch, median of {args.reps}"
     t0 = time.perf_counter()
     best, rows = tune(ctx, wl, knobs, rounds=args.rounds, reps=args.reps, log=say)
-    out = {"tuning": best, "device": torch.cuda.get_device_name(0), "compute_units": n_cu,
-           "workload": what, "seconds": time.perf_counter() - t0, "measurements": rows}
+    table = {}
+    if os.path.exists(args.out):
+        with open(args.out) as fh:
+            table = json.load(fh)
+    table[args.workload] = {
+        "tuning": best, "device": torch.cuda.get_device_name(0), "compute_units": n_cu, "workload": what,
+        "seconds": round(time.perf_counter() - t0, 1),
+        "measurements": [[r["round"], r["knob"], r["value"], round(r["ms"], 4)] for r in rows]}
     with open(args.out, "w") as fh:
-        json.dump(out, fh, indent=1)
+        json.dump(table, fh, indent=1)
     print(json.dumps({"tuning": best}), flush=True)
 
 

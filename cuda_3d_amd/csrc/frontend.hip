@@ -546,7 +546,7 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
     if (bil->mode != GCA_BILATERAL_NONE) GCA_TRY(build_lut(ctx, bil, &fp));
     GCA_TRY(cloud_reserve(out, P));
     const int64_t nblocks = (int64_t)fp.nbx * H;
-    const bool onepass = ctx->tune[GCA_TUNE_FRONTEND] != 2;
+    const bool onepass = ctx->tune[GCA_TUNE_FRONTEND] == 1;
     float *bpart = (float *)ctx_scratch(ctx, 20, sizeof(float) * 6 * nblocks, false);
     if (!bpart) return GCA_ERR_OUT_OF_MEMORY;
     // LDS of the pixel work: weight tables (fixed mode; the range table while it is short)

@@ -267,6 +267,10 @@ int32_t icp_plan_launch(gca_ctx ctx, IcpPlan *plan);
 // state's sizes from its clouds' headers, so it may be enqueued before they are final; results),
 // for overlapping state s's registration with the building of state s + 1's source.
 int32_t icp_plan_init(gca_ctx ctx, IcpPlan *plan);
+// Whether state s's target cloud still has the grid (geometry and arrays) the plan was made
+// with: a plan reads the target's grid through pointers and a geometry fixed at its creation.
+bool icp_plan_target_current(const IcpPlan *plan, int s, gca_cloud tgt);
+bool grid_same_geometry(const GridDesc &a, const GridDesc &b);
 int32_t icp_plan_persist(gca_ctx ctx, IcpPlan *plan, int s);
 // Results into the plan's pinned host memory (one kernel); extra: an optional caller word copied
 // to host_extra (pinned) by the same kernel.

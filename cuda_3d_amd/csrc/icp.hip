@@ -499,7 +499,7 @@ __device__ __forceinline__ int32_t nn_group(const PairDesc &P, float qx, float q
                     b = P.start[row + hi[2] + 1];
                 }
                 rs[r] = a;
-                cum[r + 1] = b - a;
+                cum[r + 1] = b > a ? b - a : 0u;  // a start table never decreases
             }
             cum[0] = 0;
 #pragma unroll
@@ -1709,7 +1709,24 @@ struct IcpPlan {
     uint32_t *host_err = nullptr;
     std::vector<int32_t> tail;
     std::vector<double> T0h;
+    std::vector<PairDesc> desc;  // host copy: the target grids the plan reads
 };
+
+bool grid_same_geometry(const GridDesc &a, const GridDesc &b)
+{
+    for (int k = 0; k < 3; ++k)
+        if (a.origin[k] != b.origin[k] || a.dims[k] != b.dims[k]) return false;
+    return a.cell == b.cell && a.ncells == b.ncells;
+}
+
+bool icp_plan_target_current(const IcpPlan *p, int s, gca_cloud tgt)
+{
+    if (!p || s < 0 || s >= p->n_states) return false;
+    const PairDesc &P = p->desc[(size_t)s];
+    const Grid &G = tgt->grid;
+    return G.valid && P.start == G.start && P.order == G.order && P.sxyzi == G.sxyzi && P.snrm == G.snrm &&
+           P.sgrad == G.sgrad && grid_same_geometry(P.g, G.desc);
+}
 
 void icp_plan_destroy(IcpPlan *p)
 {
@@ -1761,6 +1778,7 @@ int32_t icp_plan_create(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud 
     std::vector<StateInit> I;
     states_describe(n_pairs, n_levels, srcs, tgts, params, lanes, unit_cap, p->partials, p->nnprev, nn_off, P, I,
                     p->tail);
+    p->desc = P;
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess ||
         (e = hipMemcpy(p->pairs, P.data(), sizeof(PairDesc) * n_states, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(p->init, I.data(), sizeof(StateInit) * n_states, hipMemcpyHostToDevice)) != hipSuccess ||

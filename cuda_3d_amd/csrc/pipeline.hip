@@ -105,6 +105,14 @@ int32_t make_plan(gca_pipeline p, int slot)
     for (int l = 0; l < p->nl; ++l) {
         srcs[l] = p->lvc[slot][l];
         tgts[l] = p->lvc[1 - slot][l];
+        // the plan keeps its target's grid geometry for every later step of this parity, while
+        // the target is rebuilt each step with the pipeline's geometry: a target built before a
+        // geometry change (a tuned cell edge) is re-binned into the current geometry first
+        gca_cloud t = tgts[l];
+        if (t->grid.valid && !grid_same_geometry(t->grid.desc, p->gdesc[l])) {
+            GCA_TRY(launch_grid_build(p->ctx, t, p->gdesc[l], false));
+            GCA_TRY(grid_gather_attributes(p->ctx, t));
+        }
         prm[l] = p->lv[l].icp;
         gr[l] = p->lv[l].gradient_radius;
     }
@@ -148,6 +156,8 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
             // Enqueued here, not after every level's build, because a graph launch submits its
             // nodes in this order and a late submission is a late start.
             gca_ctx ic = p->lctx[0];
+            // the plan reads the target's grid through what it saw at its creation
+            if (!icp_plan_target_current(plan, l, p->lvc[1 - slot][l])) return GCA_ERR_DEVICE;
             if (l > 0) {
                 GCA_HIP(hipEventRecord(p->ev_built[l], lc->stream));
                 GCA_HIP(hipStreamWaitEvent(ic->stream, p->ev_built[l], 0));

@@ -176,8 +176,8 @@ enum {
                                       per iteration; 3 = one resident launch per pair and level.  Every
                                       strategy gives the same bits for the same unit partition */
     GCA_TUNE_ICP_LANES_BATCH = 7,  /* lanes per correspondence query in calls with several pairs (0 = 1) */
-    GCA_TUNE_FRONTEND = 8,         /* 0/1 = one-pass front end (pixels, look-back offset, compaction in
-                                      one kernel), 2 = pixels, then compaction (two kernels); same bits */
+    GCA_TUNE_FRONTEND = 8,         /* 0/2 = pixels, then compaction (two kernels); 1 = one pass (pixels,
+                                      look-back offset and compaction in one kernel); same bits */
     GCA_TUNE_GRID_CELL = 9,        /* neighbour-grid cell edge in quarters of the search radius, 4..12
                                       (0 = 4: cell = radius); any edge >= the radius gives the same
                                       neighbours.  Exact grids (gca_grid_cells) keep cell = radius */

@@ -12,7 +12,7 @@ class FakeCtx:
     def __init__(self):
         self.t = {"normals_lanes": 0, "icp_lanes_small": 0, "icp_lanes_medium": 0, "icp_lanes_large": 0,
                   "icp_max_units": 0, "icp_persist_blocks": 0, "icp_launch": 0, "icp_lanes_batch": 0,
-                  "frontend": 0, "grid_cell": 0}
+                  "frontend": 0, "grid_cell": 0, "voxel": 0}
         self.on = False
         self.calls = 0
 
@@ -59,7 +59,19 @@ def test_coordinate_descent_finds_separable_minimum():
 def test_table_round_trip(tmp_path):
     ctx = FakeCtx()
     path = tmp_path / "t.json"
-    path.write_text(json.dumps({"tuning": {"normals_lanes": 8, "icp_max_units": 128, "not_a_knob": 3}}))
+    path.write_text(json.dumps({"frame": {"tuning": {"normals_lanes": 8, "icp_max_units": 128, "not_a_knob": 3}},
+                                "batch": {"tuning": {"icp_lanes_batch": 2}}}))
     applied = autotune.apply(ctx, str(path))
     assert applied == {"normals_lanes": 8, "icp_max_units": 128}
     assert ctx.t["normals_lanes"] == 8 and ctx.t["icp_max_units"] == 128
+    assert autotune.apply(ctx, str(path), "batch") == {"icp_lanes_batch": 2} and ctx.t["icp_lanes_batch"] == 2
+    assert autotune.load(str(tmp_path / "missing.json")) == {}
+
+
+def test_committed_table_is_loadable():
+    """The MI355X table holds both workloads' settings, every key a knob of this build."""
+    from cuda_3d_amd import _lib as L
+
+    for w in ("frame", "batch"):
+        s = autotune.load(autotune.DEFAULT_TABLE, w)
+        assert s and all(k in L.TUNE_KEYS for k in s)

@@ -72,12 +72,12 @@ def level(frame):
 def test_frontend_bitexact(frame, ctx, mode, kernels):
     """fixed: 5x5 with its weight tables in LDS; fixed_wide: 15x15 with a range table too long for
     LDS (sigma_r = 200 mm: ~5500 entries), read from global memory.  onepass: pixels, look-back
-    offset and compaction in one kernel (default); two_pass: pixels to staging, then compaction."""
+    offset and compaction in one kernel; two_pass: pixels to staging, then compaction (default)."""
     import cuda_3d_amd as g
 
     print(f"[gpu-parity] front end {kernels} {mode}", flush=True)
-    if kernels == "two_pass":
-        with tuned(ctx, frontend=2):
+    if kernels == "onepass":
+        with tuned(ctx, frontend=1):
             _frontend_bitexact(g, frame, ctx, mode)
     else:
         _frontend_bitexact(g, frame, ctx, mode)
@@ -1165,10 +1165,13 @@ def test_pipeline_graph_replays_match_eager(ctx):
     graph.close()
 
 
-def test_pipeline_graph_follows_new_frame_buffers(ctx):
+@pytest.mark.parametrize("frontend", [0, 1])
+def test_pipeline_graph_follows_new_frame_buffers(ctx, frontend):
     """Frames arriving in four different device buffers (each frame of the pair in two copies,
     used in turn): the replayed graphs follow the new buffers through a node update, so exactly
-    two captures happen (one per frame parity), and every pose equals the eager pipeline's."""
+    two captures happen (one per frame parity), and every pose equals the eager pipeline's.
+    frontend 0: the two-kernel front end; 1: the one-pass kernel (its node carries other
+    arguments)."""
     import cuda_3d_amd as g
     from cuda_3d_amd.pipeline import FramePipeline
 
@@ -1178,6 +1181,9 @@ def test_pipeline_graph_follows_new_frame_buffers(ctx):
     bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
     eager = FramePipeline(ctx, cam, bil, graph=False)
     graph = FramePipeline(ctx, cam, bil, graph=True)
+    if frontend:
+        eager.ctx.set_tuning(frontend=frontend)
+        graph.ctx.set_tuning(frontend=frontend)
     for k in range(14):
         d, c = bufs[k % 4] if k < 8 else bufs[(k * 3) % 4]
         eager.step(d.data_ptr(), c.data_ptr())
@@ -1218,6 +1224,39 @@ def test_pipeline_tuning_change_rebuilds(ctx):
             continue
         np.testing.assert_allclose(Tp, Tr, rtol=0, atol=1e-10)
         assert [r.iterations for r in rp] == [r.iterations for r in rr]
+    ref.close()
+    pipe.close()
+
+
+def test_pipeline_grid_cell_change_mid_sequence(ctx):
+    """Regression: a registration plan reads its target's grid through the geometry and arrays
+    it saw when it was made.  Changing the grid's cell edge between steps (as the autotuner does)
+    once left the plan made at the change with the old geometry while later steps rebuilt its
+    target with the new one, an inconsistent cell table.  Now the previous frame's grids are
+    re-binned into the new geometry before the plan is made (and a plan whose target grid has
+    changed under it is refused, never launched).  The neighbour results, and so the poses, do
+    not depend on the cell edge: every step equals an untouched pipeline's, bit for bit."""
+    import cuda_3d_amd as g
+    from cuda_3d_amd.pipeline import FramePipeline
+
+    cam, dev, _, _ = _pipeline_frames(g, 320, 240, 24)
+    bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
+    ref = FramePipeline(ctx, cam, bil, graph=False)
+    pipe = FramePipeline(ctx, cam, bil, graph=True)
+    cells = {3: 5, 6: 8, 9: 4, 12: 6, 15: 0}
+    for k in range(18):
+        if k in cells:
+            print(f"[gpu-parity] step {k}: grid_cell {cells[k]}", flush=True)
+            pipe.ctx.set_tuning(grid_cell=cells[k])
+        d, c = dev[k % 2]
+        ref.step(d.data_ptr(), c.data_ptr())
+        Tr, rr = ref.result()
+        pipe.step(d.data_ptr(), c.data_ptr())
+        Tp, rp = pipe.result()
+        if k == 0:
+            continue
+        np.testing.assert_array_equal(Tp, Tr)
+        assert [rep_key(r) for r in rp] == [rep_key(r) for r in rr]
     ref.close()
     pipe.close()
 
