@@ -118,6 +118,12 @@ def level_iteration_us(reports):
     return [r.active_us / r.iterations if r.iterations else 0.0 for r in reports]
 
 
+def level_timeline_us(reports):
+    """Per level: (start, finish) of its registration in device-clock us after the step's
+    registration set-up (which follows the front end)."""
+    return [(r.latency_us - r.active_us, r.latency_us) for r in reports]
+
+
 def run_fixed(g, ctx, cam, bil, ptrs, levels, iters, steps, graph):
     """The headline workload with the stop thresholds at 0 (every level runs `iters`)."""
     import numpy as np
@@ -340,15 +346,18 @@ def main():
     ms = (t1 - t0) * 1e3 / args.steps
     log(f"{ms:.3f} ms/frame")
     # latency of one frame when the host waits for each pose
-    lat, per_it = [], []
+    lat, per_it, tl = [], [], []
     for _ in range(min(args.steps, 20)):
         t2 = time.perf_counter()
         step()
         _, reps = pipe.result()
         lat.append((time.perf_counter() - t2) * 1e3)
         per_it.append(level_iteration_us(reps))
+        tl.append(level_timeline_us(reps))
     lat_ms = float(np.median(lat))
     it_us = np.median(np.array(per_it), axis=0).tolist()
+    timeline = np.median(np.array(tl), axis=0).round(1).tolist()
+    log(f"registration start / finish by level (us after its set-up): {timeline}")
     log(f"{lat_ms:.3f} ms per frame with a pose readback after each frame; "
         f"ICP us per iteration by level {[round(x, 1) for x in it_us]}")
 
@@ -437,6 +446,7 @@ def main():
                    "iterations_per_level": iters_seen[-2:] if iters_seen else None,
                    "hip_graph": not args.no_graph, "latency_ms_per_frame_with_readback": lat_ms,
                    "icp_us_per_iteration_per_level": it_us,
+                   "icp_start_finish_us_per_level": timeline,
                    "points_full": full_n, "points_per_level": sizes, "tuning": tuning},
         "roofline": roof, "cpu_baseline": cpu, "fixed_iterations": fixed, "sequence": seq, "h2d": h2d,
         "batch": batch, "clustering": clustering, "stages": stages,

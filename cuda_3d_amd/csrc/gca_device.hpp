@@ -76,6 +76,36 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds, u
     return res;
 }
 
+// ---- runs of equal keys in a wavefront ---------------------------------------------------
+// Segment id of each lane: the lane index of the first lane of its run of equal keys (runs
+// are maximal sequences of consecutive lanes with the same key).
+__device__ __forceinline__ int run_head(int64_t key)
+{
+    const int lane = lane_id();
+    const int64_t prev = __shfl_up(key, 1, kWave);
+    int head = (lane == 0 || prev != key) ? lane : 0;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int o = __shfl_up(head, d, kWave);
+        if (lane >= d) head = max(head, o);
+    }
+    return head;
+}
+
+// In-run suffix reduction: afterwards the run head holds the run's total.
+template <typename T, typename Op>
+__device__ __forceinline__ T run_reduce(T v, int head, Op op)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T o = __shfl_down(v, d, kWave);
+        const int oh = __shfl_down(head, d, kWave);
+        if (lane + d < kWave && oh == head) v = op(v, o);
+    }
+    return v;
+}
+
 // ---- decoupled look-back of one-pass scans ----------------------------------------------
 // Chunks (workgroups) are handed out in start order by a counter, so a chunk only ever waits on
 // chunks whose workgroups are already running.  Each chunk publishes one 64-bit word per call:

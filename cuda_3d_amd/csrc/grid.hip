@@ -49,12 +49,23 @@ __global__ __launch_bounds__(256) void k_grid_scatter(const float4 *__restrict__
                                                       float4 *__restrict__ sxyzi)
 {
     const int n = hdr->n;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t lin = cell_of[i];
-        uint32_t slot = atomicSub(&count[lin], 1u) - 1u;
-        uint32_t pos = start[lin] + slot;
-        order[pos] = i;
-        sxyzi[pos] = xyzi[i];
+    const int lane = lane_id();
+    for (int b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {  // block-uniform
+        const int i = b + (int)threadIdx.x;
+        const bool on = i < n;
+        const uint32_t lin = on ? cell_of[i] : 0u;
+        // a run of equal cells in the wavefront takes its slots with one atomic (the order of
+        // points within a cell is free: every query breaks ties by original index)
+        const int head = run_head(on ? (int64_t)lin : -1 - (int64_t)lane);
+        const uint32_t len = run_reduce(on ? 1u : 0u, head, [](uint32_t x, uint32_t y) { return x + y; });
+        uint32_t top = 0u;
+        if (on && head == lane) top = atomicSub(&count[lin], len);
+        top = __shfl(top, head, kWave);
+        if (on) {
+            const uint32_t pos = start[lin] + top - 1u - (uint32_t)(lane - head);
+            order[pos] = i;
+            sxyzi[pos] = xyzi[i];
+        }
     }
 }
 

@@ -84,35 +84,6 @@ __device__ __forceinline__ int64_t voxel_key(const VoxelParams &vp, float4 p)
     return (k[0] * vp.dims[1] + k[1]) * vp.dims[2] + k[2];
 }
 
-// Segment id of each lane: the lane index of the first lane of its run of equal keys (runs
-// are maximal sequences of consecutive lanes with the same key).
-__device__ __forceinline__ int run_head(int64_t key)
-{
-    const int lane = lane_id();
-    const int64_t prev = __shfl_up(key, 1, kWave);
-    int head = (lane == 0 || prev != key) ? lane : 0;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const int o = __shfl_up(head, d, kWave);
-        if (lane >= d) head = max(head, o);
-    }
-    return head;
-}
-
-// In-run suffix reduction: afterwards the run head holds the run's total.
-template <typename T, typename Op>
-__device__ __forceinline__ T run_reduce(T v, int head, Op op)
-{
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const T o = __shfl_down(v, d, kWave);
-        const int oh = __shfl_down(head, d, kWave);
-        if (lane + d < kWave && oh == head) v = op(v, o);
-    }
-    return v;
-}
-
 // Raster-ordered points hit the same voxel in runs, so each wavefront first merges runs and
 // only the first lane of a run issues the atomics.  Every workgroup derives the lattice from the
 // header itself; workgroup 0 also stores it for the later kernels of the call and flags a
@@ -156,10 +127,12 @@ __global__ __launch_bounds__(256) void k_voxel_mark(const float4 *__restrict__ x
 // probing); it then hands each distinct voxel's sums to global memory once.  Device-scope 64-bit
 // atomics bound this kernel, so the hand-over takes one compare-and-swap on the voxel's claim
 // word: the workgroup that wins it stores its sums plainly into the voxel's "A" record, the
-// others (and points whose probe sequence found no slot) add theirs atomically into the "B"
-// record.  Raster chunks of 2048 points meet 1.3 / 1.9 / 3.0 workgroups per voxel at 1 / 2 / 4
-// cm on a 640x480 frame, so most voxels take one CAS instead of seven atomics.  Integer sums
-// are exact, so neither the arrival order nor the split into A and B changes the result.
+// others add theirs atomically into the "B" record (a point whose probe sequence found no slot
+// does the same with its own values).  Raster chunks of 2048 points meet 1.3 / 1.9 / 3.0
+// workgroups per voxel at 1 / 2 / 4 cm on a 640x480 frame, so most voxels take one CAS instead
+// of seven atomics.  Integer sums are exact, so neither the arrival order nor the split into A
+// and B changes the result.  Every voxel's A record is written by its claimant, so only B and
+// the claim need clearing after use.
 // Per-voxel record (kVoxRec words): A sums [0, 7), lattice key [7], B sums [8, 15), claim [15].
 constexpr int kVoxRec = 16;
 constexpr int kVoxChunk = 2048;
@@ -219,8 +192,12 @@ __global__ __launch_bounds__(kVoxBlock) void k_voxel_accum(const float4 *__restr
                 for (int k = 0; k < 7; ++k) atomicAdd(&t_sum[k][slot], v[k]);
             } else {
                 unsigned long long *a = acc + kVoxRec * (int64_t)r;
+                const bool own = atomicCAS(&a[15], 0ull, 1ull) == 0ull;
 #pragma unroll
-                for (int k = 0; k < 7; ++k) atomicAdd(&a[8 + k], v[k]);
+                for (int k = 0; k < 7; ++k) {
+                    if (own) a[k] = v[k];
+                    else atomicAdd(&a[8 + k], v[k]);
+                }
                 a[7] = (unsigned long long)L;
             }
         }
@@ -258,31 +235,43 @@ __global__ __launch_bounds__(256) void k_voxel_finalize(unsigned long long *__re
         out_hdr->bmin_key[threadIdx.x] = in_hdr->bmin_key[threadIdx.x];
         out_hdr->bmax_key[threadIdx.x] = in_hdr->bmax_key[threadIdx.x];
     }
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x) {
-        unsigned long long *a = acc + kVoxRec * (int64_t)r;
-        long long sum[7];
+    for (int b = blockIdx.x * blockDim.x; b < m; b += gridDim.x * blockDim.x) {  // block-uniform
+        const int r = b + (int)threadIdx.x;
+        const bool on = r < m;
+        int64_t cell = -1 - (int64_t)lane_id();  // unique keys for inactive lanes
+        if (on) {
+            unsigned long long *a = acc + kVoxRec * (int64_t)r;
+            long long sum[7];
 #pragma unroll
-        for (int c = 0; c < 7; ++c) sum[c] = (long long)(a[c] + a[8 + c]);  // A + B (two's complement)
-        const double cnt = (double)sum[6];
-        float o[6];
+            for (int c = 0; c < 7; ++c) sum[c] = (long long)(a[c] + a[8 + c]);  // A + B (two's complement)
+            const double cnt = (double)sum[6];
+            float o[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) o[c] = (float)(((double)sum[c] / cnt) * (1.0 / kFx30));
-        out_xyzi[r] = make_float4(o[0], o[1], o[2], intensity_of(o[3], o[4], o[5]));
-        out_rgb[r] = make_float4(o[3], o[4], o[5], 0.0f);
-        if (gc_count) {  // the grid's count pass for this point (same cell rule as k_grid_count)
-            const int32_t cx = grid_cell(o[0], gd.origin[0], gd.cell, gd.dims[0]);
-            const int32_t cy = grid_cell(o[1], gd.origin[1], gd.cell, gd.dims[1]);
-            const int32_t cz = grid_cell(o[2], gd.origin[2], gd.cell, gd.dims[2]);
-            const uint32_t lin = (uint32_t)(((int64_t)cx * gd.dims[1] + cy) * gd.dims[2] + cz);
-            gc_cell_of[r] = lin;
-            atomicAdd(&gc_count[lin], 1u);
+            for (int c = 0; c < 6; ++c) o[c] = (float)(((double)sum[c] / cnt) * (1.0 / kFx30));
+            out_xyzi[r] = make_float4(o[0], o[1], o[2], intensity_of(o[3], o[4], o[5]));
+            out_rgb[r] = make_float4(o[3], o[4], o[5], 0.0f);
+            if (gc_count) {  // the grid's count pass for this point (same cell rule as k_grid_count)
+                const int32_t cx = grid_cell(o[0], gd.origin[0], gd.cell, gd.dims[0]);
+                const int32_t cy = grid_cell(o[1], gd.origin[1], gd.cell, gd.dims[1]);
+                const int32_t cz = grid_cell(o[2], gd.origin[2], gd.cell, gd.dims[2]);
+                const uint32_t lin = (uint32_t)(((int64_t)cx * gd.dims[1] + cy) * gd.dims[2] + cz);
+                gc_cell_of[r] = lin;
+                cell = lin;
+            }
+            if (bitmap) {
+                const int64_t L = (int64_t)a[7];
+                bitmap[L >> 5] = 0u;
+            }
+#pragma unroll
+            for (int c = 8; c < kVoxRec; ++c) a[c] = 0ull;  // B and the claim (A is always rewritten)
         }
-        if (bitmap) {
-            const int64_t L = (int64_t)a[7];
-            bitmap[L >> 5] = 0u;
+        if (gc_count) {
+            // voxels come out in (kx, ky, kz) order, so neighbours often share a grid cell: one
+            // atomic per run of equal cells in the wavefront
+            const int head = run_head(cell);
+            const uint32_t len = run_reduce(on ? 1u : 0u, head, [](uint32_t x, uint32_t y) { return x + y; });
+            if (on && head == lane_id()) atomicAdd(&gc_count[cell], len);
         }
-#pragma unroll
-        for (int c = 0; c < kVoxRec; ++c) a[c] = 0ull;
     }
 }
 
@@ -337,8 +326,8 @@ __global__ __launch_bounds__(256) void k_vsort_flag(const unsigned long long *__
 
 // Segmented reduce: each wavefront merges the runs of equal rank among its 64 sorted positions;
 // the first lane of a run stores its sums into the voxel's A record when the whole voxel lies in
-// this wavefront (one writer), and adds them atomically into its B record otherwise (voxels
-// that cross wavefronts; the records are zero, since the finalize pass clears what it reads).
+// this wavefront (one writer); a voxel that crosses wavefronts is handed over by claim, as in
+// the bitmap path.
 __global__ __launch_bounds__(256) void k_vsort_reduce(const float4 *__restrict__ xyzi, const float4 *__restrict__ rgb,
                                                       const uint32_t *__restrict__ idx,
                                                       const uint32_t *__restrict__ flag,
@@ -378,12 +367,13 @@ __global__ __launch_bounds__(256) void k_vsort_reduce(const float4 *__restrict__
         v[k] = run_reduce(v[k], head, [](unsigned long long a, unsigned long long c) { return a + c; });
     if (on && head == lane) {
         unsigned long long *a = acc + kVoxRec * r;
-        if (f && ends) {
+        // a voxel inside this wavefront has one writer; a crossing one is claimed as in the
+        // bitmap path (the claimant's A record, the others' B record)
+        const bool own = (f && ends) || atomicCAS(&a[15], 0ull, 1ull) == 0ull;
 #pragma unroll
-            for (int k = 0; k < 7; ++k) a[k] = v[k];  // A record
-        } else {
-#pragma unroll
-            for (int k = 0; k < 7; ++k) atomicAdd(&a[8 + k], v[k]);  // B record
+        for (int k = 0; k < 7; ++k) {
+            if (own) a[k] = v[k];
+            else atomicAdd(&a[8 + k], v[k]);
         }
     }
 }

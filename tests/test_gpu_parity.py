@@ -1282,3 +1282,4 @@ def test_pipeline_rejects_bad_arguments(ctx):
     assert L.lib.gca_pipeline_result(h, T.ctypes.data_as(C.c_void_p), None, C.byref(valid)) == 0
     assert valid.value == 0 and np.array_equal(T.reshape(4, 4), np.eye(4))
     assert L.lib.gca_pipeline_destroy(h) == 0
+
