@@ -12,10 +12,11 @@
 // Here the components come from a lock-free union-find on the device:
 //   1. init     parent[i] = i, size[i] = 0
 //   2. hook     every point, walked in cell order for locality, links its root and the root of
-//               each lower-indexed neighbour within the tolerance (the radius search's own
-//               predicate: squared FP64 distance < r^2, and for the residual form the residual
-//               gate); a root is only ever linked to a smaller one by compare-and-swap, so every
-//               tree's root is its component's lowest index, which is the reference's seed
+//               each neighbour within the tolerance that comes before it in cell order (the
+//               radius search's own predicate: squared FP64 distance < r^2, and for the residual
+//               form the residual gate); a root is only ever linked to a smaller one by
+//               compare-and-swap, so every tree's root is its component's lowest index, which is
+//               the reference's seed
 //   3. compress parent[i] = root (a read-only walk, then a store to the point's own entry),
 //               size[root] += 1 (one atomic per wave and root run)
 //   4. cut      only for components above max_cluster_size (listed by k_cc_over): the
@@ -115,6 +116,27 @@ __device__ __forceinline__ bool residual_gate(const float *__restrict__ res, int
     return res == nullptr || (double)fabsf(res[a] - res[b]) < 0.003;
 }
 
+// Hook: kCcLanes lanes per point (points in cell order), each lane a share of the point's
+// candidates from the 27 cells around it, kCcUnroll loads in flight per lane.  Each edge is taken
+// from its later endpoint in cell order only (candidates at sorted positions before the point's
+// own), which halves the candidates and changes nothing: the components, and their roots (each
+// root is its component's lowest original index), do not depend on which end links an edge.
+constexpr int kCcLanes = 8;
+constexpr int kCcUnroll = 4;
+constexpr int kCcRuns = 9;  // z-runs of cells per point when the cell is at least the radius
+
+__device__ __forceinline__ void cc_link(int32_t *parent, int32_t &ri, int32_t j)
+{
+    const int32_t rj = cc_find(parent, j);
+    if (rj != ri) {
+        ri = cc_find(parent, ri);  // short walk: ri was a root not long ago
+        if (rj != ri) {
+            cc_union(parent, ri, rj);  // a compare-and-swap only for distinct roots
+            ri = cc_find(parent, ri);
+        }
+    }
+}
+
 __global__ __launch_bounds__(kCcBlock) void k_cc_hook(const CloudHdr *__restrict__ h, GridDesc g,
                                                      const uint32_t *__restrict__ start,
                                                      const float4 *__restrict__ sxyzi,
@@ -122,33 +144,66 @@ __global__ __launch_bounds__(kCcBlock) void k_cc_hook(const CloudHdr *__restrict
                                                      const float *__restrict__ res, int32_t *parent)
 {
     const int n = h->n;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+    const int gl = (int)threadIdx.x & (kCcLanes - 1);
+    const int s = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kCcLanes);  // the group's point
+    if (s >= n) return;  // the whole group
     const double r2 = (double)g.radius * (double)g.radius;
     const float4 p = sxyzi[s];
     const int32_t i = order[s];
     int32_t lo[3], hi[3];
     if (!grid_range(g, p.x, p.y, p.z, lo, hi)) return;
+    const int nry = hi[1] - lo[1] + 1;
+    const int nruns = (hi[0] - lo[0] + 1) * nry;
     int32_t ri = cc_find(parent, i);  // this point's root, refreshed after each link
-    for (int32_t ix = lo[0]; ix <= hi[0]; ++ix)
-        for (int32_t iy = lo[1]; iy <= hi[1]; ++iy) {
-            const int64_t row = ((int64_t)ix * g.dims[1] + iy) * g.dims[2];
-            const uint32_t e = start[row + hi[2] + 1];
-            for (uint32_t t = start[row + lo[2]]; t < e; ++t) {
-                const float4 q = sxyzi[t];
-                const int32_t j = order[t];
-                if (j < i && dist2(q.x, q.y, q.z, p.x, p.y, p.z) < r2 && residual_gate(res, i, j)) {
-                    const int32_t rj = cc_find(parent, j);
-                    if (rj != ri) {
-                        ri = cc_find(parent, ri);  // short walk: ri was a root not long ago
-                        if (rj != ri) {
-                            cc_union(parent, ri, rj);  // a compare-and-swap only for distinct roots
-                            ri = cc_find(parent, ri);
-                        }
-                    }
+    auto consider = [&](const float4 &q, int32_t j) {
+        if (dist2(q.x, q.y, q.z, p.x, p.y, p.z) < r2 && residual_gate(res, i, j)) cc_link(parent, ri, j);
+    };
+    if (nruns <= kCcRuns) {
+        // run bounds up front, each run clamped to the positions before s; one flattened walk
+        uint32_t rs[kCcRuns], cum[kCcRuns + 1];
+        cum[0] = 0;
+#pragma unroll
+        for (int r = 0; r < kCcRuns; ++r) {
+            uint32_t a = 0, b = 0;
+            if (r < nruns) {
+                const int ix = lo[0] + r / nry, iy = lo[1] + r % nry;
+                const int64_t row = ((int64_t)ix * g.dims[1] + iy) * g.dims[2];
+                a = start[row + lo[2]];
+                b = min(start[row + hi[2] + 1], (uint32_t)s);
+            }
+            rs[r] = a;
+            cum[r + 1] = cum[r] + (b > a ? b - a : 0u);
+        }
+        const uint32_t total = cum[kCcRuns];
+        for (uint32_t f0 = gl; f0 < total; f0 += kCcLanes * kCcUnroll) {
+            float4 q[kCcUnroll];
+            int32_t jj[kCcUnroll];
+#pragma unroll
+            for (int u = 0; u < kCcUnroll; ++u) {
+                const uint32_t f = f0 + u * kCcLanes;
+                jj[u] = -1;
+                if (f < total) {
+                    uint32_t t = rs[0] + f;
+#pragma unroll
+                    for (int r = 1; r < kCcRuns; ++r)
+                        if (f >= cum[r]) t = rs[r] + (f - cum[r]);
+                    q[u] = sxyzi[t];
+                    jj[u] = order[t];
                 }
             }
+#pragma unroll
+            for (int u = 0; u < kCcUnroll; ++u)
+                if (jj[u] >= 0) consider(q[u], jj[u]);
         }
+    } else {
+        // cell smaller than the radius (not produced by grid_build): plain run walk
+        for (int32_t ix = lo[0]; ix <= hi[0]; ++ix)
+            for (int32_t iy = lo[1]; iy <= hi[1]; ++iy) {
+                const int64_t row = ((int64_t)ix * g.dims[1] + iy) * g.dims[2];
+                const uint32_t e = min(start[row + hi[2] + 1], (uint32_t)s);
+                for (uint32_t t = start[row + lo[2]] + gl; t < e; t += kCcLanes) consider(sxyzi[t], order[t]);
+            }
+    }
 }
 
 __global__ __launch_bounds__(kCcBlock) void k_cc_compress(const CloudHdr *__restrict__ h, int32_t *parent,
@@ -398,7 +453,9 @@ int32_t launch_euclidean_clustering(gca_ctx ctx, gca_cloud c, float tolerance, u
     if (!size || !kept || !bcnt || !boff || !jobs) return GCA_ERR_OUT_OF_MEMORY;
     const dim3 grid((unsigned)nb), block(kCcBlock);
     hipLaunchKernelGGL(k_cc_init, grid, block, 0, ctx->stream, c->hdr, label, size, counts_dev + 1);
-    hipLaunchKernelGGL(k_cc_hook, grid, block, 0, ctx->stream, c->hdr, G.desc, G.start, G.sxyzi, G.order, res, label);
+    const dim3 grid_hook((unsigned)std::max<int64_t>(1, (c->cap * kCcLanes + kCcBlock - 1) / kCcBlock));
+    hipLaunchKernelGGL(k_cc_hook, grid_hook, block, 0, ctx->stream, c->hdr, G.desc, G.start, G.sxyzi, G.order, res,
+                       label);
     hipLaunchKernelGGL(k_cc_compress, grid, block, 0, ctx->stream, c->hdr, label, size);
     hipLaunchKernelGGL(k_cc_over, grid, block, 0, ctx->stream, c->hdr, label, size, max_size, jobs, counts_dev + 1);
     GCA_HIP(hipGetLastError());
