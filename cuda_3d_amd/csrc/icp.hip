@@ -99,14 +99,14 @@ struct PairDesc {
     float sl, sc;        // sqrt(lambda), sqrt(1 - lambda)
     int32_t nblk;        // single-pair kernels: workgroups
     int32_t unit_cap;    // cap on work units per iteration
-    int32_t lanes[3];    // lanes per query for n < 16384, n < 131072, larger
+    int32_t lanes[3];    // lanes per query for n < 16384, n < 49152, larger
 };
 
 // Launch-shape rules shared by host and device: the unit partition fixes the summation order, so
 // it must come out the same wherever it is computed.
 __host__ __device__ inline int icp_lanes_for(const int32_t lanes[3], int64_t n)
 {
-    return n < 16384 ? lanes[0] : (n < 131072 ? lanes[1] : lanes[2]);
+    return n < 16384 ? lanes[0] : (n < 49152 ? lanes[1] : lanes[2]);
 }
 
 __host__ __device__ inline int32_t icp_units_for(int64_t n, int G, int32_t cap)
@@ -1369,7 +1369,7 @@ static int32_t icp_blocks(int64_t n) { return (int32_t)std::min<int64_t>(1024, s
 // overrides them.
 static void icp_shape(gca_ctx ctx, int n_pairs, int32_t lanes[3], int32_t *unit_cap)
 {
-    static const int32_t def[3] = {8, 4, 4};
+    static const int32_t def[3] = {8, 4, 1};
     for (int k = 0; k < 3; ++k) {
         if (n_pairs > 1) lanes[k] = ctx->tune[GCA_TUNE_ICP_LANES_BATCH] ? ctx->tune[GCA_TUNE_ICP_LANES_BATCH] : 1;
         else lanes[k] = ctx->tune[GCA_TUNE_ICP_LANES_SMALL + k] ? ctx->tune[GCA_TUNE_ICP_LANES_SMALL + k] : def[k];

@@ -166,7 +166,7 @@ int32_t gca_ctx_profile(gca_ctx ctx, int32_t enable);  /* enable != 0 also clear
 enum {
     GCA_TUNE_ICP_LANES_SMALL = 0,  /* lanes per correspondence query in one-pair calls, clouds < 16k
                                       points: 1,2,4,8,16,32,64 */
-    GCA_TUNE_ICP_LANES_MEDIUM = 1, /* ... clouds < 128k points */
+    GCA_TUNE_ICP_LANES_MEDIUM = 1, /* ... clouds < 48k points */
     GCA_TUNE_ICP_LANES_LARGE = 2,  /* ... larger clouds */
     GCA_TUNE_ICP_MAX_UNITS = 3,    /* cap on work units (FP64 partials) per pair and iteration, 1..1024 */
     GCA_TUNE_NORMALS_LANES = 4,    /* lanes per normal/gradient query: 1,2,4,8,16,32,64 */
