@@ -124,35 +124,44 @@ def level_timeline_us(reports):
     return [(r.latency_us - r.active_us, r.latency_us) for r in reports]
 
 
-def run_fixed(g, ctx, cam, bil, ptrs, levels, iters, steps, graph):
-    """The headline workload with the stop thresholds at 0 (every level runs `iters`)."""
+def run_fixed(g, ctx, cam, bil, ptrs, levels, iters, steps, graph, overlap):
+    """The headline workload with the stop thresholds at 0 (every level runs `iters`), in the
+    headline's mode and sequentially."""
     import numpy as np
 
     from cuda_3d_amd import autotune
     from cuda_3d_amd.pipeline import FramePipeline
 
-    pipe = FramePipeline(ctx, cam, bil, levels, iters, conv_rotation=0.0, conv_translation_sq=0.0, graph=graph)
-    autotune.apply(pipe.ctx, workload="frame")
-    for k in range(6):
-        pipe.step(*ptrs[k % 2])
-    pipe.ctx.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps):
-        pipe.step(*ptrs[k % 2])
-    pipe.ctx.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / steps
-    per_it = []
-    for k in range(6):
-        pipe.step(*ptrs[k % 2])
-        per_it.append(level_iteration_us(pipe.result()[1]))
-    its = pipe.iterations()
-    pipe.close()
-    return {"ms_per_frame": ms, "iterations_per_level": its, "steps": steps,
-            "icp_us_per_iteration_per_level": np.median(np.array(per_it), axis=0).tolist()}
+    out = {}
+    for ov in ((True, False) if overlap else (False,)):
+        pipe = FramePipeline(ctx, cam, bil, levels, iters, conv_rotation=0.0, conv_translation_sq=0.0, graph=graph,
+                             overlap=ov)
+        autotune.apply(pipe.ctx, workload="frame")
+        for k in range(8):
+            pipe.step(*ptrs[k % 2])
+        pipe.ctx.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            pipe.step(*ptrs[k % 2])
+        pipe.ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        out["ms_per_frame" if ov or not overlap else "sequential_ms_per_frame"] = ms
+        if not ov:  # per-iteration device times from a sequential pipeline's reports
+            per_it = []
+            for k in range(6):
+                pipe.step(*ptrs[k % 2])
+                per_it.append(level_iteration_us(pipe.result()[1]))
+            out["iterations_per_level"] = pipe.iterations()
+            out["icp_us_per_iteration_per_level"] = np.median(np.array(per_it), axis=0).tolist()
+        pipe.close()
+    out["steps"] = steps
+    return out
 
 
-def run_sequence(g, ctx, cam, bil, levels, iters, W, H, n_frames, passes, graph):
-    """n_frames distinct frames stepped in order (each onto the one before), `passes` times."""
+def run_sequence(g, ctx, cam, bil, levels, iters, W, H, n_frames, passes, graph, overlap):
+    """n_frames distinct frames stepped in order (each onto the one before), `passes` times: the
+    pose error against the generating poses from a sequential pass with a readback per frame, then
+    the timed passes in the headline's mode."""
     import numpy as np
 
     from cuda_3d_amd import autotune, scene
@@ -161,12 +170,12 @@ def run_sequence(g, ctx, cam, bil, levels, iters, W, H, n_frames, passes, graph)
     frames, poses, _ = scene.frame_sequence(31, n_frames, W, H)
     dev = [(g.device_tensor(d.view(np.int16)), g.device_tensor(c)) for d, c in frames]
     ptrs = [(d.data_ptr(), c.data_ptr()) for d, c in dev]
-    pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=graph)
-    autotune.apply(pipe.ctx, workload="frame")
     errs, its = [], []
-    for k in range(n_frames + 1):  # warm-up pass with a readback per frame: pose error vs truth
-        pipe.step(*ptrs[k % n_frames])
-        T, reps = pipe.result()
+    seq = FramePipeline(ctx, cam, bil, levels, iters, graph=graph)
+    autotune.apply(seq.ctx, workload="frame")
+    for k in range(n_frames + 1):
+        seq.step(*ptrs[k % n_frames])
+        T, reps = seq.result()
         if T is None:
             continue
         a, b = poses[(k - 1) % n_frames], poses[k % n_frames]
@@ -174,6 +183,11 @@ def run_sequence(g, ctx, cam, bil, levels, iters, W, H, n_frames, passes, graph)
         dR = T[:3, :3] @ truth[:3, :3].T
         errs.append(float(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1))))
         its.append([r.iterations for r in reps])
+    seq.close()
+    pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=graph, overlap=overlap)
+    autotune.apply(pipe.ctx, workload="frame")
+    for k in range(n_frames + 1):  # eager steps of every slot, then graph capture
+        pipe.step(*ptrs[k % n_frames])
     pipe.ctx.synchronize()
     t0 = time.perf_counter()
     steps = n_frames * passes
@@ -183,7 +197,7 @@ def run_sequence(g, ctx, cam, bil, levels, iters, W, H, n_frames, passes, graph)
     ms = (time.perf_counter() - t0) * 1e3 / steps
     captures, updates = pipe.graph_stats()
     pipe.close()
-    return {"frames": n_frames, "steps": steps, "ms_per_frame": ms, "graph_captures": captures,
+    return {"frames": n_frames, "steps": steps, "ms_per_frame": ms, "overlap": overlap, "graph_captures": captures,
             "graph_node_updates": updates, "iterations_per_level": its,
             "rotation_err_vs_truth_rad_median": float(np.median(errs)), "rotation_err_vs_truth_rad_max": float(max(errs))}
 
@@ -294,6 +308,9 @@ def main():
     ap.add_argument("--no-sequence", action="store_true", help="skip the 8-frame sequence run")
     ap.add_argument("--no-roofline-pass", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly (no hipGraph replay)")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="headline in sequential steps (default: each step registers the previous frame beside "
+                         "the new frame's build)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (reference traffic for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE) and exit")
@@ -319,38 +336,50 @@ def main():
     bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
     dev = [(g.device_tensor(td.view(np.int16)), g.device_tensor(tc)), (g.device_tensor(sd.view(np.int16)), g.device_tensor(sc))]
     ptrs = [(d.data_ptr(), c.data_ptr()) for d, c in dev]
-    pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=not args.no_graph)
-    pctx = pipe.ctx  # the pipeline's own stream, profiler and tuning
+    overlap = not args.no_overlap
+    # the headline pipeline, and a sequential one for the per-frame latency, the registration
+    # timeline and the roofline pass (which read each frame's own registration after its step)
+    pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=not args.no_graph, overlap=overlap)
+    seqp = FramePipeline(ctx, cam, bil, levels, iters, graph=not args.no_graph)
+    pctx = seqp.ctx  # the sequential pipeline's own stream, profiler and tuning
     # the launch shapes autotuned on MI355X (cuda_3d_amd/tuning_mi355x.json), per workload
     tuning = {"table": os.path.relpath(autotune.DEFAULT_TABLE, ROOT),
-              "frame": autotune.apply(pctx, workload="frame"), "batch": autotune.apply(ctx, workload="batch")}
+              "frame": autotune.apply(pipe.ctx, workload="frame"), "batch": autotune.apply(ctx, workload="batch")}
+    autotune.apply(pctx, workload="frame")
 
-    k = 0
+    counters = {}
 
-    def step():
-        nonlocal k
-        d, c = ptrs[k % 2]
-        pipe.step(d, c)
-        k += 1
+    def step(p):
+        i = counters.get(id(p), 0)
+        d, c = ptrs[i % 2]
+        p.step(d, c)
+        counters[id(p)] = i + 1
 
-    for _ in range(max(args.warmup, 5)):  # eager steps of both frame parities, then graph capture
-        step()
-    pctx.synchronize()
-    torch.cuda.synchronize()
-    log(f"warmup done; timing {args.steps} steps ({'graph replay' if not args.no_graph else 'eager'})")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    pctx.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
+    def timed(p, n):
+        p.ctx.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            step(p)
+        p.ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / n
+
+    for _ in range(max(args.warmup, 8)):  # eager steps of every frame slot, then graph capture
+        step(pipe)
+        step(seqp)
+    log(f"warmup done; timing {args.steps} steps ({'graph replay' if not args.no_graph else 'eager'}, "
+        f"{'overlapped' if overlap else 'sequential'})")
+    ms = timed(pipe, args.steps)
     log(f"{ms:.3f} ms/frame")
-    # latency of one frame when the host waits for each pose
+    seq_ms = timed(seqp, args.steps) if overlap else ms
+    if overlap:
+        log(f"{seq_ms:.3f} ms/frame in sequential steps")
+    # latency of one frame when the host waits for each pose (sequential steps)
     lat, per_it, tl = [], [], []
     for _ in range(min(args.steps, 20)):
         t2 = time.perf_counter()
-        step()
-        _, reps = pipe.result()
+        step(seqp)
+        _, reps = seqp.result()
         lat.append((time.perf_counter() - t2) * 1e3)
         per_it.append(level_iteration_us(reps))
         tl.append(level_timeline_us(reps))
@@ -361,26 +390,26 @@ def main():
     log(f"{lat_ms:.3f} ms per frame with a pose readback after each frame; "
         f"ICP us per iteration by level {[round(x, 1) for x in it_us]}")
 
-    sizes = [pipe.level_sizes(s) for s in (0, 1)]
-    full_n = [pipe.full_size(s) for s in (0, 1)]
+    sizes = [seqp.level_sizes(s) for s in (0, 1)]
+    full_n = [seqp.full_size(s) for s in (0, 1)]
     roof = None
     stages = None
     iters_seen = []
     if not args.no_roofline_pass:
         # the same kernels launched eagerly, each stage bracketed by hipEvents on the stream
-        pipe.set_graph(False)
+        seqp.set_graph(False)
         pctx.profile(True)
         algo = 0.0
         for _ in range(args.steps):
-            new = pipe.slot
-            step()
-            it = pipe.iterations()
+            new = seqp.slot
+            step(seqp)
+            it = seqp.iterations()
             iters_seen.append(it)
             src, tgt = new, 1 - new
             algo += sum(it[l] * (24.0 * sizes[src][l] + 48.0 * sizes[tgt][l]) for l in range(len(levels)))
         prof = pctx.profile_read()
         pctx.profile(False)
-        pipe.set_graph(not args.no_graph)
+        seqp.set_graph(not args.no_graph)
         n_icp, icp_ms = prof["icp"]
         per_launch_bytes = algo / max(n_icp, 1)
         avg_launch_ms = icp_ms / max(n_icp, 1)
@@ -407,13 +436,15 @@ def main():
 
     fixed = None
     if not args.no_fixed:
-        fixed = run_fixed(g, ctx, cam, bil, ptrs, levels, iters, args.steps, not args.no_graph)
-        log(f"fixed {iters} iterations per level: {fixed['ms_per_frame']:.3f} ms/frame; ICP us per iteration "
-            f"by level {[round(x, 1) for x in fixed['icp_us_per_iteration_per_level']]}")
+        fixed = run_fixed(g, ctx, cam, bil, ptrs, levels, iters, max(10, args.steps // 4), not args.no_graph, overlap)
+        log(f"fixed {iters} iterations per level: {fixed['ms_per_frame']:.3f} ms/frame"
+            + (f" ({fixed['sequential_ms_per_frame']:.3f} sequential)" if overlap else "")
+            + f"; ICP us per iteration by level {[round(x, 1) for x in fixed['icp_us_per_iteration_per_level']]}")
 
     seq = None
     if not args.no_sequence:
-        seq = run_sequence(g, ctx, cam, bil, levels, iters, W, H, 8, max(2, args.steps // 8), not args.no_graph)
+        seq = run_sequence(g, ctx, cam, bil, levels, iters, W, H, 8, max(2, args.steps // 8), not args.no_graph,
+                           overlap)
         log(f"sequence of {seq['frames']} distinct frames: {seq['ms_per_frame']:.3f} ms/frame "
             f"({seq['graph_captures']} captures, {seq['graph_node_updates']} node updates); rotation error vs "
             f"truth median {seq['rotation_err_vs_truth_rad_median']:.2e} rad")
@@ -441,10 +472,12 @@ def main():
         "data": "synthetic seeded RGB-D scene (3 planes + 4 spheres, 2 mm noise, 5% holes), frames resident in HBM",
         "config": {"workload": "640x480 depth+RGB -> bilateral 5x5 -> voxel 0.04/0.02/0.01 -> normals r=2.5v K=30 "
                                "+ colour gradient r=2v -> 3-level colored ICP, max 30 it/level, reference "
-                               "stopping rule",
+                               "stopping rule" + ("; overlapped steps: frame k built while frame k-1 registers"
+                                                  if overlap else ""),
                    "width": W, "height": H, "levels": list(levels), "max_iters_per_level": iters,
                    "iterations_per_level": iters_seen[-2:] if iters_seen else None,
-                   "hip_graph": not args.no_graph, "latency_ms_per_frame_with_readback": lat_ms,
+                   "hip_graph": not args.no_graph, "overlapped_steps": overlap,
+                   "sequential_ms_per_frame": seq_ms, "latency_ms_per_frame_with_readback": lat_ms,
                    "icp_us_per_iteration_per_level": it_us,
                    "icp_start_finish_us_per_level": timeline,
                    "points_full": full_n, "points_per_level": sizes, "tuning": tuning},

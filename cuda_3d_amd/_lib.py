@@ -141,6 +141,8 @@ _SIGS = {
     "gca_pipeline_ctx": (i32, [vp, P(vp)]),
     "gca_pipeline_sizes": (i32, [vp, i32, P(i64), vp]),
     "gca_pipeline_graph_stats": (i32, [vp, P(i64), P(i64)]),
+    "gca_pipeline_set_overlap": (i32, [vp, i32]),
+    "gca_pipeline_flush": (i32, [vp]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)

@@ -42,6 +42,8 @@
 using namespace gca;
 
 constexpr int kPipeMaxLevels = 8;
+constexpr int kPipeSlots = 3;  // frame slots: two for sequential steps, three for overlapped ones
+constexpr int kErrStride = 16;  // uint32 words between the slots' error words (own 64-byte lines)
 
 struct gca_pipeline_s {
     gca_ctx ctx = nullptr;  // private: front end, registration, every cloud
@@ -53,31 +55,40 @@ struct gca_pipeline_s {
     float tmin = 0.f, tmax = 0.f;
     int nl = 0;
     gca_level_params lv[kPipeMaxLevels]{};
-    gca_cloud full[2] = {nullptr, nullptr};
-    gca_cloud lvc[2][kPipeMaxLevels] = {};
+    gca_ctx rctx = nullptr;  // private: the registration of the previous frame in overlapped steps
+    hipEvent_t ev_rfork = nullptr, ev_rjoin = nullptr;
+    gca_cloud full[kPipeSlots] = {};
+    gca_cloud lvc[kPipeSlots][kPipeMaxLevels] = {};
     GridDesc gdesc[kPipeMaxLevels]{};
     double max_bits[kPipeMaxLevels] = {};
-    IcpPlan *plan[2] = {nullptr, nullptr};   // plan[s]: frame in slot s onto the frame in 1 - s
-    uint32_t *err_dev = nullptr;              // voxel lattice refusals during a step
-    uint32_t *err_host = nullptr;             // pinned copy
+    IcpPlan *plan[kPipeSlots] = {};  // plan[s]: the frame in slot s onto the frame in slot prev(s)
+    uint32_t *err_dev = nullptr;     // per slot (kErrStride words apart): voxel lattice refusals of its build
+    uint32_t *err_host = nullptr;    // pinned copies
     int64_t frames = 0;
-    int last = -1;
+    int nslots = 2;                  // frame slots in use: 2 sequential, 3 overlapped
+    bool overlap = false;
+    int last = -1;                   // slot of the frame whose registration result() reports
+    int pending = -1;                // overlapped: slot of the stepped frame not yet registered
     bool use_graph = false;
     int eager_since_reset = 0;  // eager steps since plans were (re)built
     uint64_t tune_version = 0;
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    hipGraph_t graph[2] = {nullptr, nullptr};       // kept for its front-end node
-    gca::FrontLaunch *front[2] = {nullptr, nullptr};  // the front end's recorded launch per parity
-    hipGraphNode_t front_node[2] = {nullptr, nullptr};
-    const void *cap_depth[2] = {nullptr, nullptr}, *cap_bgr[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[kPipeSlots] = {};
+    hipGraph_t graph[kPipeSlots] = {};          // kept for its front-end node
+    gca::FrontLaunch *front[kPipeSlots] = {};  // the front end's recorded launch per slot
+    hipGraphNode_t front_node[kPipeSlots] = {};
+    const void *cap_depth[kPipeSlots] = {}, *cap_bgr[kPipeSlots] = {};
     int64_t captures = 0, updates = 0;
 };
 
 namespace {
 
+int prev_slot(const gca_pipeline_s *p, int s) { return (s + p->nslots - 1) % p->nslots; }
+uint32_t *err_dev_of(gca_pipeline p, int s) { return p->err_dev + kErrStride * s; }
+uint32_t *err_host_of(gca_pipeline p, int s) { return p->err_host + kErrStride * s; }
+
 void drop_graphs(gca_pipeline p)
 {
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kPipeSlots; ++s) {
         if (p->exec[s]) hipGraphExecDestroy(p->exec[s]);
         if (p->graph[s]) hipGraphDestroy(p->graph[s]);
         p->exec[s] = nullptr;
@@ -90,7 +101,7 @@ void drop_graphs(gca_pipeline p)
 void drop_plans(gca_pipeline p)
 {
     drop_graphs(p);
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kPipeSlots; ++s) {
         icp_plan_destroy(p->plan[s]);
         p->plan[s] = nullptr;
     }
@@ -104,7 +115,7 @@ int32_t make_plan(gca_pipeline p, int slot)
     float gr[kPipeMaxLevels];
     for (int l = 0; l < p->nl; ++l) {
         srcs[l] = p->lvc[slot][l];
-        tgts[l] = p->lvc[1 - slot][l];
+        tgts[l] = p->lvc[prev_slot(p, slot)][l];
         // the plan keeps its target's grid geometry for every later step of this parity, while
         // the target is rebuilt each step with the pipeline's geometry: a target built before a
         // geometry change (a tuned cell edge) is re-binned into the current geometry first
@@ -120,13 +131,41 @@ int32_t make_plan(gca_pipeline p, int slot)
     return icp_plan_create(p->ctx, 1, p->nl, srcs, tgts, prm, gr, I, &p->plan[slot]);
 }
 
-// Everything a step enqueues (no host round trip, no allocation once warm).
-int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *bgr)
+// The registration of the frame in slot reg onto the frame before it, on the registration
+// stream, forked from the pipeline stream (so after everything enqueued there before); the
+// caller makes the pipeline stream wait for ev_rjoin.  Overlapped steps only.
+int32_t enqueue_registration(gca_pipeline p, int reg)
+{
+    IcpPlan *plan = p->plan[reg];
+    gca_ctx rc = p->rctx;
+    GCA_HIP(hipEventRecord(p->ev_rfork, p->ctx->stream));
+    GCA_HIP(hipStreamWaitEvent(rc->stream, p->ev_rfork, 0));
+    GCA_TRY(icp_plan_init(rc, plan));
+    for (int l = 0; l < p->nl; ++l) {
+        // the plan reads the target's grid through what it saw at its creation
+        if (!icp_plan_target_current(plan, l, p->lvc[prev_slot(p, reg)][l])) return GCA_ERR_DEVICE;
+        GCA_TRY(icp_plan_persist(rc, plan, l));
+    }
+    // results and the registered frame's lattice error word (its build ended a step earlier)
+    GCA_TRY(icp_plan_copy_results(rc, plan, err_dev_of(p, reg), err_host_of(p, reg)));
+    GCA_HIP(hipEventRecord(p->ev_rjoin, rc->stream));
+    return GCA_OK;
+}
+
+// Everything a step enqueues (no host round trip, no allocation once warm): the new frame's
+// front end and every level's build into slot `slot`, and the registration of the frame in slot
+// `reg` (-1: none).  Sequential steps (reg == slot): each level registers onto the previous frame
+// as soon as it is built.  Overlapped steps (reg: the previous frame, built by the previous
+// step): its registration runs on the registration stream beside this frame's front end and
+// build, which touch neither of the two frames it reads.
+int32_t enqueue(gca_pipeline p, int slot, int reg, const uint16_t *depth, const uint8_t *bgr)
 {
     gca_ctx ctx = p->ctx;
-    // the front end's first kernel also clears the step's voxel-lattice error word
-    GCA_TRY(launch_from_rgbd(ctx, depth, bgr, &p->cam, &p->bil, p->tmin, p->tmax, p->full[slot], p->err_dev));
-    IcpPlan *plan = p->frames >= 1 ? p->plan[slot] : nullptr;
+    const bool overlapped = p->overlap && reg >= 0;
+    IcpPlan *plan = reg >= 0 && !p->overlap ? p->plan[reg] : nullptr;  // the inline chain
+    if (overlapped) GCA_TRY(enqueue_registration(p, reg));
+    // the front end's first kernel also clears this slot's voxel-lattice error word
+    GCA_TRY(launch_from_rgbd(ctx, depth, bgr, &p->cam, &p->bil, p->tmin, p->tmax, p->full[slot], err_dev_of(p, slot)));
     if (plan) GCA_TRY(icp_plan_init(ctx, plan));
     GCA_HIP(hipEventRecord(p->ev_fork, ctx->stream));
     auto build = [&](int l) -> int32_t {
@@ -138,7 +177,7 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         GCA_TRY(cloud_reserve(c, std::max<int64_t>(p->full[slot]->cap, 1)));
         GCA_TRY(grid_prepare(lc, c, p->gdesc[l]));
         const GridCount gc{p->gdesc[l], c->grid.count, c->grid.cell_of};
-        GCA_TRY(launch_voxel_planned(lc, p->full[slot], L.voxel_size, c, p->max_bits[l], p->err_dev, &gc));
+        GCA_TRY(launch_voxel_planned(lc, p->full[slot], L.voxel_size, c, p->max_bits[l], err_dev_of(p, slot), &gc));
         GCA_TRY(launch_grid_build(lc, c, p->gdesc[l], true));
         if (L.gradient_max_nn == 0 && L.gradient_radius <= L.normal_radius) {
             // the solve writes the grid-ordered copies registration reads
@@ -160,12 +199,12 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
         }
         return GCA_OK;
     };
-    // the registration chain runs on level 0's stream (after level 0's build): level l once it
-    // is built, right after level l - 1's launch, which hands it the pose
-    auto reg = [&](int l) -> int32_t {
+    // the sequential registration chain runs on level 0's stream (after level 0's build): level
+    // l once it is built, right after level l - 1's launch, which hands it the pose
+    auto reg_level = [&](int l) -> int32_t {
         gca_ctx ic = p->lctx[0];
         // the plan reads the target's grid through what it saw at its creation
-        if (!icp_plan_target_current(plan, l, p->lvc[1 - slot][l])) return GCA_ERR_DEVICE;
+        if (!icp_plan_target_current(plan, l, p->lvc[prev_slot(p, slot)][l])) return GCA_ERR_DEVICE;
         if (l > 0) GCA_HIP(hipStreamWaitEvent(ic->stream, p->ev_built[l], 0));
         return icp_plan_persist(ic, plan, l);
     };
@@ -174,37 +213,41 @@ int32_t enqueue(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *
     // level's build first, the chain after the coarsest: 0.328 -> 0.355 ms per 640x480 frame)
     for (int l = 0; l < p->nl; ++l) {
         GCA_TRY(build(l));
-        if (plan) GCA_TRY(reg(l));
+        if (plan) GCA_TRY(reg_level(l));
     }
     if (plan) {
         // the registration results and the step's lattice error word, by one kernel right after
         // the last registration launch on its stream (every level's build is ordered before that
         // launch, so the error word is final)
-        GCA_TRY(icp_plan_copy_results(p->lctx[0], plan, p->err_dev, p->err_host));
+        GCA_TRY(icp_plan_copy_results(p->lctx[0], plan, err_dev_of(p, slot), err_host_of(p, slot)));
         GCA_HIP(hipEventRecord(p->ev_join[0], p->lctx[0]->stream));
         GCA_HIP(hipStreamWaitEvent(ctx->stream, p->ev_join[0], 0));
-    } else {
-        GCA_HIP(hipMemcpyAsync(p->err_host, p->err_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    } else if (!p->overlap) {
+        GCA_HIP(hipMemcpyAsync(err_host_of(p, slot), err_dev_of(p, slot), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
     }
+    if (overlapped) GCA_HIP(hipStreamWaitEvent(ctx->stream, p->ev_rjoin, 0));
     return GCA_OK;
 }
 
-int32_t capture(gca_pipeline p, int slot, const uint16_t *depth, const uint8_t *bgr)
+int32_t capture(gca_pipeline p, int slot, int reg, const uint16_t *depth, const uint8_t *bgr)
 {
     gca_ctx ctx = p->ctx;
     hipGraph_t g = nullptr;
     if (!p->front[slot] && !(p->front[slot] = front_launch_new())) return GCA_ERR_OUT_OF_MEMORY;
     ctx->capturing = true;
     ctx->front_rec = p->front[slot];
+    p->rctx->capturing = true;
     for (int l = 0; l < p->nl; ++l) p->lctx[l]->capturing = true;
     hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
-    int32_t st = e == hipSuccess ? enqueue(p, slot, depth, bgr) : hip_status(e);
+    int32_t st = e == hipSuccess ? enqueue(p, slot, reg, depth, bgr) : hip_status(e);
     if (e == hipSuccess) {
         const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
         if (st == GCA_OK && e2 != hipSuccess) st = hip_status(e2);
     }
     ctx->capturing = false;
     ctx->front_rec = nullptr;
+    p->rctx->capturing = false;
     for (int l = 0; l < p->nl; ++l) p->lctx[l]->capturing = false;
     if (st == GCA_OK && !g) st = GCA_ERR_DEVICE;
     hipGraphExec_t ex = nullptr;
@@ -280,10 +323,11 @@ int32_t gca_pipeline_destroy(gca_pipeline p)
 {
     if (!p) return GCA_ERR_INVALID_VALUE;
     if (p->ctx) hipStreamSynchronize(p->ctx->stream);
+    if (p->rctx) hipStreamSynchronize(p->rctx->stream);
     for (int l = 0; l < kPipeMaxLevels; ++l)
         if (p->lctx[l]) hipStreamSynchronize(p->lctx[l]->stream);
     drop_plans(p);
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < kPipeSlots; ++s) {
         front_launch_free(p->front[s]);
         if (p->full[s]) gca_cloud_destroy(p->full[s]);
         for (int l = 0; l < p->nl; ++l)
@@ -292,6 +336,9 @@ int32_t gca_pipeline_destroy(gca_pipeline p)
     hipFree(p->err_dev);
     hipHostFree(p->err_host);
     if (p->ev_fork) hipEventDestroy(p->ev_fork);
+    if (p->ev_rfork) hipEventDestroy(p->ev_rfork);
+    if (p->ev_rjoin) hipEventDestroy(p->ev_rjoin);
+    if (p->rctx) gca_ctx_destroy(p->rctx);
     for (int l = 0; l < kPipeMaxLevels; ++l) {
         if (p->ev_join[l]) hipEventDestroy(p->ev_join[l]);
         if (p->ev_built[l]) hipEventDestroy(p->ev_built[l]);
@@ -343,15 +390,20 @@ int32_t gca_pipeline_create(gca_ctx ctx, const gca_camera *cam, const gca_bilate
                              hipEventCreateWithFlags(&p->ev_built[l], hipEventDisableTiming) != hipSuccess))
             st = GCA_ERR_DEVICE;
     }
-    if (st == GCA_OK && hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess) st = GCA_ERR_DEVICE;
+    if (st == GCA_OK) st = gca_ctx_create(ctx->device, &p->rctx);
+    if (st == GCA_OK) p->rctx->prof_owner = p->ctx;
+    if (st == GCA_OK && (hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                         hipEventCreateWithFlags(&p->ev_rfork, hipEventDisableTiming) != hipSuccess ||
+                         hipEventCreateWithFlags(&p->ev_rjoin, hipEventDisableTiming) != hipSuccess))
+        st = GCA_ERR_DEVICE;
     const int64_t cap = (int64_t)cam->width * cam->height;
-    for (int s = 0; s < 2 && st == GCA_OK; ++s) {
+    for (int s = 0; s < kPipeSlots && st == GCA_OK; ++s) {
         st = gca_cloud_create(p->ctx, cap, &p->full[s]);
         for (int l = 0; l < n_levels && st == GCA_OK; ++l) st = gca_cloud_create(p->ctx, cap, &p->lvc[s][l]);
     }
     if (st == GCA_OK && hipMalloc(&p->err_dev, 256) != hipSuccess) st = GCA_ERR_OUT_OF_MEMORY;
     if (st == GCA_OK && hipHostMalloc(&p->err_host, 256) != hipSuccess) st = GCA_ERR_OUT_OF_MEMORY;
-    if (st == GCA_OK) *p->err_host = 0u;
+    if (st == GCA_OK) memset(p->err_host, 0, 256);
     if (st != GCA_OK) {
         gca_pipeline_destroy(p);
         return st;
@@ -380,23 +432,31 @@ int32_t gca_pipeline_ctx(gca_pipeline p, gca_ctx *ctx)
 int32_t gca_pipeline_step(gca_pipeline p, const uint16_t *depth_dev, const uint8_t *bgr_dev)
 {
     if (!p || !depth_dev || !bgr_dev) return GCA_ERR_INVALID_VALUE;
-    const int slot = (int)(p->frames & 1);
+    const int slot = (int)(p->frames % p->nslots);
     if (p->ctx->tune_version != p->tune_version) {  // plans and graphs bake the launch shapes in
         GCA_HIP(hipStreamSynchronize(p->ctx->stream));
+        GCA_HIP(hipStreamSynchronize(p->rctx->stream));
         for (int l = 0; l < p->nl; ++l) GCA_HIP(hipStreamSynchronize(p->lctx[l]->stream));
         drop_plans(p);
         set_geometry(p);  // the grid cell may be a tuned one
         p->tune_version = p->ctx->tune_version;
     }
-    for (int l = 0; l < p->nl; ++l)  // the level ctxs launch with the pipeline ctx's shapes
-        memcpy(p->lctx[l]->tune, p->ctx->tune, sizeof(p->ctx->tune));
-    if (p->frames >= 1 && !p->plan[slot]) GCA_TRY(make_plan(p, slot));
+    // the level and registration ctxs launch with the pipeline ctx's shapes
+    for (int l = 0; l < p->nl; ++l) memcpy(p->lctx[l]->tune, p->ctx->tune, sizeof(p->ctx->tune));
+    memcpy(p->rctx->tune, p->ctx->tune, sizeof(p->ctx->tune));
+    // the registration this step runs: sequential, the new frame's (from its second frame on);
+    // overlapped, the previous frame's (unless a flush ran it)
+    const int reg = p->overlap ? p->pending : (p->frames >= 1 ? slot : -1);
+    if (reg >= 0 && !p->plan[reg]) GCA_TRY(make_plan(p, reg));
     // a graph is captured only once every buffer it will point into exists: after an eager
-    // step of each frame parity with its plan in place
-    const bool graph = p->use_graph && p->frames >= 1 && p->eager_since_reset >= 2 && p->plan[0] && p->plan[1];
+    // step of each slot with its plan in place
+    bool plans = true;
+    for (int s = 0; s < p->nslots; ++s) plans = plans && p->plan[s];
+    const bool graph = p->use_graph && plans && p->eager_since_reset >= p->nslots &&
+                       (p->overlap ? reg == prev_slot(p, slot) : reg == slot);
     if (graph) {
         if (!p->exec[slot]) {
-            GCA_TRY(capture(p, slot, depth_dev, bgr_dev));
+            GCA_TRY(capture(p, slot, reg, depth_dev, bgr_dev));
         } else if (p->cap_depth[slot] != depth_dev || p->cap_bgr[slot] != bgr_dev) {
             // new frame buffers: point the pixel kernel's node at them (capture again only if
             // the update is refused)
@@ -408,19 +468,61 @@ int32_t gca_pipeline_step(gca_pipeline p, const uint16_t *depth_dev, const uint8
                 ++p->updates;
             } else {
                 (void)hipGetLastError();
-                GCA_TRY(capture(p, slot, depth_dev, bgr_dev));
+                GCA_TRY(capture(p, slot, reg, depth_dev, bgr_dev));
             }
         }
         GCA_HIP(hipGraphLaunch(p->exec[slot], p->ctx->stream));
     } else {
-        GCA_TRY(enqueue(p, slot, depth_dev, bgr_dev));
-        if (p->frames >= 1) ++p->eager_since_reset;
+        GCA_TRY(enqueue(p, slot, reg, depth_dev, bgr_dev));
+        if (reg >= 0) ++p->eager_since_reset;
     }
     // host bookkeeping of a replayed step: the slot's clouds changed on the device
     p->full[slot]->host_valid = false;
     for (int l = 0; l < p->nl; ++l) p->lvc[slot][l]->host_valid = false;
+    if (p->overlap) {
+        if (reg >= 0) p->last = reg;
+        p->pending = p->frames >= 1 ? slot : -1;  // the first frame has nothing to register onto
+    } else {
+        p->last = slot;
+    }
     ++p->frames;
-    p->last = slot;
+    return GCA_OK;
+}
+
+int32_t gca_pipeline_flush(gca_pipeline p)
+{
+    if (!p) return GCA_ERR_INVALID_VALUE;
+    if (!p->overlap || p->pending < 0) return GCA_OK;
+    const int reg = p->pending;
+    if (p->ctx->tune_version != p->tune_version) {
+        GCA_HIP(hipStreamSynchronize(p->ctx->stream));
+        GCA_HIP(hipStreamSynchronize(p->rctx->stream));
+        for (int l = 0; l < p->nl; ++l) GCA_HIP(hipStreamSynchronize(p->lctx[l]->stream));
+        drop_plans(p);
+        set_geometry(p);
+        p->tune_version = p->ctx->tune_version;
+        memcpy(p->rctx->tune, p->ctx->tune, sizeof(p->ctx->tune));
+    }
+    if (!p->plan[reg]) GCA_TRY(make_plan(p, reg));
+    GCA_TRY(enqueue_registration(p, reg));
+    GCA_HIP(hipStreamWaitEvent(p->ctx->stream, p->ev_rjoin, 0));
+    p->last = reg;
+    p->pending = -1;
+    return GCA_OK;
+}
+
+int32_t gca_pipeline_set_overlap(gca_pipeline p, int32_t enable)
+{
+    if (!p) return GCA_ERR_INVALID_VALUE;
+    GCA_HIP(hipStreamSynchronize(p->ctx->stream));
+    GCA_HIP(hipStreamSynchronize(p->rctx->stream));
+    for (int l = 0; l < p->nl; ++l) GCA_HIP(hipStreamSynchronize(p->lctx[l]->stream));
+    drop_plans(p);
+    p->overlap = enable != 0;
+    p->nslots = p->overlap ? 3 : 2;
+    p->frames = 0;
+    p->pending = -1;
+    p->last = -1;
     return GCA_OK;
 }
 
@@ -430,9 +532,11 @@ int32_t gca_pipeline_result(gca_pipeline p, double T[16], gca_icp_report *report
     *valid = 0;
     GCA_HIP(hipStreamSynchronize(p->ctx->stream));
     GCA_TRY(ctx_take_device_error(p->ctx));
+    GCA_TRY(ctx_take_device_error(p->rctx));
     for (int l = 0; l < p->nl; ++l) GCA_TRY(ctx_take_device_error(p->lctx[l]));
-    if (*p->err_host) return GCA_ERR_INVALID_VALUE;  // a voxel lattice was refused
-    if (p->frames < 2) {
+    if (p->last >= 0 && *err_host_of(p, p->last)) return GCA_ERR_INVALID_VALUE;  // a voxel lattice was refused
+    const bool registered = p->overlap ? p->last >= 0 : p->frames >= 2;
+    if (!registered) {
         static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
         memcpy(T, I, sizeof(I));
         return GCA_OK;
@@ -452,7 +556,7 @@ int32_t gca_pipeline_graph_stats(gca_pipeline p, int64_t *captures, int64_t *nod
 
 int32_t gca_pipeline_sizes(gca_pipeline p, int32_t slot, int64_t *full_n, int64_t *level_n)
 {
-    if (!p || slot < 0 || slot > 1 || !full_n) return GCA_ERR_INVALID_VALUE;
+    if (!p || slot < 0 || slot >= p->nslots || !full_n) return GCA_ERR_INVALID_VALUE;
     GCA_TRY(cloud_sync_header(p->full[slot]));
     *full_n = p->full[slot]->host_hdr.n;
     if (level_n)

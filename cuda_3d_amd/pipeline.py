@@ -11,7 +11,13 @@ without a host round trip:
 This is the loop of the reference's demo (apps/app.cpp:111-372: create_from_rgbd ->
 voxel_grid_down_sample -> estimate_normals -> color_icp.align against the previous cloud) with
 the coarse-to-fine schedule of apps/test_color_icp.cpp:378-414.  Every buffer is allocated
-once (two frame slots); with graph=True each frame parity's step is replayed as a hipGraph.
+once (two frame slots, three when overlapped); with graph=True each slot's step is replayed as a
+hipGraph.
+
+overlap=True: a step builds the new frame and, beside it on a second stream, registers the
+previous frame (built by the previous step); the poses are the sequential pipeline's, one step
+later.  result() then reports the frame before the one just stepped, and flush() registers the
+last stepped frame on its own.
 """
 from __future__ import annotations
 
@@ -27,7 +33,7 @@ class FramePipeline:
     def __init__(self, ctx: Context, camera: L.Camera, bilateral: L.Bilateral,
                  levels=(0.04, 0.02, 0.01), iters=30, normal_max_nn=30, tmin=0.2, tmax=3.5,
                  conv_rotation=1e-6, conv_translation_sq=1e-6, accumulate_mode=L.ACCUM_COMPENSATED,
-                 graph=True):
+                 graph=True, overlap=False):
         """conv_*: the stopping rule; the defaults are the reference's fixed thresholds
         (color_icp.hpp:45-47), 0, 0 runs every level to `iters`."""
         self.levels = tuple(levels)
@@ -45,16 +51,34 @@ class FramePipeline:
         self.ctx = Context.borrow(c, ctx.device)  # the pipeline's own stream, profiler and tuning
         self.set_graph(graph)
         self.frames = 0
+        self.overlap = False
+        if overlap:
+            self.set_overlap(True)
         self.reports = (L.IcpReport * nl)()
         self._T = np.zeros(16)
 
     def set_graph(self, enable: bool):
         L.check(L.lib.gca_pipeline_set_graph(self.handle, 1 if enable else 0), "gca_pipeline_set_graph")
 
+    def set_overlap(self, enable: bool):
+        """Overlapped or sequential steps; synchronises and starts a new sequence."""
+        L.check(L.lib.gca_pipeline_set_overlap(self.handle, 1 if enable else 0), "gca_pipeline_set_overlap")
+        self.overlap = bool(enable)
+        self.frames = 0
+
+    def flush(self):
+        """Overlapped steps: register the last stepped frame now (its pose is otherwise computed
+        by the next step)."""
+        L.check(L.lib.gca_pipeline_flush(self.handle), "gca_pipeline_flush")
+
+    @property
+    def slots(self) -> int:
+        return 3 if self.overlap else 2
+
     @property
     def slot(self) -> int:
         """Frame slot the next step uses."""
-        return self.frames % 2
+        return self.frames % self.slots
 
     def step(self, depth_ptr: int, bgr_ptr: int):
         """Enqueue a new frame (asynchronous)."""
@@ -63,8 +87,9 @@ class FramePipeline:
         self.frames += 1
 
     def result(self):
-        """Waits for the last step: the pose mapping the last frame onto the one before (None for
-        the first frame) and the per-level reports."""
+        """Waits for the last step: the pose mapping the last registered frame onto the one before
+        (sequential: the frame just stepped; overlapped: the one before it, or the last stepped
+        frame after flush()), None before any registration, and the per-level reports."""
         valid = C.c_int32()
         L.check(L.lib.gca_pipeline_result(self.handle, _ptr(self._T), self.reports, C.byref(valid)),
                 "gca_pipeline_result")

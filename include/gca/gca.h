@@ -479,9 +479,21 @@ int32_t gca_pipeline_set_graph(gca_pipeline pipeline, int32_t enable);
 int32_t gca_pipeline_graph_stats(gca_pipeline pipeline, int64_t *captures, int64_t *node_updates);
 /* The pipeline's own ctx (its stream, stage profiler and tuning). */
 int32_t gca_pipeline_ctx(gca_pipeline pipeline, gca_ctx *ctx);
-/* Point counts of a frame slot (0 / 1; the last step used slot (frames - 1) % 2): the full
- * cloud and level_n[n_levels] (may be NULL).  Synchronises. */
+/* Point counts of a frame slot (0 .. 1 sequential, 0 .. 2 overlapped; the last step used slot
+ * (frames - 1) % slots): the full cloud and level_n[n_levels] (may be NULL).  Synchronises. */
 int32_t gca_pipeline_sizes(gca_pipeline pipeline, int32_t slot, int64_t *full_n, int64_t *level_n);
+/* Overlapped steps (no reference counterpart: the reference's loop runs one frame at a time).
+ * enable != 0: each step builds the new frame (front end, every level) and, beside it on a
+ * second stream, registers the previous frame, whose clouds the previous step built; three frame
+ * slots, so the two never touch the same clouds.  A step's work is the same, and every pose
+ * equals the sequential pipeline's bit for bit, but gca_pipeline_result after a step reports
+ * the registration of the frame before the one just stepped; gca_pipeline_flush registers the
+ * last stepped frame on its own.  Changing the mode synchronises and starts a new sequence
+ * (the next frame is a first frame). */
+int32_t gca_pipeline_set_overlap(gca_pipeline pipeline, int32_t enable);
+/* Overlapped steps: enqueue the registration still pending for the last stepped frame, so that
+ * gca_pipeline_result reports it.  No-op in sequential mode or when nothing is pending. */
+int32_t gca_pipeline_flush(gca_pipeline pipeline);
 
 #ifdef __cplusplus
 }

@@ -1283,3 +1283,43 @@ def test_pipeline_rejects_bad_arguments(ctx):
     assert valid.value == 0 and np.array_equal(T.reshape(4, 4), np.eye(4))
     assert L.lib.gca_pipeline_destroy(h) == 0
 
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_pipeline_overlap_matches_sequential(ctx, graph):
+    """Overlapped steps (the previous frame registered beside the new frame's build, three frame
+    slots) give every pose of the sequential pipeline, bit for bit, one step later; flush()
+    registers the last frame.  Frames from four buffers, so the replayed graphs also follow new
+    frame buffers."""
+    import cuda_3d_amd as g
+    from cuda_3d_amd.pipeline import FramePipeline
+
+    cam, dev, _, _ = _pipeline_frames(g, 320, 240, 26)
+    copies = [(d.clone(), c.clone()) for d, c in dev]
+    bufs = [dev[0], dev[1], copies[0], copies[1]]
+    bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
+    seq = FramePipeline(ctx, cam, bil, graph=False)
+    ovl = FramePipeline(ctx, cam, bil, graph=graph, overlap=True)
+    want = []
+    for k in range(13):
+        d, c = bufs[k % 4]
+        seq.step(d.data_ptr(), c.data_ptr())
+        Ts, rs = seq.result()
+        want.append((Ts, [rep_key(r) for r in rs]) if Ts is not None else None)
+        ovl.step(d.data_ptr(), c.data_ptr())
+        To, ro = ovl.result()
+        if k < 2:
+            assert To is None  # nothing registered yet: frame k - 1 has no predecessor for k < 2
+            continue
+        np.testing.assert_array_equal(To, want[k - 1][0])
+        assert [rep_key(r) for r in ro] == want[k - 1][1]
+    ovl.flush()
+    To, ro = ovl.result()
+    np.testing.assert_array_equal(To, want[-1][0])
+    assert [rep_key(r) for r in ro] == want[-1][1]
+    captures, updates = ovl.graph_stats()
+    print(f"[gpu-parity] overlapped steps (graph {graph}): {captures} captures, {updates} node updates", flush=True)
+    if graph:
+        assert captures == 3 and updates >= 3
+    seq.close()
+    ovl.close()
