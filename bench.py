@@ -137,7 +137,7 @@ def run_fixed(g, ctx, cam, bil, ptrs, levels, iters, steps, graph, overlap):
         pipe = FramePipeline(ctx, cam, bil, levels, iters, conv_rotation=0.0, conv_translation_sq=0.0, graph=graph,
                              overlap=ov)
         autotune.apply(pipe.ctx, workload="frame")
-        for k in range(8):
+        for k in range(12):  # eager steps of every frame slot, then graph capture
             pipe.step(*ptrs[k % 2])
         pipe.ctx.synchronize()
         t0 = time.perf_counter()
@@ -186,7 +186,7 @@ def run_sequence(g, ctx, cam, bil, levels, iters, W, H, n_frames, passes, graph,
     seq.close()
     pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=graph, overlap=overlap)
     autotune.apply(pipe.ctx, workload="frame")
-    for k in range(n_frames + 1):  # eager steps of every slot, then graph capture
+    for k in range(max(n_frames + 1, 12)):  # eager steps of every slot, then graph capture
         pipe.step(*ptrs[k % n_frames])
     pipe.ctx.synchronize()
     t0 = time.perf_counter()
@@ -364,7 +364,7 @@ def main():
         p.ctx.synchronize()
         return (time.perf_counter() - t0) * 1e3 / n
 
-    for _ in range(max(args.warmup, 8)):  # eager steps of every frame slot, then graph capture
+    for _ in range(max(args.warmup, 12)):  # eager steps of every frame slot, then graph capture
         step(pipe)
         step(seqp)
     log(f"warmup done; timing {args.steps} steps ({'graph replay' if not args.no_graph else 'eager'}, "

@@ -143,6 +143,7 @@ _SIGS = {
     "gca_pipeline_graph_stats": (i32, [vp, P(i64), P(i64)]),
     "gca_pipeline_set_overlap": (i32, [vp, i32]),
     "gca_pipeline_flush": (i32, [vp]),
+    "gca_pipeline_slots": (i32, [vp, P(i32)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)

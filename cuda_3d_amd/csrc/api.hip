@@ -144,17 +144,19 @@ constexpr int64_t kScanWideFrom = 128;
 template <bool POPC>
 __device__ __forceinline__ uint32_t scan_val(uint32_t v) { return POPC ? (uint32_t)__popc(v) : v; }
 
-template <bool POPC, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_scan_onepass(const uint32_t *__restrict__ in, int64_t n,
-                                                             int nchunks, unsigned long long *flags, ScanCtl *ctl,
-                                                             uint32_t *__restrict__ out,
-                                                             uint32_t *__restrict__ total_dev,
-                                                             uint32_t *__restrict__ out_end, uint32_t *dev_err)
+template <bool POPC, int BLOCK, int N>
+__global__ __launch_bounds__(BLOCK) void k_scan_onepass(Jobs<ScanJob, N> J)
 {
+    const ScanJob &A = J.a[N == 1 ? 0 : blockIdx.y];
+    if ((int)blockIdx.x >= A.nchunks) return;  // a smaller job's surplus workgroups
+    const uint32_t *__restrict__ in = A.in;
+    uint32_t *__restrict__ out = A.out;
+    const int64_t n = A.n;
+    const int nchunks = A.nchunks;
     __shared__ uint32_t lds[BLOCK / kWave + 1];
     __shared__ uint32_t s_chunk, s_seq, s_excl;
     if (threadIdx.x == 0) {
-        const unsigned long long w = __hip_atomic_fetch_add(&ctl->ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long w = __hip_atomic_fetch_add(&A.ctl->ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_chunk = (uint32_t)w;
         s_seq = (uint32_t)(w >> 32);
     }
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_onepass(const uint32_t *__restri
     const int c = (int)s_chunk;
     const uint32_t seq = s_seq;
     if (c >= nchunks) {  // only after an interrupted call left the counter behind
-        if (threadIdx.x == 0) __hip_atomic_store(dev_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (threadIdx.x == 0) __hip_atomic_store(A.dev_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         return;
     }
     const int64_t base = (int64_t)c * (BLOCK * kScanPer) + (int64_t)threadIdx.x * kScanPer;
@@ -192,13 +194,13 @@ __global__ __launch_bounds__(BLOCK) void k_scan_onepass(const uint32_t *__restri
     uint32_t run = block_excl_scan<BLOCK>(s, lds, &tot);
     if (threadIdx.x < kWave) {
         const int lane = lane_id();
-        const uint32_t excl = lookback_exclusive(flags, c, seq, tot, dev_err);
+        const uint32_t excl = lookback_exclusive(A.flags, c, seq, tot, A.dev_err);
         if (lane == 0) {
             s_excl = excl;
             if (c == nchunks - 1) {
-                if (total_dev) *total_dev = excl + tot;
-                if (out_end) *out_end = excl + tot;
-                __hip_atomic_store(&ctl->ctr, (unsigned long long)(seq + 1u) << 32, __ATOMIC_RELAXED,
+                if (A.total_dev) *A.total_dev = excl + tot;
+                if (A.out_end) *A.out_end = excl + tot;
+                __hip_atomic_store(&A.ctl->ctr, (unsigned long long)(seq + 1u) << 32, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
@@ -222,10 +224,11 @@ __global__ __launch_bounds__(BLOCK) void k_scan_onepass(const uint32_t *__restri
     }
 }
 
-int32_t exclusive_scan_u32(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n,
-                           uint32_t *total_dev, bool popcount)
+bool scan_wide(int64_t n) { return (n + 256 * kScanPer - 1) / (256 * kScanPer) > kScanWideFrom; }
+
+int32_t scan_job(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total_dev, bool wide,
+                 ScanJob *job)
 {
-    const bool wide = (n + 256 * kScanPer - 1) / (256 * kScanPer) > kScanWideFrom;
     const int64_t chunk = (wide ? 1024 : 256) * kScanPer;
     int64_t nchunks = (n + chunk - 1) / chunk;
     if (nchunks < 1) nchunks = 1;
@@ -233,21 +236,47 @@ int32_t exclusive_scan_u32(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64
     unsigned long long *flags = (unsigned long long *)ctx_scratch(ctx, 15, sizeof(unsigned long long) * nchunks, true);
     ScanCtl *ctl = (ScanCtl *)ctx_scratch(ctx, 23, sizeof(ScanCtl), true);
     if (!flags || !ctl) return GCA_ERR_OUT_OF_MEMORY;
-    const dim3 grid((unsigned)nchunks);
-    if (popcount && wide)
-        hipLaunchKernelGGL((k_scan_onepass<true, 1024>), grid, dim3(1024), 0, ctx->stream, in, n, (int)nchunks, flags,
-                           ctl, out, total_dev, out + n, ctx->dev_err);
-    else if (popcount)
-        hipLaunchKernelGGL((k_scan_onepass<true, 256>), grid, dim3(256), 0, ctx->stream, in, n, (int)nchunks, flags,
-                           ctl, out, total_dev, out + n, ctx->dev_err);
-    else if (wide)
-        hipLaunchKernelGGL((k_scan_onepass<false, 1024>), grid, dim3(1024), 0, ctx->stream, in, n, (int)nchunks, flags,
-                           ctl, out, total_dev, out + n, ctx->dev_err);
-    else
-        hipLaunchKernelGGL((k_scan_onepass<false, 256>), grid, dim3(256), 0, ctx->stream, in, n, (int)nchunks, flags,
-                           ctl, out, total_dev, out + n, ctx->dev_err);
+    *job = ScanJob{in, n, (int32_t)nchunks, 0, flags, ctl, out, total_dev, out + n, ctx->dev_err};
+    return GCA_OK;
+}
+
+template <bool POPC, int BLOCK, int N>
+static void scan_launch_n(hipStream_t stream, const ScanJob *jobs, int nj)
+{
+    Jobs<ScanJob, N> J{};
+    int gx = 1;
+    for (int j = 0; j < nj; ++j) {
+        J.a[j] = jobs[j];
+        gx = std::max(gx, (int)jobs[j].nchunks);
+    }
+    hipLaunchKernelGGL((k_scan_onepass<POPC, BLOCK, N>), dim3((unsigned)gx, (unsigned)nj), dim3(BLOCK), 0, stream, J);
+}
+
+template <bool POPC, int BLOCK>
+static void scan_launch_b(hipStream_t stream, const ScanJob *jobs, int nj)
+{
+    if (nj == 1) scan_launch_n<POPC, BLOCK, 1>(stream, jobs, nj);
+    else scan_launch_n<POPC, BLOCK, kMaxJobs>(stream, jobs, nj);
+}
+
+int32_t scan_launch(hipStream_t stream, const ScanJob *jobs, int nj, bool popcount, bool wide)
+{
+    if (nj < 1 || nj > kMaxJobs) return GCA_ERR_INVALID_VALUE;
+    if (popcount && wide) scan_launch_b<true, 1024>(stream, jobs, nj);
+    else if (popcount) scan_launch_b<true, 256>(stream, jobs, nj);
+    else if (wide) scan_launch_b<false, 1024>(stream, jobs, nj);
+    else scan_launch_b<false, 256>(stream, jobs, nj);
     GCA_HIP(hipGetLastError());
     return GCA_OK;
+}
+
+int32_t exclusive_scan_u32(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n,
+                           uint32_t *total_dev, bool popcount)
+{
+    const bool wide = scan_wide(n);
+    ScanJob job;
+    GCA_TRY(scan_job(ctx, in, out, n, total_dev, wide, &job));
+    return scan_launch(ctx->stream, &job, 1, popcount, wide);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -464,6 +493,11 @@ int32_t gca_ctx_synchronize(gca_ctx c)
 {
     if (!c) return GCA_ERR_INVALID_VALUE;
     GCA_HIP(hipStreamSynchronize(c->stream));
+    for (gca_ctx o : c->sync_also)
+        if (o) {
+            GCA_HIP(hipStreamSynchronize(o->stream));
+            GCA_TRY(ctx_take_device_error(o));
+        }
     return ctx_take_device_error(c);
 }
 

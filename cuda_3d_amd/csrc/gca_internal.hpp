@@ -87,6 +87,16 @@ struct GridCount {
 };
 
 struct FrontLaunch;  // a recorded front-end launch (pipeline graph updates)
+struct ScanCtl;
+
+// One launch for several pyramid levels ("jobs"): job j's arguments are a[j], blockIdx.y = j,
+// and grid.x is the largest job's; a job's surplus workgroups find no work (grid-stride loops
+// over device sizes) or return at once.  N = 1 for single-level calls.
+constexpr int kMaxJobs = 4;
+template <class A, int N>
+struct Jobs {
+    A a[N];
+};
 
 }  // namespace gca
 
@@ -136,6 +146,9 @@ struct gca_ctx_s {
     // (gca::NoAllocScratch), set by gca_ctx_create
     void *noalloc_scratch = nullptr;
     gca_ctx_s *prof_owner = nullptr;  // stage records go to this ctx's profiler when set
+    // work this ctx's users wait for as well: synchronising the ctx also synchronises these (a
+    // frame pipeline's other streams in two-lane steps)
+    gca_ctx_s *sync_also[3] = {nullptr, nullptr, nullptr};
     struct ProfRec {
         int stage;
         hipEvent_t a, b;
@@ -283,10 +296,49 @@ void icp_plan_destroy(IcpPlan *plan);
 // cell_quarters > 4: cell edge = r * cell_quarters / 4 (a tuned cell; results do not change).
 void grid_params(const float bmin[3], const float bmax[3], float r, GridDesc *d, int32_t cell_quarters = 0);
 
-// Block-wide exclusive scan of uint32 arrays (3 kernels, reduce-then-scan)
+// Exclusive scan of uint32 arrays in one pass (decoupled look-back)
 // out has n + 1 entries (out[n] = total); popcount = scan the popcounts of the words.
 int32_t exclusive_scan_u32(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n,
                            uint32_t *total_dev, bool popcount = false);
+// The same scan as one job of a multi-level launch: its look-back words are ctx's scratch;
+// wide = 1024-thread chunks (scan_wide(n) is the single-call choice).
+struct ScanJob {
+    const uint32_t *in;
+    int64_t n;
+    int32_t nchunks, pad;
+    unsigned long long *flags;
+    ScanCtl *ctl;
+    uint32_t *out, *total_dev, *out_end, *dev_err;
+};
+bool scan_wide(int64_t n);
+int32_t scan_job(gca_ctx ctx, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total_dev, bool wide,
+                 ScanJob *job);
+// jobs[0, nj) in one launch on stream (nj <= kMaxJobs, all of one kind).
+int32_t scan_launch(hipStream_t stream, const ScanJob *jobs, int nj, bool popcount, bool wide);
+
+// Lockstep build of several levels (each kernel of the sequence launched once for all levels, on
+// the launch ctx's stream; scratch from each level's own ctx, whose stream must be the launch
+// stream while these run).  Results equal the single-level calls'.
+struct VoxelLevelJob {
+    gca_ctx ctx;  // the level's scratch
+    gca_cloud in, out;
+    float voxel;
+    double max_bits;
+    uint32_t *err;
+    GridCount gc;  // the grid count of the output points (grid_prepare'd)
+};
+int32_t launch_voxel_levels(gca_ctx launch, const VoxelLevelJob *jobs, int nj);
+// Counted grids (the voxel finalize did the count pass).
+int32_t launch_grid_levels(gca_ctx launch, const gca_ctx *ctxs, const gca_cloud *cs, const GridDesc *ds, int nj);
+struct NormalsLevelJob {
+    gca_ctx ctx;
+    gca_cloud c;
+    float radius;
+    int32_t max_nn;
+    float gradient_radius;  // <= radius
+};
+// launch_normals_gradient(..., grid_attrs = true) per level.
+int32_t launch_normals_gradient_levels(gca_ctx launch, const NormalsLevelJob *jobs, int nj);
 // Stable LSD radix sort of n (64-bit key, 32-bit value) pairs by the low `bits` key bits, 8 per
 // pass; the result ends in (k0, v0) if *sorted_in_first, else in (k1, v1).
 int32_t radix_sort_u64(gca_ctx ctx, unsigned long long *k0, uint32_t *v0, unsigned long long *k1, uint32_t *v1,

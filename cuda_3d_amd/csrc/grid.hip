@@ -40,15 +40,26 @@ __global__ __launch_bounds__(256) void k_grid_count(const float4 *__restrict__ x
     }
 }
 
-__global__ __launch_bounds__(256) void k_grid_scatter(const float4 *__restrict__ xyzi,
-                                                      const CloudHdr *__restrict__ hdr,
-                                                      uint32_t *__restrict__ count,
-                                                      const uint32_t *__restrict__ start,
-                                                      const uint32_t *__restrict__ cell_of,
-                                                      int32_t *__restrict__ order,
-                                                      float4 *__restrict__ sxyzi)
+struct ScatterArgs {
+    const float4 *xyzi;
+    const CloudHdr *hdr;
+    uint32_t *count;
+    const uint32_t *start, *cell_of;
+    int32_t *order;
+    float4 *sxyzi;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_grid_scatter(Jobs<ScatterArgs, N> J)
 {
-    const int n = hdr->n;
+    const ScatterArgs &A = J.a[N == 1 ? 0 : blockIdx.y];
+    const float4 *__restrict__ xyzi = A.xyzi;
+    uint32_t *__restrict__ count = A.count;
+    const uint32_t *__restrict__ start = A.start;
+    const uint32_t *__restrict__ cell_of = A.cell_of;
+    int32_t *__restrict__ order = A.order;
+    float4 *__restrict__ sxyzi = A.sxyzi;
+    const int n = A.hdr->n;
     const int lane = lane_id();
     for (int b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {  // block-uniform
         const int i = b + (int)threadIdx.x;
@@ -117,12 +128,44 @@ int32_t launch_grid_build(gca_ctx ctx, gca_cloud c, const GridDesc &d, bool coun
         GCA_HIP(hipGetLastError());
     }
     GCA_TRY(exclusive_scan_u32(ctx, G.count, G.start, d.ncells, nullptr));
-    hipLaunchKernelGGL(k_grid_scatter, dim3(nb), dim3(256), 0, ctx->stream, c->xyzi, c->hdr, G.count,
-                       G.start, G.cell_of, G.order, G.sxyzi);
+    const Jobs<ScatterArgs, 1> sc{{{c->xyzi, c->hdr, G.count, G.start, G.cell_of, G.order, G.sxyzi}}};
+    hipLaunchKernelGGL(k_grid_scatter<1>, dim3(nb), dim3(256), 0, ctx->stream, sc);
     GCA_HIP(hipGetLastError());
     G.desc = d;
     G.valid = true;
     G.attrs_valid = false;
+    return GCA_OK;
+}
+
+int32_t launch_grid_levels(gca_ctx launch, const gca_ctx *ctxs, const gca_cloud *cs, const GridDesc *ds, int nj)
+{
+    if (nj < 1 || nj > kMaxJobs) return GCA_ERR_INVALID_VALUE;
+    bool wide = false;
+    for (int j = 0; j < nj; ++j) {
+        GCA_TRY(grid_prepare(ctxs[j], cs[j], ds[j]));
+        wide = wide || scan_wide(ds[j].ncells);
+    }
+    ScanJob scans[kMaxJobs];
+    Jobs<ScatterArgs, kMaxJobs> sc{};
+    unsigned nb = 1;
+    for (int j = 0; j < nj; ++j) {
+        const Grid &G = cs[j]->grid;
+        GCA_TRY(scan_job(ctxs[j], G.count, G.start, ds[j].ncells, nullptr, wide, &scans[j]));
+        sc.a[j] = ScatterArgs{cs[j]->xyzi, cs[j]->hdr, G.count, G.start, G.cell_of, G.order, G.sxyzi};
+        nb = std::max(nb, (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, (cs[j]->cap + 255) / 256)));
+    }
+    {
+        StageTimer timer(launch, GCA_STAGE_GRID);
+        GCA_TRY(scan_launch(launch->stream, scans, nj, false, wide));
+        hipLaunchKernelGGL(k_grid_scatter<kMaxJobs>, dim3(nb, (unsigned)nj), dim3(256), 0, launch->stream, sc);
+        GCA_HIP(hipGetLastError());
+    }
+    for (int j = 0; j < nj; ++j) {
+        Grid &G = cs[j]->grid;
+        G.desc = ds[j];
+        G.valid = true;
+        G.attrs_valid = false;
+    }
     return GCA_OK;
 }
 

@@ -440,17 +440,35 @@ __global__ __launch_bounds__(kNbBlock) void k_moments(const float4 *__restrict__
 // moment sets equal those of separate walks.  A query with more than K > 0 points within the
 // normals' radius goes to the overflow list for its normal moments; its gradient sums are
 // complete here.
-__global__ __launch_bounds__(kNbBlock) void k_moments_pair(const float4 *__restrict__ xyzi,
-                                                           const CloudHdr *__restrict__ hdr, GridDesc g,
-                                                           const uint32_t *__restrict__ start,
-                                                           const int32_t *__restrict__ order,
-                                                           const float4 *__restrict__ sxyzi, int32_t K, int gs,
-                                                           double rg, Moments *__restrict__ mom,
-                                                           Moments *__restrict__ mom_g, int32_t *__restrict__ ovf,
-                                                           uint32_t *__restrict__ ovf_n)
+struct PairArgs {
+    const float4 *xyzi;
+    const CloudHdr *hdr;
+    GridDesc g;
+    const uint32_t *start;
+    const int32_t *order;
+    const float4 *sxyzi;
+    int32_t K, gs;
+    double rg;
+    Moments *mom, *mom_g;
+    int32_t *ovf;
+    uint32_t *ovf_n;
+};
+
+template <int N>
+__global__ __launch_bounds__(kNbBlock) void k_moments_pair(Jobs<PairArgs, N> J)
 {
-    const GridRefs G{g, start, order, sxyzi};
-    const int n = hdr->n;
+    const PairArgs &A = J.a[N == 1 ? 0 : blockIdx.y];
+    const float4 *__restrict__ xyzi = A.xyzi;
+    const GridDesc &g = A.g;
+    const int32_t K = A.K;
+    const int gs = A.gs;
+    const double rg = A.rg;
+    Moments *__restrict__ mom = A.mom;
+    Moments *__restrict__ mom_g = A.mom_g;
+    int32_t *__restrict__ ovf = A.ovf;
+    uint32_t *__restrict__ ovf_n = A.ovf_n;
+    const GridRefs G{g, A.start, A.order, A.sxyzi};
+    const int n = A.hdr->n;
     const int lg = __builtin_ctz((unsigned)gs);
     const int gl = (int)threadIdx.x & (gs - 1);
     const int per_block = kNbBlock >> lg;
@@ -497,18 +515,34 @@ __global__ __launch_bounds__(kNbBlock) void k_moments_pair(const float4 *__restr
 }
 
 // Pass 2: the overflow queries, one wavefront each, with exact K-nearest selection.
-template <int KIND>
-__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_moments_overflow(
-    const float4 *__restrict__ xyzi, const float4 *__restrict__ nrm, const CloudHdr *__restrict__ hdr, GridDesc g,
-    const uint32_t *__restrict__ start, const int32_t *__restrict__ order, const float4 *__restrict__ sxyzi,
-    int32_t K, const int32_t *__restrict__ ovf, const uint32_t *__restrict__ ovf_n, Moments *__restrict__ mom)
+struct OvfArgs {
+    const float4 *xyzi, *nrm;
+    const CloudHdr *hdr;
+    GridDesc g;
+    const uint32_t *start;
+    const int32_t *order;
+    const float4 *sxyzi;
+    int32_t K, pad;
+    const int32_t *ovf;
+    const uint32_t *ovf_n;
+    Moments *mom;
+};
+
+template <int KIND, int N>
+__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_moments_overflow(Jobs<OvfArgs, N> J)
 {
+    const OvfArgs &A = J.a[N == 1 ? 0 : blockIdx.y];
+    const float4 *__restrict__ xyzi = A.xyzi;
+    const GridDesc &g = A.g;
+    const int32_t K = A.K;
+    const int32_t *__restrict__ ovf = A.ovf;
+    Moments *__restrict__ mom = A.mom;
     __shared__ NbBuf bufs[kWavesPerBlock];
     const int w = threadIdx.x / kWave;
     NbBuf &buf = bufs[w];
-    const GridRefs G{g, start, order, sxyzi};
-    const int n = hdr->n;
-    const uint32_t m = *ovf_n;
+    const GridRefs G{g, A.start, A.order, A.sxyzi};
+    const int n = A.hdr->n;
+    const uint32_t m = *A.ovf_n;
     const double inv_r = 1.0 / (double)g.radius;
     for (uint32_t e = blockIdx.x * kWavesPerBlock + w; e < m; e += gridDim.x * kWavesPerBlock) {
         const int qi = ovf[e];
@@ -600,16 +634,30 @@ __global__ __launch_bounds__(kNbBlock) void k_solve(const CloudHdr *__restrict__
 // stored, as a separate gradient call would read it back).
 // order != nullptr: thread s solves point order[s] (the grid's sorted slot s) and also writes
 // its normal and gradient to snrm[s] / sgrad[s], the grid-ordered copies registration reads.
-__global__ __launch_bounds__(kNbBlock) void k_solve_pair(const CloudHdr *__restrict__ hdr,
-                                                         const Moments *__restrict__ mom,
-                                                         const Moments *__restrict__ mom_g, double rg,
-                                                         float4 *__restrict__ nrm, float4 *__restrict__ grad,
-                                                         const int32_t *__restrict__ order,
-                                                         float4 *__restrict__ snrm, float4 *__restrict__ sgrad,
-                                                         uint32_t *__restrict__ ovf_n)
+struct SolveArgs {
+    const CloudHdr *hdr;
+    const Moments *mom, *mom_g;
+    double rg;
+    float4 *nrm, *grad;
+    const int32_t *order;
+    float4 *snrm, *sgrad;
+    uint32_t *ovf_n;
+};
+
+template <int N>
+__global__ __launch_bounds__(kNbBlock) void k_solve_pair(Jobs<SolveArgs, N> J)
 {
-    const int n = hdr->n;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *ovf_n = 0u;
+    const SolveArgs &A = J.a[N == 1 ? 0 : blockIdx.y];
+    const Moments *__restrict__ mom = A.mom;
+    const Moments *__restrict__ mom_g = A.mom_g;
+    const double rg = A.rg;
+    float4 *__restrict__ nrm = A.nrm;
+    float4 *__restrict__ grad = A.grad;
+    const int32_t *__restrict__ order = A.order;
+    float4 *__restrict__ snrm = A.snrm;
+    float4 *__restrict__ sgrad = A.sgrad;
+    const int n = A.hdr->n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *A.ovf_n = 0u;
     for (int s = blockIdx.x * kNbBlock + threadIdx.x; s < n; s += gridDim.x * kNbBlock) {
         const int i = order ? order[s] : s;
         long long S[9];
@@ -651,10 +699,11 @@ static int32_t launch_moments_pipeline(gca_ctx ctx, gca_cloud c, float radius, i
     StageTimer timer(ctx, KIND == 0 ? GCA_STAGE_NORMALS : GCA_STAGE_GRADIENT);
     hipLaunchKernelGGL(k_moments<KIND>, dim3(grid_for(cap, kNbBlock / gs, 4096)), dim3(kNbBlock), 0, ctx->stream,
                        c->xyzi, c->nrm, c->hdr, d, G.start, G.order, G.sxyzi, K, gs, mom, ovf, ovf_n);
-    if (K > 0)
-        hipLaunchKernelGGL(k_moments_overflow<KIND>, dim3(grid_for(cap, kWavesPerBlock, 1024)),
-                           dim3(kWave * kWavesPerBlock), 0, ctx->stream, c->xyzi, c->nrm, c->hdr, d, G.start,
-                           G.order, G.sxyzi, K, (const int32_t *)ovf, (const uint32_t *)ovf_n, mom);
+    if (K > 0) {
+        const Jobs<OvfArgs, 1> oj{{{c->xyzi, c->nrm, c->hdr, d, G.start, G.order, G.sxyzi, K, 0, ovf, ovf_n, mom}}};
+        hipLaunchKernelGGL((k_moments_overflow<KIND, 1>), dim3(grid_for(cap, kWavesPerBlock, 1024)),
+                           dim3(kWave * kWavesPerBlock), 0, ctx->stream, oj);
+    }
     hipLaunchKernelGGL(k_solve<KIND>, dim3(grid_for(cap, kNbBlock, 2048)), dim3(kNbBlock), 0, ctx->stream, c->hdr,
                        (const Moments *)mom, (const float4 *)c->nrm, (double)radius, out, ovf_n);
     GCA_HIP(hipGetLastError());
@@ -671,8 +720,16 @@ int32_t launch_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K)
     return launch_moments_pipeline<1>(ctx, c, radius, K, c->grad);
 }
 
-int32_t launch_normals_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K, float gradient_radius,
-                                bool grid_attrs)
+// The three kernels of one joint normals + gradient call, as jobs.
+struct PairJobs {
+    PairArgs pair;
+    OvfArgs ovf;
+    SolveArgs solve;
+    unsigned nb_pair, nb_ovf, nb_solve;
+};
+
+static int32_t pair_prepare(gca_ctx ctx, gca_cloud c, float radius, int32_t K, float gradient_radius, bool grid_attrs,
+                            PairJobs *pj)
 {
     if (!(gradient_radius <= radius)) return GCA_ERR_INVALID_VALUE;
     Grid &G = c->grid;
@@ -689,19 +746,69 @@ int32_t launch_normals_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t 
     if (!mom || !ovf || !ovf_n || !mom_g) return GCA_ERR_OUT_OF_MEMORY;
     const GridDesc d = query_desc(G, radius);
     const double rg = (double)gradient_radius;
-    StageTimer timer(ctx, GCA_STAGE_NORMALS);
-    hipLaunchKernelGGL(k_moments_pair, dim3(grid_for(cap, kNbBlock / gs, 4096)), dim3(kNbBlock), 0, ctx->stream,
-                       c->xyzi, c->hdr, d, G.start, G.order, G.sxyzi, K, gs, rg, mom, mom_g, ovf, ovf_n);
-    if (K > 0)
-        hipLaunchKernelGGL(k_moments_overflow<0>, dim3(grid_for(cap, kWavesPerBlock, 1024)),
-                           dim3(kWave * kWavesPerBlock), 0, ctx->stream, c->xyzi, c->nrm, c->hdr, d, G.start,
-                           G.order, G.sxyzi, K, (const int32_t *)ovf, (const uint32_t *)ovf_n, mom);
-    hipLaunchKernelGGL(k_solve_pair, dim3(grid_for(cap, kNbBlock, 2048)), dim3(kNbBlock), 0, ctx->stream, c->hdr,
-                       (const Moments *)mom, (const Moments *)mom_g, rg, c->nrm, c->grad,
-                       grid_attrs ? (const int32_t *)G.order : nullptr, grid_attrs ? G.snrm : nullptr,
-                       grid_attrs ? G.sgrad : nullptr, ovf_n);
+    pj->pair = PairArgs{c->xyzi, c->hdr, d, G.start, G.order, G.sxyzi, K, gs, rg, mom, mom_g, ovf, ovf_n};
+    pj->ovf = OvfArgs{c->xyzi, c->nrm, c->hdr, d, G.start, G.order, G.sxyzi, K, 0, ovf, ovf_n, mom};
+    pj->solve = SolveArgs{c->hdr, mom, mom_g, rg, c->nrm, c->grad, grid_attrs ? (const int32_t *)G.order : nullptr,
+                          grid_attrs ? G.snrm : nullptr, grid_attrs ? G.sgrad : nullptr, ovf_n};
+    pj->nb_pair = grid_for(cap, kNbBlock / gs, 4096);
+    pj->nb_ovf = K > 0 ? grid_for(cap, kWavesPerBlock, 1024) : 0;
+    pj->nb_solve = grid_for(cap, kNbBlock, 2048);
+    return GCA_OK;
+}
+
+// moments (one walk), overflow K-selection (jobs with K > 0), solve: each kernel once for all nj
+// calls (blockIdx.y = call), on stream.  A job without K selection contributes no overflow
+// workgroups' work: its overflow list stays empty (count zero), so its rows of the overflow
+// launch find nothing.
+template <int N>
+static int32_t pair_launch(hipStream_t stream, const PairJobs *pj, int nj)
+{
+    Jobs<PairArgs, N> pair{};
+    Jobs<OvfArgs, N> ovf{};
+    Jobs<SolveArgs, N> solve{};
+    unsigned nb_pair = 1, nb_ovf = 0, nb_solve = 1;
+    for (int j = 0; j < nj; ++j) {
+        pair.a[j] = pj[j].pair;
+        ovf.a[j] = pj[j].ovf;
+        solve.a[j] = pj[j].solve;
+        nb_pair = std::max(nb_pair, pj[j].nb_pair);
+        nb_ovf = std::max(nb_ovf, pj[j].nb_ovf);
+        nb_solve = std::max(nb_solve, pj[j].nb_solve);
+    }
+    hipLaunchKernelGGL(k_moments_pair<N>, dim3(nb_pair, (unsigned)nj), dim3(kNbBlock), 0, stream, pair);
+    if (nb_ovf > 0)
+        hipLaunchKernelGGL((k_moments_overflow<0, N>), dim3(nb_ovf, (unsigned)nj), dim3(kWave * kWavesPerBlock), 0,
+                           stream, ovf);
+    hipLaunchKernelGGL(k_solve_pair<N>, dim3(nb_solve, (unsigned)nj), dim3(kNbBlock), 0, stream, solve);
     GCA_HIP(hipGetLastError());
-    if (grid_attrs) G.attrs_valid = true;
+    return GCA_OK;
+}
+
+int32_t launch_normals_gradient(gca_ctx ctx, gca_cloud c, float radius, int32_t K, float gradient_radius,
+                                bool grid_attrs)
+{
+    PairJobs pj;
+    GCA_TRY(pair_prepare(ctx, c, radius, K, gradient_radius, grid_attrs, &pj));
+    {
+        StageTimer timer(ctx, GCA_STAGE_NORMALS);
+        GCA_TRY(pair_launch<1>(ctx->stream, &pj, 1));
+    }
+    if (grid_attrs) c->grid.attrs_valid = true;
+    return GCA_OK;
+}
+
+int32_t launch_normals_gradient_levels(gca_ctx launch, const NormalsLevelJob *jobs, int nj)
+{
+    if (nj < 1 || nj > kMaxJobs) return GCA_ERR_INVALID_VALUE;
+    PairJobs pj[kMaxJobs];
+    for (int j = 0; j < nj; ++j)
+        GCA_TRY(pair_prepare(jobs[j].ctx, jobs[j].c, jobs[j].radius, jobs[j].max_nn, jobs[j].gradient_radius, true,
+                             &pj[j]));
+    {
+        StageTimer timer(launch, GCA_STAGE_NORMALS);
+        GCA_TRY(pair_launch<kMaxJobs>(launch->stream, pj, nj));
+    }
+    for (int j = 0; j < nj; ++j) jobs[j].c->grid.attrs_valid = true;
     return GCA_OK;
 }
 

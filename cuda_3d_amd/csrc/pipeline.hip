@@ -25,6 +25,13 @@
 // and query ranges are clamped alike).  So after one eager step per frame parity, each parity's
 // step is captured into a hipGraph and replayed.
 //
+// Overlapped steps register the previous frame beside the new frame's build.  When the levels
+// can be built in lockstep (each build kernel launched once for all levels), consecutive steps
+// alternate between two lanes, each with its own pipeline stream, registration stream and level
+// scratch, over four frame slots; each step is then two single-stream graphs (the build, the
+// registration), ordered across lanes by events the host waits for and records around them, so
+// two frames' steps also run side by side.
+//
 // The pipeline owns private ctxs (one for the front end and registration, one per level, each
 // with its stream and scratch): the scratch buffers a captured graph points into are used by
 // nothing else, so no other call can move them.  Every cloud belongs to the main ctx; a level's
@@ -42,7 +49,7 @@
 using namespace gca;
 
 constexpr int kPipeMaxLevels = 8;
-constexpr int kPipeSlots = 3;  // frame slots: two for sequential steps, three for overlapped ones
+constexpr int kPipeSlots = 4;  // frame slots: 2 sequential, 3 overlapped in one lane, 4 in two lanes
 constexpr int kErrStride = 16;  // uint32 words between the slots' error words (own 64-byte lines)
 
 struct gca_pipeline_s {
@@ -78,11 +85,55 @@ struct gca_pipeline_s {
     hipGraphNode_t front_node[kPipeSlots] = {};
     const void *cap_depth[kPipeSlots] = {}, *cap_bgr[kPipeSlots] = {};
     int64_t captures = 0, updates = 0;
+    // overlapped steps in two lanes (lane 0 is ctx / rctx / lctx): lane 1's pipeline stream,
+    // registration stream and level scratch (created on first use), and the cross-lane events
+    int lanes = 1;
+    gca_ctx ctx1 = nullptr, rctx1 = nullptr;
+    gca_ctx lctx1[kPipeMaxLevels] = {};
+    hipEvent_t ev_slot_built[kPipeSlots] = {};  // slot s's build done (on its lane)
+    hipEvent_t ev_slot_reg[kPipeSlots] = {};    // the registration of slot s's frame done
+    bool slot_built_recorded[kPipeSlots] = {}, slot_reg_recorded[kPipeSlots] = {};
+    hipGraphExec_t exec_reg[kPipeSlots] = {};  // two lanes: the registration of slot s's frame
+    hipGraph_t graph_reg[kPipeSlots] = {};
 };
 
 namespace {
 
 int prev_slot(const gca_pipeline_s *p, int s) { return (s + p->nslots - 1) % p->nslots; }
+
+// The streams and scratch one lane of steps runs on.
+struct Lane {
+    gca_ctx main, reg;
+    gca_ctx *levels;
+};
+Lane lane(gca_pipeline p, int j) { return j ? Lane{p->ctx1, p->rctx1, p->lctx1} : Lane{p->ctx, p->rctx, p->lctx}; }
+// Two lanes: slots 0, 2 on lane 0 and 1, 3 on lane 1 (step k on lane k mod 2).
+int lane_of(const gca_pipeline_s *p, int slot) { return p->lanes == 2 ? (slot & 1) : 0; }
+
+// Every ctx of the pipeline (both lanes), for f(ctx).
+template <class F>
+void each_ctx(gca_pipeline p, F f)
+{
+    gca_ctx all[2 + 2 * kPipeMaxLevels + 2] = {p->ctx, p->rctx, p->ctx1, p->rctx1};
+    int n = 4;
+    for (int l = 0; l < kPipeMaxLevels; ++l) {
+        all[n++] = p->lctx[l];
+        all[n++] = p->lctx1[l];
+    }
+    for (int i = 0; i < n; ++i)
+        if (all[i]) f(all[i]);
+}
+
+int32_t sync_all(gca_pipeline p)
+{
+    hipError_t e = hipSuccess;
+    each_ctx(p, [&](gca_ctx c) {
+        const hipError_t e1 = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = e1;
+    });
+    return e == hipSuccess ? GCA_OK : hip_status(e);
+}
+
 uint32_t *err_dev_of(gca_pipeline p, int s) { return p->err_dev + kErrStride * s; }
 uint32_t *err_host_of(gca_pipeline p, int s) { return p->err_host + kErrStride * s; }
 
@@ -95,6 +146,10 @@ void drop_graphs(gca_pipeline p)
         p->graph[s] = nullptr;
         p->front_node[s] = nullptr;
         p->cap_depth[s] = p->cap_bgr[s] = nullptr;
+        if (p->exec_reg[s]) hipGraphExecDestroy(p->exec_reg[s]);
+        if (p->graph_reg[s]) hipGraphDestroy(p->graph_reg[s]);
+        p->exec_reg[s] = nullptr;
+        p->graph_reg[s] = nullptr;
     }
 }
 
@@ -131,15 +186,10 @@ int32_t make_plan(gca_pipeline p, int slot)
     return icp_plan_create(p->ctx, 1, p->nl, srcs, tgts, prm, gr, I, &p->plan[slot]);
 }
 
-// The registration of the frame in slot reg onto the frame before it, on the registration
-// stream, forked from the pipeline stream (so after everything enqueued there before); the
-// caller makes the pipeline stream wait for ev_rjoin.  Overlapped steps only.
-int32_t enqueue_registration(gca_pipeline p, int reg)
+// The registration plan's launches for the frame in slot reg onto the frame before it, on rc.
+int32_t registration_launches(gca_pipeline p, int reg, gca_ctx rc)
 {
     IcpPlan *plan = p->plan[reg];
-    gca_ctx rc = p->rctx;
-    GCA_HIP(hipEventRecord(p->ev_rfork, p->ctx->stream));
-    GCA_HIP(hipStreamWaitEvent(rc->stream, p->ev_rfork, 0));
     GCA_TRY(icp_plan_init(rc, plan));
     for (int l = 0; l < p->nl; ++l) {
         // the plan reads the target's grid through what it saw at its creation
@@ -147,9 +197,138 @@ int32_t enqueue_registration(gca_pipeline p, int reg)
         GCA_TRY(icp_plan_persist(rc, plan, l));
     }
     // results and the registered frame's lattice error word (its build ended a step earlier)
-    GCA_TRY(icp_plan_copy_results(rc, plan, err_dev_of(p, reg), err_host_of(p, reg)));
-    GCA_HIP(hipEventRecord(p->ev_rjoin, rc->stream));
+    return icp_plan_copy_results(rc, plan, err_dev_of(p, reg), err_host_of(p, reg));
+}
+
+// One lane: the registration of the frame in slot reg onto the frame before it, on the
+// registration stream, forked from the pipeline stream (so after everything enqueued there
+// before); the caller makes the pipeline stream wait for ev_rjoin.  Overlapped steps only.
+int32_t enqueue_registration(gca_pipeline p, int reg)
+{
+    GCA_HIP(hipEventRecord(p->ev_rfork, p->ctx->stream));
+    GCA_HIP(hipStreamWaitEvent(p->rctx->stream, p->ev_rfork, 0));
+    GCA_TRY(registration_launches(p, reg, p->rctx));
+    GCA_HIP(hipEventRecord(p->ev_rjoin, p->rctx->stream));
     return GCA_OK;
+}
+
+// Two lanes: the cross-lane order of a step's two parts (the registration, the build), as
+// events the host waits for and records around each part (the parts themselves are plain kernel
+// sequences on one stream each, replayed as graphs).  A wait for an event never recorded yet (the
+// first steps) is skipped.
+int32_t wait_slot(gca_ctx c, hipEvent_t e, bool recorded)
+{
+    if (recorded) GCA_HIP(hipStreamWaitEvent(c->stream, e, 0));
+    return GCA_OK;
+}
+
+// Before the registration of the frame in slot reg (built on the other lane) onto the frame before
+// it (built on this lane), on lane j's registration stream: both builds done.
+int32_t before_registration(gca_pipeline p, int reg, int j)
+{
+    gca_ctx rc = lane(p, j).reg;
+    const int tgt = prev_slot(p, reg);
+    GCA_TRY(wait_slot(rc, p->ev_slot_built[reg], p->slot_built_recorded[reg]));
+    return wait_slot(rc, p->ev_slot_built[tgt], p->slot_built_recorded[tgt]);
+}
+
+int32_t after_registration(gca_pipeline p, int reg, int j)
+{
+    GCA_HIP(hipEventRecord(p->ev_slot_reg[reg], lane(p, j).reg->stream));
+    p->slot_reg_recorded[reg] = true;
+    return GCA_OK;
+}
+
+// Before the build of slot `slot` (frame k) on lane j's pipeline stream: the two registrations
+// that last read the slot's clouds are done (frame k - 4's own, run on the other lane, and frame
+// k - 3's, which read it as its target, run on this lane's registration stream).
+int32_t before_build(gca_pipeline p, int slot, int j)
+{
+    gca_ctx m = lane(p, j).main;
+    const int next = (slot + 1) % p->nslots;
+    GCA_TRY(wait_slot(m, p->ev_slot_reg[slot], p->slot_reg_recorded[slot]));
+    return wait_slot(m, p->ev_slot_reg[next], p->slot_reg_recorded[next]);
+}
+
+int32_t after_build(gca_pipeline p, int slot, int j)
+{
+    GCA_HIP(hipEventRecord(p->ev_slot_built[slot], lane(p, j).main->stream));
+    p->slot_built_recorded[slot] = true;
+    return GCA_OK;
+}
+
+// Lockstep build: each kernel of the levels' build (voxel, grid, joint normals + gradient)
+// launched once for all levels, on one stream.  Possible when every level fits one multi-level
+// launch; overlapped steps then run in two lanes (each a pipeline stream and a registration
+// stream: per-level streams in two lanes would need more hardware queues than the device gives
+// a process).
+bool lockstep_ok(const gca_pipeline_s *p)
+{
+    if (p->nl > kMaxJobs) return false;
+    for (int l = 0; l < p->nl; ++l)
+        if (p->lv[l].gradient_max_nn != 0 || !(p->lv[l].gradient_radius <= p->lv[l].normal_radius)) return false;
+    return true;
+}
+
+// While it lives, the level ctxs enqueue on stream s (the lockstep build's scratch set-up:
+// first-use clears and re-allocations are then ordered with its launches).
+struct LevelStreamLease {
+    gca_ctx *lc;
+    int n;
+    hipStream_t saved[kPipeMaxLevels] = {};
+    LevelStreamLease(gca_ctx *levels, int nl, hipStream_t s) : lc(levels), n(nl)
+    {
+        for (int l = 0; l < n; ++l) {
+            saved[l] = lc[l]->stream;
+            lc[l]->stream = s;
+        }
+    }
+    ~LevelStreamLease()
+    {
+        for (int l = 0; l < n; ++l) lc[l]->stream = saved[l];
+    }
+};
+
+// Slot `slot`'s levels built in lockstep on lane j's pipeline stream, with its level scratch.
+int32_t build_lockstep(gca_pipeline p, int slot, int j)
+{
+    const Lane L = lane(p, j);
+    gca_ctx ctx = L.main;
+    LevelStreamLease lease(L.levels, p->nl, ctx->stream);
+    VoxelLevelJob vj[kMaxJobs];
+    NormalsLevelJob nj[kMaxJobs];
+    gca_cloud cs[kMaxJobs];
+    for (int l = 0; l < p->nl; ++l) {
+        gca_ctx lc = L.levels[l];
+        gca_cloud c = p->lvc[slot][l];
+        const gca_level_params &V = p->lv[l];
+        GCA_TRY(cloud_reserve(c, std::max<int64_t>(p->full[slot]->cap, 1)));
+        GCA_TRY(grid_prepare(lc, c, p->gdesc[l]));
+        vj[l] = VoxelLevelJob{lc, p->full[slot], c, V.voxel_size, p->max_bits[l], err_dev_of(p, slot),
+                              GridCount{p->gdesc[l], c->grid.count, c->grid.cell_of}};
+        nj[l] = NormalsLevelJob{lc, c, V.normal_radius, V.normal_max_nn, V.gradient_radius};
+        cs[l] = c;
+    }
+    GCA_TRY(launch_voxel_levels(ctx, vj, p->nl));
+    GCA_TRY(launch_grid_levels(ctx, L.levels, cs, p->gdesc, p->nl));
+    GCA_TRY(launch_normals_gradient_levels(ctx, nj, p->nl));
+    for (int l = 0; l < p->nl; ++l) {
+        cs[l]->has_normals = cs[l]->has_gradient = true;
+        GCA_TRY(grid_gather_attributes(ctx, cs[l]));  // no launch: the solve wrote them
+    }
+    return GCA_OK;
+}
+
+// Two lanes (step k on lane k mod 2): frame k's front end and lockstep build into slot `slot` on
+// lane j's pipeline stream.  It runs beside the registration of frame k - 1 (this lane's
+// registration stream) and beside the other lane's step; before_build / after_build order it
+// across lanes.
+int32_t lane_build(gca_pipeline p, int slot, int j, const uint16_t *depth, const uint8_t *bgr)
+{
+    const Lane L = lane(p, j);
+    GCA_TRY(launch_from_rgbd(L.main, depth, bgr, &p->cam, &p->bil, p->tmin, p->tmax, p->full[slot],
+                             err_dev_of(p, slot)));
+    return build_lockstep(p, slot, j);
 }
 
 // Everything a step enqueues (no host round trip, no allocation once warm): the new frame's
@@ -160,6 +339,17 @@ int32_t enqueue_registration(gca_pipeline p, int reg)
 // build, which touch neither of the two frames it reads.
 int32_t enqueue(gca_pipeline p, int slot, int reg, const uint16_t *depth, const uint8_t *bgr)
 {
+    if (p->lanes == 2) {  // eager two-lane step: the registration, then the build (see gca_pipeline_step)
+        const int j = lane_of(p, slot);
+        if (reg >= 0) {
+            GCA_TRY(before_registration(p, reg, j));
+            GCA_TRY(registration_launches(p, reg, lane(p, j).reg));
+            GCA_TRY(after_registration(p, reg, j));
+        }
+        GCA_TRY(before_build(p, slot, j));
+        GCA_TRY(lane_build(p, slot, j, depth, bgr));
+        return after_build(p, slot, j);
+    }
     gca_ctx ctx = p->ctx;
     const bool overlapped = p->overlap && reg >= 0;
     IcpPlan *plan = reg >= 0 && !p->overlap ? p->plan[reg] : nullptr;  // the inline chain
@@ -230,25 +420,20 @@ int32_t enqueue(gca_pipeline p, int slot, int reg, const uint16_t *depth, const 
     return GCA_OK;
 }
 
-int32_t capture(gca_pipeline p, int slot, int reg, const uint16_t *depth, const uint8_t *bgr)
+// What work() enqueues on origin's stream (and on streams it forks from there), captured and
+// instantiated.
+template <class F>
+int32_t capture_graph(gca_pipeline p, gca_ctx origin, F work, hipGraph_t *g_out, hipGraphExec_t *ex_out)
 {
-    gca_ctx ctx = p->ctx;
     hipGraph_t g = nullptr;
-    if (!p->front[slot] && !(p->front[slot] = front_launch_new())) return GCA_ERR_OUT_OF_MEMORY;
-    ctx->capturing = true;
-    ctx->front_rec = p->front[slot];
-    p->rctx->capturing = true;
-    for (int l = 0; l < p->nl; ++l) p->lctx[l]->capturing = true;
-    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
-    int32_t st = e == hipSuccess ? enqueue(p, slot, reg, depth, bgr) : hip_status(e);
+    each_ctx(p, [](gca_ctx c) { c->capturing = true; });
+    hipError_t e = hipStreamBeginCapture(origin->stream, hipStreamCaptureModeThreadLocal);
+    int32_t st = e == hipSuccess ? work() : hip_status(e);
     if (e == hipSuccess) {
-        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        const hipError_t e2 = hipStreamEndCapture(origin->stream, &g);
         if (st == GCA_OK && e2 != hipSuccess) st = hip_status(e2);
     }
-    ctx->capturing = false;
-    ctx->front_rec = nullptr;
-    p->rctx->capturing = false;
-    for (int l = 0; l < p->nl; ++l) p->lctx[l]->capturing = false;
+    each_ctx(p, [](gca_ctx c) { c->capturing = false; });
     if (st == GCA_OK && !g) st = GCA_ERR_DEVICE;
     hipGraphExec_t ex = nullptr;
     if (st == GCA_OK) {
@@ -259,6 +444,27 @@ int32_t capture(gca_pipeline p, int slot, int reg, const uint16_t *depth, const 
         if (g) hipGraphDestroy(g);
         return st;
     }
+    *g_out = g;
+    *ex_out = ex;
+    return GCA_OK;
+}
+
+// The step of slot `slot` as a graph (two lanes: its build part; the registration part is
+// capture_registration's).
+int32_t capture(gca_pipeline p, int slot, int reg, const uint16_t *depth, const uint8_t *bgr)
+{
+    const int j = lane_of(p, slot);
+    gca_ctx ctx = lane(p, j).main;
+    if (!p->front[slot] && !(p->front[slot] = front_launch_new())) return GCA_ERR_OUT_OF_MEMORY;
+    ctx->front_rec = p->front[slot];
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    const int32_t st = capture_graph(
+        p, ctx,
+        [&]() { return p->lanes == 2 ? lane_build(p, slot, j, depth, bgr) : enqueue(p, slot, reg, depth, bgr); }, &g,
+        &ex);
+    ctx->front_rec = nullptr;
+    GCA_TRY(st);
     if (p->exec[slot]) hipGraphExecDestroy(p->exec[slot]);
     if (p->graph[slot]) hipGraphDestroy(p->graph[slot]);
     p->exec[slot] = ex;
@@ -268,6 +474,21 @@ int32_t capture(gca_pipeline p, int slot, int reg, const uint16_t *depth, const 
     p->front_node[slot] = front_launch_node(p->front[slot], g);
     p->cap_depth[slot] = depth;
     p->cap_bgr[slot] = bgr;
+    ++p->captures;
+    return GCA_OK;
+}
+
+// Two lanes: the registration of slot reg's frame (run by lane j) as a graph.
+int32_t capture_registration(gca_pipeline p, int reg, int j)
+{
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    gca_ctx rc = lane(p, j).reg;
+    GCA_TRY(capture_graph(p, rc, [&]() { return registration_launches(p, reg, rc); }, &g, &ex));
+    if (p->exec_reg[reg]) hipGraphExecDestroy(p->exec_reg[reg]);
+    if (p->graph_reg[reg]) hipGraphDestroy(p->graph_reg[reg]);
+    p->exec_reg[reg] = ex;
+    p->graph_reg[reg] = g;
     ++p->captures;
     return GCA_OK;
 }
@@ -315,6 +536,36 @@ void set_geometry(gca_pipeline p)
     }
 }
 
+// Lane 1's ctxs and events (two-lane overlapped steps), made on first use.
+int32_t make_lane1(gca_pipeline p)
+{
+    if (p->ctx1) return GCA_OK;
+    const int dev = p->ctx->device;
+    GCA_TRY(gca_ctx_create(dev, &p->ctx1));
+    p->ctx1->prof_owner = p->ctx;
+    GCA_TRY(gca_ctx_create(dev, &p->rctx1));
+    p->rctx1->prof_owner = p->ctx;
+    for (int l = 0; l < p->nl; ++l) {
+        GCA_TRY(gca_ctx_create(dev, &p->lctx1[l]));
+        p->lctx1[l]->prof_owner = p->ctx;
+    }
+    return GCA_OK;
+}
+
+// Plans, graphs and the geometry follow a tuning change; every ctx launches with the pipeline
+// ctx's shapes.
+int32_t follow_tuning(gca_pipeline p)
+{
+    if (p->ctx->tune_version != p->tune_version) {  // plans and graphs bake the launch shapes in
+        GCA_TRY(sync_all(p));
+        drop_plans(p);
+        set_geometry(p);  // the grid cell may be a tuned one
+        p->tune_version = p->ctx->tune_version;
+    }
+    each_ctx(p, [&](gca_ctx c) { memcpy(c->tune, p->ctx->tune, sizeof(p->ctx->tune)); });
+    return GCA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -322,10 +573,7 @@ extern "C" {
 int32_t gca_pipeline_destroy(gca_pipeline p)
 {
     if (!p) return GCA_ERR_INVALID_VALUE;
-    if (p->ctx) hipStreamSynchronize(p->ctx->stream);
-    if (p->rctx) hipStreamSynchronize(p->rctx->stream);
-    for (int l = 0; l < kPipeMaxLevels; ++l)
-        if (p->lctx[l]) hipStreamSynchronize(p->lctx[l]->stream);
+    sync_all(p);
     drop_plans(p);
     for (int s = 0; s < kPipeSlots; ++s) {
         front_launch_free(p->front[s]);
@@ -338,11 +586,18 @@ int32_t gca_pipeline_destroy(gca_pipeline p)
     if (p->ev_fork) hipEventDestroy(p->ev_fork);
     if (p->ev_rfork) hipEventDestroy(p->ev_rfork);
     if (p->ev_rjoin) hipEventDestroy(p->ev_rjoin);
+    for (int s = 0; s < kPipeSlots; ++s) {
+        if (p->ev_slot_built[s]) hipEventDestroy(p->ev_slot_built[s]);
+        if (p->ev_slot_reg[s]) hipEventDestroy(p->ev_slot_reg[s]);
+    }
     if (p->rctx) gca_ctx_destroy(p->rctx);
+    if (p->rctx1) gca_ctx_destroy(p->rctx1);
+    if (p->ctx1) gca_ctx_destroy(p->ctx1);
     for (int l = 0; l < kPipeMaxLevels; ++l) {
         if (p->ev_join[l]) hipEventDestroy(p->ev_join[l]);
         if (p->ev_built[l]) hipEventDestroy(p->ev_built[l]);
         if (p->lctx[l]) gca_ctx_destroy(p->lctx[l]);
+        if (p->lctx1[l]) gca_ctx_destroy(p->lctx1[l]);
     }
     if (p->ctx) gca_ctx_destroy(p->ctx);
     delete p;
@@ -396,6 +651,10 @@ int32_t gca_pipeline_create(gca_ctx ctx, const gca_camera *cam, const gca_bilate
                          hipEventCreateWithFlags(&p->ev_rfork, hipEventDisableTiming) != hipSuccess ||
                          hipEventCreateWithFlags(&p->ev_rjoin, hipEventDisableTiming) != hipSuccess))
         st = GCA_ERR_DEVICE;
+    for (int s = 0; s < kPipeSlots && st == GCA_OK; ++s)
+        if (hipEventCreateWithFlags(&p->ev_slot_built[s], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&p->ev_slot_reg[s], hipEventDisableTiming) != hipSuccess)
+            st = GCA_ERR_DEVICE;
     const int64_t cap = (int64_t)cam->width * cam->height;
     for (int s = 0; s < kPipeSlots && st == GCA_OK; ++s) {
         st = gca_cloud_create(p->ctx, cap, &p->full[s]);
@@ -433,28 +692,25 @@ int32_t gca_pipeline_step(gca_pipeline p, const uint16_t *depth_dev, const uint8
 {
     if (!p || !depth_dev || !bgr_dev) return GCA_ERR_INVALID_VALUE;
     const int slot = (int)(p->frames % p->nslots);
-    if (p->ctx->tune_version != p->tune_version) {  // plans and graphs bake the launch shapes in
-        GCA_HIP(hipStreamSynchronize(p->ctx->stream));
-        GCA_HIP(hipStreamSynchronize(p->rctx->stream));
-        for (int l = 0; l < p->nl; ++l) GCA_HIP(hipStreamSynchronize(p->lctx[l]->stream));
-        drop_plans(p);
-        set_geometry(p);  // the grid cell may be a tuned one
-        p->tune_version = p->ctx->tune_version;
-    }
-    // the level and registration ctxs launch with the pipeline ctx's shapes
-    for (int l = 0; l < p->nl; ++l) memcpy(p->lctx[l]->tune, p->ctx->tune, sizeof(p->ctx->tune));
-    memcpy(p->rctx->tune, p->ctx->tune, sizeof(p->ctx->tune));
+    GCA_TRY(follow_tuning(p));
     // the registration this step runs: sequential, the new frame's (from its second frame on);
     // overlapped, the previous frame's (unless a flush ran it)
     const int reg = p->overlap ? p->pending : (p->frames >= 1 ? slot : -1);
-    if (reg >= 0 && !p->plan[reg]) GCA_TRY(make_plan(p, reg));
     // a graph is captured only once every buffer it will point into exists: after an eager
     // step of each slot with its plan in place
     bool plans = true;
-    for (int s = 0; s < p->nslots; ++s) plans = plans && p->plan[s];
+    for (int s = 0; s < p->nslots; ++s) plans = plans && (p->plan[s] || s == reg);
     const bool graph = p->use_graph && plans && p->eager_since_reset >= p->nslots &&
                        (p->overlap ? reg == prev_slot(p, slot) : reg == slot);
+    // Two lanes: buffers are (re)allocated on the host only by eager steps and captures, and
+    // one lane's allocation must not free what the other lane's queued work still reads, so
+    // those steps start from an idle pipeline (a replayed step allocates nothing).
+    const bool replay = graph && p->exec[slot] && (p->lanes == 1 || p->exec_reg[reg]);
+    if (p->lanes == 2 && !replay) GCA_TRY(sync_all(p));
+    if (reg >= 0 && !p->plan[reg]) GCA_TRY(make_plan(p, reg));
+    const int j = lane_of(p, slot);
     if (graph) {
+        if (p->lanes == 2 && !p->exec_reg[reg]) GCA_TRY(capture_registration(p, reg, j));
         if (!p->exec[slot]) {
             GCA_TRY(capture(p, slot, reg, depth_dev, bgr_dev));
         } else if (p->cap_depth[slot] != depth_dev || p->cap_bgr[slot] != bgr_dev) {
@@ -471,7 +727,16 @@ int32_t gca_pipeline_step(gca_pipeline p, const uint16_t *depth_dev, const uint8
                 GCA_TRY(capture(p, slot, reg, depth_dev, bgr_dev));
             }
         }
-        GCA_HIP(hipGraphLaunch(p->exec[slot], p->ctx->stream));
+        if (p->lanes == 2) {
+            GCA_TRY(before_registration(p, reg, j));
+            GCA_HIP(hipGraphLaunch(p->exec_reg[reg], lane(p, j).reg->stream));
+            GCA_TRY(after_registration(p, reg, j));
+            GCA_TRY(before_build(p, slot, j));
+            GCA_HIP(hipGraphLaunch(p->exec[slot], lane(p, j).main->stream));
+            GCA_TRY(after_build(p, slot, j));
+        } else {
+            GCA_HIP(hipGraphLaunch(p->exec[slot], p->ctx->stream));
+        }
     } else {
         GCA_TRY(enqueue(p, slot, reg, depth_dev, bgr_dev));
         if (reg >= 0) ++p->eager_since_reset;
@@ -494,18 +759,21 @@ int32_t gca_pipeline_flush(gca_pipeline p)
     if (!p) return GCA_ERR_INVALID_VALUE;
     if (!p->overlap || p->pending < 0) return GCA_OK;
     const int reg = p->pending;
-    if (p->ctx->tune_version != p->tune_version) {
-        GCA_HIP(hipStreamSynchronize(p->ctx->stream));
-        GCA_HIP(hipStreamSynchronize(p->rctx->stream));
-        for (int l = 0; l < p->nl; ++l) GCA_HIP(hipStreamSynchronize(p->lctx[l]->stream));
-        drop_plans(p);
-        set_geometry(p);
-        p->tune_version = p->ctx->tune_version;
-        memcpy(p->rctx->tune, p->ctx->tune, sizeof(p->ctx->tune));
+    GCA_TRY(follow_tuning(p));
+    if (!p->plan[reg]) {
+        if (p->lanes == 2) GCA_TRY(sync_all(p));  // see gca_pipeline_step
+        GCA_TRY(make_plan(p, reg));
     }
-    if (!p->plan[reg]) GCA_TRY(make_plan(p, reg));
-    GCA_TRY(enqueue_registration(p, reg));
-    GCA_HIP(hipStreamWaitEvent(p->ctx->stream, p->ev_rjoin, 0));
+    if (p->lanes == 2) {
+        // on the lane the next step would take (the one that would register this frame)
+        const int j = (reg + 1) & 1;
+        GCA_TRY(before_registration(p, reg, j));
+        GCA_TRY(registration_launches(p, reg, lane(p, j).reg));
+        GCA_TRY(after_registration(p, reg, j));
+    } else {
+        GCA_TRY(enqueue_registration(p, reg));
+        GCA_HIP(hipStreamWaitEvent(p->ctx->stream, p->ev_rjoin, 0));
+    }
     p->last = reg;
     p->pending = -1;
     return GCA_OK;
@@ -514,12 +782,18 @@ int32_t gca_pipeline_flush(gca_pipeline p)
 int32_t gca_pipeline_set_overlap(gca_pipeline p, int32_t enable)
 {
     if (!p) return GCA_ERR_INVALID_VALUE;
-    GCA_HIP(hipStreamSynchronize(p->ctx->stream));
-    GCA_HIP(hipStreamSynchronize(p->rctx->stream));
-    for (int l = 0; l < p->nl; ++l) GCA_HIP(hipStreamSynchronize(p->lctx[l]->stream));
+    GCA_TRY(sync_all(p));
     drop_plans(p);
     p->overlap = enable != 0;
-    p->nslots = p->overlap ? 3 : 2;
+    // overlapped steps run in two lanes when the levels can be built in lockstep
+    p->lanes = p->overlap && lockstep_ok(p) ? 2 : 1;
+    if (p->lanes == 2) GCA_TRY(make_lane1(p));
+    // synchronising the pipeline ctx waits for both lanes (registrations are joined by events)
+    p->ctx->sync_also[0] = p->lanes == 2 ? p->rctx : nullptr;
+    p->ctx->sync_also[1] = p->lanes == 2 ? p->ctx1 : nullptr;
+    p->ctx->sync_also[2] = p->lanes == 2 ? p->rctx1 : nullptr;
+    p->nslots = p->lanes == 2 ? 4 : (p->overlap ? 3 : 2);
+    for (int s = 0; s < kPipeSlots; ++s) p->slot_built_recorded[s] = p->slot_reg_recorded[s] = false;
     p->frames = 0;
     p->pending = -1;
     p->last = -1;
@@ -530,10 +804,13 @@ int32_t gca_pipeline_result(gca_pipeline p, double T[16], gca_icp_report *report
 {
     if (!p || !T || !valid) return GCA_ERR_INVALID_VALUE;
     *valid = 0;
-    GCA_HIP(hipStreamSynchronize(p->ctx->stream));
-    GCA_TRY(ctx_take_device_error(p->ctx));
-    GCA_TRY(ctx_take_device_error(p->rctx));
-    for (int l = 0; l < p->nl; ++l) GCA_TRY(ctx_take_device_error(p->lctx[l]));
+    GCA_TRY(sync_all(p));
+    int32_t dev_st = GCA_OK;
+    each_ctx(p, [&](gca_ctx c) {
+        const int32_t e = ctx_take_device_error(c);
+        if (dev_st == GCA_OK) dev_st = e;
+    });
+    GCA_TRY(dev_st);
     if (p->last >= 0 && *err_host_of(p, p->last)) return GCA_ERR_INVALID_VALUE;  // a voxel lattice was refused
     const bool registered = p->overlap ? p->last >= 0 : p->frames >= 2;
     if (!registered) {
@@ -543,6 +820,13 @@ int32_t gca_pipeline_result(gca_pipeline p, double T[16], gca_icp_report *report
     }
     GCA_TRY(icp_plan_results(p->ctx, p->plan[p->last], T, reports));
     *valid = 1;
+    return GCA_OK;
+}
+
+int32_t gca_pipeline_slots(gca_pipeline p, int32_t *slots)
+{
+    if (!p || !slots) return GCA_ERR_INVALID_VALUE;
+    *slots = p->nslots;
     return GCA_OK;
 }
 
@@ -557,6 +841,7 @@ int32_t gca_pipeline_graph_stats(gca_pipeline p, int64_t *captures, int64_t *nod
 int32_t gca_pipeline_sizes(gca_pipeline p, int32_t slot, int64_t *full_n, int64_t *level_n)
 {
     if (!p || slot < 0 || slot >= p->nslots || !full_n) return GCA_ERR_INVALID_VALUE;
+    GCA_TRY(sync_all(p));  // the slot may be another lane's
     GCA_TRY(cloud_sync_header(p->full[slot]));
     *full_n = p->full[slot]->host_hdr.n;
     if (level_n)

@@ -88,18 +88,30 @@ __device__ __forceinline__ int64_t voxel_key(const VoxelParams &vp, float4 p)
 // only the first lane of a run issues the atomics.  Every workgroup derives the lattice from the
 // header itself; workgroup 0 also stores it for the later kernels of the call and flags a
 // refused lattice (*err).
-__global__ __launch_bounds__(256) void k_voxel_mark(const float4 *__restrict__ xyzi,
-                                                    const CloudHdr *__restrict__ hdr, float voxel,
-                                                    double max_bits, VoxelParams *__restrict__ vp_out,
-                                                    uint32_t *__restrict__ err, uint32_t *__restrict__ bitmap,
-                                                    int64_t *__restrict__ lin)
+struct MarkArgs {
+    const float4 *xyzi;
+    const CloudHdr *hdr;
+    float voxel;
+    double max_bits;
+    VoxelParams *vp_out;
+    uint32_t *err, *bitmap;
+    int64_t *lin;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_voxel_mark(Jobs<MarkArgs, N> J)
 {
+    const MarkArgs &A = J.a[N == 1 ? 0 : blockIdx.y];
+    const float4 *__restrict__ xyzi = A.xyzi;
+    const CloudHdr *__restrict__ hdr = A.hdr;
+    uint32_t *__restrict__ bitmap = A.bitmap;
+    int64_t *__restrict__ lin = A.lin;
     __shared__ VoxelParams s_vp;
     if (threadIdx.x == 0) {
-        s_vp = voxel_params_of(hdr, voxel, max_bits);
+        s_vp = voxel_params_of(hdr, A.voxel, A.max_bits);
         if (blockIdx.x == 0) {
-            *vp_out = s_vp;
-            if (s_vp.bad && err) atomicOr(err, 1u);
+            *A.vp_out = s_vp;
+            if (s_vp.bad && A.err) atomicOr(A.err, 1u);
         }
     }
     __syncthreads();
@@ -141,19 +153,29 @@ constexpr int kVoxTableBits = 10;
 constexpr int kVoxTable = 1 << kVoxTableBits;
 constexpr int kVoxProbe = 16;
 
-__global__ __launch_bounds__(kVoxBlock) void k_voxel_accum(const float4 *__restrict__ xyzi,
-                                                     const float4 *__restrict__ rgb,
-                                                     const CloudHdr *__restrict__ hdr,
-                                                     const VoxelParams *__restrict__ vpp,
-                                                     const uint32_t *__restrict__ bitmap,
-                                                     const uint32_t *__restrict__ word_pre,
-                                                     const int64_t *__restrict__ lin,
-                                                     unsigned long long *__restrict__ acc)
+struct AccumArgs {
+    const float4 *xyzi, *rgb;
+    const CloudHdr *hdr;
+    const VoxelParams *vpp;
+    const uint32_t *bitmap, *word_pre;
+    const int64_t *lin;
+    unsigned long long *acc;
+};
+
+template <int N>
+__global__ __launch_bounds__(kVoxBlock) void k_voxel_accum(Jobs<AccumArgs, N> J)
 {
+    const AccumArgs &A = J.a[N == 1 ? 0 : blockIdx.y];
+    const float4 *__restrict__ xyzi = A.xyzi;
+    const float4 *__restrict__ rgb = A.rgb;
+    const uint32_t *__restrict__ bitmap = A.bitmap;
+    const uint32_t *__restrict__ word_pre = A.word_pre;
+    const int64_t *__restrict__ lin = A.lin;
+    unsigned long long *__restrict__ acc = A.acc;
     __shared__ int32_t t_rank[kVoxTable];
     __shared__ long long t_key[kVoxTable];
     __shared__ unsigned long long t_sum[7][kVoxTable];
-    const int n = vpp->bad ? 0 : hdr->n;
+    const int n = A.vpp->bad ? 0 : A.hdr->n;
     for (int h = threadIdx.x; h < kVoxTable; h += blockDim.x) {
         t_rank[h] = -1;
 #pragma unroll
@@ -221,15 +243,29 @@ __global__ __launch_bounds__(kVoxBlock) void k_voxel_accum(const float4 *__restr
 
 // gc_count != nullptr: each output point is also counted into its cell of the grid gc (and its
 // cell recorded), which is the first kernel of that grid's build.
-__global__ __launch_bounds__(256) void k_voxel_finalize(unsigned long long *__restrict__ acc,
-                                                        const CloudHdr *__restrict__ in_hdr,
-                                                        CloudHdr *__restrict__ out_hdr,
-                                                        uint32_t *__restrict__ bitmap,
-                                                        float4 *__restrict__ out_xyzi,
-                                                        float4 *__restrict__ out_rgb, GridDesc gd,
-                                                        uint32_t *__restrict__ gc_count,
-                                                        uint32_t *__restrict__ gc_cell_of)
+struct FinalArgs {
+    unsigned long long *acc;
+    const CloudHdr *in_hdr;
+    CloudHdr *out_hdr;
+    uint32_t *bitmap;
+    float4 *out_xyzi, *out_rgb;
+    GridDesc gd;
+    uint32_t *gc_count, *gc_cell_of;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_voxel_finalize(Jobs<FinalArgs, N> J)
 {
+    const FinalArgs &A = J.a[N == 1 ? 0 : blockIdx.y];
+    unsigned long long *__restrict__ acc = A.acc;
+    const CloudHdr *__restrict__ in_hdr = A.in_hdr;
+    CloudHdr *__restrict__ out_hdr = A.out_hdr;
+    uint32_t *__restrict__ bitmap = A.bitmap;
+    float4 *__restrict__ out_xyzi = A.out_xyzi;
+    float4 *__restrict__ out_rgb = A.out_rgb;
+    const GridDesc &gd = A.gd;
+    uint32_t *__restrict__ gc_count = A.gc_count;
+    uint32_t *__restrict__ gc_cell_of = A.gc_cell_of;
     const int m = out_hdr->n;
     if (blockIdx.x == 0 && threadIdx.x < 3) {
         out_hdr->bmin_key[threadIdx.x] = in_hdr->bmin_key[threadIdx.x];
@@ -418,17 +454,27 @@ static int32_t voxel_sort_enqueue(gca_ctx ctx, gca_cloud in, const VoxelParams &
     GCA_TRY(exclusive_scan_u32(ctx, flag, seg, n, (uint32_t *)&out->hdr->n));  // n = 0: M = 0
     hipLaunchKernelGGL(k_vsort_reduce, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->rgb, is, flag, seg, n, acc);
     GCA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_voxel_finalize, dim3((unsigned)std::min<int64_t>(4096, nb)), dim3(256), 0, ctx->stream, acc,
-                       in->hdr, out->hdr, nullptr, out->xyzi, out->rgb, GridDesc{}, nullptr, nullptr);
+    const Jobs<FinalArgs, 1> fin{{{acc, in->hdr, out->hdr, nullptr, out->xyzi, out->rgb, GridDesc{}, nullptr, nullptr}}};
+    hipLaunchKernelGGL(k_voxel_finalize<1>, dim3((unsigned)std::min<int64_t>(4096, nb)), dim3(256), 0, ctx->stream, fin);
     GCA_HIP(hipGetLastError());
     cloud_points_changed(out);
     return GCA_OK;
 }
 
-// Kernel sequence of one voxel call: mark (with the device lattice), popcount scan over `words` bitmap
-// words (>= the lattice's), accumulate, finalize.  Scratch is sized for `n_cap` input points.
-static int32_t voxel_enqueue(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, int64_t n_cap, int64_t words,
-                             double max_bits, uint32_t *err, const GridCount *gc = nullptr)
+// The kernels of one bitmap-path voxel call, as jobs.
+struct VoxelJobs {
+    MarkArgs mark;
+    ScanJob scan;
+    AccumArgs accum;
+    FinalArgs fin;
+    unsigned nb, nb_acc;
+};
+
+// Host side of one voxel call: the ctx's bitmap and scratch for `n_cap` input points and a
+// popcount scan over `words` bitmap words (>= the lattice's); enqueues nothing but a first-use
+// clear of the bitmap on ctx's stream.
+static int32_t voxel_prepare(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, int64_t n_cap, int64_t words,
+                             double max_bits, uint32_t *err, const GridCount *gc, bool wide, VoxelJobs *vj)
 {
     if (ctx->bitmap_words < words || ctx->bitmap_dirty) {
         GCA_HIP(hipStreamSynchronize(ctx->stream));
@@ -449,20 +495,56 @@ static int32_t voxel_enqueue(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud o
     unsigned long long *acc = (unsigned long long *)ctx_scratch(ctx, 7, sizeof(unsigned long long) * kVoxRec * nc, true);
     VoxelParams *vp = (VoxelParams *)ctx_scratch(ctx, 22, sizeof(VoxelParams), false);
     if (!lin || !word_pre || !acc || !vp) return GCA_ERR_OUT_OF_MEMORY;
-    const unsigned nb = (unsigned)std::min<int64_t>(4096, (nc + 255) / 256);
+    GCA_TRY(scan_job(ctx, ctx->bitmap, word_pre, words, (uint32_t *)&out->hdr->n, wide, &vj->scan));
+    vj->nb = (unsigned)std::min<int64_t>(4096, (nc + 255) / 256);
+    vj->nb_acc = (unsigned)std::min<int64_t>(4096, (nc + kVoxChunk - 1) / kVoxChunk);
+    vj->mark = MarkArgs{in->xyzi, in->hdr, voxel, max_bits, vp, err, ctx->bitmap, lin};
+    vj->accum = AccumArgs{in->xyzi, in->rgb, in->hdr, vp, ctx->bitmap, word_pre, lin, acc};
+    vj->fin = FinalArgs{acc, in->hdr, out->hdr, ctx->bitmap, out->xyzi, out->rgb, gc ? gc->d : GridDesc{},
+                        gc ? gc->count : nullptr, gc ? gc->cell_of : nullptr};
+    return GCA_OK;
+}
+
+// mark (with the device lattice), popcount scan, accumulate, finalize: each kernel once for all
+// nj calls (blockIdx.y = call), on stream.
+template <int N>
+static int32_t voxel_launch(hipStream_t stream, const VoxelJobs *vj, int nj, bool wide)
+{
+    Jobs<MarkArgs, N> mark{};
+    Jobs<AccumArgs, N> accum{};
+    Jobs<FinalArgs, N> fin{};
+    ScanJob scans[kMaxJobs];
+    unsigned nb = 1, nb_acc = 1;
+    for (int j = 0; j < nj; ++j) {
+        mark.a[j] = vj[j].mark;
+        accum.a[j] = vj[j].accum;
+        fin.a[j] = vj[j].fin;
+        scans[j] = vj[j].scan;
+        nb = std::max(nb, vj[j].nb);
+        nb_acc = std::max(nb_acc, vj[j].nb_acc);
+    }
+    hipLaunchKernelGGL(k_voxel_mark<N>, dim3(nb, (unsigned)nj), dim3(256), 0, stream, mark);
+    GCA_HIP(hipGetLastError());
+    GCA_TRY(scan_launch(stream, scans, nj, true, wide));
+    hipLaunchKernelGGL(k_voxel_accum<N>, dim3(nb_acc, (unsigned)nj), dim3(kVoxBlock), 0, stream, accum);
+    hipLaunchKernelGGL(k_voxel_finalize<N>, dim3(nb, (unsigned)nj), dim3(256), 0, stream, fin);
+    GCA_HIP(hipGetLastError());
+    return GCA_OK;
+}
+
+// Kernel sequence of one voxel call: mark (with the device lattice), popcount scan over `words` bitmap
+// words (>= the lattice's), accumulate, finalize.  Scratch is sized for `n_cap` input points.
+static int32_t voxel_enqueue(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud out, int64_t n_cap, int64_t words,
+                             double max_bits, uint32_t *err, const GridCount *gc = nullptr)
+{
+    const bool wide = scan_wide(words);
+    VoxelJobs vj;
+    GCA_TRY(voxel_prepare(ctx, in, voxel, out, n_cap, words, max_bits, err, gc, wide, &vj));
     ctx->bitmap_dirty = true;  // until the whole sequence is enqueued
-    StageTimer timer(ctx, GCA_STAGE_VOXEL);
-    hipLaunchKernelGGL(k_voxel_mark, dim3(nb), dim3(256), 0, ctx->stream, in->xyzi, in->hdr, voxel, max_bits, vp, err,
-                       ctx->bitmap, lin);
-    GCA_HIP(hipGetLastError());
-    GCA_TRY(exclusive_scan_u32(ctx, ctx->bitmap, word_pre, words, (uint32_t *)&out->hdr->n, true));
-    const unsigned nb_acc = (unsigned)std::min<int64_t>(4096, (nc + kVoxChunk - 1) / kVoxChunk);
-    hipLaunchKernelGGL(k_voxel_accum, dim3(nb_acc), dim3(kVoxBlock), 0, ctx->stream, in->xyzi, in->rgb, in->hdr,
-                       (const VoxelParams *)vp, ctx->bitmap, word_pre, lin, acc);
-    hipLaunchKernelGGL(k_voxel_finalize, dim3(nb), dim3(256), 0, ctx->stream, acc, in->hdr, out->hdr,
-                       ctx->bitmap, out->xyzi, out->rgb, gc ? gc->d : GridDesc{}, gc ? gc->count : nullptr,
-                       gc ? gc->cell_of : nullptr);
-    GCA_HIP(hipGetLastError());
+    {
+        StageTimer timer(ctx, GCA_STAGE_VOXEL);
+        GCA_TRY(voxel_launch<1>(ctx->stream, &vj, 1, wide));
+    }
     ctx->bitmap_dirty = false;
     cloud_points_changed(out);
     return GCA_OK;
@@ -496,6 +578,33 @@ int32_t launch_voxel_planned(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud o
     const int64_t words = ((int64_t)max_bits + 31) / 32;
     GCA_TRY(cloud_reserve(out, std::max<int64_t>(in->cap, 1)));
     return voxel_enqueue(ctx, in, voxel, out, in->cap, words, max_bits, err, gc);
+}
+
+int32_t launch_voxel_levels(gca_ctx launch, const VoxelLevelJob *jobs, int nj)
+{
+    if (nj < 1 || nj > kMaxJobs) return GCA_ERR_INVALID_VALUE;
+    int64_t words[kMaxJobs];
+    bool wide = false;
+    for (int j = 0; j < nj; ++j) {
+        words[j] = ((int64_t)jobs[j].max_bits + 31) / 32;
+        wide = wide || scan_wide(words[j]);
+    }
+    VoxelJobs vj[kMaxJobs];
+    for (int j = 0; j < nj; ++j) {
+        const VoxelLevelJob &J = jobs[j];
+        GCA_TRY(cloud_reserve(J.out, std::max<int64_t>(J.in->cap, 1)));
+        GCA_TRY(voxel_prepare(J.ctx, J.in, J.voxel, J.out, J.in->cap, words[j], J.max_bits, J.err, &J.gc, wide, &vj[j]));
+        J.ctx->bitmap_dirty = true;
+    }
+    {
+        StageTimer timer(launch, GCA_STAGE_VOXEL);
+        GCA_TRY(voxel_launch<kMaxJobs>(launch->stream, vj, nj, wide));
+    }
+    for (int j = 0; j < nj; ++j) {
+        jobs[j].ctx->bitmap_dirty = false;
+        cloud_points_changed(jobs[j].out);
+    }
+    return GCA_OK;
 }
 
 }  // namespace gca

@@ -11,13 +11,15 @@ without a host round trip:
 This is the loop of the reference's demo (apps/app.cpp:111-372: create_from_rgbd ->
 voxel_grid_down_sample -> estimate_normals -> color_icp.align against the previous cloud) with
 the coarse-to-fine schedule of apps/test_color_icp.cpp:378-414.  Every buffer is allocated
-once (two frame slots, three when overlapped); with graph=True each slot's step is replayed as a
-hipGraph.
+once (two frame slots; `slots` when overlapped); with graph=True each slot's step is replayed
+as a hipGraph.
 
 overlap=True: a step builds the new frame and, beside it on a second stream, registers the
-previous frame (built by the previous step); the poses are the sequential pipeline's, one step
-later.  result() then reports the frame before the one just stepped, and flush() registers the
-last stepped frame on its own.
+previous frame (built by the previous step); with up to four levels that use the joint normals
++ gradient walk, the levels are built in lockstep and consecutive steps alternate between two
+lanes of streams, so two frames' steps also run side by side (four slots).  The poses are the
+sequential pipeline's, one step later.  result() then reports the frame before the one just
+stepped, and flush() registers the last stepped frame on its own.
 """
 from __future__ import annotations
 
@@ -73,7 +75,10 @@ class FramePipeline:
 
     @property
     def slots(self) -> int:
-        return 3 if self.overlap else 2
+        """Frame slots of the current mode (step k uses slot k % slots)."""
+        n = C.c_int32()
+        L.check(L.lib.gca_pipeline_slots(self.handle, C.byref(n)), "gca_pipeline_slots")
+        return int(n.value)
 
     @property
     def slot(self) -> int:

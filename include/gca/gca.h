@@ -477,19 +477,28 @@ int32_t gca_pipeline_set_graph(gca_pipeline pipeline, int32_t enable);
 /* Graph captures and front-end node updates since the pipeline was created (no reference
  * counterpart: replay bookkeeping). */
 int32_t gca_pipeline_graph_stats(gca_pipeline pipeline, int64_t *captures, int64_t *node_updates);
-/* The pipeline's own ctx (its stream, stage profiler and tuning). */
+/* The pipeline's own ctx (its stream, stage profiler and tuning; synchronising it waits for
+ * every step enqueued). */
 int32_t gca_pipeline_ctx(gca_pipeline pipeline, gca_ctx *ctx);
-/* Point counts of a frame slot (0 .. 1 sequential, 0 .. 2 overlapped; the last step used slot
- * (frames - 1) % slots): the full cloud and level_n[n_levels] (may be NULL).  Synchronises. */
+/* Frame slots of the current mode (no reference counterpart): 2 sequential, 4 overlapped (3 when
+ * the levels cannot be built in lockstep; see gca_pipeline_set_overlap).  Step k uses slot
+ * k % slots. */
+int32_t gca_pipeline_slots(gca_pipeline pipeline, int32_t *slots);
+/* Point counts of a frame slot (0 .. slots - 1; the last step used slot (frames - 1) % slots):
+ * the full cloud and level_n[n_levels] (may be NULL).  Synchronises. */
 int32_t gca_pipeline_sizes(gca_pipeline pipeline, int32_t slot, int64_t *full_n, int64_t *level_n);
 /* Overlapped steps (no reference counterpart: the reference's loop runs one frame at a time).
  * enable != 0: each step builds the new frame (front end, every level) and, beside it on a
- * second stream, registers the previous frame, whose clouds the previous step built; three frame
- * slots, so the two never touch the same clouds.  A step's work is the same, and every pose
- * equals the sequential pipeline's bit for bit, but gca_pipeline_result after a step reports
- * the registration of the frame before the one just stepped; gca_pipeline_flush registers the
- * last stepped frame on its own.  Changing the mode synchronises and starts a new sequence
- * (the next frame is a first frame). */
+ * second stream, registers the previous frame, whose clouds the previous step built.  When the
+ * pipeline has at most 4 levels and each uses the joint normals + gradient walk (gradient max_nn
+ * 0, gradient radius <= normal radius), the levels are built in lockstep (each build kernel
+ * launched once for all levels) and consecutive steps alternate between two lanes of streams
+ * and scratch, so two frames' steps also run side by side; four frame slots (else one lane,
+ * per-level streams, three slots).  No two concurrent parts touch the same clouds.  A step's
+ * work is the same, and every pose equals the sequential pipeline's bit for bit, but
+ * gca_pipeline_result after a step reports the registration of the frame before the one just
+ * stepped; gca_pipeline_flush registers the last stepped frame on its own.  Changing the mode
+ * synchronises and starts a new sequence (the next frame is a first frame). */
 int32_t gca_pipeline_set_overlap(gca_pipeline pipeline, int32_t enable);
 /* Overlapped steps: enqueue the registration still pending for the last stepped frame, so that
  * gca_pipeline_result reports it.  No-op in sequential mode or when nothing is pending. */

@@ -1285,41 +1285,57 @@ def test_pipeline_rejects_bad_arguments(ctx):
 
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_pipeline_overlap_matches_sequential(ctx, graph):
-    """Overlapped steps (the previous frame registered beside the new frame's build, three frame
-    slots) give every pose of the sequential pipeline, bit for bit, one step later; flush()
-    registers the last frame.  Frames from four buffers, so the replayed graphs also follow new
-    frame buffers."""
+@pytest.mark.parametrize("graph,size,levels", [(False, (320, 240), 3), (True, (320, 240), 3), (True, (640, 480), 3),
+                                               (False, (320, 240), 5), (True, (320, 240), 5)])
+def test_pipeline_overlap_matches_sequential(ctx, graph, size, levels):
+    """Overlapped steps (the previous frame registered beside the new frame's build) give every
+    pose of the sequential pipeline, bit for bit, one step later, and every frame's level sizes;
+    flush() registers the last frame.  Three levels: lockstep builds in two lanes, four frame
+    slots; five levels (beyond one multi-level launch): one lane, per-level streams, three slots.
+    Poses are read only every few steps, so that several steps (both lanes) are in flight
+    between reads.  Frames from six buffers, so the replayed graphs also follow new frame
+    buffers."""
     import cuda_3d_amd as g
     from cuda_3d_amd.pipeline import FramePipeline
 
-    cam, dev, _, _ = _pipeline_frames(g, 320, 240, 26)
-    copies = [(d.clone(), c.clone()) for d, c in dev]
-    bufs = [dev[0], dev[1], copies[0], copies[1]]
+    vox = (0.04, 0.02, 0.01) if levels == 3 else (0.08, 0.04, 0.03, 0.02, 0.01)
+    cam, dev, _, _ = _pipeline_frames(g, size[0], size[1], 26)
+    bufs = list(dev) + [(d.clone(), c.clone()) for _ in range(2) for d, c in dev]  # frame k % 2 in bufs[k % 6]
     bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
-    seq = FramePipeline(ctx, cam, bil, graph=False)
-    ovl = FramePipeline(ctx, cam, bil, graph=graph, overlap=True)
-    want = []
-    for k in range(13):
-        d, c = bufs[k % 4]
+    seq = FramePipeline(ctx, cam, bil, vox, graph=False)
+    ovl = FramePipeline(ctx, cam, bil, vox, graph=graph, overlap=True)
+    slots = 4 if levels == 3 else 3
+    assert ovl.slots == slots
+    want, sizes = [], []
+    for k in range(16):
+        d, c = bufs[k % 6]
         seq.step(d.data_ptr(), c.data_ptr())
         Ts, rs = seq.result()
         want.append((Ts, [rep_key(r) for r in rs]) if Ts is not None else None)
+        sizes.append(seq.sizes(k % 2))
+    checked = 0
+    for k in range(16):
+        d, c = bufs[k % 6]
         ovl.step(d.data_ptr(), c.data_ptr())
+        if k not in (0, 1, 2, 5, 6, 7, 11, 15):
+            continue
         To, ro = ovl.result()
+        for j in range(max(0, k - slots + 1), k + 1):  # the slots still holding frames j
+            assert ovl.sizes(j % slots) == sizes[j]
         if k < 2:
             assert To is None  # nothing registered yet: frame k - 1 has no predecessor for k < 2
             continue
         np.testing.assert_array_equal(To, want[k - 1][0])
         assert [rep_key(r) for r in ro] == want[k - 1][1]
+        checked += 1
     ovl.flush()
     To, ro = ovl.result()
     np.testing.assert_array_equal(To, want[-1][0])
     assert [rep_key(r) for r in ro] == want[-1][1]
     captures, updates = ovl.graph_stats()
-    print(f"[gpu-parity] overlapped steps (graph {graph}): {captures} captures, {updates} node updates", flush=True)
-    if graph:
-        assert captures == 3 and updates >= 3
+    print(f"[gpu-parity] overlapped steps {size[0]}x{size[1]}, {levels} levels (graph {graph}): {checked + 1} poses "
+          f"checked, {captures} captures, {updates} node updates", flush=True)
+    if graph:  # two lanes: a build graph and a registration graph per slot
+        assert captures == (2 * slots if slots == 4 else slots) and updates >= 3
     seq.close()
     ovl.close()
