@@ -136,7 +136,7 @@ def run_fixed(g, ctx, cam, bil, ptrs, levels, iters, steps, graph, overlap):
     for ov in ((True, False) if overlap else (False,)):
         pipe = FramePipeline(ctx, cam, bil, levels, iters, conv_rotation=0.0, conv_translation_sq=0.0, graph=graph,
                              overlap=ov)
-        autotune.apply(pipe.ctx, workload="frame")
+        autotune.apply(pipe.ctx, workload="frame_overlapped" if ov else "frame")
         for k in range(12):  # eager steps of every frame slot, then graph capture
             pipe.step(*ptrs[k % 2])
         pipe.ctx.synchronize()
@@ -185,7 +185,7 @@ def run_sequence(g, ctx, cam, bil, levels, iters, W, H, n_frames, passes, graph,
         its.append([r.iterations for r in reps])
     seq.close()
     pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=graph, overlap=overlap)
-    autotune.apply(pipe.ctx, workload="frame")
+    autotune.apply(pipe.ctx, workload="frame_overlapped" if overlap else "frame")
     for k in range(max(n_frames + 1, 12)):  # eager steps of every slot, then graph capture
         pipe.step(*ptrs[k % n_frames])
     pipe.ctx.synchronize()
@@ -344,8 +344,11 @@ def main():
     pctx = seqp.ctx  # the sequential pipeline's own stream, profiler and tuning
     # the launch shapes autotuned on MI355X (cuda_3d_amd/tuning_mi355x.json), per workload
     tuning = {"table": os.path.relpath(autotune.DEFAULT_TABLE, ROOT),
-              "frame": autotune.apply(pipe.ctx, workload="frame"), "batch": autotune.apply(ctx, workload="batch")}
-    autotune.apply(pctx, workload="frame")
+              "frame": autotune.apply(pctx, workload="frame"), "batch": autotune.apply(ctx, workload="batch")}
+    if overlap:  # overlapped steps have their own shapes (tuned on throughput, not one frame's stages)
+        tuning["frame_overlapped"] = autotune.apply(pipe.ctx, workload="frame_overlapped")
+    else:
+        autotune.apply(pipe.ctx, workload="frame")
 
     counters = {}
 

@@ -11,9 +11,13 @@ voxel and grid output are exact whatever the shape or cell), so the tuner only m
 
 Method: coordinate descent.  Starting from the current values, each knob in turn is swept over
 its candidates with the others held, and the value with the lowest median time of the stages
-it drives (ctx stage profiler: hipEvents on the library's stream) is kept; `rounds` passes.
+it drives (ctx stage profiler: hipEvents on the library's stream) is kept; `rounds` passes.  The
+overlapped frame workload is scored by wall time per frame instead: its steps run side by side
+(two lanes, graph replay), so the device time a shape takes from the other frames in flight
+counts, not one frame's stage latency (fewer lanes per query use fewer workgroup slots).
 
     python -m cuda_3d_amd.autotune --workload frame --out cuda_3d_amd/tuning_mi355x.json  # on the GPU
+    python -m cuda_3d_amd.autotune --workload frame_overlapped --out cuda_3d_amd/tuning_mi355x.json
     python -m cuda_3d_amd.autotune --workload batch --out cuda_3d_amd/tuning_mi355x.json
 
 each writes its workload's entry {"tuning": {...}, "workload": ..., "measurements": [[round, knob,
@@ -62,6 +66,7 @@ STAGES_OF = {
 # never reads the one-pair lanes or the normals shape)
 UNUSED_BY = {
     "frame": ("icp_lanes_batch",),
+    "frame_overlapped": ("icp_lanes_batch", "icp_persist_blocks"),
     "batch": ("normals_lanes", "icp_lanes_small", "icp_lanes_medium", "icp_lanes_large", "grid_cell"),
 }
 
@@ -78,9 +83,23 @@ def measure(ctx, workload: Callable[[], None], stages: Sequence[str], reps: int)
     return statistics.median(times)
 
 
+def measure_wall(ctx, workload: Callable[[], None], stages: Sequence[str], reps: int, warm: int = 2) -> float:
+    """Median over `reps` runs of the wall time (ms) of one workload run, after `warm` unmeasured
+    runs (a tuning change drops a pipeline's graphs: the first steps after it are eager)."""
+    for _ in range(warm):
+        workload()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        workload()
+        times.append((time.perf_counter() - t0) * 1e3)
+    return statistics.median(times)
+
+
 def tune(ctx, workload: Callable[[], None], knobs: Dict[str, List[int]], rounds: int = 2, reps: int = 5,
-         log: Callable[[str], None] = print) -> tuple:
-    """Coordinate descent; returns (best settings, list of measurements)."""
+         log: Callable[[str], None] = print, score=measure) -> tuple:
+    """Coordinate descent; returns (best settings, list of measurements).  score(ctx, workload,
+    stages, reps) -> ms is the quantity minimised."""
     best = {k: v for k, v in ctx.get_tuning().items() if k in knobs}
     rows = []
     workload()  # warm
@@ -91,7 +110,7 @@ def tune(ctx, workload: Callable[[], None], knobs: Dict[str, List[int]], rounds:
             scores = {}
             for v in cands:
                 ctx.set_tuning(**{knob: v})
-                t = measure(ctx, workload, stages, reps)
+                t = score(ctx, workload, stages, reps)
                 scores[v] = t
                 rows.append({"round": rnd, "knob": knob, "value": v, "ms": t, "settings": dict(best, **{knob: v})})
             v_best = min(scores, key=scores.get)
@@ -135,8 +154,9 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--steps", type=int, default=6, help="frames per measured run")
     ap.add_argument("--knobs", default="", help="comma-separated subset of knobs")
-    ap.add_argument("--workload", choices=("frame", "batch"), default="frame",
-                    help="frame: 640x480 odometry step; batch: 64 uneven pairs x 3 levels, one launch")
+    ap.add_argument("--workload", choices=("frame", "frame_overlapped", "batch"), default="frame",
+                    help="frame: 640x480 odometry step (stage times); frame_overlapped: the same in overlapped "
+                         "steps (wall time per frame); batch: 64 uneven pairs x 3 levels, one launch")
     args = ap.parse_args(argv)
 
     from . import core as g
@@ -152,11 +172,19 @@ def main(argv=None):
         keep = set(args.knobs.split(","))
         knobs = {k: v for k, v in knobs.items() if k in keep}
     say = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    score = measure
     if args.workload == "frame":
         wl = workloads.frame_workload(ctx, steps=args.steps)
         ctx = wl.ctx  # the pipeline runs (and is tuned) on its own ctx
         what = (f"640x480 frame pipeline (3 levels, normals K30, 3-level colored ICP), {args.steps} frames "
                 f"per run, median of {args.reps}")
+    elif args.workload == "frame_overlapped":
+        steps = max(args.steps, 24)
+        wl = workloads.frame_workload(ctx, steps=steps, overlap=True)
+        ctx = wl.ctx
+        score = measure_wall
+        what = (f"640x480 frame pipeline in overlapped steps (two lanes, graph replay), wall time of {steps} "
+                f"frames per run, median of {args.reps}")
     else:
         bp = workloads.BatchPairs(ctx, log=say)
 
@@ -165,7 +193,7 @@ def main(argv=None):
             ctx.synchronize()
         what = f"64 uneven pairs x 3 levels in one persistent launch, median of {args.reps}"
     t0 = time.perf_counter()
-    best, rows = tune(ctx, wl, knobs, rounds=args.rounds, reps=args.reps, log=say)
+    best, rows = tune(ctx, wl, knobs, rounds=args.rounds, reps=args.reps, log=say, score=score)
     table = {}
     if os.path.exists(args.out):
         with open(args.out) as fh:

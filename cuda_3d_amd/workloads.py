@@ -20,10 +20,13 @@ LEVELS = (0.04, 0.02, 0.01)
 LAMBDA = 0.968
 
 
-def frame_workload(ctx, steps: int = 6, W: int = 640, H: int = 480, seed: int = 12, levels=LEVELS, iters=30):
-    """Returns run() -> None that pushes `steps` frames through a FramePipeline (eager launches:
-    the stage profiler brackets each launch) and syncs.  run.pipe is the pipeline, run.ctx its
-    own ctx (where its tuning and profiling live)."""
+def frame_workload(ctx, steps: int = 6, W: int = 640, H: int = 480, seed: int = 12, levels=LEVELS, iters=30,
+                   overlap: bool = False):
+    """Returns run() -> None that pushes `steps` frames through a FramePipeline and syncs.
+    overlap=False: sequential eager steps (the stage profiler brackets each launch); overlap=True:
+    overlapped, graph-replayed steps (their throughput is what counts: run() returns nothing, the
+    caller times it).  run.pipe is the pipeline, run.ctx its own ctx (where its tuning and
+    profiling live)."""
     from . import core as g
     from . import scene
     from ._lib import BILATERAL_FIXED
@@ -35,7 +38,7 @@ def frame_workload(ctx, steps: int = 6, W: int = 640, H: int = 480, seed: int = 
     dev = [(g.device_tensor(td.view(np.int16)), g.device_tensor(tc)),
            (g.device_tensor(sd.view(np.int16)), g.device_tensor(sc))]
     ptrs = [(d.data_ptr(), c.data_ptr()) for d, c in dev]
-    pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=False)
+    pipe = FramePipeline(ctx, cam, bil, levels, iters, graph=overlap, overlap=overlap)
     state = {"k": 0}
 
     def run():

@@ -1,6 +1,7 @@
 """Autotuner logic on the CPU: coordinate descent over a fake ctx whose stage times are a known
 function of the launch shape."""
 import json
+import time
 
 from cuda_3d_amd import autotune
 
@@ -69,9 +70,22 @@ def test_table_round_trip(tmp_path):
 
 
 def test_committed_table_is_loadable():
-    """The MI355X table holds both workloads' settings, every key a knob of this build."""
+    """The MI355X table holds every workload's settings, every key a knob of this build."""
     from cuda_3d_amd import _lib as L
 
-    for w in ("frame", "batch"):
+    for w in ("frame", "frame_overlapped", "batch"):
         s = autotune.load(autotune.DEFAULT_TABLE, w)
         assert s and all(k in L.TUNE_KEYS for k in s)
+
+
+def test_wall_score_times_runs_after_warm_ones():
+    """The overlapped workload's score: wall time of a run, after unmeasured warm runs (a tuning
+    change drops a pipeline's graphs, so its first steps after it are eager)."""
+    ran = []
+
+    def run():
+        ran.append(1)
+        time.sleep(0.01 if len(ran) <= 2 else 0.001)
+
+    ms = autotune.measure_wall(FakeCtx(), run, ("icp",), reps=3, warm=2)
+    assert len(ran) == 5 and 0.5 <= ms < 8.0
