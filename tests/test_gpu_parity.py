@@ -1261,6 +1261,46 @@ def test_pipeline_grid_cell_change_mid_sequence(ctx):
     pipe.close()
 
 
+def test_pipeline_overlap_tuning_changes(ctx):
+    """Two-lane overlapped steps across tuning changes (grid cell edge, lanes per query): a change
+    synchronises both lanes, drops the plans and graphs, and the following steps run eagerly and
+    capture again.  Every pose still equals the sequential pipeline's one step later: to the last
+    bits for the cell edge and the normals' lanes (exact neighbour sets and fixed-point moments),
+    within 1e-10 for the ICP's lanes (they only reorder its FP64 partial sums), with the same
+    iteration counts.  Poses are read every third step, so steps are in flight across a change."""
+    import cuda_3d_amd as g
+    from cuda_3d_amd.pipeline import FramePipeline
+
+    cam, dev, _, _ = _pipeline_frames(g, 320, 240, 24)
+    bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
+    ref = FramePipeline(ctx, cam, bil, graph=False)
+    ovl = FramePipeline(ctx, cam, bil, graph=True, overlap=True)
+    assert ovl.slots == 4
+    changes = {4: dict(grid_cell=5), 9: dict(normals_lanes=4, icp_lanes_small=2, icp_lanes_medium=2),
+               14: dict(grid_cell=4, normals_lanes=8)}
+    want = []
+    for k in range(20):
+        if k in changes:
+            print(f"[gpu-parity] step {k}: {changes[k]}", flush=True)
+            ovl.ctx.set_tuning(**changes[k])
+        d, c = dev[k % 2]
+        ref.step(d.data_ptr(), c.data_ptr())
+        Tr, rr = ref.result()
+        want.append((Tr, [r.iterations for r in rr]))  # the reports are views of the pipeline's buffer
+        ovl.step(d.data_ptr(), c.data_ptr())
+        if k % 3 != 2:
+            continue
+        To, ro = ovl.result()
+        np.testing.assert_allclose(To, want[k - 1][0], rtol=0, atol=1e-10)
+        assert [r.iterations for r in ro] == want[k - 1][1]
+    ovl.flush()
+    To, ro = ovl.result()
+    np.testing.assert_allclose(To, want[-1][0], rtol=0, atol=1e-10)
+    assert [r.iterations for r in ro] == want[-1][1]
+    ref.close()
+    ovl.close()
+
+
 def test_pipeline_rejects_bad_arguments(ctx):
     import ctypes as C
 
