@@ -89,6 +89,7 @@ struct gca_pipeline_s {
     // overlapped steps in two lanes (lane 0 is ctx / rctx / lctx): lane 1's pipeline stream,
     // registration stream and level scratch (created on first use), and the cross-lane events
     int lanes = 1;
+    bool lane1_ready = false;
     gca_ctx ctx1 = nullptr, rctx1 = nullptr;
     gca_ctx lctx1[kPipeMaxLevels] = {};
     hipEvent_t ev_slot_built[kPipeSlots] = {};  // slot s's build done (on its lane)
@@ -543,7 +544,8 @@ void set_geometry(gca_pipeline p)
 // four distinct queues, or parts meant to run side by side are serialised by the queue they
 // share (measured: 0.215 ms per 640x480 frame for the first pipeline of a process, about 0.26 for
 // later ones, by the queues they happened to get).  A stream made with a CU mask gets a queue of
-// its own; the mask here is every CU of the device.
+// its own; the mask here is every CU of the device.  A runtime that refuses such a stream leaves
+// c on its shared one (slower, not wrong).
 int32_t own_queue(gca_ctx c)
 {
     hipDeviceProp_t prop;
@@ -552,7 +554,10 @@ int32_t own_queue(gca_ctx c)
     std::vector<uint32_t> mask((size_t)(n + 31) / 32, 0xffffffffu);
     if (n % 32) mask.back() = (1u << (n % 32)) - 1u;
     hipStream_t s = nullptr;
-    GCA_HIP(hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()));
+    if (hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()) != hipSuccess || !s) {
+        (void)hipGetLastError();
+        return GCA_OK;
+    }
     const hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         hipStreamDestroy(s);
@@ -567,17 +572,20 @@ int32_t own_queue(gca_ctx c)
 // of two-lane steps on queues of their own.
 int32_t make_lane1(gca_pipeline p)
 {
-    if (p->ctx1) return GCA_OK;
     const int dev = p->ctx->device;
-    GCA_TRY(gca_ctx_create(dev, &p->ctx1));
-    p->ctx1->prof_owner = p->ctx;
-    GCA_TRY(gca_ctx_create(dev, &p->rctx1));
-    p->rctx1->prof_owner = p->ctx;
-    for (int l = 0; l < p->nl; ++l) {
-        GCA_TRY(gca_ctx_create(dev, &p->lctx1[l]));
-        p->lctx1[l]->prof_owner = p->ctx;
-    }
+    auto make = [&](gca_ctx *c) -> int32_t {
+        if (!*c) {
+            GCA_TRY(gca_ctx_create(dev, c));
+            (*c)->prof_owner = p->ctx;
+        }
+        return GCA_OK;
+    };
+    if (p->lane1_ready) return GCA_OK;
+    GCA_TRY(make(&p->ctx1));
+    GCA_TRY(make(&p->rctx1));
+    for (int l = 0; l < p->nl; ++l) GCA_TRY(make(&p->lctx1[l]));
     for (gca_ctx c : {p->ctx, p->rctx, p->ctx1, p->rctx1}) GCA_TRY(own_queue(c));
+    p->lane1_ready = true;
     return GCA_OK;
 }
 
@@ -816,7 +824,17 @@ int32_t gca_pipeline_set_overlap(gca_pipeline p, int32_t enable)
     p->overlap = enable != 0;
     // overlapped steps run in two lanes when the levels can be built in lockstep
     p->lanes = p->overlap && lockstep_ok(p) ? 2 : 1;
-    if (p->lanes == 2) GCA_TRY(make_lane1(p));
+    if (p->lanes == 2) {
+        const int32_t st = make_lane1(p);
+        if (st != GCA_OK) {  // stay a sequential pipeline
+            p->overlap = false;
+            p->lanes = 1;
+            p->nslots = 2;
+            p->frames = 0;
+            p->pending = p->last = -1;
+            return st;
+        }
+    }
     // synchronising the pipeline ctx waits for both lanes (registrations are joined by events)
     p->ctx->sync_also[0] = p->lanes == 2 ? p->rctx : nullptr;
     p->ctx->sync_also[1] = p->lanes == 2 ? p->ctx1 : nullptr;
