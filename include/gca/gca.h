@@ -470,9 +470,10 @@ int32_t gca_pipeline_step(gca_pipeline pipeline, const uint16_t *depth_dev, cons
  * GCA_ERR_INVALID_VALUE if a level's voxel lattice was refused (point_cloud.cu:213-218). */
 int32_t gca_pipeline_result(gca_pipeline pipeline, double T[16], gca_icp_report *reports,
                             int32_t *valid);
-/* Replay each frame parity's step as a hipGraph (captured after one eager step per parity, and
- * again when the ctx tuning changes).  A step with other frame buffers than the graph's updates
- * the one kernel node that reads them (hipGraphExecKernelNodeSetParams) instead of capturing. */
+/* Replay each frame slot's step as a hipGraph (captured after one eager step per slot, and again
+ * when the ctx tuning changes; two-lane overlapped steps are two graphs per slot, the build and
+ * the registration).  A step with other frame buffers than the graph's updates the one kernel node
+ * that reads them (hipGraphExecKernelNodeSetParams) instead of capturing. */
 int32_t gca_pipeline_set_graph(gca_pipeline pipeline, int32_t enable);
 /* Graph captures and front-end node updates since the pipeline was created (no reference
  * counterpart: replay bookkeeping). */
