@@ -7,14 +7,14 @@ include/gca/gca.h).  This package is the Python mirror of the reference's host c
 from ._lib import (ACCUM_COMPENSATED, ACCUM_FP64, ACCUM_NAIVE_FP32, BILATERAL_FIXED,
                    BILATERAL_NONE, BILATERAL_REFERENCE_ADAPTIVE, RESIDUAL_ABS, RESIDUAL_PAIR,
                    RESIDUAL_RMSE, GcaError, IcpParams, IcpReport, LevelParams)
-from .core import (ColorICP, Context, PointCloud, batch_icp, batch_multiscale_icp, bilateral, build_gauss_newton,
-                   compute_residuals, default_context, device_tensor, icp_params, level_params,
+from .core import (BatchPlan, ColorICP, Context, PointCloud, batch_icp, batch_multiscale_icp, bilateral,
+                   build_gauss_newton, compute_residuals, default_context, device_tensor, icp_params, level_params,
                    make_camera, multiscale_icp, objective_grad, solve_gauss_newton)
 
 __all__ = [
     "ACCUM_COMPENSATED", "ACCUM_FP64", "ACCUM_NAIVE_FP32", "BILATERAL_FIXED", "BILATERAL_NONE",
     "BILATERAL_REFERENCE_ADAPTIVE", "RESIDUAL_ABS", "RESIDUAL_PAIR", "RESIDUAL_RMSE", "GcaError",
-    "IcpParams", "IcpReport", "LevelParams", "ColorICP", "Context", "PointCloud", "batch_icp", "batch_multiscale_icp",
+    "IcpParams", "IcpReport", "LevelParams", "BatchPlan", "ColorICP", "Context", "PointCloud", "batch_icp", "batch_multiscale_icp",
     "bilateral", "build_gauss_newton", "compute_residuals", "default_context", "device_tensor",
     "icp_params", "level_params", "make_camera", "multiscale_icp", "objective_grad",
     "solve_gauss_newton",

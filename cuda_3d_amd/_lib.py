@@ -132,6 +132,12 @@ _SIGS = {
     "gca_color_icp_objective_grad": (i32, [vp, vp, vp, vp, f32, vp, vp, vp, vp, vp]),
     "gca_color_icp_batch": (i32, [vp, i32, vp, vp, P(IcpParams), f32, vp, vp, vp]),
     "gca_color_icp_batch_multiscale": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
+    "gca_icp_batch_plan_create": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, i32, P(vp)]),
+    "gca_icp_batch_plan_enqueue": (i32, [vp]),
+    "gca_icp_batch_plan_set_initial": (i32, [vp, vp]),
+    "gca_icp_batch_plan_results": (i32, [vp, vp, vp]),
+    "gca_icp_batch_plan_captures": (i32, [vp, P(i64)]),
+    "gca_icp_batch_plan_destroy": (i32, [vp]),
     "gca_multiscale_icp": (i32, [vp, vp, vp, i32, P(LevelParams), vp, vp, vp]),
     "gca_pipeline_create": (i32, [vp, P(Camera), P(Bilateral), f32, f32, i32, vp, P(vp)]),
     "gca_pipeline_destroy": (i32, [vp]),

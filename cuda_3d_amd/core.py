@@ -623,6 +623,61 @@ def batch_multiscale_icp(srcs, tgts, params, T0s, gradient_radii):
     return T.reshape(n, 4, 4), [list(reps[k * nl:(k + 1) * nl]) for k in range(n)]
 
 
+class BatchPlan:
+    """gca_icp_batch_plan: batch_multiscale_icp set up once, run many times.  srcs[k][l],
+    tgts[k][l], params, gradient_radii and T0s as for batch_multiscale_icp.  enqueue() queues a
+    run without a host round trip (graph=True: the plan replays a graph of its own after the
+    first run); results() synchronises and returns (T [n,4,4], reports [n][levels])."""
+
+    def __init__(self, srcs, tgts, params, T0s, gradient_radii, graph: bool = True):
+        n, nl = len(srcs), len(params)
+        if any(len(s) != nl for s in srcs) or any(len(t_) != nl for t_ in tgts) or len(tgts) != n:
+            raise ValueError("srcs / tgts must be n_pairs x n_levels")
+        self.ctx = srcs[0][0].ctx
+        self.n, self.nl = n, nl
+        self._clouds = (srcs, tgts)   # the plan reads them: keep them alive
+        hs = (C.c_void_p * (n * nl))(*[c.handle.value for row in srcs for c in row])
+        ht = (C.c_void_p * (n * nl))(*[c.handle.value for row in tgts for c in row])
+        prm = (L.IcpParams * nl)(*params)
+        gr = (C.c_float * nl)(*[float(r) for r in gradient_radii])
+        T0 = np.ascontiguousarray(np.asarray(T0s, np.float64).reshape(n, 16))
+        h = C.c_void_p()
+        check(lib.gca_icp_batch_plan_create(self.ctx.handle, n, nl, hs, ht, prm, gr, _ptr(T0), int(graph),
+                                            C.byref(h)), "gca_icp_batch_plan_create")
+        self.handle = h
+
+    def enqueue(self) -> None:
+        check(lib.gca_icp_batch_plan_enqueue(self.handle), "gca_icp_batch_plan_enqueue")
+
+    def set_initial(self, T0s) -> None:
+        T0 = np.ascontiguousarray(np.asarray(T0s, np.float64).reshape(self.n, 16))
+        check(lib.gca_icp_batch_plan_set_initial(self.handle, _ptr(T0)), "gca_icp_batch_plan_set_initial")
+
+    def results(self):
+        T = np.zeros((self.n, 16))
+        reps = (L.IcpReport * (self.n * self.nl))()
+        check(lib.gca_icp_batch_plan_results(self.handle, _ptr(T), reps), "gca_icp_batch_plan_results")
+        nl = self.nl
+        return T.reshape(self.n, 4, 4), [list(reps[k * nl:(k + 1) * nl]) for k in range(self.n)]
+
+    @property
+    def captures(self) -> int:
+        v = C.c_int64()
+        check(lib.gca_icp_batch_plan_captures(self.handle, C.byref(v)), "gca_icp_batch_plan_captures")
+        return v.value
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib.gca_icp_batch_plan_destroy(self.handle), "gca_icp_batch_plan_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def level_params(voxel, normal_radius=None, normal_max_nn=30, gradient_radius=None,
                  gradient_max_nn=0, icp: Optional[L.IcpParams] = None) -> L.LevelParams:
     """One level of the coarse-to-fine schedule; radii default to 2.5 v (normals,

@@ -269,13 +269,17 @@ int32_t launch_copy_cloud(gca_ctx ctx, gca_cloud src, gca_cloud dst);
 int32_t launch_upload_finish(gca_ctx ctx, gca_cloud c, int64_t n);
 
 // Registration plans (icp.hip): n_pairs chains of n_levels states set up once (descriptors,
-// buffers, pinned result copies); each launch enqueues set-up + one persistent launch + the
+// buffers, pinned result copies); each run enqueues set-up + the registration launches + the
 // result copy without a host round trip, so it can be captured in a graph.  Results are read
 // after the ctx stream has been synchronised.
 struct IcpPlan;
 int32_t icp_plan_create(gca_ctx ctx, int n_pairs, int n_levels, const gca_cloud *srcs, const gca_cloud *tgts,
                         const gca_icp_params *params, const float *grad_r, const double *T0, IcpPlan **out);
-int32_t icp_plan_launch(gca_ctx ctx, IcpPlan *plan);
+int32_t icp_plan_run(gca_ctx ctx, IcpPlan *plan);
+// Whether every cloud still has the buffers (src: pointers and capacity src_cap[s]; tgt: arrays,
+// grid and gathered attributes) the plan's descriptors were made from.
+bool icp_plan_clouds_current(const IcpPlan *plan, const gca_cloud *srcs, const int64_t *src_cap,
+                             const gca_cloud *tgts);
 // The same launch in parts (state set-up; one state's persistent launch, which takes the
 // state's sizes from its clouds' headers, so it may be enqueued before they are final; results),
 // for overlapping state s's registration with the building of state s + 1's source.

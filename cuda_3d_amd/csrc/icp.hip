@@ -1812,15 +1812,16 @@ int32_t icp_plan_persist(gca_ctx ctx, IcpPlan *p, int s)
     return GCA_OK;
 }
 
-// The states, the launch's error word and an optional caller word, stored by one workgroup
-// straight into pinned host memory (device-accessible): one launch instead of a copy each.
+// The states, the launch's error word and an optional caller word, stored straight into pinned
+// host memory (device-accessible) by a grid-stride loop: one launch instead of a copy each.
 __global__ __launch_bounds__(256) void k_results_to_host(const uint4 *__restrict__ states, int n16,
                                                          const uint32_t *__restrict__ err,
                                                          const uint32_t *__restrict__ extra, uint4 *host_states,
                                                          uint32_t *host_err, uint32_t *host_extra)
 {
-    for (int k = threadIdx.x; k < n16; k += blockDim.x) host_states[k] = states[k];
-    if (threadIdx.x == 0) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += gridDim.x * blockDim.x)
+        host_states[k] = states[k];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         *host_err = *err;
         if (extra) *host_extra = *extra;
     }
@@ -1830,24 +1831,42 @@ int32_t icp_plan_copy_results(gca_ctx ctx, IcpPlan *p, const uint32_t *extra, ui
 {
     static_assert(sizeof(IcpState) % 16 == 0, "states are copied as 16-byte words");
     const int n16 = (int)(sizeof(IcpState) * p->n_states / 16);
-    hipLaunchKernelGGL(k_results_to_host, dim3(1), dim3(256), 0, ctx->stream, (const uint4 *)p->states, n16,
+    const unsigned nb = (unsigned)std::min(64, (n16 + 255) / 256);
+    hipLaunchKernelGGL(k_results_to_host, dim3(nb), dim3(256), 0, ctx->stream, (const uint4 *)p->states, n16,
                        (const uint32_t *)p->err, extra, (uint4 *)p->host_states, p->host_err, host_extra);
     GCA_HIP(hipGetLastError());
     return GCA_OK;
 }
 
-// Set-up + persistent launch + results into pinned memory, all on the ctx stream.
-int32_t icp_plan_launch(gca_ctx ctx, IcpPlan *p)
+// One whole run of a plan on the ctx stream: states set up, every pair and level registered
+// (one resident launch per state for a single pair, else one persistent launch over all), the
+// states and the error word stored into the plan's pinned host memory.  No host round trip and
+// no allocation, so it can be captured.
+int32_t icp_plan_run(gca_ctx ctx, IcpPlan *p)
 {
-    hipLaunchKernelGGL(k_state_init, dim3((p->n_states + 63) / 64), dim3(64), 0, ctx->stream, p->states,
-                       (const PairDesc *)p->pairs, (const StateInit *)p->init, (const double *)p->T0, p->n_states,
-                       p->err);
-    launch_persist_any(ctx, p->mode, p->pairs, p->states, p->n_states, p->err);
+    GCA_TRY(icp_plan_init(ctx, p));
+    if (use_resident(ctx, p->n_pairs, p->unit_cap))
+        for (int s = 0; s < p->n_states; ++s)
+            launch_resident_any(ctx, p->mode, p->pairs, p->states, s, 0, p->unit_cap, p->err);
+    else
+        launch_persist_any(ctx, p->mode, p->pairs, p->states, p->n_states, p->err);
     GCA_HIP(hipGetLastError());
-    GCA_HIP(hipMemcpyAsync(p->host_states, p->states, sizeof(IcpState) * p->n_states, hipMemcpyDeviceToHost,
-                           ctx->stream));
-    GCA_HIP(hipMemcpyAsync(p->host_err, p->err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    return GCA_OK;
+    return icp_plan_copy_results(ctx, p, nullptr, nullptr);
+}
+
+// Whether every cloud of a plan still has the buffers (and, for a target, the neighbour grid
+// and its gathered attributes) the plan's descriptors point at.
+bool icp_plan_clouds_current(const IcpPlan *p, const gca_cloud *srcs, const int64_t *src_cap, const gca_cloud *tgts)
+{
+    for (int s = 0; s < p->n_states; ++s) {
+        const PairDesc &P = p->desc[(size_t)s];
+        const gca_cloud src = srcs[s], tgt = tgts[s];
+        if (P.src != src->xyzi || P.src_hdr != src->hdr || src->cap != src_cap[s]) return false;
+        if (!tgt->has_normals || !tgt->has_gradient || !tgt->grid.attrs_valid) return false;
+        if (P.tgt_hdr != tgt->hdr || P.nrm != tgt->nrm || P.grad != tgt->grad) return false;
+        if (!icp_plan_target_current(p, s, tgt)) return false;
+    }
+    return true;
 }
 
 // After the ctx stream has been synchronised.
@@ -2063,6 +2082,127 @@ int32_t gca_color_icp_batch_multiscale(gca_ctx ctx, int32_t n_pairs, int32_t n_l
     if (n_pairs > 0 && (!srcs || !tgts || !T0 || !T_out)) return GCA_ERR_INVALID_VALUE;
     if (n_pairs == 0) return GCA_OK;
     return run_states(ctx, n_pairs, n_levels, srcs, tgts, params, gradient_radius, T0, T_out, reports);
+}
+
+}  // extern "C"
+
+// A batch registration set up once: its plan, the clouds it reads, and (use_graph) one run
+// captured as a graph.
+struct gca_icp_batch_plan_s {
+    gca_ctx ctx = nullptr;
+    IcpPlan *plan = nullptr;
+    std::vector<gca_cloud> src, tgt;
+    std::vector<int64_t> src_cap;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t captures = 0;
+    bool ran = false;
+};
+
+static int32_t batch_plan_capture(gca_icp_batch_plan b)
+{
+    gca_ctx ctx = b->ctx;
+    hipGraph_t g = nullptr;
+    ctx->capturing = true;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    int32_t st = e == hipSuccess ? icp_plan_run(ctx, b->plan) : hip_status(e);
+    if (e == hipSuccess) {
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        if (st == GCA_OK && e2 != hipSuccess) st = hip_status(e2);
+    }
+    ctx->capturing = false;
+    if (st == GCA_OK && !g) st = GCA_ERR_DEVICE;
+    hipGraphExec_t ex = nullptr;
+    if (st == GCA_OK && (e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0)) != hipSuccess) st = hip_status(e);
+    if (st != GCA_OK) {
+        if (g) hipGraphDestroy(g);
+        return st;
+    }
+    b->graph = g;
+    b->exec = ex;
+    ++b->captures;
+    return GCA_OK;
+}
+
+extern "C" {
+
+int32_t gca_icp_batch_plan_create(gca_ctx ctx, int32_t n_pairs, int32_t n_levels, const gca_cloud *srcs,
+                                  const gca_cloud *tgts, const gca_icp_params *params, const float *gradient_radius,
+                                  const double *T0, int32_t use_graph, gca_icp_batch_plan *out)
+{
+    if (!out) return GCA_ERR_INVALID_VALUE;
+    *out = nullptr;
+    if (!ctx || n_pairs < 1 || n_levels < 1 || n_levels > 16 || !params || !gradient_radius || !srcs || !tgts || !T0)
+        return GCA_ERR_INVALID_VALUE;
+    const int n_states = n_pairs * n_levels;
+    gca_icp_batch_plan b = new (std::nothrow) gca_icp_batch_plan_s();
+    if (!b) return GCA_ERR_OUT_OF_MEMORY;
+    b->ctx = ctx;
+    b->use_graph = use_graph != 0;
+    const int32_t st = icp_plan_create(ctx, n_pairs, n_levels, srcs, tgts, params, gradient_radius, T0, &b->plan);
+    if (st != GCA_OK) {
+        delete b;
+        return st;
+    }
+    b->src.assign(srcs, srcs + n_states);
+    b->tgt.assign(tgts, tgts + n_states);
+    for (int s = 0; s < n_states; ++s) b->src_cap.push_back(srcs[s]->cap);
+    *out = b;
+    return GCA_OK;
+}
+
+int32_t gca_icp_batch_plan_enqueue(gca_icp_batch_plan b)
+{
+    if (!b) return GCA_ERR_INVALID_VALUE;
+    if (!icp_plan_clouds_current(b->plan, b->src.data(), b->src_cap.data(), b->tgt.data()))
+        return GCA_ERR_INVALID_VALUE;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    GCA_HIP(hipStreamIsCapturing(b->ctx->stream, &cs));
+    if (!b->use_graph || cs != hipStreamCaptureStatusNone) {  // eager (or into the caller's capture)
+        GCA_TRY(icp_plan_run(b->ctx, b->plan));
+    } else {
+        if (!b->exec) GCA_TRY(batch_plan_capture(b));
+        GCA_HIP(hipGraphLaunch(b->exec, b->ctx->stream));
+    }
+    b->ran = true;
+    return GCA_OK;
+}
+
+int32_t gca_icp_batch_plan_set_initial(gca_icp_batch_plan b, const double *T0)
+{
+    if (!b || !T0) return GCA_ERR_INVALID_VALUE;
+    IcpPlan *p = b->plan;
+    GCA_HIP(hipStreamSynchronize(b->ctx->stream));
+    GCA_HIP(hipMemcpy(p->T0, T0, sizeof(double) * 16 * p->n_pairs, hipMemcpyHostToDevice));
+    p->T0h.assign(T0, T0 + 16 * p->n_pairs);
+    return GCA_OK;
+}
+
+int32_t gca_icp_batch_plan_results(gca_icp_batch_plan b, double *T_out, gca_icp_report *reports)
+{
+    if (!b || !T_out) return GCA_ERR_INVALID_VALUE;
+    if (!b->ran) return GCA_ERR_NOT_READY;
+    GCA_HIP(hipStreamSynchronize(b->ctx->stream));
+    return icp_plan_results(b->ctx, b->plan, T_out, reports);
+}
+
+int32_t gca_icp_batch_plan_captures(gca_icp_batch_plan b, int64_t *captures)
+{
+    if (!b || !captures) return GCA_ERR_INVALID_VALUE;
+    *captures = b->captures;
+    return GCA_OK;
+}
+
+int32_t gca_icp_batch_plan_destroy(gca_icp_batch_plan b)
+{
+    if (!b) return GCA_OK;
+    const hipError_t e = hipStreamSynchronize(b->ctx->stream);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    icp_plan_destroy(b->plan);
+    delete b;
+    return e == hipSuccess ? GCA_OK : hip_status(e);
 }
 
 int32_t gca_multiscale_icp(gca_ctx ctx, gca_cloud src_full, gca_cloud tgt_full, int32_t n_levels,

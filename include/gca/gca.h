@@ -440,6 +440,35 @@ int32_t gca_color_icp_batch_multiscale(gca_ctx ctx, int32_t n_pairs, int32_t n_l
                                        const gca_icp_params *params, const float *gradient_radius,
                                        const double *T0, double *T_out, gca_icp_report *reports);
 
+/* Batch plan: gca_color_icp_batch_multiscale set up once and run many times (same arguments,
+ * same poses and reports).  create checks the pairs, readies the targets' neighbour grids and
+ * allocates every buffer a run needs.  enqueue queues one run on ctx's stream: state set-up,
+ * the persistent launch over every pair and level, the results into pinned host memory.  It
+ * makes no host round trip and no allocation, so a caller may capture it into a graph of its
+ * own.  With use_graph != 0 the plan captures the run as a graph of its own at the first
+ * enqueue and replays it after (a stream the caller is capturing gets the plain launches).
+ * A run reads the clouds as they are when it executes: sources rewritten in place are
+ * registered afresh.  enqueue returns GCA_ERR_INVALID_VALUE, queueing nothing, once a cloud of
+ * the plan was re-allocated, or a target's neighbour grid, normals or gradient was invalidated
+ * since create: make a new plan then.  set_initial replaces the T0 table (n_pairs x 16) for
+ * later runs; it synchronises ctx's stream.  results synchronises ctx's stream and reads the
+ * last run's poses (n_pairs x 16) and reports (n_pairs * n_levels, may be NULL);
+ * GCA_ERR_NOT_READY before any run.  Runs on one stream are ordered, and each overwrites the
+ * previous one's results.  destroy synchronises and frees (before ctx is destroyed).
+ * No reference counterpart: the reference registers pair by pair (color_icp::align,
+ * src/registration/color_icp.cu:151-234); the batch is this library's. */
+typedef struct gca_icp_batch_plan_s *gca_icp_batch_plan;
+int32_t gca_icp_batch_plan_create(gca_ctx ctx, int32_t n_pairs, int32_t n_levels,
+                                  const gca_cloud *srcs, const gca_cloud *tgts,
+                                  const gca_icp_params *params, const float *gradient_radius,
+                                  const double *T0, int32_t use_graph, gca_icp_batch_plan *plan);
+int32_t gca_icp_batch_plan_enqueue(gca_icp_batch_plan plan);
+int32_t gca_icp_batch_plan_set_initial(gca_icp_batch_plan plan, const double *T0);
+int32_t gca_icp_batch_plan_results(gca_icp_batch_plan plan, double *T_out,
+                                   gca_icp_report *reports);
+int32_t gca_icp_batch_plan_captures(gca_icp_batch_plan plan, int64_t *captures);
+int32_t gca_icp_batch_plan_destroy(gca_icp_batch_plan plan);
+
 /* Multi-scale (coarse-to-fine) colored ICP between two full-resolution clouds: per level,
  * voxel downsample both, normals + colour gradient of the target, then align, carrying the
  * pose.  Reference: the coarse-to-fine schedule of apps/test_color_icp.cpp:378-414. */

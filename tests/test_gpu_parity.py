@@ -775,6 +775,58 @@ def test_launch_strategies_identical(ctx, uneven_pairs):
     assert [rep_key(r) for r in res[0][1]] == [rep_key(r) for r in res[1][1]]
 
 
+def test_batch_plan_matches_batch_calls(ctx, uneven_pairs):
+    """A batch plan's runs (eager, and a graph captured once then replayed) are bit-identical to
+    one-shot batch calls: repeated runs, new initial poses, sources moved in place between runs.
+    A plan whose target lost its gradient refuses to run."""
+    import cuda_3d_amd as g
+    from cuda_3d_amd import scene
+
+    srcs0, tgts = uneven_pairs
+    n = len(srcs0)
+    p = icp_p(g, 30, 0.05)
+    T0b = [scene.random_rigid(31 + k, 3.0, 0.02) for k in range(n)]
+    M = scene.random_rigid(77, 2.0, 0.01)
+    for graph in (False, True):
+        srcs = [s.copy() for s in srcs0]
+        rows_s, rows_t = [[s] for s in srcs], [[t] for t in tgts]
+        want_a = g.batch_multiscale_icp(rows_s, rows_t, [p], [np.eye(4)] * n, [0.0])
+        want_b = g.batch_multiscale_icp(rows_s, rows_t, [p], T0b, [0.0])
+        plan = g.BatchPlan(rows_s, rows_t, [p], [np.eye(4)] * n, [0.0], graph=graph)
+        with pytest.raises(g.GcaError):
+            plan.results()  # no run yet
+        for rep in range(3):
+            print(f"[gpu-parity] batch plan graph={graph} run {rep}", flush=True)
+            plan.enqueue()
+            T, reps = plan.results()
+            np.testing.assert_array_equal(T, want_a[0])
+            assert [rep_key(r[0]) for r in reps] == [rep_key(r[0]) for r in want_a[1]]
+        plan.set_initial(T0b)
+        plan.enqueue()
+        plan.enqueue()  # back to back on one stream: the later run's results
+        T, reps = plan.results()
+        np.testing.assert_array_equal(T, want_b[0])
+        assert [rep_key(r[0]) for r in reps] == [rep_key(r[0]) for r in want_b[1]]
+        for s in srcs:  # rewritten in place: the plan registers the moved points
+            s.transform(M)
+        want_c = g.batch_multiscale_icp(rows_s, rows_t, [p], T0b, [0.0])
+        plan.enqueue()
+        T, _ = plan.results()
+        np.testing.assert_array_equal(T, want_c[0])
+        assert not np.array_equal(want_c[0], want_b[0])
+        assert plan.captures == (1 if graph else 0)
+        plan.close()
+    # a target whose gradient was recomputed: its gathered copy is stale, the plan refuses
+    t = tgts[1].copy()
+    plan = g.BatchPlan([[srcs0[1]]], [[t]], [p], [np.eye(4)], [0.0])
+    plan.enqueue()
+    plan.results()
+    t.compute_color_gradient(0.04, 0)
+    with pytest.raises(g.GcaError):
+        plan.enqueue()
+    plan.close()
+
+
 def test_batch_multiscale_matches_sequential(ctx, frame):
     """Chained levels in one launch == level-by-level calls carrying the pose on the host."""
     import cuda_3d_amd as g

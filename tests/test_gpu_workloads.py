@@ -44,11 +44,18 @@ def oracle_pair(host, levels, iters, conv, lam):
     return T, [r.iterations for r in reps]
 
 
-def test_batch64_matches_oracle(ctx):
-    from cuda_3d_amd.workloads import LAMBDA, BatchPairs
+@pytest.fixture(scope="module")
+def batch64(ctx):
+    from cuda_3d_amd.workloads import BatchPairs
 
     print("[gpu-workloads] batch64: building 64 pairs (seed 2026)", flush=True)
-    bp = BatchPairs(ctx, 64, seed=2026, keep_host=True)
+    return BatchPairs(ctx, 64, seed=2026, keep_host=True)
+
+
+def test_batch64_matches_oracle(batch64):
+    from cuda_3d_amd.workloads import LAMBDA
+
+    bp = batch64
     T, reps = bp.batched()
     print("[gpu-workloads] batch64: batched launch done; oracle per pair", flush=True)
     O.lib()
@@ -64,3 +71,23 @@ def test_batch64_matches_oracle(ctx):
           f"iterations equal in {sum(a == b for a, b in zip(its_gpu, its_orc))}/64 pairs", flush=True)
     assert errs[:, 0].max() <= 1e-5 and errs[:, 1].max() <= 1e-5
     assert its_gpu == its_orc
+
+
+def test_batch64_plan_graph_matches_batched(batch64):
+    """The 64-pair batch as a plan whose run is captured once as a graph and replayed: each run
+    gives the one-shot batch call's poses and reports bit for bit (and so the oracle's, to the
+    tolerance above)."""
+    import cuda_3d_amd as g
+
+    bp = batch64
+    T, reps = bp.batched()
+    plan = g.BatchPlan(bp.src, bp.tgt, bp.params, bp.T0, bp.grad_r, graph=True)
+    key = lambda rs: [[(r.iterations, r.converged, r.n_correspondences, r.rmse) for r in row] for row in rs]
+    for run in range(3):
+        print(f"[gpu-workloads] batch64 plan: graph run {run}", flush=True)
+        plan.enqueue()
+        Tp, rp = plan.results()
+        np.testing.assert_array_equal(Tp, T)
+        assert key(rp) == key(reps)
+    assert plan.captures == 1
+    plan.close()
