@@ -2159,7 +2159,12 @@ int32_t gca_icp_batch_plan_enqueue(gca_icp_batch_plan b)
         return GCA_ERR_INVALID_VALUE;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     GCA_HIP(hipStreamIsCapturing(b->ctx->stream, &cs));
-    if (!b->use_graph || cs != hipStreamCaptureStatusNone) {  // eager (or into the caller's capture)
+    if (cs != hipStreamCaptureStatusNone) {  // into the caller's capture: no profiler events in it
+        b->ctx->capturing = true;
+        const int32_t st = icp_plan_run(b->ctx, b->plan);
+        b->ctx->capturing = false;
+        GCA_TRY(st);
+    } else if (!b->use_graph) {
         GCA_TRY(icp_plan_run(b->ctx, b->plan));
     } else {
         if (!b->exec) GCA_TRY(batch_plan_capture(b));

@@ -827,6 +827,40 @@ def test_batch_plan_matches_batch_calls(ctx, uneven_pairs):
     plan.close()
 
 
+def test_batch_plan_in_caller_graph(ctx, uneven_pairs):
+    """A plan's enqueue captured into the caller's own graph (torch's, on the ctx stream): each
+    replay, with eager work and synchronisations between them, gives the one-shot call's poses."""
+    import torch
+
+    import cuda_3d_amd as g
+
+    srcs, tgts = uneven_pairs
+    n = len(srcs)
+    p = icp_p(g, 30, 0.05)
+    rows_s, rows_t = [[s] for s in srcs], [[t] for t in tgts]
+    want = g.batch_multiscale_icp(rows_s, rows_t, [p], [np.eye(4)] * n, [0.0])
+    plan = g.BatchPlan(rows_s, rows_t, [p], [np.eye(4)] * n, [0.0], graph=True)
+    plan.enqueue()
+    plan.results()  # warm: the plan's own graph
+    assert plan.captures == 1
+    graph = torch.cuda.CUDAGraph()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    with torch.cuda.graph(graph, stream=stream):
+        plan.enqueue()  # the stream is capturing: plain launches into the caller's graph
+    assert plan.captures == 1
+    for rep in range(3):
+        print(f"[gpu-parity] batch plan in a caller graph: replay {rep}", flush=True)
+        x = torch.arange(1 << 16, device="cuda", dtype=torch.float32)
+        assert float(x.sum()) == float((1 << 16) * ((1 << 16) - 1) // 2)
+        graph.replay()
+        torch.cuda.synchronize()
+        T, reps = plan.results()
+        np.testing.assert_array_equal(T, want[0])
+        assert [rep_key(r[0]) for r in reps] == [rep_key(r[0]) for r in want[1]]
+    del graph
+    plan.close()
+
+
 def test_batch_multiscale_matches_sequential(ctx, frame):
     """Chained levels in one launch == level-by-level calls carrying the pose on the host."""
     import cuda_3d_amd as g
