@@ -95,7 +95,6 @@ struct MarkArgs {
     double max_bits;
     VoxelParams *vp_out;
     uint32_t *err, *bitmap;
-    int64_t *lin;
 };
 
 template <int N>
@@ -105,7 +104,6 @@ __global__ __launch_bounds__(256) void k_voxel_mark(Jobs<MarkArgs, N> J)
     const float4 *__restrict__ xyzi = A.xyzi;
     const CloudHdr *__restrict__ hdr = A.hdr;
     uint32_t *__restrict__ bitmap = A.bitmap;
-    int64_t *__restrict__ lin = A.lin;
     __shared__ VoxelParams s_vp;
     if (threadIdx.x == 0) {
         s_vp = voxel_params_of(hdr, A.voxel, A.max_bits);
@@ -121,10 +119,7 @@ __global__ __launch_bounds__(256) void k_voxel_mark(Jobs<MarkArgs, N> J)
     for (int b = blockIdx.x * blockDim.x; b < n; b += stride) {  // block-uniform trip count
         const int i = b + threadIdx.x;
         int64_t L = -1 - (int64_t)lane_id();  // unique keys for inactive lanes
-        if (i < n) {
-            L = voxel_key(vp, xyzi[i]);
-            lin[i] = L;
-        }
+        if (i < n) L = voxel_key(vp, xyzi[i]);
         const int64_t word = L >= 0 ? (L >> 5) : L;
         const int head = run_head(word);
         uint32_t bits = L >= 0 ? (1u << (uint32_t)(L & 31)) : 0u;
@@ -133,8 +128,11 @@ __global__ __launch_bounds__(256) void k_voxel_mark(Jobs<MarkArgs, N> J)
     }
 }
 
-// Accumulation with workgroup pre-aggregation.  A workgroup takes kVoxChunk consecutive points
-// (raster order: a few image rows, whose points share few voxels) and sums their fixed-point
+// Accumulation with workgroup pre-aggregation.  Each point's voxel key is recomputed from the
+// point (the marking pass stores none: the point is read here anyway, and the key costs a few
+// FP32 operations against 16 bytes written and read back per point).  A workgroup takes
+// kVoxChunk consecutive points (raster order: a few image rows, whose points share few voxels)
+// and sums their fixed-point
 // values per voxel in an LDS hash table keyed by the voxel's rank (open addressing, linear
 // probing); it then hands each distinct voxel's sums to global memory once.  Device-scope 64-bit
 // atomics bound this kernel, so the hand-over takes one compare-and-swap on the voxel's claim
@@ -158,7 +156,6 @@ struct AccumArgs {
     const CloudHdr *hdr;
     const VoxelParams *vpp;
     const uint32_t *bitmap, *word_pre;
-    const int64_t *lin;
     unsigned long long *acc;
 };
 
@@ -170,26 +167,29 @@ __global__ __launch_bounds__(kVoxBlock) void k_voxel_accum(Jobs<AccumArgs, N> J)
     const float4 *__restrict__ rgb = A.rgb;
     const uint32_t *__restrict__ bitmap = A.bitmap;
     const uint32_t *__restrict__ word_pre = A.word_pre;
-    const int64_t *__restrict__ lin = A.lin;
     unsigned long long *__restrict__ acc = A.acc;
     __shared__ int32_t t_rank[kVoxTable];
     __shared__ long long t_key[kVoxTable];
     __shared__ unsigned long long t_sum[7][kVoxTable];
-    const int n = A.vpp->bad ? 0 : A.hdr->n;
+    __shared__ VoxelParams s_vp;
+    if (threadIdx.x == 0) s_vp = *A.vpp;
     for (int h = threadIdx.x; h < kVoxTable; h += blockDim.x) {
         t_rank[h] = -1;
 #pragma unroll
         for (int k = 0; k < 7; ++k) t_sum[k][h] = 0ull;
     }
+    __syncthreads();
+    const VoxelParams vp = s_vp;
+    const int n = vp.bad ? 0 : A.hdr->n;
     for (int base = blockIdx.x * kVoxChunk; base < n; base += gridDim.x * kVoxChunk) {  // block-uniform
         __syncthreads();  // table clear (initial, or by the previous chunk's flush)
         const int end = min(base + kVoxChunk, n);
         for (int i = base + (int)threadIdx.x; i < end; i += blockDim.x) {
-            const int64_t L = lin[i];
+            const float4 p = xyzi[i], c = rgb[i];
+            const int64_t L = voxel_key(vp, p);  // the marking pass's key, recomputed
             const uint32_t w = bitmap[L >> 5];
             const uint32_t below = w & ((1u << (uint32_t)(L & 31)) - 1u);
             const int32_t r = (int32_t)(word_pre[L >> 5] + (uint32_t)__popc(below));
-            const float4 p = xyzi[i], c = rgb[i];
             unsigned long long v[7];
             v[0] = (unsigned long long)fx((double)p.x * kFx30);
             v[1] = (unsigned long long)fx((double)p.y * kFx30);
@@ -490,16 +490,15 @@ static int32_t voxel_prepare(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud o
         ctx->bitmap_dirty = false;
     }
     const int64_t nc = std::max<int64_t>(n_cap, 1);
-    int64_t *lin = (int64_t *)ctx_scratch(ctx, 5, sizeof(int64_t) * nc, false);
     uint32_t *word_pre = (uint32_t *)ctx_scratch(ctx, 6, sizeof(uint32_t) * (words + 1), false);
     unsigned long long *acc = (unsigned long long *)ctx_scratch(ctx, 7, sizeof(unsigned long long) * kVoxRec * nc, true);
     VoxelParams *vp = (VoxelParams *)ctx_scratch(ctx, 22, sizeof(VoxelParams), false);
-    if (!lin || !word_pre || !acc || !vp) return GCA_ERR_OUT_OF_MEMORY;
+    if (!word_pre || !acc || !vp) return GCA_ERR_OUT_OF_MEMORY;
     GCA_TRY(scan_job(ctx, ctx->bitmap, word_pre, words, (uint32_t *)&out->hdr->n, wide, &vj->scan));
     vj->nb = (unsigned)std::min<int64_t>(4096, (nc + 255) / 256);
     vj->nb_acc = (unsigned)std::min<int64_t>(4096, (nc + kVoxChunk - 1) / kVoxChunk);
-    vj->mark = MarkArgs{in->xyzi, in->hdr, voxel, max_bits, vp, err, ctx->bitmap, lin};
-    vj->accum = AccumArgs{in->xyzi, in->rgb, in->hdr, vp, ctx->bitmap, word_pre, lin, acc};
+    vj->mark = MarkArgs{in->xyzi, in->hdr, voxel, max_bits, vp, err, ctx->bitmap};
+    vj->accum = AccumArgs{in->xyzi, in->rgb, in->hdr, vp, ctx->bitmap, word_pre, acc};
     vj->fin = FinalArgs{acc, in->hdr, out->hdr, ctx->bitmap, out->xyzi, out->rgb, gc ? gc->d : GridDesc{},
                         gc ? gc->count : nullptr, gc ? gc->cell_of : nullptr};
     return GCA_OK;
