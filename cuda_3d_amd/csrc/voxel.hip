@@ -132,9 +132,9 @@ __global__ __launch_bounds__(256) void k_voxel_mark(Jobs<MarkArgs, N> J)
 // point (the marking pass stores none: the point is read here anyway, and the key costs a few
 // FP32 operations against 16 bytes written and read back per point).  A workgroup takes
 // kVoxChunk consecutive points (raster order: a few image rows, whose points share few voxels)
-// and sums their fixed-point
-// values per voxel in an LDS hash table keyed by the voxel's rank (open addressing, linear
-// probing); it then hands each distinct voxel's sums to global memory once.  Device-scope 64-bit
+// and sums their fixed-point values per voxel in an LDS hash table keyed by the voxel's rank
+// (open addressing, linear probing); it then hands each distinct voxel's sums to global memory
+// once.  Device-scope 64-bit
 // atomics bound this kernel, so the hand-over takes one compare-and-swap on the voxel's claim
 // word: the workgroup that wins it stores its sums plainly into the voxel's "A" record, the
 // others add theirs atomically into the "B" record (a point whose probe sequence found no slot
