@@ -330,7 +330,7 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
+        print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)

@@ -139,7 +139,7 @@ def load(path: str = DEFAULT_TABLE, workload: str = "frame") -> dict:
 
 
 def apply(ctx, path: str = DEFAULT_TABLE, workload: str = "frame") -> dict:
-    """Apply a saved table's settings for `workload` to a ctx (keys unknown to this build are
+    """Apply a saved table's settings for `workload` to a ctx (keys the library does not know are
     ignored)."""
     settings = load(path, workload)
     if settings:
