@@ -827,6 +827,45 @@ def test_batch_plan_matches_batch_calls(ctx, uneven_pairs):
     plan.close()
 
 
+def test_batch_plan_levels_with_skipped_level(ctx, uneven_pairs):
+    """A two-level plan whose second level has max_iterations 0 (skipped: the chain ends at the
+    first level) and a three-level plan, each against the one-shot call, bit for bit."""
+    import cuda_3d_amd as g
+
+    srcs0, tgts0 = uneven_pairs
+    vs = (0.04, 0.02, 0.01)
+    S, Tg = [], []
+    for s, t in zip(srcs0, tgts0):
+        S.append([s.voxel_grid_down_sample(v) for v in vs])
+        row = []
+        for v in vs:
+            c = t.voxel_grid_down_sample(v)
+            c.estimate_normals(2.5 * v, 30)
+            c.compute_color_gradient(2.0 * v, 0)
+            row.append(c)
+        Tg.append(row)
+    n = len(S)
+    T0 = [np.eye(4)] * n
+    for levels, iters in (((0, 1), (10, 0)), ((0, 1, 2), (10, 10, 10))):
+        src = [[S[k][l] for l in levels] for k in range(n)]
+        tgt = [[Tg[k][l] for l in levels] for k in range(n)]
+        prm = [icp_p(g, it, 2.5 * vs[l], conv=(1e-6, 1e-6)) for l, it in zip(levels, iters)]
+        gr = [2.0 * vs[l] for l in levels]
+        want = g.batch_multiscale_icp(src, tgt, prm, T0, gr)
+        for graph in (False, True):
+            print(f"[gpu-parity] batch plan levels {levels} iterations {iters} graph={graph}", flush=True)
+            plan = g.BatchPlan(src, tgt, prm, T0, gr, graph=graph)
+            for _ in range(2):
+                plan.enqueue()
+                T, reps = plan.results()
+                np.testing.assert_array_equal(T, want[0])
+                assert [[rep_key(r) for r in row] for row in reps] == [[rep_key(r) for r in row] for row in want[1]]
+            plan.close()
+        if iters[-1] == 0:
+            assert all(row[-1].iterations == 0 for row in want[1])
+            assert any(row[0].iterations > 0 for row in want[1])
+
+
 def test_batch_plan_in_caller_graph(ctx, uneven_pairs):
     """A plan's enqueue captured into the caller's own graph (torch's, on the ctx stream): each
     replay, with eager work and synchronisations between them, gives the one-shot call's poses."""
