@@ -467,7 +467,15 @@ int32_t capture(gca_pipeline p, int slot, int reg, const uint16_t *depth, const 
         &ex);
     ctx->front_rec = nullptr;
     GCA_TRY(st);
-    if (p->exec[slot]) hipGraphExecDestroy(p->exec[slot]);
+    if (p->exec[slot]) {  // a replacement: an earlier launch of the old graph may still run
+        const int32_t sst = sync_all(p);
+        if (sst != GCA_OK) {
+            hipGraphExecDestroy(ex);
+            hipGraphDestroy(g);
+            return sst;
+        }
+        hipGraphExecDestroy(p->exec[slot]);
+    }
     if (p->graph[slot]) hipGraphDestroy(p->graph[slot]);
     p->exec[slot] = ex;
     p->graph[slot] = g;
@@ -487,7 +495,15 @@ int32_t capture_registration(gca_pipeline p, int reg, int j)
     hipGraphExec_t ex = nullptr;
     gca_ctx rc = lane(p, j).reg;
     GCA_TRY(capture_graph(p, rc, [&]() { return registration_launches(p, reg, rc); }, &g, &ex));
-    if (p->exec_reg[reg]) hipGraphExecDestroy(p->exec_reg[reg]);
+    if (p->exec_reg[reg]) {  // a replacement: an earlier launch of the old graph may still run
+        const int32_t sst = sync_all(p);
+        if (sst != GCA_OK) {
+            hipGraphExecDestroy(ex);
+            hipGraphDestroy(g);
+            return sst;
+        }
+        hipGraphExecDestroy(p->exec_reg[reg]);
+    }
     if (p->graph_reg[reg]) hipGraphDestroy(p->graph_reg[reg]);
     p->exec_reg[reg] = ex;
     p->graph_reg[reg] = g;
@@ -714,7 +730,10 @@ int32_t gca_pipeline_set_graph(gca_pipeline p, int32_t enable)
 {
     if (!p) return GCA_ERR_INVALID_VALUE;
     p->use_graph = enable != 0;
-    if (!p->use_graph) drop_graphs(p);
+    if (!p->use_graph) {
+        GCA_TRY(sync_all(p));  // no executable graph is destroyed while a launch of it may run
+        drop_graphs(p);
+    }
     return GCA_OK;
 }
 

@@ -1426,6 +1426,40 @@ def test_pipeline_overlap_tuning_changes(ctx):
     ovl.close()
 
 
+def test_pipeline_overlap_graph_toggled(ctx):
+    """Two-lane overlapped steps with graph replay switched off (steps in flight: the graphs are
+    dropped after the lanes drain) and on again: every pose still equals the sequential
+    pipeline's one step later, to the last bits, with the same iteration counts."""
+    import cuda_3d_amd as g
+    from cuda_3d_amd.pipeline import FramePipeline
+
+    cam, dev, _, _ = _pipeline_frames(g, 320, 240, 31)
+    bil = g.bilateral(g.BILATERAL_FIXED, 2, 2.0, 30.0)
+    ref = FramePipeline(ctx, cam, bil, graph=False)
+    ovl = FramePipeline(ctx, cam, bil, graph=True, overlap=True)
+    want = []
+    for k in range(18):
+        if k in (7, 12):
+            print(f"[gpu-parity] step {k}: graph {'off' if k == 7 else 'on'}", flush=True)
+            ovl.set_graph(k == 12)
+        d, c = dev[k % 2]
+        ref.step(d.data_ptr(), c.data_ptr())
+        Tr, rr = ref.result()
+        want.append((Tr, [r.iterations for r in rr]))
+        ovl.step(d.data_ptr(), c.data_ptr())
+        if k % 4 != 3:
+            continue
+        To, ro = ovl.result()
+        np.testing.assert_array_equal(To, want[k - 1][0])
+        assert [r.iterations for r in ro] == want[k - 1][1]
+    assert ovl.graph_stats()[0] > 0
+    ovl.flush()
+    To, ro = ovl.result()
+    np.testing.assert_array_equal(To, want[-1][0])
+    ref.close()
+    ovl.close()
+
+
 def test_pipeline_rejects_bad_arguments(ctx):
     import ctypes as C
 
