@@ -47,14 +47,35 @@ struct FrontParams {
     int32_t wr_len[6];
 };
 
+// Back-projection of pixel (x, y) at depth z: x = ((u - 0.5 - cx) z) / fx in the reference's
+// FP32 operation order (cuda_point_cloud_factory.cu:189-200).
+__device__ __forceinline__ void backproject(int x, int y, float z, const gca_intrinsics &di, float &X, float &Y)
+{
+    const float u = (float)x - 0.5f, v = (float)y - 0.5f;
+    X = __fdiv_rn(__fmul_rn(__fsub_rn(u, di.cx), z), di.fx);
+    Y = __fdiv_rn(__fmul_rn(__fsub_rn(v, di.cy), z), di.fy);
+}
+
+// A point's xyzi and rgb records from its position and its colour pixel's B, G, R bytes
+// (bits 0-7, 8-15, 16-23 of bgr3): colour = byte / 255 (cuda_point_cloud_factory.cu:256-261).
+__device__ __forceinline__ void point_records(float X, float Y, float Z, uint32_t bgr3, float4 &oxyz, float4 &orgb)
+{
+    const float inv255 = 1.0f / 255.0f;
+    const float b = __fmul_rn((float)(bgr3 & 0xffu), inv255);
+    const float g = __fmul_rn((float)((bgr3 >> 8) & 0xffu), inv255);
+    const float rr = __fmul_rn((float)((bgr3 >> 16) & 0xffu), inv255);
+    oxyz = make_float4(X, Y, Z, intensity_of(rr, g, b));
+    orgb = make_float4(rr, g, b, 0.0f);
+}
+
 // The pixel work of a workgroup (the row segment x0 .. x0 + 255 of row y): stage the depth rows
 // y-R .. y+R and the weight tables in LDS, then for this thread's pixel the bilateral filter,
 // back-projection, depth -> colour re-projection and BGR sampling in the reference's FP32
-// operation order.  Returns whether the pixel is a valid point; contains a barrier (every
-// thread of the workgroup calls it).
+// operation order.  Returns whether the pixel is a valid point (then oxyz / orgb are its records
+// and bgr3 its colour pixel's bytes); contains a barrier (every thread of the workgroup calls it).
 __device__ __forceinline__ bool front_pixel(const uint16_t *__restrict__ depth, const uint8_t *__restrict__ bgr,
                                             const double *__restrict__ lut, const FrontParams &fp, int y, int x0,
-                                            int n_ws, int n_wr, float4 &oxyz, float4 &orgb)
+                                            int n_ws, int n_wr, float4 &oxyz, float4 &orgb, uint32_t &bgr3)
 {
     // dynamic LDS, sized by the host for this filter: the spatial and range weight tables (fixed
     // mode: every tap reads one of each) followed by the depth tile of the filter's own radius
@@ -110,6 +131,7 @@ __device__ __forceinline__ bool front_pixel(const uint16_t *__restrict__ depth, 
     bool valid = false;
     oxyz = make_float4(0.f, 0.f, 0.f, 0.f);
     orgb = make_float4(0.f, 0.f, 0.f, 0.f);
+    bgr3 = 0u;
     if (x < W) {
         const uint16_t d0 = tile[R * cols + threadIdx.x + R];
         const float z0 = (float)d0 * fp.scale;
@@ -152,9 +174,8 @@ __device__ __forceinline__ bool front_pixel(const uint16_t *__restrict__ depth, 
             }
             const float z = (float)f * fp.scale;
             if (!(z < fp.tmin || z > fp.tmax)) {
-                const float u = (float)x - 0.5f, v = (float)y - 0.5f;
-                const float X = __fdiv_rn(__fmul_rn(__fsub_rn(u, fp.di.cx), z), fp.di.fx);
-                const float Y = __fdiv_rn(__fmul_rn(__fsub_rn(v, fp.di.cy), z), fp.di.fy);
+                float X, Y;
+                backproject(x, y, z, fp.di, X, Y);
                 const float Z = z;
                 const float c0 = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(fp.R[0], X), __fmul_rn(fp.R[3], Y)),
                                                      __fmul_rn(fp.R[6], Z)), fp.t[0]);
@@ -170,12 +191,8 @@ __device__ __forceinline__ bool front_pixel(const uint16_t *__restrict__ depth, 
                     if (fu > -1.0f && fu < (float)W && fv > -1.0f && fv < (float)H) {
                         const int tx = (int)fu, ty = (int)fv;
                         const uint8_t *c = bgr + 3 * ((int64_t)ty * W + tx);
-                        const float inv255 = 1.0f / 255.0f;
-                        const float b = __fmul_rn((float)c[0], inv255);
-                        const float g = __fmul_rn((float)c[1], inv255);
-                        const float rr = __fmul_rn((float)c[2], inv255);
-                        oxyz = make_float4(X, Y, Z, intensity_of(rr, g, b));
-                        orgb = make_float4(rr, g, b, 0.0f);
+                        bgr3 = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16);
+                        point_records(X, Y, Z, bgr3, oxyz, orgb);
                         valid = true;
                     }
                 }
@@ -188,9 +205,7 @@ __device__ __forceinline__ bool front_pixel(const uint16_t *__restrict__ depth, 
 __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restrict__ depth,
                                                          const uint8_t *__restrict__ bgr,
                                                          const double *__restrict__ lut, FrontParams fp,
-                                                         float4 *__restrict__ st_xyzi,
-                                                         float4 *__restrict__ st_rgb,
-                                                         uint8_t *__restrict__ st_valid,
+                                                         uint2 *__restrict__ st_pix,
                                                          uint32_t *__restrict__ block_cnt,
                                                          CloudHdr *__restrict__ hdr, uint32_t *clear_word,
                                                          int n_ws, int n_wr)
@@ -208,13 +223,10 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_pixels(const uint16_t *__restr
         }
     }
     float4 oxyz, orgb;
-    const bool valid = front_pixel(depth, bgr, lut, fp, y, x0, n_ws, n_wr, oxyz, orgb);
-    if (x < W) {
-        const int64_t p = (int64_t)y * W + x;
-        st_xyzi[p] = oxyz;
-        st_rgb[p] = orgb;
-        st_valid[p] = valid ? 1 : 0;
-    }
+    uint32_t bgr3;
+    const bool valid = front_pixel(depth, bgr, lut, fp, y, x0, n_ws, n_wr, oxyz, orgb, bgr3);
+    if (x < W)  // 8 bytes per pixel: the depth in metres, the colour bytes and the valid bit
+        st_pix[(int64_t)y * W + x] = make_uint2(__float_as_uint(oxyz.z), valid ? (bgr3 | (1u << 24)) : 0u);
     uint64_t m = __ballot(valid);
     if (lane_id() == 0) wcount[threadIdx.x / kWave] = (uint32_t)__popcll(m);
     __syncthreads();
@@ -284,7 +296,8 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_onepass(const uint16_t *__rest
     const int y = c / fp.nbx, x0 = (c - y * fp.nbx) * kRowSeg;
     if (c == 0 && threadIdx.x == 0 && clear_word) *clear_word = 0u;
     float4 oxyz, orgb;
-    const bool valid = front_pixel(depth, bgr, lut, fp, y, x0, n_ws, n_wr, oxyz, orgb);
+    uint32_t bgr3;
+    const bool valid = front_pixel(depth, bgr, lut, fp, y, x0, n_ws, n_wr, oxyz, orgb, bgr3);
     uint32_t tot;
     const uint32_t rank = block_excl_scan<kRowSeg>(valid ? 1u : 0u, lds, &tot);
     const uint32_t excl = lookback_exclusive_block<kRowSeg>(flags, c, seq, tot, dev_err, lb);
@@ -300,11 +313,12 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_onepass(const uint16_t *__rest
     block_bounds(valid, oxyz, bpart + (int64_t)c * 6);
 }
 
-__global__ __launch_bounds__(kRowSeg) void k_rgbd_compact(const float4 *__restrict__ st_xyzi,
-                                                          const float4 *__restrict__ st_rgb,
-                                                          const uint8_t *__restrict__ st_valid,
+// Stable compaction of the valid pixels; each point's records are rebuilt from its staged depth
+// and colour bytes with the pixel kernel's own operations (the same bits as records staged whole,
+// at 8 bytes of staging per pixel instead of 33).
+__global__ __launch_bounds__(kRowSeg) void k_rgbd_compact(const uint2 *__restrict__ st_pix,
                                                           const uint32_t *__restrict__ block_cnt,
-                                                          int32_t W, int32_t nbx,
+                                                          gca_intrinsics di, int32_t W, int32_t nbx,
                                                           float4 *__restrict__ out_xyzi,
                                                           float4 *__restrict__ out_rgb,
                                                           float *__restrict__ bpart, CloudHdr *__restrict__ hdr)
@@ -320,7 +334,8 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_compact(const float4 *__restri
     uint32_t pre = 0;
     for (int64_t k = threadIdx.x; k < b; k += kRowSeg) pre += block_cnt[k];
     pre = wave_sum(pre);
-    bool valid = (x < W) && st_valid[p];
+    const uint2 pix = x < W ? st_pix[p] : make_uint2(0u, 0u);
+    const bool valid = (pix.y >> 24) != 0u;
     uint64_t m = __ballot(valid);
     const int w = threadIdx.x / kWave;
     if (lane_id() == 0) {
@@ -343,9 +358,13 @@ __global__ __launch_bounds__(kRowSeg) void k_rgbd_compact(const float4 *__restri
     if (valid) {
         uint64_t below = m & ((1ull << lane_id()) - 1ull);
         uint32_t dst = woff[w] + (uint32_t)__popcll(below);
-        float4 a = st_xyzi[p];
+        const float z = __uint_as_float(pix.x);
+        float X, Y;
+        backproject(x, y, z, di, X, Y);
+        float4 a, c;
+        point_records(X, Y, z, pix.y, a, c);
         out_xyzi[dst] = a;
-        out_rgb[dst] = st_rgb[p];
+        out_rgb[dst] = c;
         mn[0] = mx[0] = a.x;
         mn[1] = mx[1] = a.y;
         mn[2] = mx[2] = a.z;
@@ -461,8 +480,7 @@ struct FrontLaunch {
     const uint8_t *bgr = nullptr;
     const double *lut = nullptr;
     FrontParams fp{};
-    float4 *st_xyzi = nullptr, *st_rgb = nullptr;
-    uint8_t *st_valid = nullptr;
+    uint2 *st_pix = nullptr;
     uint32_t *bcnt = nullptr;
     CloudHdr *hdr = nullptr;
     uint32_t *clear_word = nullptr;
@@ -511,8 +529,8 @@ hipError_t front_launch_update(FrontLaunch *f, hipGraphExec_t exec, hipGraphNode
     if (!f || !f->valid || !exec || !node) return hipErrorInvalidValue;
     f->depth = depth;
     f->bgr = bgr;
-    void *pix[] = {&f->depth, &f->bgr, &f->lut, &f->fp, &f->st_xyzi, &f->st_rgb, &f->st_valid,
-                   &f->bcnt, &f->hdr, &f->clear_word, &f->n_ws, &f->n_wr};
+    void *pix[] = {&f->depth, &f->bgr, &f->lut, &f->fp, &f->st_pix, &f->bcnt, &f->hdr, &f->clear_word, &f->n_ws,
+                   &f->n_wr};
     void *one[] = {&f->depth, &f->bgr, &f->lut, &f->fp, &f->flags, &f->ctl, &f->nchunks, &f->out_xyzi,
                    &f->out_rgb, &f->bpart, &f->hdr, &f->clear_word, &f->n_ws, &f->n_wr, &f->dev_err};
     hipKernelNodeParams kp{};
@@ -595,11 +613,9 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
         GCA_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, ctx->stream, (const float *)bpart, nblocks, out->hdr);
     } else {
-        float4 *st_xyzi = (float4 *)ctx_scratch(ctx, 0, sizeof(float4) * P, false);
-        float4 *st_rgb = (float4 *)ctx_scratch(ctx, 1, sizeof(float4) * P, false);
-        uint8_t *st_valid = (uint8_t *)ctx_scratch(ctx, 2, (size_t)P, false);
+        uint2 *st_pix = (uint2 *)ctx_scratch(ctx, 0, sizeof(uint2) * P, false);
         uint32_t *bcnt = (uint32_t *)ctx_scratch(ctx, 3, sizeof(uint32_t) * (nblocks + 1), false);
-        if (!st_xyzi || !st_rgb || !st_valid || !bcnt) return GCA_ERR_OUT_OF_MEMORY;
+        if (!st_pix || !bcnt) return GCA_ERR_OUT_OF_MEMORY;
         dim3 grid(fp.nbx, H);
         if (FrontLaunch *r = ctx->front_rec) {
             *r = FrontLaunch{};
@@ -610,9 +626,7 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
             r->bgr = bgr;
             r->lut = (const double *)ctx->lut;
             r->fp = fp;
-            r->st_xyzi = st_xyzi;
-            r->st_rgb = st_rgb;
-            r->st_valid = st_valid;
+            r->st_pix = st_pix;
             r->bcnt = bcnt;
             r->hdr = out->hdr;
             r->clear_word = clear_word;
@@ -621,11 +635,10 @@ int32_t launch_from_rgbd(gca_ctx ctx, const uint16_t *depth, const uint8_t *bgr,
             r->valid = true;
         }
         hipLaunchKernelGGL(k_rgbd_pixels, grid, dim3(kRowSeg), lds, ctx->stream, depth, bgr,
-                           (const double *)ctx->lut, fp, st_xyzi, st_rgb, st_valid, bcnt, out->hdr, clear_word, n_ws,
-                           n_wr);
+                           (const double *)ctx->lut, fp, st_pix, bcnt, out->hdr, clear_word, n_ws, n_wr);
         GCA_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_rgbd_compact, grid, dim3(kRowSeg), 0, ctx->stream, st_xyzi, st_rgb, st_valid,
-                           (const uint32_t *)bcnt, W, fp.nbx, out->xyzi, out->rgb, bpart, out->hdr);
+        hipLaunchKernelGGL(k_rgbd_compact, grid, dim3(kRowSeg), 0, ctx->stream, (const uint2 *)st_pix,
+                           (const uint32_t *)bcnt, fp.di, W, fp.nbx, out->xyzi, out->rgb, bpart, out->hdr);
         hipLaunchKernelGGL(k_bounds_fold, dim3(1), dim3(256), 0, ctx->stream, (const float *)bpart, nblocks, out->hdr);
     }
     GCA_HIP(hipGetLastError());
