@@ -495,8 +495,11 @@ static int32_t voxel_prepare(gca_ctx ctx, gca_cloud in, float voxel, gca_cloud o
     VoxelParams *vp = (VoxelParams *)ctx_scratch(ctx, 22, sizeof(VoxelParams), false);
     if (!word_pre || !acc || !vp) return GCA_ERR_OUT_OF_MEMORY;
     GCA_TRY(scan_job(ctx, ctx->bitmap, word_pre, words, (uint32_t *)&out->hdr->n, wide, &vj->scan));
-    vj->nb = (unsigned)std::min<int64_t>(4096, (nc + 255) / 256);
-    vj->nb_acc = (unsigned)std::min<int64_t>(4096, (nc + kVoxChunk - 1) / kVoxChunk);
+    // grid widths in multiples of 8: workgroups go to the 8 XCDs by linear id mod 8, so in an
+    // all-levels launch (level = blockIdx.y) the levels' workgroups on the same points then share
+    // an XCD and its L2, and the points are fetched from HBM once rather than once per level
+    vj->nb = (unsigned)((std::min<int64_t>(4096, (nc + 255) / 256) + 7) / 8 * 8);
+    vj->nb_acc = (unsigned)((std::min<int64_t>(4096, (nc + kVoxChunk - 1) / kVoxChunk) + 7) / 8 * 8);
     vj->mark = MarkArgs{in->xyzi, in->hdr, voxel, max_bits, vp, err, ctx->bitmap};
     vj->accum = AccumArgs{in->xyzi, in->rgb, in->hdr, vp, ctx->bitmap, word_pre, acc};
     vj->fin = FinalArgs{acc, in->hdr, out->hdr, ctx->bitmap, out->xyzi, out->rgb, gc ? gc->d : GridDesc{},
