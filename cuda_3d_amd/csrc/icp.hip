@@ -66,6 +66,15 @@ __host__ __device__ inline uint32_t subq_units(int nvb, int x)
     return nvb > x ? (uint32_t)((nvb - x + kSubQ - 1) / kSubQ) : 0u;
 }
 
+// Unit c of sub-queue x: the sub-queues hold consecutive ranges of units (in the order above:
+// sizes q + 1 for the first nvb mod kSubQ, q after), so the workgroups of one XCD, which serve
+// their own sub-queue first, register neighbouring source points against one target region.
+__host__ __device__ inline int subq_unit(int nvb, int x, int c)
+{
+    const int q = nvb / kSubQ, r = nvb % kSubQ;
+    return x * q + (x < r ? x : r) + c;
+}
+
 struct IcpState {
     double T[16];
     double sys[kNSys];
@@ -870,7 +879,7 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_persist(con
                 __hip_atomic_fetch_add(&st->q[x].w, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const uint32_t e = (uint32_t)(old >> 32), c = (uint32_t)old;
             const bool ok = e < kEpochWait && c < subq_units(refresh ? s_nvb : st->nvb, x);
-            s_vb = ok ? x + kSubQ * (int)c : -1;
+            s_vb = ok ? subq_unit(refresh ? s_nvb : st->nvb, x, (int)c) : -1;
             s_epoch = e;
             if (ok && e == 0 && c == 0 && x == 0) st->t_start = wall_clock64();  // unit 0 of the first epoch
         } else if (threadIdx.x >= kWave && threadIdx.x < kWave + 12) {
