@@ -10,6 +10,21 @@ namespace gca {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Workgroups are dealt to the device's 8 XCDs round-robin (linear workgroup id mod 8), and each
+// XCD has its own L2.  xcd_runs<G>(b, m) permutes [0, m) so that each XCD takes runs of G
+// consecutive logical blocks, the runs dealt to the XCDs in turn: neighbouring blocks (which
+// read the same neighbourhoods) share an L2 while the work stays spread over every XCD.  Blocks
+// beyond the last whole round of 8 runs keep their place.
+constexpr int kXcds = 8;
+template <int G>
+__device__ __forceinline__ int xcd_runs(int b, int m)
+{
+    const int whole = m / (kXcds * G) * (kXcds * G);
+    if (b >= whole) return b;
+    const int x = b % kXcds, i = b / kXcds;
+    return ((i / G) * kXcds + x) * G + i % G;
+}
+
 // Device-scope ordering for the last-block (ticket) pattern.  __threadfence() is a seq_cst
 // fence, which on gfx950 both writes back AND invalidates the XCD's L2: issued by every block
 // it evicts the data the other blocks are still reading.  Writers only need release

@@ -1037,8 +1037,11 @@ __global__ __launch_bounds__(kIcpBlock) GCA_ICP_OCCUPANCY void k_icp_resident(co
         Tc[t - 64] = st->T[t - 64];
     }
     __syncthreads();
-    const int nvb = s_nvb, vb = (int)blockIdx.x;
-    if (vb >= nvb || f_stop) return;
+    const int nvb = s_nvb;
+    if ((int)blockIdx.x >= nvb || f_stop) return;
+    // runs of 4 units (neighbouring source points) per XCD: the runs' target reads share an L2
+    // (the partition, and so the sums, do not change)
+    const int vb = xcd_runs<4>((int)blockIdx.x, nvb);
     if (vb == 0 && t == 0) st->t_start = wall_clock64();
     __builtin_amdgcn_s_setprio(3);  // a chain of dependent round trips; see k_icp_persist
     const int grp = vb % kSubQ;
